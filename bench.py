@@ -1,0 +1,230 @@
+"""Benchmark of the betweenness hot path on one MI355X.
+
+A step is one batched launch sequence over one batch of source vertices
+(label scan -> layered dependencies -> BC accumulation) on the seeded
+road-like 320,000-vertex graph with its MHL index resident in HBM.  The value
+is whole-job full-BC throughput in sources/s; the plain one-source-per-launch
+form, the RQ serving latencies and the CPU baseline are reported beside it.
+
+  python bench.py [--steps K] [--warmup W] [--workload road320k] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="road320k")
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--plain-sources", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rq", action="store_true")
+    ap.add_argument("--rq-requests", type=int, default=6)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by source ranges, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import gpu_betweennesscentrality_amd as bcx
+    from gpu_betweennesscentrality_amd import graphgen as gg
+
+    t0 = time.time()
+    g = gg.make(args.workload)
+    G = bcx.Graph.from_arcs(g.n, g.u, g.v, g.w)
+    log(f"[bench] graph {args.workload}: n={g.n} arcs={g.arcs} ({time.time() - t0:.1f}s)")
+    ix = bcx.Index.build_mhl(G)
+    st = ix.stats()
+    lbar = ix.mean_scan(1 << 18, 1)
+    log(f"[bench] index: height={st['tree_height']} width={st['tree_width']} "
+        f"entries={st['label_entries']} Lbar={lbar:.1f} build={st['build_seconds']:.1f}s")
+    ctx = bcx.Context(G, ix, 0)
+    n = g.n
+    B = min(args.batch, n)
+    ctx.set_tuning(batch=B)
+
+    # Batches are blocks of the tree order (how a full sweep visits sources),
+    # spread evenly over it so the sample covers shallow and deep subtrees.
+    tree_perm = ix.tree_order()
+
+    def tree_batch(k: int, nb: int) -> np.ndarray:
+        start = (k * (n - B)) // max(1, nb - 1) if nb > 1 else 0
+        start -= start % 64
+        return tree_perm[start:start + B]
+
+    total = args.warmup + args.steps
+    for k in range(args.warmup):
+        ctx.bc_sources(tree_batch(k, total), bcx.MODE_BATCHED)
+    ctx.reset_timing()
+    bc = np.zeros(n)
+    t = time.perf_counter()
+    for k in range(args.warmup, total):
+        bc, ovf = ctx.bc_sources(tree_batch(k, total), bcx.MODE_BATCHED, bc)
+    el = time.perf_counter() - t
+    value = args.steps * B / el
+    log(f"[bench] batched: {args.steps} x {B} sources in {el:.3f}s -> {value:.1f} sources/s")
+
+    # Per-kernel timing with HIP events on the context stream (one batch).
+    ctx.set_profiling(True)
+    ctx.reset_timing()
+    ctx.bc_sources(tree_batch(total // 2, total), bcx.MODE_BATCHED)
+    tm = ctx.timing()
+    ctx.set_profiling(False)
+    scan_ms = tm["scan_ms"]
+    dep_ms = tm["dep_ms"]
+    red_ms = tm["reduce_ms"]
+    log(f"[bench] per batch: scan {scan_ms:.2f} ms, dep {dep_ms:.2f} ms, reduce {red_ms:.2f} ms")
+    S = 64
+    scan_bytes = B * (16.0 * n * lbar / S + 16.0 * n)
+    achieved = scan_bytes / (scan_ms * 1e-3) / 1e9
+
+    # Plain form: one source per launch sequence.
+    ps = tree_perm[: args.plain_sources]
+    ctx.bc_sources(ps[:2], bcx.MODE_PLAIN)
+    t = time.perf_counter()
+    ctx.bc_sources(ps, bcx.MODE_PLAIN)
+    plain = len(ps) / (time.perf_counter() - t)
+    log(f"[bench] plain: {plain:.1f} sources/s ({value / plain:.2f}x batched/plain)")
+
+    rq = None
+    if not args.no_rq:
+        rq = _rq_bench(bcx, gg, ctx, g, args.rq_requests)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = _cpu_baseline(g, n, args.cpu_seconds)
+
+    out = {
+        "metric": "sources/sec for full BC (batched vs plain) + RQ p50/p99 latency, 1 MI355X",
+        "value": round(value, 2),
+        "unit": "sources/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * el / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32+u64 (labels, counts) / f64 (dependencies, BC)",
+        "data": "synthetic (seeded road-like Delaunay graph, generated in-process)",
+        "config": {
+            "workload": f"{args.workload}: batched MHL-QBC full BC, n={n}, arcs={g.arcs}, "
+                        f"{B} sources per batched launch sequence",
+            "n": n, "arcs": int(g.arcs), "batch": B, "tile": S,
+            "tree_height": st["tree_height"], "tree_width": st["tree_width"],
+            "label_entries": st["label_entries"], "Lbar": round(lbar, 2),
+            "plain_sources_per_s": round(plain, 2),
+            "batched_over_plain": round(value / plain, 2),
+            "full_bc_seconds_est": round(n / value, 2),
+            "scan_ms_per_batch": round(scan_ms, 3), "dep_ms_per_batch": round(dep_ms, 3),
+            "reduce_ms_per_batch": round(red_ms, 3),
+            "count_overflow": bool(ovf),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "scan_tile_kernel",
+        },
+        "cpu_baseline": cpu,
+        "rq": rq,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def _rq_bench(bcx, gg, ctx, g, nreq: int) -> dict:
+    """Continuous-batching RQ serving: Poisson arrivals, admission by budget."""
+    n = g.n
+    row = np.zeros(n + 1, np.int64)
+    np.add.at(row, g.u.astype(np.int64) + 1, 1)
+    row = np.cumsum(row)
+    sets = gg.query_sets(n, nreq, seed=4, adjacency=(row, g.v))
+    free, _ = ctx.mem_info()
+    srv = bcx.RQServer(ctx, int(free * 0.25))
+    # Saturating arrival rate: mean gap = one batch time / 2.
+    rng_gap = gg.uniform01(4, 30, nreq)
+    t_batch = None
+    arrivals = []
+    submitted = {}
+    done = {}
+    t0 = time.perf_counter()
+    pending = list(range(nreq))
+    next_arrival = 0.0
+    while len(done) < nreq:
+        now = time.perf_counter() - t0
+        while pending and now >= next_arrival:
+            i = pending[0]
+            rid = srv.submit(sets[i])
+            if rid is None:
+                break
+            pending.pop(0)
+            submitted[rid] = (i, time.perf_counter())
+            gap = -np.log(1.0 - rng_gap[i]) * (t_batch or 0.01) * 0.5
+            next_arrival = now + gap
+        ts = time.perf_counter()
+        srv.step()
+        t_step = time.perf_counter() - ts
+        t_batch = t_step if t_batch is None else 0.9 * t_batch + 0.1 * t_step
+        for rid in list(submitted):
+            r = srv.poll(rid)
+            if r is not None:
+                i, ta = submitted.pop(rid)
+                done[i] = time.perf_counter() - ta
+        if not submitted and pending:
+            next_arrival = min(next_arrival, time.perf_counter() - t0)
+    wall = time.perf_counter() - t0
+    lat = np.array([done[i] for i in range(nreq)])
+    qs = np.array([len(s) for s in sets])
+    res = {"requests": nreq, "p50_s": round(float(np.percentile(lat, 50)), 4),
+           "p99_s": round(float(np.percentile(lat, 99)), 4),
+           "p99_over_p50": round(float(np.percentile(lat, 99) / np.percentile(lat, 50)), 3),
+           "requests_per_s": round(nreq / wall, 4), "queries_per_s": round(float(qs.sum()) / wall, 1),
+           "q_sizes": qs.tolist(), "batch_seconds": round(t_batch or 0.0, 4)}
+    log(f"[bench] rq: {res}")
+    return res
+
+
+def _cpu_baseline(g, n: int, budget_s: float) -> dict:
+    """Oracle Dijkstra-Brandes (one thread) on a bounded sample of sources."""
+    import oracle
+    from gpu_betweennesscentrality_amd import graphgen as gg
+
+    csr = oracle.CSR.from_arcgraph(g)
+    src = gg.permutation(9, 1, n).astype(np.int32)
+    t = time.perf_counter()
+    oracle.brandes(csr, src[:2])
+    per = max(1e-4, (time.perf_counter() - t) / 2)
+    k = int(max(4, min(n, budget_s / per)))
+    t = time.perf_counter()
+    oracle.brandes(csr, src[:k])
+    el = time.perf_counter() - t
+    res = {"value": round(k / el, 3), "unit": "sources/s", "cores": 1, "kind": "port",
+           "sample": f"{k} seeded random sources of the same graph, Dijkstra + Brandes "
+                     f"(oracle/oracle.c), {el:.1f}s"}
+    log(f"[bench] cpu baseline: {res}")
+    return res
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,369 @@
+"""bcx — MI355X-native exact betweenness centrality (QBC / MHL-QBC / RQ path).
+
+Python mirror of the C ABI in include/bcx.h (ctypes, numpy arrays).  The
+native library ``libbcx.so`` next to this file holds the HIP kernels and the
+host index builder; importing this package never falls back to anything else:
+a missing or stale library raises ``BcxError``.
+
+Reference entry points mirrored (CSU-WRZ/GPU_BetweennessCentrality):
+  Graph.read            Graph::ReadGraph (src/Graph.cpp:4)
+  Index.build_mhl       TreeDecomp Reduce/MakeTree/MakeIndex (MHLQBC.cu:497-501)
+  Index.build_order     CHconsorderMT/makeTree/makeIndex (QBC.cu:421-423)
+  Context.full_bc       Graph::brandesBCIndexGPU_5 (MHLQBC.cu:495, QBC.cu:419)
+  Context.subset_bc     Graph::brandesBCIndexGPU_PointBatch (RQ.cu:465)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+__all__ = ["BcxError", "Graph", "Index", "Context", "RQServer", "lib", "MODE_PLAIN", "MODE_BATCHED"]
+
+MODE_PLAIN = 0
+MODE_BATCHED = 1
+_PKG = Path(__file__).resolve().parent
+_LIB = None
+
+
+class BcxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bcx error {code}: {msg}")
+        self.code = code
+
+
+class IndexStats(C.Structure):
+    _fields_ = [("n", C.c_int32), ("tree_height", C.c_int32), ("tree_width", C.c_int32),
+                ("count_overflow", C.c_int32), ("label_entries", C.c_int64),
+                ("mean_depth", C.c_double), ("build_seconds", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile", C.c_int32), ("tchunk", C.c_int32), ("bp_threads", C.c_int32),
+                ("batch", C.c_int32)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("scan_ms", C.c_double), ("dep_ms", C.c_double), ("reduce_ms", C.c_double),
+                ("scan_launches", C.c_int64), ("dep_launches", C.c_int64), ("sources", C.c_int64)]
+
+
+_P = C.c_void_p
+_I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_U32P = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_U64P = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+_SIGS = {
+    "bcx_last_error": (C.c_char_p, []),
+    "bcx_version": (C.c_char_p, []),
+    "bcx_graph_read": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "bcx_graph_from_arcs": (C.c_int, [C.c_int32, C.c_int64, _I32P, _I32P, _I32P, C.POINTER(_P)]),
+    "bcx_graph_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "bcx_graph_connected": (C.c_int, [_P]),
+    "bcx_graph_free": (None, [_P]),
+    "bcx_index_build_mhl": (C.c_int, [_P, C.POINTER(_P)]),
+    "bcx_index_build_order": (C.c_int, [_P, C.c_char_p, C.POINTER(_P)]),
+    "bcx_index_build_with_order": (C.c_int, [_P, _I32P, C.POINTER(_P)]),
+    "bcx_index_stats_get": (C.c_int, [_P, C.POINTER(IndexStats)]),
+    "bcx_index_query": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "bcx_index_mean_scan": (C.c_int, [_P, C.c_int64, C.c_uint64, C.POINTER(C.c_double)]),
+    "bcx_index_tree_order": (C.c_int, [_P, _I32P]),
+    "bcx_index_save": (C.c_int, [_P, C.c_char_p]),
+    "bcx_index_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "bcx_index_free": (None, [_P]),
+    "bcx_ctx_create": (C.c_int, [C.c_int, _P, _P, C.POINTER(_P)]),
+    "bcx_ctx_free": (None, [_P]),
+    "bcx_ctx_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "bcx_ctx_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "bcx_ctx_workspace_bytes": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_size_t)]),
+    "bcx_ctx_mem_info": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "bcx_ctx_timing": (C.c_int, [_P, C.POINTER(Timing)]),
+    "bcx_ctx_timing_reset": (None, [_P]),
+    "bcx_ctx_set_profiling": (C.c_int, [_P, C.c_int]),
+    "bcx_full_bc": (C.c_int, [_P, C.c_int, _F64P, C.POINTER(C.c_uint32)]),
+    "bcx_bc_sources": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, _F64P, C.POINTER(C.c_uint32)]),
+    "bcx_query_dist_count": (C.c_int, [_P, C.c_int32, _I32P, _U64P, _U32P]),
+    "bcx_subset_bc": (C.c_int, [_P, _I32P, C.c_int32, _F64P]),
+    "bcx_rq_create": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "bcx_rq_free": (None, [_P]),
+    "bcx_rq_submit": (C.c_int, [_P, _I32P, C.c_int32, C.POINTER(C.c_int64)]),
+    "bcx_rq_cancel": (C.c_int, [_P, C.c_int64]),
+    "bcx_rq_step": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "bcx_rq_poll": (C.c_int, [_P, C.c_int64, _F64P]),
+    "bcx_rq_bytes_needed": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_size_t)]),
+}
+
+
+def lib() -> C.CDLL:
+    """Load libbcx.so (in-tree build) and declare every C ABI signature."""
+    global _LIB
+    if _LIB is None:
+        path = Path(os.environ.get("BCX_LIBRARY", _PKG / "libbcx.so"))
+        if not path.exists():
+            raise BcxError(-5, f"{path} not built; run `python -m gpu_betweennesscentrality_amd.build`")
+        L = C.CDLL(str(path))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise BcxError(rc, lib().bcx_last_error().decode(errors="replace"))
+    return rc
+
+
+class Graph:
+    """Parsed graph (host memory)."""
+
+    def __init__(self, handle: int):
+        self._h = _P(handle)
+
+    @classmethod
+    def read(cls, path: str | os.PathLike) -> "Graph":
+        h = _P()
+        _check(lib().bcx_graph_read(str(path).encode(), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_arcs(cls, n: int, u, v, w) -> "Graph":
+        """0-based arc arrays; every edge listed both ways with one weight >= 1."""
+        u = np.ascontiguousarray(u, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.int32)
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        h = _P()
+        _check(lib().bcx_graph_from_arcs(int(n), len(u), u, v, w, C.byref(h)))
+        return cls(h.value)
+
+    @property
+    def n(self) -> int:
+        n = C.c_int32()
+        _check(lib().bcx_graph_info(self._h, C.byref(n), None))
+        return n.value
+
+    @property
+    def arcs(self) -> int:
+        m = C.c_int64()
+        _check(lib().bcx_graph_info(self._h, None, C.byref(m)))
+        return m.value
+
+    def connected(self) -> bool:
+        return bool(_check(lib().bcx_graph_connected(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().bcx_graph_free(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Index:
+    """Tree-decomposition label index (distance, uint64 count, max-hop)."""
+
+    def __init__(self, handle: int):
+        self._h = _P(handle)
+
+    @classmethod
+    def build_mhl(cls, g: Graph) -> "Index":
+        h = _P()
+        _check(lib().bcx_index_build_mhl(g._h, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def build_order(cls, g: Graph, order) -> "Index":
+        """order: path of an order file, or an array order[v] = contraction rank."""
+        h = _P()
+        if isinstance(order, (str, os.PathLike)):
+            _check(lib().bcx_index_build_order(g._h, str(order).encode(), C.byref(h)))
+        else:
+            o = np.ascontiguousarray(order, dtype=np.int32)
+            _check(lib().bcx_index_build_with_order(g._h, o, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load(cls, path) -> "Index":
+        h = _P()
+        _check(lib().bcx_index_load(str(path).encode(), C.byref(h)))
+        return cls(h.value)
+
+    def save(self, path) -> None:
+        _check(lib().bcx_index_save(self._h, str(path).encode()))
+
+    def stats(self) -> dict:
+        st = IndexStats()
+        _check(lib().bcx_index_stats_get(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in IndexStats._fields_}
+
+    def query(self, s: int, t: int) -> tuple[int, int, int, bool]:
+        d, c, h = C.c_int32(), C.c_uint64(), C.c_uint32()
+        rc = _check(lib().bcx_index_query(self._h, s, t, C.byref(d), C.byref(c), C.byref(h)))
+        return d.value, c.value, h.value, rc == 1
+
+    def tree_order(self) -> np.ndarray:
+        """Vertex ids in DFS preorder of the decomposition tree."""
+        n = self.stats()["n"]
+        out = np.zeros(n, np.int32)
+        _check(lib().bcx_index_tree_order(self._h, out))
+        return out
+
+    def mean_scan(self, pairs: int = 1 << 18, seed: int = 1) -> float:
+        m = C.c_double()
+        _check(lib().bcx_index_mean_scan(self._h, pairs, seed, C.byref(m)))
+        return m.value
+
+    def close(self) -> None:
+        if self._h:
+            lib().bcx_index_free(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """Device-resident graph + index on one GPU."""
+
+    def __init__(self, g: Graph, ix: Index, device: int = 0):
+        h = _P()
+        _check(lib().bcx_ctx_create(device, g._h, ix._h, C.byref(h)))
+        self._h = h
+        self.n = g.n
+
+    def set_tuning(self, **kw) -> None:
+        t = self.tuning()
+        t.update(kw)
+        tn = Tuning(**t)
+        _check(lib().bcx_ctx_set_tuning(self._h, C.byref(tn)))
+
+    def tuning(self) -> dict:
+        tn = Tuning()
+        _check(lib().bcx_ctx_get_tuning(self._h, C.byref(tn)))
+        return {k: getattr(tn, k) for k, _ in Tuning._fields_}
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib().bcx_ctx_set_profiling(self._h, int(on)))
+
+    def timing(self) -> dict:
+        t = Timing()
+        _check(lib().bcx_ctx_timing(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+    def reset_timing(self) -> None:
+        lib().bcx_ctx_timing_reset(self._h)
+
+    def workspace_bytes(self, batch: int) -> int:
+        b = C.c_size_t()
+        _check(lib().bcx_ctx_workspace_bytes(self._h, batch, C.byref(b)))
+        return b.value
+
+    def mem_info(self) -> tuple[int, int]:
+        f, t = C.c_size_t(), C.c_size_t()
+        _check(lib().bcx_ctx_mem_info(self._h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def full_bc(self, mode: int = MODE_BATCHED) -> tuple[np.ndarray, bool]:
+        out = np.zeros(self.n, np.float64)
+        ovf = C.c_uint32()
+        _check(lib().bcx_full_bc(self._h, mode, out, C.byref(ovf)))
+        return out, bool(ovf.value)
+
+    def bc_sources(self, sources, mode: int = MODE_BATCHED, bc=None) -> tuple[np.ndarray, bool]:
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        out = np.zeros(self.n, np.float64) if bc is None else np.ascontiguousarray(bc, np.float64)
+        ovf = C.c_uint32()
+        _check(lib().bcx_bc_sources(self._h, mode, src, len(src), out, C.byref(ovf)))
+        return out, bool(ovf.value)
+
+    def query(self, source: int) -> tuple[np.ndarray, np.ndarray, np.ndarray, bool]:
+        d = np.zeros(self.n, np.int32)
+        s = np.zeros(self.n, np.uint64)
+        h = np.zeros(self.n, np.uint32)
+        rc = _check(lib().bcx_query_dist_count(self._h, source, d, s, h))
+        return d, s, h, rc == 1
+
+    def subset_bc(self, points) -> np.ndarray:
+        p = np.ascontiguousarray(points, dtype=np.int32)
+        out = np.zeros(len(p), np.float64)
+        _check(lib().bcx_subset_bc(self._h, p, len(p), out))
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            lib().bcx_ctx_free(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RQServer:
+    """Continuous-batching subset-BC engine over one Context (bcx_rq_*)."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int):
+        h = _P()
+        _check(lib().bcx_rq_create(ctx._h, hbm_budget_bytes, C.byref(h)))
+        self._h = h
+        self._ctx = ctx
+        self._q: dict[int, int] = {}
+
+    def bytes_needed(self, q: int) -> int:
+        b = C.c_size_t()
+        _check(lib().bcx_rq_bytes_needed(self._h, q, C.byref(b)))
+        return b.value
+
+    def submit(self, points) -> int | None:
+        """Request id, or None when it cannot be admitted now (BCX_EBUSY)."""
+        p = np.ascontiguousarray(points, dtype=np.int32)
+        rid = C.c_int64()
+        rc = lib().bcx_rq_submit(self._h, p, len(p), C.byref(rid))
+        if rc == -6:
+            return None
+        _check(rc)
+        self._q[rid.value] = len(p)
+        return rid.value
+
+    def cancel(self, rid: int) -> None:
+        _check(lib().bcx_rq_cancel(self._h, rid))
+        self._q.pop(rid, None)
+
+    def step(self) -> int:
+        a = C.c_int32()
+        _check(lib().bcx_rq_step(self._h, C.byref(a)))
+        return a.value
+
+    def poll(self, rid: int) -> np.ndarray | None:
+        out = np.zeros(self._q[rid], np.float64)
+        done = _check(lib().bcx_rq_poll(self._h, rid, out))
+        if done:
+            self._q.pop(rid, None)
+            return out
+        return None
+
+    def close(self) -> None:
+        if self._h:
+            lib().bcx_rq_free(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,195 @@
+// C ABI: graph and index entry points (host only).  Device entry points live
+// in bcx_device.hip.  Each entry point converts exceptions into a return code
+// and a thread-local message.
+#include <cstring>
+#include <new>
+#include <random>
+#include <string>
+
+#include "bcx.h"
+#include "bcx_host.hpp"
+#include "bcx_internal.hpp"
+
+struct bcx_graph {
+  bcx::Graph g;
+};
+struct bcx_index {
+  bcx::Index ix;
+};
+
+namespace bcx {
+thread_local std::string g_last_error;
+void set_error(const std::string &m) { g_last_error = m; }
+const Graph &graph_of(const bcx_graph *g) { return g->g; }
+const Index &index_of(const bcx_index *ix) { return ix->ix; }
+}  // namespace bcx
+
+extern "C" {
+
+const char *bcx_last_error(void) { return bcx::g_last_error.c_str(); }
+const char *bcx_version(void) { return "bcx 0.1 (gfx950)"; }
+
+int bcx_graph_read(const char *path, bcx_graph **out) {
+  BCX_GUARD_BEGIN
+  if (!path || !out) return bcx::fail(BCX_EINVAL, "null argument");
+  auto *h = new bcx_graph();
+  try {
+    bcx::read_graph_file(path, h->g);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_graph_from_arcs(int32_t n, int64_t arcs, const int32_t *u, const int32_t *v,
+                        const int32_t *w, bcx_graph **out) {
+  BCX_GUARD_BEGIN
+  if (!out || (arcs > 0 && (!u || !v || !w))) return bcx::fail(BCX_EINVAL, "null argument");
+  auto *h = new bcx_graph();
+  try {
+    bcx::graph_from_arcs(n, arcs, u, v, w, h->g);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_graph_info(const bcx_graph *g, int32_t *n, int64_t *arcs) {
+  if (!g) return bcx::fail(BCX_EINVAL, "null graph");
+  if (n) *n = g->g.n;
+  if (arcs) *arcs = g->g.arcs();
+  return BCX_OK;
+}
+
+int bcx_graph_connected(const bcx_graph *g) {
+  if (!g) return bcx::fail(BCX_EINVAL, "null graph");
+  return bcx::graph_connected(g->g) ? 1 : 0;
+}
+
+void bcx_graph_free(bcx_graph *g) { delete g; }
+
+static int build_common(const bcx_graph *g, const std::vector<int32_t> &order, bcx_index **out) {
+  if (!bcx::graph_connected(g->g))
+    return bcx::fail(BCX_EGRAPH, "graph is disconnected; the index needs a connected graph");
+  auto *h = new bcx_index();
+  try {
+    bcx::build_index(g->g, order, h->ix);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  return BCX_OK;
+}
+
+int bcx_index_build_mhl(const bcx_graph *g, bcx_index **out) {
+  BCX_GUARD_BEGIN
+  if (!g || !out) return bcx::fail(BCX_EINVAL, "null argument");
+  return build_common(g, {}, out);
+  BCX_GUARD_END
+}
+
+int bcx_index_build_order(const bcx_graph *g, const char *order_path, bcx_index **out) {
+  BCX_GUARD_BEGIN
+  if (!g || !order_path || !out) return bcx::fail(BCX_EINVAL, "null argument");
+  std::vector<int32_t> order;
+  bcx::read_order_file(order_path, g->g.n, order);
+  return build_common(g, order, out);
+  BCX_GUARD_END
+}
+
+int bcx_index_build_with_order(const bcx_graph *g, const int32_t *order, bcx_index **out) {
+  BCX_GUARD_BEGIN
+  if (!g || !order || !out) return bcx::fail(BCX_EINVAL, "null argument");
+  const int32_t n = g->g.n;
+  std::vector<int32_t> ord(order, order + n);
+  std::vector<char> used((size_t)n, 0);
+  for (int32_t v = 0; v < n; ++v) {
+    if (ord[(size_t)v] < 0 || ord[(size_t)v] >= n || used[(size_t)ord[(size_t)v]])
+      return bcx::fail(BCX_EINVAL, "order is not a permutation of 0..n-1");
+    used[(size_t)ord[(size_t)v]] = 1;
+  }
+  return build_common(g, ord, out);
+  BCX_GUARD_END
+}
+
+int bcx_index_stats_get(const bcx_index *ix, bcx_index_stats *st) {
+  if (!ix || !st) return bcx::fail(BCX_EINVAL, "null argument");
+  const auto &x = ix->ix;
+  st->n = x.n;
+  st->tree_height = x.tree_height;
+  st->tree_width = x.tree_width;
+  st->count_overflow = x.count_overflow;
+  st->label_entries = x.entries();
+  st->mean_depth = x.n ? (double)x.entries() / x.n : 0.0;
+  st->build_seconds = x.build_seconds;
+  return BCX_OK;
+}
+
+int bcx_index_query(const bcx_index *ix, int32_t s, int32_t t, int32_t *d, uint64_t *sigma,
+                    uint32_t *hop) {
+  if (!ix) return bcx::fail(BCX_EINVAL, "null index");
+  const auto &x = ix->ix;
+  if (s < 0 || s >= x.n || t < 0 || t >= x.n) return bcx::fail(BCX_EINVAL, "vertex out of range");
+  int32_t dd;
+  uint64_t ss;
+  uint32_t hh;
+  bool ovf;
+  bcx::index_query(x, s, t, dd, ss, hh, ovf);
+  if (d) *d = dd;
+  if (sigma) *sigma = ss;
+  if (hop) *hop = hh;
+  return ovf ? 1 : BCX_OK;
+}
+
+int bcx_index_tree_order(const bcx_index *ix, int32_t *order) {
+  if (!ix || !order) return bcx::fail(BCX_EINVAL, "null argument");
+  std::memcpy(order, ix->ix.perm.data(), ix->ix.perm.size() * sizeof(int32_t));
+  return BCX_OK;
+}
+
+int bcx_index_mean_scan(const bcx_index *ix, int64_t pairs, uint64_t seed, double *mean) {
+  if (!ix || !mean || pairs <= 0) return bcx::fail(BCX_EINVAL, "bad argument");
+  const auto &x = ix->ix;
+  std::mt19937_64 rng(seed);
+  double acc = 0;
+  for (int64_t k = 0; k < pairs; ++k) {
+    int32_t p = (int32_t)(rng() % (uint64_t)x.n), q = (int32_t)(rng() % (uint64_t)x.n);
+    acc += bcx::lca_depth(x, std::min(p, q), std::max(p, q)) + 1;
+  }
+  *mean = acc / (double)pairs;
+  return BCX_OK;
+}
+
+int bcx_index_save(const bcx_index *ix, const char *path) {
+  BCX_GUARD_BEGIN
+  if (!ix || !path) return bcx::fail(BCX_EINVAL, "null argument");
+  bcx::save_index(ix->ix, path);
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_index_load(const char *path, bcx_index **out) {
+  BCX_GUARD_BEGIN
+  if (!path || !out) return bcx::fail(BCX_EINVAL, "null argument");
+  auto *h = new bcx_index();
+  try {
+    bcx::load_index(path, h->ix);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+void bcx_index_free(bcx_index *ix) { delete ix; }
+
+}  // extern "C"

@@ -1,0 +1,878 @@
+// bcx_device.hip — device side of the QBC / MHL-QBC / RQ betweenness path for
+// MI355X (gfx950): device-resident index, label-scan, dependency and BC
+// accumulation kernels, and the C ABI entry points that drive them.
+//
+// Pipeline per batch of sources (the reference runs one fused kernel with one
+// 256-thread block per source, MHLQBC.cu:237-475):
+//   scan    distance, uint64 path count and max-hop layer from each source to
+//           every target by one min/sum/max aggregation over the common prefix
+//           of the two label rows (MHLQBC.cu:302-412).  The LCA depth comes
+//           from a sparse-table range-minimum over tree depths in DFS preorder
+//           (the preorder counterpart of the Euler-tour RMQ, MHLQBC.cu:302-331).
+//           Batched form: one wavefront lane per source of a 64-source tile, so
+//           each target row is read once per tile and is wave-uniform.
+//   dep     per source: bucket vertices by layer (counting sort), then pull
+//           dependencies layer by layer, deepest first:
+//           delta[u] = sum over DAG arcs u->v of sigma_u * (1 + delta_v) / sigma_v
+//           (MHLQBC.cu:415-466, QBC.cu:379-389), without FP64 atomics.
+//   reduce  bc[v] += 0.5 * delta_s[v] for the batch's sources in increasing
+//           source order (MHLQBC.cu:469-471 then addKernel 477-494), so the
+//           result is the same for every batch size and form.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bcx.h"
+#include "bcx_host.hpp"
+#include "bcx_internal.hpp"
+
+#define HIP_OK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      throw bcx::Error{BCX_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+namespace {
+
+constexpr int kInfDev = 999999999;
+constexpr int kWave = 64;
+
+struct DevIndex {
+  int n;
+  int levels;
+  int64_t arcs;
+  const int *depth;       // n, preorder
+  const int64_t *row_off; // n + 1
+  const int *dis;         // entries
+  const uint64_t *cnt;    // entries
+  const uint32_t *hop;    // entries
+  const int *rmq;         // levels * n: rmq[k*n + i] = min depth over [i, i + 2^k)
+  const int *grow;        // n + 1, CSR in preorder ids
+  const int *gcol;        // arcs
+  const int *gw;          // arcs
+};
+
+// Batch bookkeeping lives in device memory so that one captured graph can be
+// replayed for every batch: src list, cursor (first list index of the batch),
+// total list length.
+struct DevBatch {
+  const int *srcs;
+  int *cursor;
+  const int *total;
+};
+
+struct DevWork {
+  int *d;          // slots * n
+  uint64_t *sigma; // slots * n
+  uint32_t *hop;   // slots * n
+  int *order;      // slots * n
+  int *lcnt;       // slots * 2 * (n + 2)
+  double *delta;   // slots * n
+};
+
+__device__ __forceinline__ uint64_t mul_ovf(uint64_t a, uint64_t b, int &ovf) {
+  if (((a | b) >> 32) == 0) return a * b;
+  if (__umul64hi(a, b) != 0) {
+    ovf = 1;
+    return ~0ull;
+  }
+  return a * b;
+}
+
+__device__ __forceinline__ uint64_t add_ovf(uint64_t a, uint64_t b, int &ovf) {
+  const uint64_t r = a + b;
+  if (r < a) {
+    ovf = 1;
+    return ~0ull;
+  }
+  return r;
+}
+
+// Depth of LCA(u, v) for preorder ids: for u < v it is min(depth[u+1..v]) - 1.
+__device__ __forceinline__ int lca_depth_dev(const DevIndex &ix, int u, int v) {
+  if (u == v) return ix.depth[u];
+  const int a = min(u, v) + 1, b = max(u, v);
+  const int k = 31 - __clz(b - a + 1);
+  const int x = ix.rmq[(int64_t)k * ix.n + a];
+  const int y = ix.rmq[(int64_t)k * ix.n + b - (1 << k) + 1];
+  return min(x, y) - 1;
+}
+
+// min / count-sum / hop-max over entries 0..L of two rows.
+__device__ __forceinline__ void scan_rows(const DevIndex &ix, int64_t rs, int64_t rt, int L,
+                                          int &m, uint64_t &c, uint32_t &h, int &ovf) {
+  m = 0x7fffffff;
+  c = 0;
+  h = 0;
+  for (int i = 0; i <= L; ++i) {
+    const int v = ix.dis[rs + i] + ix.dis[rt + i];
+    if (v <= m) {
+      const uint64_t cc = mul_ovf(ix.cnt[rs + i], ix.cnt[rt + i], ovf);
+      const uint32_t hh = ix.hop[rs + i] + ix.hop[rt + i];
+      if (v < m) {
+        m = v;
+        c = cc;
+        h = hh;
+      } else {
+        c = add_ovf(c, cc, ovf);
+        h = max(h, hh);
+      }
+    }
+  }
+}
+
+// Plain form: one source, one thread per target (the reference's mapping).
+__global__ __launch_bounds__(256) void scan_plain_kernel(DevIndex ix, DevBatch bt, DevWork ws,
+                                                         int *ovf_flag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ix.n) return;
+  const int s = bt.srcs[bt.cursor[0]];
+  const int L = lca_depth_dev(ix, s, t);
+  int m, ovf = 0;
+  uint64_t c;
+  uint32_t h;
+  scan_rows(ix, ix.row_off[s], ix.row_off[t], L, m, c, h, ovf);
+  ws.d[t] = m;
+  ws.sigma[t] = c;
+  ws.hop[t] = h;
+  if (ovf) atomicOr(ovf_flag, 1);
+}
+
+// Batched form: lane = source of a 64-source tile, waves walk target chunks.
+__global__ __launch_bounds__(256) void scan_tile_kernel(DevIndex ix, DevBatch bt, DevWork ws,
+                                                        int batch, int tchunk, int *ovf_flag) {
+  const int base = bt.cursor[0];
+  const int total = bt.total[0];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int slot = blockIdx.x * kWave + lane;
+  const bool active = slot < batch && base + slot < total;
+  if (base + (int)blockIdx.x * kWave >= total) return;  // whole tile past the end
+  const int s = active ? bt.srcs[base + slot] : bt.srcs[base + blockIdx.x * kWave];
+  const int64_t rs = ix.row_off[s];
+  const int t0 = (blockIdx.y * (blockDim.x / kWave) + wave) * tchunk;
+  const int n = ix.n;
+  int ovf = 0;
+  for (int tt = 0; tt < tchunk; ++tt) {
+    const int t = __builtin_amdgcn_readfirstlane(t0 + tt);
+    if (t >= n) break;
+    const int64_t rt = ix.row_off[t];
+    const int L = lca_depth_dev(ix, s, t);
+    int m;
+    uint64_t c;
+    uint32_t h;
+    scan_rows(ix, rs, rt, L, m, c, h, ovf);
+    if (active) {
+      const int64_t o = (int64_t)slot * n + t;
+      ws.d[o] = m;
+      ws.sigma[o] = c;
+      ws.hop[o] = h;
+    }
+  }
+  if (ovf) atomicOr(ovf_flag, 1);
+}
+
+// Block-wide exclusive scan of a[0..M) in global memory; returns the total.
+__device__ int block_exclusive_scan(int *a, int M, int *sh) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int per = (M + nt - 1) / nt;
+  const int lo = min(M, tid * per), hi = min(M, lo + per);
+  int sum = 0;
+  for (int i = lo; i < hi; ++i) sum += a[i];
+  sh[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {
+    const int x = tid >= off ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += x;
+    __syncthreads();
+  }
+  int run = sh[tid] - sum;
+  const int tot = sh[nt - 1];
+  for (int i = lo; i < hi; ++i) {
+    const int x = a[i];
+    a[i] = run;
+    run += x;
+  }
+  __syncthreads();
+  return tot;
+}
+
+// One block per source slot: layer buckets + layered pull back-propagation.
+__global__ __launch_bounds__(1024) void dep_kernel(DevIndex ix, DevBatch bt, DevWork ws,
+                                                   int batch) {
+  __shared__ int sh[1024];
+  __shared__ int s_max;
+  const int slot = blockIdx.x;
+  const int base = bt.cursor[0];
+  if (slot >= batch || base + slot >= bt.total[0]) return;
+  const int n = ix.n;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int s = bt.srcs[base + slot];
+  const int *d = ws.d + (int64_t)slot * n;
+  const uint64_t *sg = ws.sigma + (int64_t)slot * n;
+  const uint32_t *hp = ws.hop + (int64_t)slot * n;
+  int *order = ws.order + (int64_t)slot * n;
+  int *start = ws.lcnt + (int64_t)slot * 2 * (n + 2);
+  int *fill = start + (n + 2);
+  double *delta = ws.delta + (int64_t)slot * n;
+
+  if (tid == 0) s_max = 0;
+  __syncthreads();
+  int mx = 0;
+  for (int v = tid; v < n; v += nt) mx = max(mx, (int)hp[v]);
+  atomicMax(&s_max, mx);
+  __syncthreads();
+  const int maxL = s_max;
+  const int M = maxL + 2;
+  for (int i = tid; i < M; i += nt) start[i] = 0;
+  __syncthreads();
+  for (int v = tid; v < n; v += nt) atomicAdd(&start[hp[v]], 1);
+  __syncthreads();
+  block_exclusive_scan(start, M, sh);
+  for (int i = tid; i < M; i += nt) fill[i] = start[i];
+  __syncthreads();
+  for (int v = tid; v < n; v += nt) order[atomicAdd(&fill[hp[v]], 1)] = v;
+  __syncthreads();
+  for (int L = maxL; L >= 1; --L) {
+    const int lo = start[L], hi = start[L + 1];
+    for (int k = lo + tid; k < hi; k += nt) {
+      const int u = order[k];
+      const int du = d[u];
+      const double su = (double)sg[u];
+      double acc = 0.0;
+      for (int a = ix.grow[u]; a < ix.grow[u + 1]; ++a) {
+        const int v = ix.gcol[a];
+        if (d[v] == du + ix.gw[a]) acc += su * (1.0 + delta[v]) / (double)sg[v];
+      }
+      delta[u] = acc;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) delta[s] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void reduce_kernel(int n, DevBatch bt, const double *delta,
+                                                     int batch, double *bc) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int cnt = min(batch, bt.total[0] - bt.cursor[0]);
+  double acc = bc[v];
+  for (int k = 0; k < cnt; ++k) acc += 0.5 * delta[(int64_t)k * n + v];
+  bc[v] = acc;
+}
+
+// Per-batch dependency sums for the RQ engine: bsum[v] = sum over the batch's
+// sources, in slot order, of 0.5 * delta_s[v].
+__global__ __launch_bounds__(256) void batch_sum_kernel(int n, DevBatch bt, const double *delta,
+                                                        int batch, double *bsum) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int cnt = min(batch, bt.total[0] - bt.cursor[0]);
+  double acc = 0.0;
+  for (int k = 0; k < cnt; ++k) acc += 0.5 * delta[(int64_t)k * n + v];
+  bsum[v] = acc;
+}
+
+// contrib[i] = bsum[points[i]] for one request and one batch.
+__global__ __launch_bounds__(256) void gather_kernel(const double *bsum, const int *points, int q,
+                                                     double *contrib) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < q) contrib[i] = bsum[points[i]];
+}
+
+// out[i] = sum over batches b = 0..nb-1 (in that order) of contrib[b*q + i].
+__global__ __launch_bounds__(256) void finish_kernel(const double *contrib, int q, int nb,
+                                                     double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= q) return;
+  double acc = 0.0;
+  for (int b = 0; b < nb; ++b) acc += contrib[(int64_t)b * q + i];
+  out[i] = acc;
+}
+
+__global__ void advance_kernel(DevBatch bt, int step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) bt.cursor[0] += step;
+}
+
+__global__ void fill_f64_kernel(double *p, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <class T>
+T *dev_upload(const std::vector<T> &h) {
+  T *p = nullptr;
+  const size_t bytes = std::max<size_t>(1, h.size() * sizeof(T));
+  HIP_OK(hipMalloc(&p, bytes));
+  if (!h.empty()) HIP_OK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+}  // namespace
+
+struct bcx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevIndex ix{};
+  std::vector<void *> owned;          // index + graph buffers
+  std::vector<int32_t> perm, iperm;   // preorder <-> vertex
+  int32_t count_overflow_index = 0;
+  bcx_tuning tn{64, 16, 256, 4096};
+  // batch workspace
+  int ws_slots = 0;
+  DevWork ws{};
+  int *d_srcs = nullptr;
+  int *d_cursor = nullptr;
+  int *d_total = nullptr;
+  int *d_ovf = nullptr;
+  double *d_bc = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_batch = 0;
+  bcx_tuning graph_tn{};
+  bool profiling = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bcx_timing timing{};
+
+  ~bcx_ctx() {
+    if (hipSetDevice(device) != hipSuccess) return;
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    free_ws();
+    for (void *p : owned) (void)hipFree(p);
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void free_ws() {
+    if (graph_exec) {
+      (void)hipGraphExecDestroy(graph_exec);
+      graph_exec = nullptr;
+    }
+    void *bufs[] = {ws.d, ws.sigma, ws.hop, ws.order, ws.lcnt, ws.delta};
+    for (void *p : bufs)
+      if (p) (void)hipFree(p);
+    ws = DevWork{};
+    ws_slots = 0;
+  }
+  static size_t slot_bytes(int n) {
+    return (size_t)n * (4 + 8 + 4 + 4 + 8) + (size_t)2 * ((size_t)n + 2) * 4;
+  }
+  void ensure_ws(int slots) {
+    if (slots <= ws_slots) return;
+    free_ws();
+    const size_t n = (size_t)ix.n;
+    HIP_OK(hipMalloc(&ws.d, n * slots * 4));
+    HIP_OK(hipMalloc(&ws.sigma, n * slots * 8));
+    HIP_OK(hipMalloc(&ws.hop, n * slots * 4));
+    HIP_OK(hipMalloc(&ws.order, n * slots * 4));
+    HIP_OK(hipMalloc(&ws.lcnt, 2 * (n + 2) * slots * 4));
+    HIP_OK(hipMalloc(&ws.delta, n * slots * 8));
+    ws_slots = slots;
+  }
+};
+
+namespace {
+
+DevBatch batch_of(bcx_ctx *c) { return DevBatch{c->d_srcs, c->d_cursor, c->d_total}; }
+
+// Launch the kernels of one batch on the context stream.  With `timed`,
+// events bracket each phase and the durations are accumulated.
+void launch_batch(bcx_ctx *c, int mode, int batch, bool timed, double *bsum = nullptr,
+                  const DevBatch *bt_override = nullptr) {
+  const int n = c->ix.n;
+  const DevBatch bt = bt_override ? *bt_override : batch_of(c);
+  if (timed) HIP_OK(hipEventRecord(c->ev[0], c->stream));
+  if (mode == BCX_MODE_PLAIN) {
+    scan_plain_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(c->ix, bt, c->ws, c->d_ovf);
+  } else {
+    const int waves = 4;
+    dim3 grid((batch + kWave - 1) / kWave,
+              (n + waves * c->tn.tchunk - 1) / (waves * c->tn.tchunk));
+    scan_tile_kernel<<<grid, waves * kWave, 0, c->stream>>>(c->ix, bt, c->ws, batch, c->tn.tchunk,
+                                                            c->d_ovf);
+  }
+  if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
+  dep_kernel<<<batch, c->tn.bp_threads, 0, c->stream>>>(c->ix, bt, c->ws, batch);
+  if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
+  if (bsum) {
+    batch_sum_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(n, bt, c->ws.delta, batch, bsum);
+  } else {
+    reduce_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(n, bt, c->ws.delta, batch, c->d_bc);
+    advance_kernel<<<1, 64, 0, c->stream>>>(bt, batch);
+  }
+  if (timed) {
+    HIP_OK(hipEventRecord(c->ev[3], c->stream));
+    HIP_OK(hipEventSynchronize(c->ev[3]));
+    float a = 0, b = 0, r = 0;
+    HIP_OK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIP_OK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    HIP_OK(hipEventElapsedTime(&r, c->ev[2], c->ev[3]));
+    c->timing.scan_ms += a;
+    c->timing.dep_ms += b;
+    c->timing.reduce_ms += r;
+  }
+  HIP_OK(hipGetLastError());
+  c->timing.scan_launches += 1;
+  c->timing.dep_launches += 1;
+}
+
+void run_sources(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, double *bc_inout,
+                 uint32_t *overflow) {
+  const int n = c->ix.n;
+  std::vector<int32_t> pre((size_t)count);
+  for (int32_t k = 0; k < count; ++k) {
+    if (sources[k] < 0 || sources[k] >= n) throw bcx::Error{BCX_EINVAL, "source out of range"};
+    pre[(size_t)k] = c->iperm[(size_t)sources[k]];
+  }
+  std::sort(pre.begin(), pre.end());
+  std::vector<double> bc_pre((size_t)n);
+  for (int32_t v = 0; v < n; ++v) bc_pre[(size_t)c->iperm[(size_t)v]] = bc_inout[v];
+  const int batch = mode == BCX_MODE_PLAIN ? 1 : std::max(1, std::min(c->tn.batch, count));
+  c->ensure_ws(batch);
+  if (count > 0)
+    HIP_OK(hipMemcpyAsync(c->d_srcs, pre.data(), (size_t)count * 4, hipMemcpyHostToDevice,
+                          c->stream));
+  const int zero = 0;
+  HIP_OK(hipMemcpyAsync(c->d_cursor, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->d_total, &count, 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->d_ovf, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->d_bc, bc_pre.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+  const int nb = (count + batch - 1) / batch;
+  const bool use_graph = mode == BCX_MODE_BATCHED && !c->profiling && nb > 1;
+  if (use_graph &&
+      (!c->graph_exec || c->graph_batch != batch ||
+       std::memcmp(&c->graph_tn, &c->tn, sizeof(bcx_tuning)) != 0)) {
+    if (c->graph_exec) {
+      HIP_OK(hipGraphExecDestroy(c->graph_exec));
+      c->graph_exec = nullptr;
+    }
+    hipGraph_t g;
+    HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    launch_batch(c, mode, batch, false);
+    HIP_OK(hipStreamEndCapture(c->stream, &g));
+    HIP_OK(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+    HIP_OK(hipGraphDestroy(g));
+    c->graph_batch = batch;
+    c->graph_tn = c->tn;
+    // capture counted launches that did not run
+    c->timing.scan_launches -= 1;
+    c->timing.dep_launches -= 1;
+  }
+  for (int b = 0; b < nb; ++b) {
+    if (use_graph) {
+      HIP_OK(hipGraphLaunch(c->graph_exec, c->stream));
+      c->timing.scan_launches += 1;
+      c->timing.dep_launches += 1;
+    } else {
+      launch_batch(c, mode, batch, c->profiling);
+    }
+  }
+  c->timing.sources += count;
+  int ovf = 0;
+  HIP_OK(hipMemcpyAsync(bc_pre.data(), c->d_bc, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(&ovf, c->d_ovf, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  for (int32_t v = 0; v < n; ++v) bc_inout[v] = bc_pre[(size_t)c->iperm[(size_t)v]];
+  if (overflow) *overflow = (uint32_t)(ovf | c->count_overflow_index);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx **out) {
+  BCX_GUARD_BEGIN
+  if (!gh || !ih || !out) return bcx::fail(BCX_EINVAL, "null argument");
+  const bcx::Graph &g = bcx::graph_of(gh);
+  const bcx::Index &ix = bcx::index_of(ih);
+  if (g.n != ix.n) return bcx::fail(BCX_EINVAL, "graph and index sizes differ");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return bcx::fail(BCX_EDEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return bcx::fail(BCX_EINVAL, "device id out of range");
+  HIP_OK(hipSetDevice(device));
+  std::unique_ptr<bcx_ctx> c(new bcx_ctx());
+  c->device = device;
+  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto &e : c->ev) HIP_OK(hipEventCreate(&e));
+  const int n = g.n;
+  c->perm = ix.perm;
+  c->iperm = ix.iperm;
+  c->count_overflow_index = ix.count_overflow;
+  // CSR in preorder ids, arcs of each vertex in file order.
+  std::vector<int32_t> grow((size_t)n + 1, 0), gcol((size_t)g.arcs()), gw((size_t)g.arcs());
+  if (g.arcs() > 0x7fffffff) return bcx::fail(BCX_EGRAPH, "more than 2^31 arcs");
+  for (int32_t p = 0; p < n; ++p) {
+    const int32_t v = ix.perm[(size_t)p];
+    grow[(size_t)p + 1] = grow[(size_t)p] + (int32_t)(g.row[(size_t)v + 1] - g.row[(size_t)v]);
+  }
+  for (int32_t p = 0; p < n; ++p) {
+    const int32_t v = ix.perm[(size_t)p];
+    int32_t o = grow[(size_t)p];
+    for (int64_t a = g.row[(size_t)v]; a < g.row[(size_t)v + 1]; ++a, ++o) {
+      gcol[(size_t)o] = ix.iperm[(size_t)g.col[(size_t)a]];
+      gw[(size_t)o] = g.w[(size_t)a];
+    }
+  }
+  int levels = 1;
+  while ((1 << levels) <= n) ++levels;
+  std::vector<int32_t> rmq((size_t)levels * n);
+  std::copy(ix.depth.begin(), ix.depth.end(), rmq.begin());
+  for (int k = 1; k < levels; ++k) {
+    const int32_t *prev = rmq.data() + (size_t)(k - 1) * n;
+    int32_t *cur = rmq.data() + (size_t)k * n;
+    const int half = 1 << (k - 1);
+    for (int32_t i = 0; i < n; ++i)
+      cur[i] = (i + half < n) ? std::min(prev[i], prev[i + half]) : prev[i];
+  }
+  auto keep = [&](void *p) {
+    c->owned.push_back(p);
+    return p;
+  };
+  DevIndex d{};
+  d.n = n;
+  d.levels = levels;
+  d.arcs = g.arcs();
+  d.depth = (const int *)keep(dev_upload(ix.depth));
+  d.row_off = (const int64_t *)keep(dev_upload(ix.row_off));
+  d.dis = (const int *)keep(dev_upload(ix.dis));
+  d.cnt = (const uint64_t *)keep(dev_upload(ix.cnt));
+  d.hop = (const uint32_t *)keep(dev_upload(ix.hop));
+  d.rmq = (const int *)keep(dev_upload(rmq));
+  d.grow = (const int *)keep(dev_upload(grow));
+  d.gcol = (const int *)keep(dev_upload(gcol));
+  d.gw = (const int *)keep(dev_upload(gw));
+  c->ix = d;
+  c->d_srcs = (int *)keep(dev_upload(std::vector<int32_t>((size_t)n, 0)));
+  c->d_cursor = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
+  c->d_total = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
+  c->d_ovf = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
+  c->d_bc = (double *)keep(dev_upload(std::vector<double>((size_t)n, 0.0)));
+  *out = c.release();
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+void bcx_ctx_free(bcx_ctx *ctx) { delete ctx; }
+
+int bcx_ctx_set_tuning(bcx_ctx *c, const bcx_tuning *tn) {
+  if (!c || !tn) return bcx::fail(BCX_EINVAL, "null argument");
+  if (tn->tile != 64) return bcx::fail(BCX_EINVAL, "tile must be 64");
+  if (tn->tchunk < 1 || tn->tchunk > 4096) return bcx::fail(BCX_EINVAL, "tchunk out of range");
+  if (tn->bp_threads < 64 || tn->bp_threads > 1024 || (tn->bp_threads & (tn->bp_threads - 1)))
+    return bcx::fail(BCX_EINVAL, "bp_threads must be a power of two in [64, 1024]");
+  if (tn->batch < 1 || tn->batch > (1 << 20)) return bcx::fail(BCX_EINVAL, "batch out of range");
+  c->tn = *tn;
+  return BCX_OK;
+}
+
+int bcx_ctx_get_tuning(const bcx_ctx *c, bcx_tuning *tn) {
+  if (!c || !tn) return bcx::fail(BCX_EINVAL, "null argument");
+  *tn = c->tn;
+  return BCX_OK;
+}
+
+int bcx_ctx_workspace_bytes(const bcx_ctx *c, int32_t batch, size_t *bytes) {
+  if (!c || !bytes || batch < 1) return bcx::fail(BCX_EINVAL, "bad argument");
+  *bytes = bcx_ctx::slot_bytes(c->ix.n) * (size_t)batch;
+  return BCX_OK;
+}
+
+int bcx_ctx_mem_info(const bcx_ctx *c, size_t *free_bytes, size_t *total_bytes) {
+  BCX_GUARD_BEGIN
+  if (!c) return bcx::fail(BCX_EINVAL, "null context");
+  HIP_OK(hipSetDevice(c->device));
+  size_t f = 0, t = 0;
+  HIP_OK(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_ctx_set_profiling(bcx_ctx *c, int on) {
+  if (!c) return bcx::fail(BCX_EINVAL, "null context");
+  c->profiling = on != 0;
+  return BCX_OK;
+}
+
+int bcx_ctx_timing(const bcx_ctx *c, bcx_timing *t) {
+  if (!c || !t) return bcx::fail(BCX_EINVAL, "null argument");
+  *t = c->timing;
+  return BCX_OK;
+}
+
+void bcx_ctx_timing_reset(bcx_ctx *c) {
+  if (c) c->timing = bcx_timing{};
+}
+
+int bcx_bc_sources(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, double *bc_inout,
+                   uint32_t *overflow) {
+  BCX_GUARD_BEGIN
+  if (!c || !bc_inout || (count > 0 && !sources) || count < 0)
+    return bcx::fail(BCX_EINVAL, "bad argument");
+  if (mode != BCX_MODE_PLAIN && mode != BCX_MODE_BATCHED) return bcx::fail(BCX_EINVAL, "bad mode");
+  if (count > c->ix.n) return bcx::fail(BCX_EINVAL, "more sources than vertices");
+  HIP_OK(hipSetDevice(c->device));
+  run_sources(c, mode, sources, count, bc_inout, overflow);
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_full_bc(bcx_ctx *c, int mode, double *bc_out, uint32_t *overflow) {
+  BCX_GUARD_BEGIN
+  if (!c || !bc_out) return bcx::fail(BCX_EINVAL, "null argument");
+  std::vector<int32_t> all((size_t)c->ix.n);
+  for (int32_t v = 0; v < c->ix.n; ++v) all[(size_t)v] = v;
+  for (int32_t v = 0; v < c->ix.n; ++v) bc_out[v] = 0.0;
+  return bcx_bc_sources(c, mode, all.data(), c->ix.n, bc_out, overflow);
+  BCX_GUARD_END
+}
+
+int bcx_query_dist_count(bcx_ctx *c, int32_t source, int32_t *d, uint64_t *sigma, uint32_t *hop) {
+  BCX_GUARD_BEGIN
+  if (!c) return bcx::fail(BCX_EINVAL, "null context");
+  const int n = c->ix.n;
+  if (source < 0 || source >= n) return bcx::fail(BCX_EINVAL, "source out of range");
+  HIP_OK(hipSetDevice(c->device));
+  c->ensure_ws(1);
+  const int32_t pre = c->iperm[(size_t)source];
+  const int zero = 0, one = 1;
+  HIP_OK(hipMemcpyAsync(c->d_srcs, &pre, 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->d_cursor, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->d_total, &one, 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->d_ovf, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  scan_plain_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(c->ix, batch_of(c), c->ws, c->d_ovf);
+  HIP_OK(hipGetLastError());
+  std::vector<int32_t> dd((size_t)n);
+  std::vector<uint64_t> ss((size_t)n);
+  std::vector<uint32_t> hh((size_t)n);
+  int ovf = 0;
+  HIP_OK(hipMemcpyAsync(dd.data(), c->ws.d, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(ss.data(), c->ws.sigma, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(hh.data(), c->ws.hop, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(&ovf, c->d_ovf, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  for (int32_t v = 0; v < n; ++v) {
+    const size_t p = (size_t)c->iperm[(size_t)v];
+    if (d) d[v] = dd[p];
+    if (sigma) sigma[v] = ss[p];
+    if (hop) hop[v] = hh[p];
+  }
+  return (ovf | c->count_overflow_index) ? 1 : BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_subset_bc(bcx_ctx *c, const int32_t *points, int32_t q, double *bc_out) {
+  BCX_GUARD_BEGIN
+  if (!c || (q > 0 && (!points || !bc_out)) || q < 0) return bcx::fail(BCX_EINVAL, "bad argument");
+  for (int32_t i = 0; i < q; ++i)
+    if (points[i] < 0 || points[i] >= c->ix.n) return bcx::fail(BCX_EINVAL, "query vertex out of range");
+  std::vector<double> bc((size_t)c->ix.n);
+  uint32_t ovf = 0;
+  int rc = bcx_full_bc(c, BCX_MODE_BATCHED, bc.data(), &ovf);
+  if (rc != BCX_OK) return rc;
+  for (int32_t i = 0; i < q; ++i) bc_out[i] = bc[(size_t)points[i]];
+  return ovf ? 1 : BCX_OK;
+  BCX_GUARD_END
+}
+
+}  // extern "C"
+
+// ---- RQ serving engine ------------------------------------------------------
+// One cyclic sweep over all sources (tree order, batches of tn.batch) serves
+// every admitted request: after each batch, each request gathers the batch's
+// dependency sums at its query vertices into its own [batch][q] table; after
+// one full cycle the table is summed in batch order 0..nb-1.  A request thus
+// sees exactly the contributions of the n sources computed while it was in
+// flight, and its result bits do not depend on its admission batch or on the
+// other requests.  The engine shares the context's workspace: calls on one
+// context must not overlap.
+struct bcx_rq {
+  struct Req {
+    int64_t id;
+    int32_t q;
+    int *points = nullptr;      // preorder ids
+    double *contrib = nullptr;  // nb * q
+    double *out = nullptr;      // q
+    int32_t done_batches = 0;
+    bool finished = false;
+    std::vector<double> result;
+    size_t bytes = 0;
+  };
+  bcx_ctx *ctx = nullptr;
+  size_t budget = 0, used = 0;
+  int32_t batch = 0, nb = 0, cur = 0;
+  int64_t next_id = 1;
+  double *bsum = nullptr;
+  int *d_srcs = nullptr, *d_cursor = nullptr, *d_total = nullptr;  // own sweep list
+  std::vector<Req> reqs;
+
+  ~bcx_rq() {
+    if (!ctx || hipSetDevice(ctx->device) != hipSuccess) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &r : reqs) release(r);
+    for (void *p : {(void *)bsum, (void *)d_srcs, (void *)d_cursor, (void *)d_total})
+      if (p) (void)hipFree(p);
+  }
+  void release(Req &r) {
+    if (r.points) (void)hipFree(r.points);
+    if (r.contrib) (void)hipFree(r.contrib);
+    if (r.out) (void)hipFree(r.out);
+    r.points = nullptr;
+    r.contrib = r.out = nullptr;
+    used -= r.bytes;
+    r.bytes = 0;
+  }
+  size_t need(int32_t q) const { return (size_t)q * (4 + 8) + (size_t)nb * q * 8; }
+};
+
+extern "C" {
+
+int bcx_rq_create(bcx_ctx *c, size_t hbm_budget_bytes, bcx_rq **out) {
+  BCX_GUARD_BEGIN
+  if (!c || !out) return bcx::fail(BCX_EINVAL, "null argument");
+  HIP_OK(hipSetDevice(c->device));
+  std::unique_ptr<bcx_rq> r(new bcx_rq());
+  r->ctx = c;
+  r->budget = hbm_budget_bytes;
+  r->batch = std::max(1, std::min(c->tn.batch, c->ix.n));
+  r->nb = (c->ix.n + r->batch - 1) / r->batch;
+  c->ensure_ws(r->batch);
+  HIP_OK(hipMalloc(&r->bsum, (size_t)c->ix.n * 8));
+  // The sweep's source list is the identity over tree order.
+  std::vector<int32_t> ident((size_t)c->ix.n);
+  for (int32_t i = 0; i < c->ix.n; ++i) ident[(size_t)i] = i;
+  HIP_OK(hipMalloc(&r->d_srcs, ident.size() * 4));
+  HIP_OK(hipMalloc(&r->d_cursor, 16));
+  HIP_OK(hipMalloc(&r->d_total, 16));
+  HIP_OK(hipMemcpy(r->d_srcs, ident.data(), ident.size() * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(r->d_total, &c->ix.n, 4, hipMemcpyHostToDevice));
+  *out = r.release();
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+void bcx_rq_free(bcx_rq *rq) { delete rq; }
+
+int bcx_rq_bytes_needed(const bcx_rq *rq, int32_t q, size_t *bytes) {
+  if (!rq || !bytes || q < 0) return bcx::fail(BCX_EINVAL, "bad argument");
+  *bytes = rq->need(q);
+  return BCX_OK;
+}
+
+int bcx_rq_submit(bcx_rq *rq, const int32_t *points, int32_t q, int64_t *id) {
+  BCX_GUARD_BEGIN
+  if (!rq || !id || q <= 0 || !points) return bcx::fail(BCX_EINVAL, "bad argument");
+  bcx_ctx *c = rq->ctx;
+  std::vector<int32_t> pre((size_t)q);
+  for (int32_t i = 0; i < q; ++i) {
+    if (points[i] < 0 || points[i] >= c->ix.n) return bcx::fail(BCX_EINVAL, "query vertex out of range");
+    pre[(size_t)i] = c->iperm[(size_t)points[i]];
+  }
+  const size_t need = rq->need(q);
+  if (rq->used + need > rq->budget)
+    return bcx::fail(BCX_EBUSY, "request does not fit the device-memory budget now");
+  HIP_OK(hipSetDevice(c->device));
+  bcx_rq::Req r;
+  r.id = rq->next_id++;
+  r.q = q;
+  try {
+    HIP_OK(hipMalloc(&r.points, (size_t)q * 4));
+    HIP_OK(hipMalloc(&r.contrib, (size_t)rq->nb * q * 8));
+    HIP_OK(hipMalloc(&r.out, (size_t)q * 8));
+    HIP_OK(hipMemcpyAsync(r.points, pre.data(), (size_t)q * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+  } catch (...) {
+    if (r.points) (void)hipFree(r.points);
+    if (r.contrib) (void)hipFree(r.contrib);
+    if (r.out) (void)hipFree(r.out);
+    throw;
+  }
+  r.bytes = need;
+  rq->used += need;
+  *id = r.id;
+  rq->reqs.push_back(std::move(r));
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_rq_cancel(bcx_rq *rq, int64_t id) {
+  BCX_GUARD_BEGIN
+  if (!rq) return bcx::fail(BCX_EINVAL, "null engine");
+  HIP_OK(hipSetDevice(rq->ctx->device));
+  HIP_OK(hipStreamSynchronize(rq->ctx->stream));
+  for (size_t k = 0; k < rq->reqs.size(); ++k)
+    if (rq->reqs[k].id == id) {
+      rq->release(rq->reqs[k]);
+      rq->reqs.erase(rq->reqs.begin() + (long)k);
+      return BCX_OK;
+    }
+  return bcx::fail(BCX_ENOTFOUND, "unknown request id");
+  BCX_GUARD_END
+}
+
+int bcx_rq_step(bcx_rq *rq, int32_t *active) {
+  BCX_GUARD_BEGIN
+  if (!rq) return bcx::fail(BCX_EINVAL, "null engine");
+  bcx_ctx *c = rq->ctx;
+  HIP_OK(hipSetDevice(c->device));
+  int32_t live = 0;
+  for (auto &r : rq->reqs)
+    if (!r.finished) ++live;
+  if (live > 0) {
+    const int32_t base = rq->cur * rq->batch;
+    HIP_OK(hipMemcpyAsync(rq->d_cursor, &base, 4, hipMemcpyHostToDevice, c->stream));
+    const DevBatch bt{rq->d_srcs, rq->d_cursor, rq->d_total};
+    c->ensure_ws(rq->batch);
+    launch_batch(c, BCX_MODE_BATCHED, rq->batch, false, rq->bsum, &bt);
+    for (auto &r : rq->reqs) {
+      if (r.finished) continue;
+      gather_kernel<<<(r.q + 255) / 256, 256, 0, c->stream>>>(
+          rq->bsum, r.points, r.q, r.contrib + (int64_t)rq->cur * r.q);
+      if (++r.done_batches == rq->nb) {
+        finish_kernel<<<(r.q + 255) / 256, 256, 0, c->stream>>>(r.contrib, r.q, rq->nb, r.out);
+        r.result.resize((size_t)r.q);
+        HIP_OK(hipMemcpyAsync(r.result.data(), r.out, (size_t)r.q * 8, hipMemcpyDeviceToHost,
+                              c->stream));
+        r.finished = true;
+      }
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->timing.sources += std::min(rq->batch, c->ix.n - base);
+    rq->cur = (rq->cur + 1) % rq->nb;
+    for (auto &r : rq->reqs)
+      if (r.finished && r.bytes) rq->release(r);  // memory returns to the budget
+  }
+  if (active) {
+    int32_t a = 0;
+    for (auto &r : rq->reqs)
+      if (!r.finished) ++a;
+    *active = a;
+  }
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_rq_poll(bcx_rq *rq, int64_t id, double *bc_out) {
+  if (!rq) return bcx::fail(BCX_EINVAL, "null engine");
+  for (size_t k = 0; k < rq->reqs.size(); ++k) {
+    auto &r = rq->reqs[k];
+    if (r.id != id) continue;
+    if (!r.finished) return 0;
+    if (bc_out) std::memcpy(bc_out, r.result.data(), r.result.size() * 8);
+    rq->reqs.erase(rq->reqs.begin() + (long)k);
+    return 1;
+  }
+  return bcx::fail(BCX_ENOTFOUND, "unknown request id");
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+// Host-side data structures of the bcx library: the parsed graph and the
+// tree-decomposition label index, both in the layout the device consumes.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bcx {
+
+// Reference INF for label distances (src/TreeCount/Util.h:35, TreeDecomp.h:58).
+constexpr int32_t kInf = 999999999;
+
+struct Graph {
+  int32_t n = 0;
+  // Directed arcs as listed in the file (0-based), grouped per tail in file
+  // order: the adjacency Graph::ReadGraph builds (src/Graph.cpp:22-36).
+  std::vector<int64_t> row;  // n + 1
+  std::vector<int32_t> col;  // arcs
+  std::vector<int32_t> w;    // arcs
+  int64_t arcs() const { return (int64_t)col.size(); }
+};
+
+// Tree-decomposition label index.  Tree nodes are numbered in DFS preorder of
+// the decomposition tree (children in creation order), so the subtree of node
+// p is the contiguous id range [p, p + subtree_size[p]) and every ancestor of p
+// has a smaller id.  Row p holds depth[p] + 1 entries: entry i < depth[p] is
+// the label towards p's ancestor at depth i (root first), the last entry is a
+// self entry (0, 1, 0) so that a pair query is one aggregation over the common
+// prefix of two rows.
+struct Index {
+  int32_t n = 0;
+  std::vector<int32_t> perm;    // preorder id -> vertex id
+  std::vector<int32_t> iperm;   // vertex id -> preorder id
+  std::vector<int32_t> depth;   // root = 0
+  std::vector<int32_t> parent;  // preorder id of parent, -1 for the root
+  std::vector<int32_t> subtree; // subtree sizes
+  std::vector<int64_t> row_off; // n + 1 offsets into the label arrays
+  std::vector<int32_t> dis;     // restricted (up-path) distance
+  std::vector<uint64_t> cnt;    // number of shortest up-paths
+  std::vector<uint32_t> hop;    // max hops among those paths
+  // Bags (upward shortcut neighbours) per node, preorder ids, for tools.
+  std::vector<int64_t> bag_off;
+  std::vector<int32_t> bag;
+  int32_t tree_height = 0;  // max depth + 1
+  int32_t tree_width = 0;   // max bag size
+  int32_t count_overflow = 0;
+  double build_seconds = 0.0;
+  int64_t entries() const { return (int64_t)dis.size(); }
+};
+
+// Error carrier for the host side; the C ABI maps it to codes + message.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+// graph_io.cpp
+void read_graph_file(const char *path, Graph &g);
+void graph_from_arcs(int32_t n, int64_t arcs, const int32_t *u, const int32_t *v,
+                     const int32_t *w, Graph &g);
+bool graph_connected(const Graph &g);
+void read_order_file(const char *path, int32_t n, std::vector<int32_t> &order);
+
+// index_build.cpp
+// Elimination order: empty -> min-degree (reference Reduce policy); else
+// order[v] = contraction rank of v (0 contracts first).
+void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix);
+void index_query(const Index &ix, int32_t s, int32_t t, int32_t &d,
+                 uint64_t &sigma, uint32_t &hop, bool &ovf);
+int32_t lca_depth(const Index &ix, int32_t p, int32_t q);  // preorder ids
+void save_index(const Index &ix, const char *path);
+void load_index(const char *path, Index &ix);
+
+}  // namespace bcx

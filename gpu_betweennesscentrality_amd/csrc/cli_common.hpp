@@ -1,0 +1,88 @@
+// Shared helpers of the QBC / MHLQBC / RQ executables.
+#pragma once
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "bcx.h"
+
+namespace cli {
+
+[[noreturn]] inline void die(const std::string &what) {
+  std::cerr << "error: " << what << ": " << bcx_last_error() << std::endl;
+  std::exit(1);
+}
+
+inline void check(int rc, const char *what) {
+  if (rc < 0) die(what);
+}
+
+inline int env_int(const char *name, int dflt) {
+  const char *v = std::getenv(name);
+  return (v && *v) ? std::atoi(v) : dflt;
+}
+
+inline int parse_device(const char *s) {
+  try {
+    return std::stoi(s);
+  } catch (...) {
+    std::cerr << "error: device id must be an integer: " << s << std::endl;
+    std::exit(1);
+  }
+}
+
+// Knobs that the reference fixes at compile time come from the environment so
+// the positional command line stays identical: BCX_MODE=plain|batched,
+// BCX_TCHUNK, BCX_BP_THREADS, BCX_BATCH.
+inline int apply_env_tuning(bcx_ctx *ctx) {
+  bcx_tuning tn;
+  check(bcx_ctx_get_tuning(ctx, &tn), "bcx_ctx_get_tuning");
+  tn.tchunk = env_int("BCX_TCHUNK", tn.tchunk);
+  tn.bp_threads = env_int("BCX_BP_THREADS", tn.bp_threads);
+  tn.batch = env_int("BCX_BATCH", tn.batch);
+  check(bcx_ctx_set_tuning(ctx, &tn), "bcx_ctx_set_tuning");
+  const char *m = std::getenv("BCX_MODE");
+  return (m && std::string(m) == "plain") ? BCX_MODE_PLAIN : BCX_MODE_BATCHED;
+}
+
+// "vertexID  BCValue" then "id value" with 2 decimals (QBC.cu:614-626).
+inline void write_bc_file(const std::string &path, const std::vector<double> &bc) {
+  std::ofstream ofs;
+  ofs.precision(2);
+  ofs.setf(std::ios::fixed, std::ios::floatfield);
+  ofs.open(path, std::ios::out);
+  if (!ofs) {
+    std::cerr << "error: cannot write " << path << std::endl;
+    std::exit(1);
+  }
+  ofs << "vertexID  BCValue" << std::endl;
+  for (size_t i = 0; i < bc.size(); ++i) ofs << i + 1 << " " << bc[i] << std::endl;
+  ofs.close();
+  std::cout << "The BC value has been save: " << path << std::endl;
+}
+
+// Optional full-precision dump for parity checks (BCX_BC_DUMP=path): raw
+// little-endian doubles, one per vertex in file-id order.
+inline void maybe_dump_bc(const std::vector<double> &bc) {
+  const char *p = std::getenv("BCX_BC_DUMP");
+  if (!p || !*p) return;
+  FILE *f = std::fopen(p, "wb");
+  if (!f || std::fwrite(bc.data(), sizeof(double), bc.size(), f) != bc.size()) {
+    std::cerr << "error: cannot write " << p << std::endl;
+    std::exit(1);
+  }
+  std::fclose(f);
+}
+
+inline double seconds_since(std::chrono::high_resolution_clock::time_point t0) {
+  return std::chrono::duration_cast<std::chrono::duration<double>>(
+             std::chrono::high_resolution_clock::now() - t0)
+      .count();
+}
+
+}  // namespace cli

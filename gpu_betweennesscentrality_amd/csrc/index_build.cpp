@@ -1,0 +1,460 @@
+// Host construction of the tree-decomposition label index.
+//
+// Method (the reference's TreeCount variant, src/TreeCount/TreeDecomp.cpp):
+//  1. Vertex elimination (Reduce, TreeDecomp.cpp:54-208).  Eliminating x joins
+//     every pair of its live neighbours (u, v) by a shortcut of length
+//     l(u,x) + l(x,v); the shortcut keeps the minimum length, the number of
+//     shortest restricted paths (counts multiply along x, add on ties) and the
+//     maximum hop count among them (hops add along x, max on ties).  The
+//     min-degree policy picks the smallest (degree, fill-degree, id) with the
+//     reference's lazy re-keying of changed vertices (TreeDecomp.cpp:65-112).
+//     An explicit order (order file, buildIndex.cpp:4-111) may replace it.
+//  2. Tree (MakeTree, TreeDecomp.cpp:210-238): the last eliminated vertex is
+//     the root, every other vertex hangs below the bag neighbour eliminated
+//     first after it (GetParentNode, TreeDecomp.cpp:630-664).
+//  3. Labels (makeIndexDFS, TreeDecomp.cpp:355-477): for node p and ancestor
+//     a_j at depth j, dis_p[j] = min over bag vertices x at depth k >= j of
+//     l(p,x) (k == j) or l(p,x) + dis_x[j] (k > j), with counts multiplied
+//     and summed over ties and hops added and maxed over ties.
+// Layout differs from the reference on purpose: nodes are renumbered in DFS
+// preorder, rows are one flat SoA (dis / cnt / hop) with an extra self entry,
+// counts are uint64 with overflow detection instead of uint32, and labels are
+// computed level by level (all nodes of one depth in parallel) instead of by a
+// recursive DFS, since a row depends only on its ancestors' rows.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <set>
+#include <tuple>
+
+#include "bcx.h"
+#include "bcx_host.hpp"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace bcx {
+
+namespace {
+
+struct Arc {
+  int32_t nb;
+  int32_t len;
+  uint64_t cnt;
+  uint32_t hop;
+};
+
+inline uint64_t mul_sat(uint64_t a, uint64_t b, int32_t &ovf) {
+  uint64_t r;
+  if (__builtin_mul_overflow(a, b, &r)) {
+    ovf = 1;
+    return UINT64_MAX;
+  }
+  return r;
+}
+inline uint64_t add_sat(uint64_t a, uint64_t b, int32_t &ovf) {
+  uint64_t r;
+  if (__builtin_add_overflow(a, b, &r)) {
+    ovf = 1;
+    return UINT64_MAX;
+  }
+  return r;
+}
+
+inline std::vector<Arc>::iterator find_arc(std::vector<Arc> &a, int32_t nb) {
+  auto it = std::lower_bound(a.begin(), a.end(), nb,
+                             [](const Arc &x, int32_t k) { return x.nb < k; });
+  return (it != a.end() && it->nb == nb) ? it : a.end();
+}
+
+// Elimination; returns elimination sequence and bags (sorted by vertex id).
+void eliminate(const Graph &g, const std::vector<int32_t> &order,
+               std::vector<int32_t> &elim, std::vector<std::vector<Arc>> &bags,
+               int32_t &width, int32_t &ovf) {
+  const int32_t n = g.n;
+  std::vector<std::vector<Arc>> adj((size_t)n);
+  for (int32_t x = 0; x < n; ++x) {
+    auto &a = adj[(size_t)x];
+    for (int64_t k = g.row[(size_t)x]; k < g.row[(size_t)x + 1]; ++k)
+      a.push_back(Arc{g.col[(size_t)k], g.w[(size_t)k], 1, 1});
+    std::sort(a.begin(), a.end(), [](const Arc &p, const Arc &q) { return p.nb < q.nb; });
+  }
+  elim.clear();
+  elim.reserve((size_t)n);
+  bags.assign((size_t)n, {});
+  width = 0;
+  std::vector<char> alive((size_t)n, 1);
+
+  const bool min_degree = order.empty();
+  std::vector<int32_t> DD, DD2, tDD, tDD2;
+  std::vector<char> changed;
+  std::set<std::tuple<int32_t, int32_t, int32_t>> pq;
+  std::vector<int32_t> seq;
+  if (min_degree) {
+    DD.resize((size_t)n);
+    DD2.resize((size_t)n);
+    tDD.resize((size_t)n);
+    tDD2.resize((size_t)n);
+    changed.assign((size_t)n, 0);
+    for (int32_t x = 0; x < n; ++x) {
+      DD[(size_t)x] = DD2[(size_t)x] = tDD[(size_t)x] = tDD2[(size_t)x] =
+          (int32_t)adj[(size_t)x].size();
+      pq.insert(std::make_tuple(DD[(size_t)x], DD2[(size_t)x], x));
+    }
+  } else {
+    seq.assign((size_t)n, -1);
+    for (int32_t v = 0; v < n; ++v) seq[(size_t)order[(size_t)v]] = v;
+  }
+
+  for (int32_t step = 0; step < n; ++step) {
+    int32_t x;
+    if (min_degree) {
+      x = std::get<2>(*pq.begin());
+      while (changed[(size_t)x]) {  // lazy re-key, TreeDecomp.cpp:103-112
+        pq.erase(std::make_tuple(DD[(size_t)x], DD2[(size_t)x], x));
+        DD[(size_t)x] = tDD[(size_t)x];
+        DD2[(size_t)x] = tDD2[(size_t)x];
+        pq.insert(std::make_tuple(DD[(size_t)x], DD2[(size_t)x], x));
+        changed[(size_t)x] = 0;
+        x = std::get<2>(*pq.begin());
+      }
+      pq.erase(pq.begin());
+    } else {
+      x = seq[(size_t)step];
+    }
+    elim.push_back(x);
+    alive[(size_t)x] = 0;
+    std::vector<Arc> nb;
+    nb.swap(adj[(size_t)x]);  // only live neighbours remain in adjacency lists
+    for (const Arc &y : nb) {
+      auto &ay = adj[(size_t)y.nb];
+      auto it = find_arc(ay, x);
+      if (it != ay.end()) ay.erase(it);
+      if (min_degree) {
+        tDD[(size_t)y.nb] = (int32_t)ay.size();
+        changed[(size_t)y.nb] = 1;
+      }
+    }
+    for (size_t i = 0; i < nb.size(); ++i) {
+      for (size_t j = i + 1; j < nb.size(); ++j) {
+        const Arc &a = nb[i], &b = nb[j];
+        const int32_t L = a.len + b.len;  // < kInf: total weight is checked at load
+        const uint64_t c = mul_sat(a.cnt, b.cnt, ovf);
+        const uint32_t h = a.hop + b.hop;
+        auto &au = adj[(size_t)a.nb];
+        auto it = find_arc(au, b.nb);
+        if (it != au.end()) {
+          auto &av = adj[(size_t)b.nb];
+          auto jt = find_arc(av, a.nb);
+          if (it->len == L) {
+            it->cnt = jt->cnt = add_sat(it->cnt, c, ovf);
+            if (h > it->hop) it->hop = jt->hop = h;
+          } else if (it->len > L) {
+            it->len = jt->len = L;
+            it->cnt = jt->cnt = c;
+            it->hop = jt->hop = h;
+          }
+        } else {
+          Arc ab{b.nb, L, c, h}, ba{a.nb, L, c, h};
+          au.insert(std::lower_bound(au.begin(), au.end(), b.nb,
+                                     [](const Arc &p, int32_t k) { return p.nb < k; }),
+                    ab);
+          auto &av = adj[(size_t)b.nb];
+          av.insert(std::lower_bound(av.begin(), av.end(), a.nb,
+                                     [](const Arc &p, int32_t k) { return p.nb < k; }),
+                    ba);
+          if (min_degree) {
+            tDD[(size_t)a.nb] = (int32_t)au.size();
+            tDD[(size_t)b.nb] = (int32_t)av.size();
+            ++tDD2[(size_t)a.nb];
+            ++tDD2[(size_t)b.nb];
+            changed[(size_t)a.nb] = changed[(size_t)b.nb] = 1;
+          }
+        }
+      }
+    }
+    if ((int32_t)nb.size() > width) width = (int32_t)nb.size();
+    bags[(size_t)x] = std::move(nb);
+  }
+}
+
+}  // namespace
+
+void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int32_t n = g.n;
+  if (!order.empty() && (int32_t)order.size() != n)
+    throw Error{BCX_EINVAL, "order has wrong length"};
+  ix = Index();
+  ix.n = n;
+  std::vector<int32_t> elim;
+  std::vector<std::vector<Arc>> bags;
+  eliminate(g, order, elim, bags, ix.tree_width, ix.count_overflow);
+
+  // Tree: creation position tpos (root = 0), parent = bag vertex of max tpos.
+  std::vector<int32_t> tpos((size_t)n), tvert((size_t)n), tparent((size_t)n, -1);
+  std::vector<std::vector<int32_t>> children((size_t)n);
+  for (int32_t k = n - 1, t = 0; k >= 0; --k, ++t) {
+    int32_t v = elim[(size_t)k];
+    tpos[(size_t)v] = t;
+    tvert[(size_t)t] = v;
+    if (k == n - 1) continue;
+    const auto &b = bags[(size_t)v];
+    if (b.empty())
+      throw Error{BCX_EGRAPH, "graph is disconnected (vertex " + std::to_string(v + 1) +
+                                  " ends a component); the index needs a connected graph"};
+    int32_t best = b[0].nb;
+    for (const Arc &a : b)
+      if (tpos[(size_t)a.nb] > tpos[(size_t)best]) best = a.nb;
+    tparent[(size_t)t] = tpos[(size_t)best];
+    children[(size_t)tpos[(size_t)best]].push_back(t);
+  }
+  // DFS preorder over creation positions, children in creation order.
+  ix.perm.resize((size_t)n);
+  ix.iperm.resize((size_t)n);
+  ix.depth.resize((size_t)n);
+  ix.parent.resize((size_t)n);
+  ix.subtree.assign((size_t)n, 1);
+  std::vector<int32_t> pre_of_t((size_t)n);
+  {
+    std::vector<std::pair<int32_t, int32_t>> st;  // (tree pos, depth)
+    st.emplace_back(0, 0);
+    int32_t next = 0;
+    while (!st.empty()) {
+      auto [t, d] = st.back();
+      st.pop_back();
+      int32_t p = next++;
+      pre_of_t[(size_t)t] = p;
+      ix.perm[(size_t)p] = tvert[(size_t)t];
+      ix.depth[(size_t)p] = d;
+      const auto &ch = children[(size_t)t];
+      for (size_t i = ch.size(); i-- > 0;) st.emplace_back(ch[i], d + 1);
+    }
+  }
+  for (int32_t p = 0; p < n; ++p) ix.iperm[(size_t)ix.perm[(size_t)p]] = p;
+  ix.tree_height = 0;
+  for (int32_t p = 0; p < n; ++p) {
+    int32_t t = tpos[(size_t)ix.perm[(size_t)p]];
+    ix.parent[(size_t)p] = tparent[(size_t)t] < 0 ? -1 : pre_of_t[(size_t)tparent[(size_t)t]];
+    ix.tree_height = std::max(ix.tree_height, ix.depth[(size_t)p] + 1);
+  }
+  for (int32_t p = n - 1; p > 0; --p) ix.subtree[(size_t)ix.parent[(size_t)p]] += ix.subtree[(size_t)p];
+
+  // Bags in preorder ids.
+  ix.bag_off.assign((size_t)n + 1, 0);
+  for (int32_t p = 0; p < n; ++p)
+    ix.bag_off[(size_t)p + 1] = ix.bag_off[(size_t)p] + (int64_t)bags[(size_t)ix.perm[(size_t)p]].size();
+  ix.bag.resize((size_t)ix.bag_off[(size_t)n]);
+  for (int32_t p = 0; p < n; ++p) {
+    const auto &b = bags[(size_t)ix.perm[(size_t)p]];
+    for (size_t i = 0; i < b.size(); ++i)
+      ix.bag[(size_t)ix.bag_off[(size_t)p] + i] = ix.iperm[(size_t)b[i].nb];
+  }
+
+  // Label rows.
+  ix.row_off.assign((size_t)n + 1, 0);
+  for (int32_t p = 0; p < n; ++p)
+    ix.row_off[(size_t)p + 1] = ix.row_off[(size_t)p] + ix.depth[(size_t)p] + 1;
+  const int64_t E = ix.row_off[(size_t)n];
+  ix.dis.resize((size_t)E);
+  ix.cnt.resize((size_t)E);
+  ix.hop.resize((size_t)E);
+  std::vector<int32_t> by_depth((size_t)n), lvl_off((size_t)ix.tree_height + 1, 0);
+  for (int32_t p = 0; p < n; ++p) lvl_off[(size_t)ix.depth[(size_t)p] + 1]++;
+  for (int32_t d = 0; d < ix.tree_height; ++d) lvl_off[(size_t)d + 1] += lvl_off[(size_t)d];
+  {
+    std::vector<int32_t> f(lvl_off.begin(), lvl_off.end() - 1);
+    for (int32_t p = 0; p < n; ++p) by_depth[(size_t)f[(size_t)ix.depth[(size_t)p]]++] = p;
+  }
+  int32_t ovf_all = ix.count_overflow;
+  for (int32_t d = 0; d < ix.tree_height; ++d) {
+    const int32_t lo = lvl_off[(size_t)d], hi = lvl_off[(size_t)d + 1];
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : ovf_all) if (hi - lo > 64)
+    for (int32_t k = lo; k < hi; ++k) {
+      const int32_t p = by_depth[(size_t)k];
+      const int32_t D = ix.depth[(size_t)p];
+      int32_t *rd = ix.dis.data() + ix.row_off[(size_t)p];
+      uint64_t *rc = ix.cnt.data() + ix.row_off[(size_t)p];
+      uint32_t *rh = ix.hop.data() + ix.row_off[(size_t)p];
+      for (int32_t j = 0; j < D; ++j) {
+        rd[j] = kInf;
+        rc[j] = 0;
+        rh[j] = 0;
+      }
+      rd[D] = 0;
+      rc[D] = 1;
+      rh[D] = 0;
+      const auto &b = bags[(size_t)ix.perm[(size_t)p]];
+      for (const Arc &a : b) {  // direct shortcut to each bag ancestor
+        const int32_t kpos = ix.depth[(size_t)ix.iperm[(size_t)a.nb]];
+        rd[kpos] = a.len;
+        rc[kpos] = a.cnt;
+        rh[kpos] = a.hop;
+      }
+      int32_t ovf = 0;
+      for (const Arc &a : b) {
+        const int32_t x = ix.iperm[(size_t)a.nb];
+        const int32_t kpos = ix.depth[(size_t)x];
+        const int32_t *xd = ix.dis.data() + ix.row_off[(size_t)x];
+        const uint64_t *xc = ix.cnt.data() + ix.row_off[(size_t)x];
+        const uint32_t *xh = ix.hop.data() + ix.row_off[(size_t)x];
+        for (int32_t j = 0; j < kpos; ++j) {
+          if (xd[j] >= kInf) continue;
+          const int64_t cand64 = (int64_t)a.len + xd[j];
+          if (cand64 > rd[j]) continue;
+          const int32_t cand = (int32_t)cand64;
+          const uint64_t c = mul_sat(a.cnt, xc[j], ovf);
+          const uint32_t h = a.hop + xh[j];
+          if (cand < rd[j]) {
+            rd[j] = cand;
+            rc[j] = c;
+            rh[j] = h;
+          } else {
+            rc[j] = add_sat(rc[j], c, ovf);
+            if (h > rh[j]) rh[j] = h;
+          }
+        }
+      }
+      ovf_all |= ovf;
+    }
+  }
+  ix.count_overflow = ovf_all;
+  ix.build_seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+int32_t lca_depth(const Index &ix, int32_t p, int32_t q) {
+  // Ancestor test on preorder ids: a is an ancestor-or-self of x iff
+  // a <= x < a + subtree[a].
+  while (!(p <= q && q < p + ix.subtree[(size_t)p])) p = ix.parent[(size_t)p];
+  return ix.depth[(size_t)p];
+}
+
+// Pair query over the common prefix of two rows: the aggregation of
+// MHLQBC.cu:356-404 / TreeDecomp.cpp:969-1017, with the ancestor case folded
+// into the self entry.
+void index_query(const Index &ix, int32_t s, int32_t t, int32_t &d, uint64_t &sigma,
+                 uint32_t &hop, bool &ovf) {
+  const int32_t p = ix.iperm[(size_t)s], q = ix.iperm[(size_t)t];
+  const int32_t L = lca_depth(ix, std::min(p, q), std::max(p, q));
+  const int64_t op = ix.row_off[(size_t)p], oq = ix.row_off[(size_t)q];
+  int32_t best = 2 * kInf + 1;
+  uint64_t c = 0;
+  uint32_t h = 0;
+  int32_t of = 0;
+  for (int32_t i = 0; i <= L; ++i) {
+    const int32_t v = ix.dis[(size_t)(op + i)] + ix.dis[(size_t)(oq + i)];
+    if (v > best) continue;
+    const uint64_t cc = mul_sat(ix.cnt[(size_t)(op + i)], ix.cnt[(size_t)(oq + i)], of);
+    const uint32_t hh = ix.hop[(size_t)(op + i)] + ix.hop[(size_t)(oq + i)];
+    if (v < best) {
+      best = v;
+      c = cc;
+      h = hh;
+    } else {
+      c = add_sat(c, cc, of);
+      if (hh > h) h = hh;
+    }
+  }
+  d = best;
+  sigma = c;
+  hop = h;
+  ovf = of != 0;
+}
+
+// ---- binary image ---------------------------------------------------------
+namespace {
+const char kMagic[8] = {'B', 'C', 'X', 'I', 'D', 'X', '0', '1'};
+template <class T>
+void wvec(FILE *f, const std::vector<T> &v) {
+  int64_t k = (int64_t)v.size();
+  if (std::fwrite(&k, sizeof k, 1, f) != 1 ||
+      (k && std::fwrite(v.data(), sizeof(T), (size_t)k, f) != (size_t)k))
+    throw Error{BCX_EIO, "index write failed"};
+}
+template <class T>
+void rvec(FILE *f, std::vector<T> &v) {
+  int64_t k;
+  if (std::fread(&k, sizeof k, 1, f) != 1 || k < 0 || k > ((int64_t)1 << 40))
+    throw Error{BCX_EIO, "index file corrupt"};
+  v.resize((size_t)k);
+  if (k && std::fread(v.data(), sizeof(T), (size_t)k, f) != (size_t)k)
+    throw Error{BCX_EIO, "index file truncated"};
+}
+}  // namespace
+
+void save_index(const Index &ix, const char *path) {
+  FILE *f = std::fopen(path, "wb");
+  if (!f) throw Error{BCX_EIO, std::string("cannot write ") + path};
+  try {
+    int32_t hdr[4] = {ix.n, ix.tree_height, ix.tree_width, ix.count_overflow};
+    if (std::fwrite(kMagic, 1, 8, f) != 8 || std::fwrite(hdr, sizeof hdr, 1, f) != 1)
+      throw Error{BCX_EIO, "index write failed"};
+    wvec(f, ix.perm);
+    wvec(f, ix.depth);
+    wvec(f, ix.parent);
+    wvec(f, ix.row_off);
+    wvec(f, ix.dis);
+    wvec(f, ix.cnt);
+    wvec(f, ix.hop);
+    wvec(f, ix.bag_off);
+    wvec(f, ix.bag);
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  if (std::fclose(f) != 0) throw Error{BCX_EIO, "index write failed"};
+}
+
+void load_index(const char *path, Index &ix) {
+  FILE *f = std::fopen(path, "rb");
+  if (!f) throw Error{BCX_EIO, std::string("cannot open ") + path};
+  ix = Index();
+  try {
+    char m[8];
+    int32_t hdr[4];
+    if (std::fread(m, 1, 8, f) != 8 || std::memcmp(m, kMagic, 8) != 0 ||
+        std::fread(hdr, sizeof hdr, 1, f) != 1)
+      throw Error{BCX_EIO, std::string(path) + " is not a bcx index image"};
+    ix.n = hdr[0];
+    ix.tree_height = hdr[1];
+    ix.tree_width = hdr[2];
+    ix.count_overflow = hdr[3];
+    rvec(f, ix.perm);
+    rvec(f, ix.depth);
+    rvec(f, ix.parent);
+    rvec(f, ix.row_off);
+    rvec(f, ix.dis);
+    rvec(f, ix.cnt);
+    rvec(f, ix.hop);
+    rvec(f, ix.bag_off);
+    rvec(f, ix.bag);
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  std::fclose(f);
+  const int32_t n = ix.n;
+  if (n <= 0 || (int32_t)ix.perm.size() != n || (int32_t)ix.depth.size() != n ||
+      (int32_t)ix.parent.size() != n || (int64_t)ix.row_off.size() != (int64_t)n + 1 ||
+      (int64_t)ix.dis.size() != ix.row_off[(size_t)n] || ix.cnt.size() != ix.dis.size() ||
+      ix.hop.size() != ix.dis.size())
+    throw Error{BCX_EIO, std::string(path) + ": inconsistent index image"};
+  ix.iperm.assign((size_t)n, -1);
+  for (int32_t p = 0; p < n; ++p) {
+    int32_t v = ix.perm[(size_t)p];
+    if (v < 0 || v >= n || ix.iperm[(size_t)v] != -1)
+      throw Error{BCX_EIO, std::string(path) + ": bad permutation"};
+    ix.iperm[(size_t)v] = p;
+  }
+  ix.subtree.assign((size_t)n, 1);
+  for (int32_t p = n - 1; p > 0; --p) {
+    int32_t a = ix.parent[(size_t)p];
+    if (a < 0 || a >= p) throw Error{BCX_EIO, std::string(path) + ": bad tree"};
+    ix.subtree[(size_t)a] += ix.subtree[(size_t)p];
+  }
+}
+
+}  // namespace bcx

@@ -1,0 +1,180 @@
+/*
+ * bcx.h — C ABI of the MI355X-native QBC / MHL-QBC / RQ betweenness path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns BCX_OK (0) or a
+ * negative BCX_E* code; bcx_last_error() returns a thread-local message for
+ * the last failing call on the calling thread.  The caller owns every host
+ * array it passes; the library copies what it needs and owns device memory
+ * per handle.  Vertex ids at this boundary are the 0-based ids of the graph
+ * file (file id - 1), as in the reference's Graph class.
+ *
+ * Reference interfaces replaced (CSU-WRZ/GPU_BetweennessCentrality):
+ *   graph reading ............ Graph::ReadGraph (src/Graph.cpp:4-85) and
+ *                              Graph_::ReadWeightedGraph (src/TreeCount/Graph.cpp:41-81)
+ *   count-index build ........ TreeDecomp ctor/Reduce/MakeTree/MakeIndex
+ *                              (src/TreeCount/TreeDecomp.cpp:12-511), called at
+ *                              MHLQBC.cu:497-501 and RQ.cu:470-474
+ *   order-file index build ... Graph::CHconsorderMT/makeTree/makeIndex
+ *                              (src/buildIndex.cpp:4-278), called at QBC.cu:421-423
+ *   device upload ............ moveCountIndexToDevice (MHLQBC.cu:6-128),
+ *                              moveDisIndexToDevice (QBC.cu:11-127),
+ *                              initGraphTravel (MHLQBC.cu:130-160)
+ *   full BC .................. Graph::brandesBCIndexGPU_5 (MHLQBC.cu:495-537,
+ *                              QBC.cu:419-584) with BC_Kernel/addKernel
+ *   subset BC ................ Graph::brandesBCIndexGPU_PointBatch (RQ.cu:465-531)
+ *   (d, sigma, hop) probe .... stage 2 of BC_Kernel (MHLQBC.cu:334-412)
+ */
+#ifndef BCX_H_
+#define BCX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  BCX_OK = 0,
+  BCX_EINVAL = -1,     /* bad argument or malformed input file            */
+  BCX_EIO = -2,        /* file cannot be opened / read / written          */
+  BCX_EGRAPH = -3,     /* graph violates a precondition (see message)     */
+  BCX_ENOMEM = -4,     /* host or device allocation failed                */
+  BCX_EDEVICE = -5,    /* HIP runtime error                               */
+  BCX_EBUSY = -6,      /* RQ server: request cannot be admitted right now */
+  BCX_ENOTFOUND = -7,  /* RQ server: unknown request id                   */
+};
+
+/* Execution forms of the full-BC path. */
+enum {
+  BCX_MODE_PLAIN = 0,   /* one source per launch sequence                 */
+  BCX_MODE_BATCHED = 1, /* source tiles, many sources per launch, graphs  */
+};
+
+typedef struct bcx_graph bcx_graph; /* parsed graph, host memory           */
+typedef struct bcx_index bcx_index; /* tree-decomposition label index      */
+typedef struct bcx_ctx bcx_ctx;     /* device-resident graph + index       */
+typedef struct bcx_rq bcx_rq;       /* RQ serving engine on one context    */
+
+typedef struct bcx_index_stats {
+  int32_t n;              /* vertices                                       */
+  int32_t tree_height;    /* max depth + 1 (reference tree_height_)         */
+  int32_t tree_width;     /* max bag size (reference tree_width_)           */
+  int32_t count_overflow; /* 1 if any label path count overflowed uint64    */
+  int64_t label_entries;  /* sum over nodes of (depth + 1)                  */
+  double mean_depth;      /* label_entries / n                              */
+  double build_seconds;   /* elimination + tree + labels, wall clock        */
+} bcx_index_stats;
+
+typedef struct bcx_tuning {
+  int32_t tile;          /* sources per label-scan tile: 16, 32 or 64       */
+  int32_t tchunk;        /* targets per wavefront work item                 */
+  int32_t bp_threads;    /* threads per dependency (back-propagation) group */
+  int32_t batch;         /* sources per batched launch sequence             */
+} bcx_tuning;
+
+typedef struct bcx_timing {
+  double scan_ms;     /* label scan (distance, count, hop) kernels         */
+  double dep_ms;      /* DAG + layered dependency kernels                  */
+  double reduce_ms;   /* BC accumulation kernels                           */
+  int64_t scan_launches;
+  int64_t dep_launches;
+  int64_t sources;
+} bcx_timing;
+
+const char *bcx_last_error(void);
+const char *bcx_version(void);
+
+/* ---- graph ------------------------------------------------------------ */
+/* Reads "n m" then m lines "u v w" (1-based ids).  Each undirected edge must
+ * be listed in both directions with equal weight w >= 1; duplicate lines are
+ * dropped first-wins as in src/Graph.cpp:29-35.  Violations return BCX_EGRAPH
+ * (the reference continues on such input; see src/Graph.cpp:7-9). */
+int bcx_graph_read(const char *path, bcx_graph **out);
+/* Same contract from memory: arcs (u[i], v[i], w[i]), 0-based ids. */
+int bcx_graph_from_arcs(int32_t n, int64_t arcs, const int32_t *u,
+                        const int32_t *v, const int32_t *w, bcx_graph **out);
+int bcx_graph_info(const bcx_graph *g, int32_t *n, int64_t *arcs);
+/* 1 if connected (every path precondition of the index needs it). */
+int bcx_graph_connected(const bcx_graph *g);
+void bcx_graph_free(bcx_graph *g);
+
+/* ---- index ------------------------------------------------------------ */
+/* Min-degree elimination index with distance, uint64 path-count and max-hop
+ * labels (TreeDecomp::Reduce/MakeTree/MakeIndex, TreeDecomp.cpp:54-511). */
+int bcx_index_build_mhl(const bcx_graph *g, bcx_index **out);
+/* Same labels, elimination in the order of an order file ("n" then
+ * "nodeID order" lines, 0-based ids; Graph::CHconsorderMT, buildIndex.cpp:4-30). */
+int bcx_index_build_order(const bcx_graph *g, const char *order_path,
+                          bcx_index **out);
+/* Same, from an in-memory order: order[v] = contraction rank of vertex v. */
+int bcx_index_build_with_order(const bcx_graph *g, const int32_t *order,
+                               bcx_index **out);
+int bcx_index_stats_get(const bcx_index *ix, bcx_index_stats *st);
+/* Host-side (d, sigma, hop) query on the labels; used by tools and tests. */
+int bcx_index_query(const bcx_index *ix, int32_t s, int32_t t, int32_t *d,
+                    uint64_t *sigma, uint32_t *hop);
+/* order[p] = vertex id of the p-th tree node in DFS preorder (n entries).
+ * Sources processed in this order share label prefixes best. */
+int bcx_index_tree_order(const bcx_index *ix, int32_t *order);
+/* Mean scan length pos(lca)+1 over `pairs` seeded random pairs. */
+int bcx_index_mean_scan(const bcx_index *ix, int64_t pairs, uint64_t seed,
+                        double *mean);
+/* Write / read a device-ready binary image of the index (own format). */
+int bcx_index_save(const bcx_index *ix, const char *path);
+int bcx_index_load(const char *path, bcx_index **out);
+void bcx_index_free(bcx_index *ix);
+
+/* ---- device context --------------------------------------------------- */
+int bcx_ctx_create(int device, const bcx_graph *g, const bcx_index *ix,
+                   bcx_ctx **out);
+void bcx_ctx_free(bcx_ctx *ctx);
+int bcx_ctx_set_tuning(bcx_ctx *ctx, const bcx_tuning *tn);
+int bcx_ctx_get_tuning(const bcx_ctx *ctx, bcx_tuning *tn);
+/* Bytes of device memory one batched sequence of `batch` sources needs. */
+int bcx_ctx_workspace_bytes(const bcx_ctx *ctx, int32_t batch, size_t *bytes);
+int bcx_ctx_mem_info(const bcx_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
+int bcx_ctx_timing(const bcx_ctx *ctx, bcx_timing *t);
+void bcx_ctx_timing_reset(bcx_ctx *ctx);
+/* on != 0: launch batches kernel by kernel with HIP events between phases
+ * (accumulated into bcx_timing) instead of replaying a captured graph. */
+int bcx_ctx_set_profiling(bcx_ctx *ctx, int on);
+
+/* ---- hot path --------------------------------------------------------- */
+/* Full BC over all sources: bc_out[n] (undirected, unnormalised, each
+ * unordered pair once; MHLQBC.cu:470 / QBC.cu:395).  overflow (may be NULL)
+ * receives 1 if any path count overflowed uint64. */
+int bcx_full_bc(bcx_ctx *ctx, int mode, double *bc_out, uint32_t *overflow);
+/* Dependencies of a source subset added into bc_inout[n] (0.5 * delta per
+ * source).  Sources are accumulated in increasing tree-order, so a result
+ * does not depend on the mode or on how the set is split into calls. */
+int bcx_bc_sources(bcx_ctx *ctx, int mode, const int32_t *sources,
+                   int32_t count, double *bc_inout, uint32_t *overflow);
+/* Distance, path count and max-hop layer from `source` to every vertex. */
+int bcx_query_dist_count(bcx_ctx *ctx, int32_t source, int32_t *d,
+                         uint64_t *sigma, uint32_t *hop);
+/* Subset BC (RQ): bc_out[q] = BC of points[i] (RQ.cu:451-459, summed). */
+int bcx_subset_bc(bcx_ctx *ctx, const int32_t *points, int32_t q,
+                  double *bc_out);
+
+/* ---- RQ serving engine ------------------------------------------------ */
+/* Continuous-batching subset-BC server on one context: requests are admitted
+ * against a device-memory budget and ride one cyclic source sweep; each
+ * completes after a full cycle and its result is summed in a fixed order, so
+ * it is bitwise the same alone or co-batched.  Driven by bcx_rq_step(). */
+int bcx_rq_create(bcx_ctx *ctx, size_t hbm_budget_bytes, bcx_rq **out);
+void bcx_rq_free(bcx_rq *rq);
+/* Admit a request; *id receives its handle.  BCX_EBUSY if it does not fit now. */
+int bcx_rq_submit(bcx_rq *rq, const int32_t *points, int32_t q, int64_t *id);
+int bcx_rq_cancel(bcx_rq *rq, int64_t id);
+/* Run one source batch of the sweep for all admitted requests.
+ * *active receives the number of requests still in flight. */
+int bcx_rq_step(bcx_rq *rq, int32_t *active);
+/* 1 and result copied (bc_out[q]) if done, 0 if still running. */
+int bcx_rq_poll(bcx_rq *rq, int64_t id, double *bc_out);
+int bcx_rq_bytes_needed(const bcx_rq *rq, int32_t q, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BCX_H_ */
