@@ -72,6 +72,7 @@ _SIGS = {
                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
     "bcx_index_mean_scan": (C.c_int, [_P, C.c_int64, C.c_uint64, C.POINTER(C.c_double)]),
     "bcx_index_tree_order": (C.c_int, [_P, _I32P]),
+    "bcx_index_tree": (C.c_int, [_P, _I32P, _I32P]),
     "bcx_index_save": (C.c_int, [_P, C.c_char_p]),
     "bcx_index_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "bcx_index_free": (None, [_P]),
@@ -217,6 +218,14 @@ class Index:
         out = np.zeros(n, np.int32)
         _check(lib().bcx_index_tree_order(self._h, out))
         return out
+
+    def tree(self) -> tuple[np.ndarray, np.ndarray]:
+        """(parent vertex id or -1, height = depth + 1) per vertex."""
+        n = self.stats()["n"]
+        par = np.zeros(n, np.int32)
+        hgt = np.zeros(n, np.int32)
+        _check(lib().bcx_index_tree(self._h, par, hgt))
+        return par, hgt
 
     def mean_scan(self, pairs: int = 1 << 18, seed: int = 1) -> float:
         m = C.c_double()

@@ -154,6 +154,17 @@ int bcx_index_tree_order(const bcx_index *ix, int32_t *order) {
   return BCX_OK;
 }
 
+int bcx_index_tree(const bcx_index *ix, int32_t *parent, int32_t *height) {
+  if (!ix || !parent || !height) return bcx::fail(BCX_EINVAL, "null argument");
+  const auto &x = ix->ix;
+  for (int32_t p = 0; p < x.n; ++p) {
+    const int32_t v = x.perm[(size_t)p];
+    parent[v] = x.parent[(size_t)p] < 0 ? -1 : x.perm[(size_t)x.parent[(size_t)p]];
+    height[v] = x.depth[(size_t)p] + 1;
+  }
+  return BCX_OK;
+}
+
 int bcx_index_mean_scan(const bcx_index *ix, int64_t pairs, uint64_t seed, double *mean) {
   if (!ix || !mean || pairs <= 0) return bcx::fail(BCX_EINVAL, "bad argument");
   const auto &x = ix->ix;

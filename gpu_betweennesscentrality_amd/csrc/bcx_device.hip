@@ -76,22 +76,19 @@ struct DevWork {
   double *delta;   // slots * n
 };
 
-__device__ __forceinline__ uint64_t mul_ovf(uint64_t a, uint64_t b, int &ovf) {
+// Saturating count arithmetic: ~0 marks "overflowed" and is absorbing under
+// products with nonzero counts and under sums, so a result overflowed iff its
+// final count is ~0 (candidates that lose the minimum do not matter).
+__device__ __forceinline__ uint64_t mul_sat(uint64_t a, uint64_t b) {
   if (((a | b) >> 32) == 0) return a * b;
-  if (__umul64hi(a, b) != 0) {
-    ovf = 1;
-    return ~0ull;
-  }
+  if (a == 0 || b == 0) return 0;
+  if (a == ~0ull || b == ~0ull || __umul64hi(a, b) != 0) return ~0ull;
   return a * b;
 }
 
-__device__ __forceinline__ uint64_t add_ovf(uint64_t a, uint64_t b, int &ovf) {
+__device__ __forceinline__ uint64_t add_sat(uint64_t a, uint64_t b) {
   const uint64_t r = a + b;
-  if (r < a) {
-    ovf = 1;
-    return ~0ull;
-  }
-  return r;
+  return r < a ? ~0ull : r;
 }
 
 // Depth of LCA(u, v) for preorder ids: for u < v it is min(depth[u+1..v]) - 1.
@@ -104,23 +101,23 @@ __device__ __forceinline__ int lca_depth_dev(const DevIndex &ix, int u, int v) {
   return min(x, y) - 1;
 }
 
-// min / count-sum / hop-max over entries 0..L of two rows.
+// min / count-sum / hop-max over entries 0..L of two rows (one thread).
 __device__ __forceinline__ void scan_rows(const DevIndex &ix, int64_t rs, int64_t rt, int L,
-                                          int &m, uint64_t &c, uint32_t &h, int &ovf) {
+                                          int &m, uint64_t &c, uint32_t &h) {
   m = 0x7fffffff;
   c = 0;
   h = 0;
   for (int i = 0; i <= L; ++i) {
     const int v = ix.dis[rs + i] + ix.dis[rt + i];
     if (v <= m) {
-      const uint64_t cc = mul_ovf(ix.cnt[rs + i], ix.cnt[rt + i], ovf);
+      const uint64_t cc = mul_sat(ix.cnt[rs + i], ix.cnt[rt + i]);
       const uint32_t hh = ix.hop[rs + i] + ix.hop[rt + i];
       if (v < m) {
         m = v;
         c = cc;
         h = hh;
       } else {
-        c = add_ovf(c, cc, ovf);
+        c = add_sat(c, cc);
         h = max(h, hh);
       }
     }
@@ -134,51 +131,110 @@ __global__ __launch_bounds__(256) void scan_plain_kernel(DevIndex ix, DevBatch b
   if (t >= ix.n) return;
   const int s = bt.srcs[bt.cursor[0]];
   const int L = lca_depth_dev(ix, s, t);
-  int m, ovf = 0;
+  int m;
   uint64_t c;
   uint32_t h;
-  scan_rows(ix, ix.row_off[s], ix.row_off[t], L, m, c, h, ovf);
+  scan_rows(ix, ix.row_off[s], ix.row_off[t], L, m, c, h);
   ws.d[t] = m;
   ws.sigma[t] = c;
   ws.hop[t] = h;
-  if (ovf) atomicOr(ovf_flag, 1);
+  if (c == ~0ull) atomicOr(ovf_flag, 1);
 }
 
-// Batched form: lane = source of a 64-source tile, waves walk target chunks.
-__global__ __launch_bounds__(256) void scan_tile_kernel(DevIndex ix, DevBatch bt, DevWork ws,
-                                                        int batch, int tchunk, int *ovf_flag) {
+__device__ __forceinline__ int wave_min(int x) {
+  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, kWave));
+  return x;
+}
+__device__ __forceinline__ int wave_max(int x) {
+  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, kWave));
+  return x;
+}
+
+// Batched form.  A block owns a tile of 64 sources (lane j <-> source j) and a
+// chunk of targets; its waves share the tile's distance rows, staged in LDS
+// transposed ([entry][lane], conflict-free) for the first kTileDepth entries.
+// For each target the loop bound is wave-uniform (max LCA depth over the
+// lanes), so the target row is read with scalar loads once per 64 sources.
+// Pass 1 keeps the minimum and the first and last entry attaining it; the
+// count/hop of that entry are fetched afterwards, and only lanes with a tie
+// (first != last, i.e. shortest paths through different top vertices) rescan
+// their range to sum counts and max hops.
+constexpr int kTileDepth = 512;
+
+__global__ __launch_bounds__(1024) void scan_tile_kernel(
+    DevIndex ix, const int *__restrict__ gdis, const uint64_t *__restrict__ gcnt,
+    const uint32_t *__restrict__ ghop, const int64_t *__restrict__ grow_off, DevBatch bt,
+    int *__restrict__ out_d, uint64_t *__restrict__ out_s, uint32_t *__restrict__ out_h,
+    int batch, int tchunk, int *ovf_flag) {
+  __shared__ int sdis[kTileDepth * kWave];
   const int base = bt.cursor[0];
   const int total = bt.total[0];
+  const int tile0 = blockIdx.x * kWave;
+  if (base + tile0 >= total) return;  // uniform over the block, before any barrier
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
-  const int slot = blockIdx.x * kWave + lane;
+  const int nwaves = blockDim.x / kWave;
+  const int slot = tile0 + lane;
   const bool active = slot < batch && base + slot < total;
-  if (base + (int)blockIdx.x * kWave >= total) return;  // whole tile past the end
-  const int s = active ? bt.srcs[base + slot] : bt.srcs[base + blockIdx.x * kWave];
-  const int64_t rs = ix.row_off[s];
-  const int t0 = (blockIdx.y * (blockDim.x / kWave) + wave) * tchunk;
+  const int s = active ? bt.srcs[base + slot] : bt.srcs[base + tile0];
+  const int64_t rs = grow_off[s];
+  const int sdepth = ix.depth[s];
+  for (int i = wave; i < kTileDepth; i += nwaves)
+    sdis[i * kWave + lane] = i <= sdepth ? gdis[rs + i] : 0;
+  __syncthreads();
   const int n = ix.n;
+  const int t0 = (blockIdx.y * nwaves + wave) * tchunk;
   int ovf = 0;
   for (int tt = 0; tt < tchunk; ++tt) {
     const int t = __builtin_amdgcn_readfirstlane(t0 + tt);
     if (t >= n) break;
-    const int64_t rt = ix.row_off[t];
+    const int64_t rt = __builtin_amdgcn_readfirstlane(grow_off[t]);
+    const int *__restrict__ dt = gdis + rt;
     const int L = lca_depth_dev(ix, s, t);
-    int m;
-    uint64_t c;
-    uint32_t h;
-    scan_rows(ix, rs, rt, L, m, c, h, ovf);
+    const int Lmin = __builtin_amdgcn_readfirstlane(wave_min(L));
+    const int Lmax = __builtin_amdgcn_readfirstlane(wave_max(L));
+    int m = 0x7fffffff, first = 0, last = 0;
+    const int e1 = min(Lmin + 1, kTileDepth);
+    int i = 0;
+    // Unmasked: every lane's range covers i (i <= Lmin), source entry in LDS.
+#pragma unroll 4
+    for (; i < e1; ++i) {
+      const int v = sdis[i * kWave + lane] + dt[i];
+      first = v < m ? i : first;
+      m = min(m, v);
+      last = v == m ? i : last;
+    }
+    // Masked tail (lanes whose LCA is deeper than the wave minimum), and
+    // entries past the LDS window read from the label row in HBM.
+    for (; i <= Lmax; ++i) {
+      if (i <= L) {
+        const int sv = i < kTileDepth ? sdis[i * kWave + lane] : gdis[rs + i];
+        const int v = sv + dt[i];
+        first = v < m ? i : first;
+        m = min(m, v);
+        last = v == m ? i : last;
+      }
+    }
+    uint64_t c = mul_sat(gcnt[rs + first], gcnt[rt + first]);
+    uint32_t h = ghop[rs + first] + ghop[rt + first];
+    for (int k = first + 1; k <= last; ++k) {  // ties only
+      if (gdis[rs + k] + gdis[rt + k] == m) {
+        c = add_sat(c, mul_sat(gcnt[rs + k], gcnt[rt + k]));
+        h = max(h, ghop[rs + k] + ghop[rt + k]);
+      }
+    }
     if (active) {
       const int64_t o = (int64_t)slot * n + t;
-      ws.d[o] = m;
-      ws.sigma[o] = c;
-      ws.hop[o] = h;
+      out_d[o] = m;
+      out_s[o] = c;
+      out_h[o] = h;
+      ovf |= c == ~0ull;
     }
   }
   if (ovf) atomicOr(ovf_flag, 1);
 }
 
-// Block-wide exclusive scan of a[0..M) in global memory; returns the total.
+// Block-wide exclusive scan of a[0..M) (LDS or global); returns the total.
 __device__ int block_exclusive_scan(int *a, int M, int *sh) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int per = (M + nt - 1) / nt;
@@ -204,32 +260,17 @@ __device__ int block_exclusive_scan(int *a, int M, int *sh) {
   return tot;
 }
 
-// One block per source slot: layer buckets + layered pull back-propagation.
-__global__ __launch_bounds__(1024) void dep_kernel(DevIndex ix, DevBatch bt, DevWork ws,
-                                                   int batch) {
-  __shared__ int sh[1024];
-  __shared__ int s_max;
-  const int slot = blockIdx.x;
-  const int base = bt.cursor[0];
-  if (slot >= batch || base + slot >= bt.total[0]) return;
+constexpr int kDepThreads = 256;
+constexpr int kLdsBins = 4096;
+
+// Counting sort of vertices by layer, then the layered pull back-propagation.
+// `start`/`fill` point to LDS when the layer count fits, else to global memory.
+__device__ __forceinline__ void dep_body(const DevIndex &ix, int s, const int *d,
+                                         const uint64_t *sg, const uint32_t *hp, int *order,
+                                         double *delta, int maxL, int *start, int *fill,
+                                         int *sh) {
   const int n = ix.n;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int s = bt.srcs[base + slot];
-  const int *d = ws.d + (int64_t)slot * n;
-  const uint64_t *sg = ws.sigma + (int64_t)slot * n;
-  const uint32_t *hp = ws.hop + (int64_t)slot * n;
-  int *order = ws.order + (int64_t)slot * n;
-  int *start = ws.lcnt + (int64_t)slot * 2 * (n + 2);
-  int *fill = start + (n + 2);
-  double *delta = ws.delta + (int64_t)slot * n;
-
-  if (tid == 0) s_max = 0;
-  __syncthreads();
-  int mx = 0;
-  for (int v = tid; v < n; v += nt) mx = max(mx, (int)hp[v]);
-  atomicMax(&s_max, mx);
-  __syncthreads();
-  const int maxL = s_max;
   const int M = maxL + 2;
   for (int i = tid; i < M; i += nt) start[i] = 0;
   __syncthreads();
@@ -256,6 +297,40 @@ __global__ __launch_bounds__(1024) void dep_kernel(DevIndex ix, DevBatch bt, Dev
     __syncthreads();
   }
   if (tid == 0) delta[s] = 0.0;
+}
+
+// One block per source slot.
+__global__ __launch_bounds__(kDepThreads) void dep_kernel(DevIndex ix, DevBatch bt, DevWork ws,
+                                                          int batch) {
+  __shared__ int sh[kDepThreads];
+  __shared__ int lstart[kLdsBins + 2];
+  __shared__ int lfill[kLdsBins + 2];
+  __shared__ int s_max;
+  const int slot = blockIdx.x;
+  const int base = bt.cursor[0];
+  if (slot >= batch || base + slot >= bt.total[0]) return;
+  const int n = ix.n;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int s = bt.srcs[base + slot];
+  const int *d = ws.d + (int64_t)slot * n;
+  const uint64_t *sg = ws.sigma + (int64_t)slot * n;
+  const uint32_t *hp = ws.hop + (int64_t)slot * n;
+  int *order = ws.order + (int64_t)slot * n;
+  double *delta = ws.delta + (int64_t)slot * n;
+  if (tid == 0) s_max = 0;
+  __syncthreads();
+  int mx = 0;
+  for (int v = tid; v < n; v += nt) mx = max(mx, (int)hp[v]);
+  mx = wave_max(mx);
+  if ((tid & (kWave - 1)) == 0) atomicMax(&s_max, mx);
+  __syncthreads();
+  const int maxL = s_max;
+  if (maxL + 2 <= kLdsBins) {
+    dep_body(ix, s, d, sg, hp, order, delta, maxL, lstart, lfill, sh);
+  } else {
+    int *gs = ws.lcnt + (int64_t)slot * 2 * (n + 2);
+    dep_body(ix, s, d, sg, hp, order, delta, maxL, gs, gs + (n + 2), sh);
+  }
 }
 
 __global__ __launch_bounds__(256) void reduce_kernel(int n, DevBatch bt, const double *delta,
@@ -323,8 +398,7 @@ struct bcx_ctx {
   DevIndex ix{};
   std::vector<void *> owned;          // index + graph buffers
   std::vector<int32_t> perm, iperm;   // preorder <-> vertex
-  int32_t count_overflow_index = 0;
-  bcx_tuning tn{64, 16, 256, 4096};
+  bcx_tuning tn{64, 32, 256, 4096};
   // batch workspace
   int ws_slots = 0;
   DevWork ws{};
@@ -392,14 +466,15 @@ void launch_batch(bcx_ctx *c, int mode, int batch, bool timed, double *bsum = nu
   if (mode == BCX_MODE_PLAIN) {
     scan_plain_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(c->ix, bt, c->ws, c->d_ovf);
   } else {
-    const int waves = 4;
+    const int waves = 16;
     dim3 grid((batch + kWave - 1) / kWave,
               (n + waves * c->tn.tchunk - 1) / (waves * c->tn.tchunk));
-    scan_tile_kernel<<<grid, waves * kWave, 0, c->stream>>>(c->ix, bt, c->ws, batch, c->tn.tchunk,
-                                                            c->d_ovf);
+    scan_tile_kernel<<<grid, waves * kWave, 0, c->stream>>>(
+        c->ix, c->ix.dis, c->ix.cnt, c->ix.hop, c->ix.row_off, bt, c->ws.d, c->ws.sigma,
+        c->ws.hop, batch, c->tn.tchunk, c->d_ovf);
   }
   if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
-  dep_kernel<<<batch, c->tn.bp_threads, 0, c->stream>>>(c->ix, bt, c->ws, batch);
+  dep_kernel<<<batch, kDepThreads, 0, c->stream>>>(c->ix, bt, c->ws, batch);
   if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
   if (bsum) {
     batch_sum_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(n, bt, c->ws.delta, batch, bsum);
@@ -480,7 +555,7 @@ void run_sources(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, do
   HIP_OK(hipMemcpyAsync(&ovf, c->d_ovf, 4, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   for (int32_t v = 0; v < n; ++v) bc_inout[v] = bc_pre[(size_t)c->iperm[(size_t)v]];
-  if (overflow) *overflow = (uint32_t)(ovf | c->count_overflow_index);
+  if (overflow) *overflow = (uint32_t)ovf;
 }
 
 }  // namespace
@@ -505,7 +580,6 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
   const int n = g.n;
   c->perm = ix.perm;
   c->iperm = ix.iperm;
-  c->count_overflow_index = ix.count_overflow;
   // CSR in preorder ids, arcs of each vertex in file order.
   std::vector<int32_t> grow((size_t)n + 1, 0), gcol((size_t)g.arcs()), gw((size_t)g.arcs());
   if (g.arcs() > 0x7fffffff) return bcx::fail(BCX_EGRAPH, "more than 2^31 arcs");
@@ -666,7 +740,7 @@ int bcx_query_dist_count(bcx_ctx *c, int32_t source, int32_t *d, uint64_t *sigma
     if (sigma) sigma[v] = ss[p];
     if (hop) hop[v] = hh[p];
   }
-  return (ovf | c->count_overflow_index) ? 1 : BCX_OK;
+  return ovf ? 1 : BCX_OK;
   BCX_GUARD_END
 }
 

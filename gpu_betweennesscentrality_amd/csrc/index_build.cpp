@@ -46,6 +46,8 @@ struct Arc {
   uint32_t hop;
 };
 
+// Saturating count arithmetic: UINT64_MAX marks "overflowed"; it stays
+// UINT64_MAX under products with nonzero counts and under sums.
 inline uint64_t mul_sat(uint64_t a, uint64_t b, int32_t &ovf) {
   uint64_t r;
   if (__builtin_mul_overflow(a, b, &r)) {
@@ -361,7 +363,10 @@ void index_query(const Index &ix, int32_t s, int32_t t, int32_t &d, uint64_t &si
   d = best;
   sigma = c;
   hop = h;
-  ovf = of != 0;
+  // Counts saturate at UINT64_MAX and saturation propagates through products
+  // and sums, so only a saturated final count means the true count overflowed
+  // (a saturated candidate that loses the minimum does not matter).
+  ovf = c == UINT64_MAX;
 }
 
 // ---- binary image ---------------------------------------------------------

@@ -60,7 +60,9 @@ typedef struct bcx_index_stats {
   int32_t n;              /* vertices                                       */
   int32_t tree_height;    /* max depth + 1 (reference tree_height_)         */
   int32_t tree_width;     /* max bag size (reference tree_width_)           */
-  int32_t count_overflow; /* 1 if any label path count overflowed uint64    */
+  int32_t count_overflow; /* 1 if some label count saturated at UINT64_MAX;
+                             informational: results are flagged only when
+                             their own final count saturates               */
   int64_t label_entries;  /* sum over nodes of (depth + 1)                  */
   double mean_depth;      /* label_entries / n                              */
   double build_seconds;   /* elimination + tree + labels, wall clock        */
@@ -117,6 +119,9 @@ int bcx_index_query(const bcx_index *ix, int32_t s, int32_t t, int32_t *d,
 /* order[p] = vertex id of the p-th tree node in DFS preorder (n entries).
  * Sources processed in this order share label prefixes best. */
 int bcx_index_tree_order(const bcx_index *ix, int32_t *order);
+/* Tree shape per vertex v: parent[v] = parent vertex id (-1 for the root),
+ * height[v] = depth + 1 (the reference's Node_::height, TreeDecomp.h:106). */
+int bcx_index_tree(const bcx_index *ix, int32_t *parent, int32_t *height);
 /* Mean scan length pos(lca)+1 over `pairs` seeded random pairs. */
 int bcx_index_mean_scan(const bcx_index *ix, int64_t pairs, uint64_t seed,
                         double *mean);
