@@ -41,6 +41,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rq", action="store_true")
     ap.add_argument("--rq-requests", type=int, default=6)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -74,6 +75,11 @@ def main() -> int:
         return tree_perm[start:start + B]
 
     total = args.warmup + args.steps
+    tuning = ctx.tuning()
+    if not args.no_autotune:
+        ta = time.time()
+        tuning = ctx.autotune(tree_batch(0, total))
+        log(f"[bench] autotune: {tuning} ({time.time() - ta:.1f}s)")
     for k in range(args.warmup):
         ctx.bc_sources(tree_batch(k, total), bcx.MODE_BATCHED)
     ctx.reset_timing()
@@ -132,6 +138,7 @@ def main() -> int:
             "workload": f"{args.workload}: batched MHL-QBC full BC, n={n}, arcs={g.arcs}, "
                         f"{B} sources per batched launch sequence",
             "n": n, "arcs": int(g.arcs), "batch": B, "tile": S,
+            "tchunk": tuning["tchunk"], "dep_threads": tuning["bp_threads"],
             "tree_height": st["tree_height"], "tree_width": st["tree_width"],
             "label_entries": st["label_entries"], "Lbar": round(lbar, 2),
             "plain_sources_per_s": round(plain, 2),

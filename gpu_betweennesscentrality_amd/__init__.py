@@ -80,6 +80,7 @@ _SIGS = {
     "bcx_ctx_free": (None, [_P]),
     "bcx_ctx_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "bcx_ctx_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "bcx_ctx_autotune": (C.c_int, [_P, _I32P, C.c_int32, C.POINTER(Tuning)]),
     "bcx_ctx_workspace_bytes": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_size_t)]),
     "bcx_ctx_mem_info": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "bcx_ctx_timing": (C.c_int, [_P, C.POINTER(Timing)]),
@@ -262,6 +263,13 @@ class Context:
     def tuning(self) -> dict:
         tn = Tuning()
         _check(lib().bcx_ctx_get_tuning(self._h, C.byref(tn)))
+        return {k: getattr(tn, k) for k, _ in Tuning._fields_}
+
+    def autotune(self, sources) -> dict:
+        """Pick target chunk and dependency-group width on a sample batch."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        tn = Tuning()
+        _check(lib().bcx_ctx_autotune(self._h, src, len(src), C.byref(tn)))
         return {k: getattr(tn, k) for k, _ in Tuning._fields_}
 
     def set_profiling(self, on: bool) -> None:

@@ -72,7 +72,7 @@ struct DevWork {
   uint64_t *sigma; // slots * n
   uint32_t *hop;   // slots * n
   int *order;      // slots * n
-  int *lcnt;       // slots * 2 * (n + 2)
+  int *lcnt;       // slots * (n + 2): layer bins when they exceed LDS
   double *delta;   // slots * n
 };
 
@@ -260,37 +260,53 @@ __device__ int block_exclusive_scan(int *a, int M, int *sh) {
   return tot;
 }
 
-constexpr int kDepThreads = 256;
-constexpr int kLdsBins = 4096;
+constexpr int kLdsBins = 2048;
 
 // Counting sort of vertices by layer, then the layered pull back-propagation.
-// `start`/`fill` point to LDS when the layer count fits, else to global memory.
-__device__ __forceinline__ void dep_body(const DevIndex &ix, int s, const int *d,
-                                         const uint64_t *sg, const uint32_t *hp, int *order,
-                                         double *delta, int maxL, int *start, int *fill,
-                                         int *sh) {
+// `bins` (LDS when the layer count fits, else global) first counts layer
+// sizes, is scanned to layer starts, then advanced by the scatter so that
+// afterwards bins[L] is the end of layer L and layer L is [bins[L-1], bins[L]).
+// Arc loads of a vertex are issued four at a time so a layer costs a few
+// dependent memory round trips instead of a few per arc; the sum keeps CSR
+// order, so the result does not depend on scheduling.
+__device__ __forceinline__ void dep_body(const DevIndex &ix, int s, const int *__restrict__ d,
+                                         const uint64_t *__restrict__ sg,
+                                         const uint32_t *__restrict__ hp, int *__restrict__ order,
+                                         double *delta, int maxL, int *bins, int *sh) {
   const int n = ix.n;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int M = maxL + 2;
-  for (int i = tid; i < M; i += nt) start[i] = 0;
+  for (int i = tid; i < M; i += nt) bins[i] = 0;
   __syncthreads();
-  for (int v = tid; v < n; v += nt) atomicAdd(&start[hp[v]], 1);
+  for (int v = tid; v < n; v += nt) atomicAdd(&bins[hp[v]], 1);
   __syncthreads();
-  block_exclusive_scan(start, M, sh);
-  for (int i = tid; i < M; i += nt) fill[i] = start[i];
+  block_exclusive_scan(bins, M, sh);
+  for (int v = tid; v < n; v += nt) order[atomicAdd(&bins[hp[v]], 1)] = v;
   __syncthreads();
-  for (int v = tid; v < n; v += nt) order[atomicAdd(&fill[hp[v]], 1)] = v;
-  __syncthreads();
+  const int *__restrict__ grow = ix.grow;
+  const int *__restrict__ gcol = ix.gcol;
+  const int *__restrict__ gw = ix.gw;
   for (int L = maxL; L >= 1; --L) {
-    const int lo = start[L], hi = start[L + 1];
+    const int lo = bins[L - 1], hi = bins[L];
     for (int k = lo + tid; k < hi; k += nt) {
       const int u = order[k];
+      const int a0 = grow[u], a1 = grow[u + 1];
       const int du = d[u];
       const double su = (double)sg[u];
       double acc = 0.0;
-      for (int a = ix.grow[u]; a < ix.grow[u + 1]; ++a) {
-        const int v = ix.gcol[a];
-        if (d[v] == du + ix.gw[a]) acc += su * (1.0 + delta[v]) / (double)sg[v];
+      for (int a = a0; a < a1; a += 4) {
+        int vv[4], dv[4], ww[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          vv[q] = a + q < a1 ? gcol[a + q] : u;
+          ww[q] = a + q < a1 ? gw[a + q] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dv[q] = d[vv[q]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (a + q < a1 && dv[q] == du + ww[q])
+            acc += su * (1.0 + delta[vv[q]]) / (double)sg[vv[q]];
       }
       delta[u] = acc;
     }
@@ -300,11 +316,11 @@ __device__ __forceinline__ void dep_body(const DevIndex &ix, int s, const int *d
 }
 
 // One block per source slot.
+template <int kDepThreads>
 __global__ __launch_bounds__(kDepThreads) void dep_kernel(DevIndex ix, DevBatch bt, DevWork ws,
                                                           int batch) {
   __shared__ int sh[kDepThreads];
-  __shared__ int lstart[kLdsBins + 2];
-  __shared__ int lfill[kLdsBins + 2];
+  __shared__ int lbins[kLdsBins];
   __shared__ int s_max;
   const int slot = blockIdx.x;
   const int base = bt.cursor[0];
@@ -325,12 +341,11 @@ __global__ __launch_bounds__(kDepThreads) void dep_kernel(DevIndex ix, DevBatch 
   if ((tid & (kWave - 1)) == 0) atomicMax(&s_max, mx);
   __syncthreads();
   const int maxL = s_max;
-  if (maxL + 2 <= kLdsBins) {
-    dep_body(ix, s, d, sg, hp, order, delta, maxL, lstart, lfill, sh);
-  } else {
-    int *gs = ws.lcnt + (int64_t)slot * 2 * (n + 2);
-    dep_body(ix, s, d, sg, hp, order, delta, maxL, gs, gs + (n + 2), sh);
-  }
+  // Two call sites so each inlined copy knows its bins' address space.
+  if (maxL + 2 <= kLdsBins)
+    dep_body(ix, s, d, sg, hp, order, delta, maxL, lbins, sh);
+  else
+    dep_body(ix, s, d, sg, hp, order, delta, maxL, ws.lcnt + (int64_t)slot * (n + 2), sh);
 }
 
 __global__ __launch_bounds__(256) void reduce_kernel(int n, DevBatch bt, const double *delta,
@@ -436,7 +451,7 @@ struct bcx_ctx {
     ws_slots = 0;
   }
   static size_t slot_bytes(int n) {
-    return (size_t)n * (4 + 8 + 4 + 4 + 8) + (size_t)2 * ((size_t)n + 2) * 4;
+    return (size_t)n * (4 + 8 + 4 + 4 + 8) + ((size_t)n + 2) * 4;
   }
   void ensure_ws(int slots) {
     if (slots <= ws_slots) return;
@@ -446,7 +461,7 @@ struct bcx_ctx {
     HIP_OK(hipMalloc(&ws.sigma, n * slots * 8));
     HIP_OK(hipMalloc(&ws.hop, n * slots * 4));
     HIP_OK(hipMalloc(&ws.order, n * slots * 4));
-    HIP_OK(hipMalloc(&ws.lcnt, 2 * (n + 2) * slots * 4));
+    HIP_OK(hipMalloc(&ws.lcnt, (n + 2) * slots * 4));
     HIP_OK(hipMalloc(&ws.delta, n * slots * 8));
     ws_slots = slots;
   }
@@ -474,7 +489,19 @@ void launch_batch(bcx_ctx *c, int mode, int batch, bool timed, double *bsum = nu
         c->ws.hop, batch, c->tn.tchunk, c->d_ovf);
   }
   if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
-  dep_kernel<<<batch, kDepThreads, 0, c->stream>>>(c->ix, bt, c->ws, batch);
+  switch (c->tn.bp_threads) {
+    case 64:
+      dep_kernel<64><<<batch, 64, 0, c->stream>>>(c->ix, bt, c->ws, batch);
+      break;
+    case 128:
+      dep_kernel<128><<<batch, 128, 0, c->stream>>>(c->ix, bt, c->ws, batch);
+      break;
+    case 512:
+      dep_kernel<512><<<batch, 512, 0, c->stream>>>(c->ix, bt, c->ws, batch);
+      break;
+    default:
+      dep_kernel<256><<<batch, 256, 0, c->stream>>>(c->ix, bt, c->ws, batch);
+  }
   if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
   if (bsum) {
     batch_sum_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(n, bt, c->ws.delta, batch, bsum);
@@ -640,8 +667,9 @@ int bcx_ctx_set_tuning(bcx_ctx *c, const bcx_tuning *tn) {
   if (!c || !tn) return bcx::fail(BCX_EINVAL, "null argument");
   if (tn->tile != 64) return bcx::fail(BCX_EINVAL, "tile must be 64");
   if (tn->tchunk < 1 || tn->tchunk > 4096) return bcx::fail(BCX_EINVAL, "tchunk out of range");
-  if (tn->bp_threads < 64 || tn->bp_threads > 1024 || (tn->bp_threads & (tn->bp_threads - 1)))
-    return bcx::fail(BCX_EINVAL, "bp_threads must be a power of two in [64, 1024]");
+  if (tn->bp_threads != 64 && tn->bp_threads != 128 && tn->bp_threads != 256 &&
+      tn->bp_threads != 512)
+    return bcx::fail(BCX_EINVAL, "bp_threads must be 64, 128, 256 or 512");
   if (tn->batch < 1 || tn->batch > (1 << 20)) return bcx::fail(BCX_EINVAL, "batch out of range");
   c->tn = *tn;
   return BCX_OK;
@@ -950,3 +978,63 @@ int bcx_rq_poll(bcx_rq *rq, int64_t id, double *bc_out) {
 }
 
 }  // extern "C"
+
+// ---- autotuner --------------------------------------------------------------
+// The two phases are tuned separately on one sample batch: the target chunk
+// of the label scan (targets per wavefront work item) by the scan time alone,
+// the width of the dependency group (threads sharing one source's layers) by
+// the dependency time alone.  Each candidate runs once to warm up and once
+// timed with HIP events on the context stream.
+extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t count,
+                                bcx_tuning *best_out) {
+  BCX_GUARD_BEGIN
+  if (!c || !sources || count <= 0) return bcx::fail(BCX_EINVAL, "bad argument");
+  HIP_OK(hipSetDevice(c->device));
+  const bcx_tuning start = c->tn;
+  const int32_t cnt = std::min(count, start.batch);
+  std::vector<double> scratch((size_t)c->ix.n, 0.0);
+  const bool was_profiling = c->profiling;
+  c->profiling = true;
+  auto trial = [&](const bcx_tuning &t, double &scan, double &dep) {
+    c->tn = t;
+    run_sources(c, BCX_MODE_BATCHED, sources, cnt, scratch.data(), nullptr);  // warm-up
+    c->timing = bcx_timing{};
+    run_sources(c, BCX_MODE_BATCHED, sources, cnt, scratch.data(), nullptr);
+    scan = c->timing.scan_ms;
+    dep = c->timing.dep_ms;
+  };
+  bcx_tuning best = start;
+  double best_scan = 1e300, best_dep = 1e300;
+  try {
+    for (int tc : {8, 16, 32, 64}) {
+      bcx_tuning t = start;
+      t.tchunk = tc;
+      double s, d;
+      trial(t, s, d);
+      if (s < best_scan) {
+        best_scan = s;
+        best.tchunk = tc;
+      }
+    }
+    for (int bp : {64, 128, 256, 512}) {
+      bcx_tuning t = best;
+      t.bp_threads = bp;
+      double s, d;
+      trial(t, s, d);
+      if (d < best_dep) {
+        best_dep = d;
+        best.bp_threads = bp;
+      }
+    }
+  } catch (...) {
+    c->tn = start;
+    c->profiling = was_profiling;
+    throw;
+  }
+  c->tn = best;
+  c->profiling = was_profiling;
+  c->timing = bcx_timing{};
+  if (best_out) *best_out = best;
+  return BCX_OK;
+  BCX_GUARD_END
+}

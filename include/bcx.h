@@ -69,9 +69,9 @@ typedef struct bcx_index_stats {
 } bcx_index_stats;
 
 typedef struct bcx_tuning {
-  int32_t tile;          /* sources per label-scan tile: 16, 32 or 64       */
+  int32_t tile;          /* sources per label-scan tile (= 64, one per lane) */
   int32_t tchunk;        /* targets per wavefront work item                 */
-  int32_t bp_threads;    /* threads per dependency (back-propagation) group */
+  int32_t bp_threads;    /* threads per dependency group: 64..512, pow2   */
   int32_t batch;         /* sources per batched launch sequence             */
 } bcx_tuning;
 
@@ -136,6 +136,12 @@ int bcx_ctx_create(int device, const bcx_graph *g, const bcx_index *ix,
 void bcx_ctx_free(bcx_ctx *ctx);
 int bcx_ctx_set_tuning(bcx_ctx *ctx, const bcx_tuning *tn);
 int bcx_ctx_get_tuning(const bcx_ctx *ctx, bcx_tuning *tn);
+/* Autotuner (no reference counterpart: the reference fixes 256x256 launches
+ * at compile time, MHLQBC.cu:3).  Times candidate target chunks and
+ * dependency-group widths on `sources` (one batch at most), keeps the fastest
+ * in the context and copies it to *best (may be NULL). */
+int bcx_ctx_autotune(bcx_ctx *ctx, const int32_t *sources, int32_t count,
+                     bcx_tuning *best);
 /* Bytes of device memory one batched sequence of `batch` sources needs. */
 int bcx_ctx_workspace_bytes(const bcx_ctx *ctx, int32_t batch, size_t *bytes);
 int bcx_ctx_mem_info(const bcx_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
