@@ -35,7 +35,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="road320k")
-    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--batch", type=int, default=4096, help="label-scan sources per batch")
+    ap.add_argument("--sweep-batch", type=int, default=16384, help="bag-sweep sources per batch")
     ap.add_argument("--plain-sources", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -69,38 +70,51 @@ def main() -> int:
     # spread evenly over it so the sample covers shallow and deep subtrees.
     tree_perm = ix.tree_order()
 
-    def tree_batch(k: int, nb: int) -> np.ndarray:
-        start = (k * (n - B)) // max(1, nb - 1) if nb > 1 else 0
+    def tree_batch(k: int, nb: int, size: int = B) -> np.ndarray:
+        size = min(size, n)
+        start = (k * (n - size)) // max(1, nb - 1) if nb > 1 else 0
         start -= start % 64
-        return tree_perm[start:start + B]
+        return tree_perm[start:start + size]
 
     total = args.warmup + args.steps
+
+    def time_mode(mode: int, label: str, size: int):
+        ctx.set_tuning(batch=size)
+        for k in range(args.warmup):
+            ctx.bc_sources(tree_batch(k, total, size), mode)
+        bc = np.zeros(n)
+        ovf = False
+        done = 0
+        t = time.perf_counter()
+        for k in range(args.warmup, total):
+            src = tree_batch(k, total, size)
+            bc, o = ctx.bc_sources(src, mode, bc)
+            done += len(src)
+            ovf |= o
+        el = time.perf_counter() - t
+        rate = done / el
+        ctx.set_profiling(True)
+        ctx.reset_timing()
+        ctx.bc_sources(tree_batch(total // 2, total, size), mode)
+        tm = ctx.timing()
+        ctx.set_profiling(False)
+        log(f"[bench] {label}: {done} sources in {el:.3f}s -> {rate:.1f} sources/s; "
+            f"phases {tm['scan_ms']:.1f} / {tm['dep_ms']:.1f} / {tm['reduce_ms']:.1f} ms")
+        return rate, el, tm, ovf
+
     tuning = ctx.tuning()
     if not args.no_autotune:
         ta = time.time()
         tuning = ctx.autotune(tree_batch(0, total))
         log(f"[bench] autotune: {tuning} ({time.time() - ta:.1f}s)")
-    for k in range(args.warmup):
-        ctx.bc_sources(tree_batch(k, total), bcx.MODE_BATCHED)
-    ctx.reset_timing()
-    bc = np.zeros(n)
-    t = time.perf_counter()
-    for k in range(args.warmup, total):
-        bc, ovf = ctx.bc_sources(tree_batch(k, total), bcx.MODE_BATCHED, bc)
-    el = time.perf_counter() - t
-    value = args.steps * B / el
-    log(f"[bench] batched: {args.steps} x {B} sources in {el:.3f}s -> {value:.1f} sources/s")
-
-    # Per-kernel timing with HIP events on the context stream (one batch).
-    ctx.set_profiling(True)
-    ctx.reset_timing()
-    ctx.bc_sources(tree_batch(total // 2, total), bcx.MODE_BATCHED)
-    tm = ctx.timing()
-    ctx.set_profiling(False)
-    scan_ms = tm["scan_ms"]
-    dep_ms = tm["dep_ms"]
-    red_ms = tm["reduce_ms"]
-    log(f"[bench] per batch: scan {scan_ms:.2f} ms, dep {dep_ms:.2f} ms, reduce {red_ms:.2f} ms")
+    scan_rate, scan_el, scan_tm, ovf1 = time_mode(bcx.MODE_BATCHED, "label-scan batched", B)
+    Bs = min(args.sweep_batch, n)
+    value, el, tm, ovf = time_mode(bcx.MODE_SWEEP, "bag-sweep batched", Bs)
+    ctx.set_tuning(batch=B)
+    ovf = ovf or ovf1
+    scan_ms = scan_tm["scan_ms"]
+    dep_ms = scan_tm["dep_ms"]
+    red_ms = scan_tm["reduce_ms"]
     S = 64
     scan_bytes = B * (16.0 * n * lbar / S + 16.0 * n)
     achieved = scan_bytes / (scan_ms * 1e-3) / 1e9
@@ -135,12 +149,14 @@ def main() -> int:
         "dtype": "int32+u64 (labels, counts) / f64 (dependencies, BC)",
         "data": "synthetic (seeded road-like Delaunay graph, generated in-process)",
         "config": {
-            "workload": f"{args.workload}: batched MHL-QBC full BC, n={n}, arcs={g.arcs}, "
-                        f"{B} sources per batched launch sequence",
-            "n": n, "arcs": int(g.arcs), "batch": B, "tile": S,
+            "workload": f"{args.workload}: MHL full BC (bag-sweep form), n={n}, arcs={g.arcs}, "
+                        f"{Bs} sources per batched launch",
+            "n": n, "arcs": int(g.arcs), "sweep_batch": Bs, "labelscan_batch": B, "tile": S,
             "tchunk": tuning["tchunk"], "dep_threads": tuning["bp_threads"],
             "tree_height": st["tree_height"], "tree_width": st["tree_width"],
             "label_entries": st["label_entries"], "Lbar": round(lbar, 2),
+            "labelscan_sources_per_s": round(scan_rate, 2),
+            "sweep_phase_ms": [round(tm["scan_ms"], 3), round(tm["dep_ms"], 3), round(tm["reduce_ms"], 3)],
             "plain_sources_per_s": round(plain, 2),
             "batched_over_plain": round(value / plain, 2),
             "full_bc_seconds_est": round(n / value, 2),

@@ -20,10 +20,12 @@ from pathlib import Path
 
 import numpy as np
 
-__all__ = ["BcxError", "Graph", "Index", "Context", "RQServer", "lib", "MODE_PLAIN", "MODE_BATCHED"]
+__all__ = ["BcxError", "Graph", "Index", "Context", "RQServer", "lib", "MODE_PLAIN", "MODE_BATCHED",
+           "MODE_SWEEP"]
 
 MODE_PLAIN = 0
 MODE_BATCHED = 1
+MODE_SWEEP = 2
 _PKG = Path(__file__).resolve().parent
 _LIB = None
 

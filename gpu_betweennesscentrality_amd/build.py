@@ -34,7 +34,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-Wall", "-Wno-unused-function",
              f"-I{INCLUDE}", f"-I{CSRC}"]
 HOST_SOURCES = ["graph_io.cpp", "index_build.cpp", "bcx_api.cpp"]
-HIP_SOURCES = ["bcx_device.hip"]
+HIP_SOURCES = ["bcx_device.hip", "bcx_sweep.hip"]
 CLIS = {"MHLQBC": "cli_mhlqbc.cpp", "QBC": "cli_qbc.cpp", "RQ": "cli_rq.cpp"}
 
 

@@ -1,0 +1,219 @@
+// bcx_dev.hpp — device-side definitions shared by the HIP translation units
+// of the bcx library: device views of the index and graph, batch and sweep
+// workspaces, the context object behind the C ABI, and small device helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "bcx.h"
+#include "bcx_host.hpp"
+#include "bcx_internal.hpp"
+
+#define HIP_OK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      throw bcx::Error{BCX_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+namespace bcx_dev {
+
+constexpr int kInfDev = 999999999;
+constexpr int kWave = 64;
+
+struct DevIndex {
+  int n;
+  int levels;
+  int64_t arcs;
+  const int *depth;       // n, preorder
+  const int64_t *row_off; // n + 1
+  const int *dis;         // entries
+  const uint64_t *cnt;    // entries
+  const uint32_t *hop;    // entries
+  const int *rmq;         // levels * n: rmq[k*n + i] = min depth over [i, i + 2^k)
+  const int *grow;        // n + 1, CSR in preorder ids
+  const int *gcol;        // arcs
+  const int *gw;          // arcs
+};
+
+// Batch bookkeeping lives in device memory so that one captured graph can be
+// replayed for every batch: src list, cursor (first list index of the batch),
+// total list length.
+struct DevBatch {
+  const int *srcs;
+  int *cursor;
+  const int *total;
+};
+
+struct DevWork {
+  int *d;          // slots * n
+  uint64_t *sigma; // slots * n
+  uint32_t *hop;   // slots * n
+  int *order;      // slots * n
+  int *lcnt;       // slots * (n + 2): layer bins when they exceed LDS
+  double *delta;   // slots * n
+};
+
+// Tree-decomposition view for the bag sweep (all ids in tree preorder).
+struct DevTree {
+  int n;
+  int height;
+  int64_t bags;            // total bag entries B
+  const int *subtree;      // n
+  const int *bag_off;      // n + 1, entries ordered by ancestor depth
+  const int *bag_v;        // B: ancestor
+  const int *bag_len;      // B: restricted shortcut length
+  const uint64_t *bag_cnt; // B: restricted shortest-path count
+  const int *ibag_off;     // n + 1: entries whose ancestor is this node
+  const int *ibag_x;       // B: the descendant owning the entry
+  const int *ibag_e;       // B: the entry index
+  const int *lvl_off;      // height + 1
+  const int *lvl_v;        // n: nodes grouped by depth
+  // Tight bag pairs (a, b of bag(z) with l(a,z) + l(z,b) == l(a,b)), grouped
+  // by the depth of z: pair q moves share frac[q] of shortcut (a,b)'s flow
+  // (entry tp_ab[q] of bag(b)) onto z and onto z's entries for a and b.
+  int64_t pairs;
+  const int *lvl_pair_off; // height + 1
+  const int *tp_ab;        // P
+  const double *tp_frac;   // P: c(a,z) c(z,b) / c(a,b)
+  const int *zp_off;       // n + 1 (by z in preorder) into zp_pair
+  const int *zp_pair;      // P
+  const int *ep_off;       // B + 1: pairs touching each bag entry
+  const int *ep_pair;      // 2P
+};
+
+// Per-tile state of the sweep, [tile][node][lane] for the lane arrays.
+struct SweepWork {
+  int *D;          // distance
+  uint64_t *S;     // path count
+  double *dD;      // dependency of the descending state (ascending, for
+                   // ancestors, after the ascending pass)
+  double *node_t;  // [tile][n]: node flow summed over the tile's lanes
+  double *phi_t;   // [tile][B]: shortcut flow summed over the tile's lanes
+  double *node;    // [n]: node flow summed over tiles
+  double *phi;     // [B]: shortcut flow summed over tiles
+  double *interior;// [n]: flow through shortcut interiors (after unpacking)
+  double *pf;      // [P]: flow share of each tight pair
+  int *tile_src;   // [tiles * 64] sources (tree ids)
+  int *cand;       // candidate lists of the ascending pass
+  int *cand_off;   // [tiles + 1]
+  int cand_cap;
+};
+
+// Saturating count arithmetic: ~0 marks "overflowed" and is absorbing under
+// products with nonzero counts and under sums, so a result overflowed iff its
+// final count is ~0 (candidates that lose the minimum do not matter).
+__device__ __forceinline__ uint64_t mul_sat(uint64_t a, uint64_t b) {
+  if (((a | b) >> 32) == 0) return a * b;
+  if (a == 0 || b == 0) return 0;
+  if (a == ~0ull || b == ~0ull || __umul64hi(a, b) != 0) return ~0ull;
+  return a * b;
+}
+
+__device__ __forceinline__ uint64_t add_sat(uint64_t a, uint64_t b) {
+  const uint64_t r = a + b;
+  return r < a ? ~0ull : r;
+}
+
+__device__ __forceinline__ int wave_min(int x) {
+  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, kWave));
+  return x;
+}
+__device__ __forceinline__ int wave_max(int x) {
+  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, kWave));
+  return x;
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+  return x;
+}
+
+template <class T>
+T *dev_upload(const std::vector<T> &h) {
+  T *p = nullptr;
+  const size_t bytes = std::max<size_t>(1, h.size() * sizeof(T));
+  HIP_OK(hipMalloc(&p, bytes));
+  if (!h.empty()) HIP_OK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+}  // namespace bcx_dev
+
+struct bcx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bcx_dev::DevIndex ix{};
+  std::vector<void *> owned;          // index + graph buffers
+  std::vector<int32_t> perm, iperm;   // preorder <-> vertex
+  bcx_tuning tn{64, 32, 256, 4096};
+  // batch workspace
+  int ws_slots = 0;
+  bcx_dev::DevWork ws{};
+  int *d_srcs = nullptr;
+  int *d_cursor = nullptr;
+  int *d_total = nullptr;
+  int *d_ovf = nullptr;
+  double *d_bc = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_batch = 0;
+  bcx_tuning graph_tn{};
+  bool profiling = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bcx_timing timing{};
+  // bag-sweep form: tree arrays (device) and host copies used for planning
+  bcx_dev::DevTree tree{};
+  std::vector<int32_t> h_parent, h_depth, h_subtree, h_lvl_off, h_lvl_pair_off;
+  int sweep_tiles = 0;
+  bcx_dev::SweepWork sw{};
+
+  ~bcx_ctx() {
+    if (hipSetDevice(device) != hipSuccess) return;
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    free_ws();
+    free_sweep();
+    for (void *p : owned) (void)hipFree(p);
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void free_sweep();
+  void ensure_sweep(int tiles);
+  void free_ws() {
+    if (graph_exec) {
+      (void)hipGraphExecDestroy(graph_exec);
+      graph_exec = nullptr;
+    }
+    void *bufs[] = {ws.d, ws.sigma, ws.hop, ws.order, ws.lcnt, ws.delta};
+    for (void *p : bufs)
+      if (p) (void)hipFree(p);
+    ws = bcx_dev::DevWork{};
+    ws_slots = 0;
+  }
+  static size_t slot_bytes(int n) {
+    return (size_t)n * (4 + 8 + 4 + 4 + 8) + ((size_t)n + 2) * 4;
+  }
+  void ensure_ws(int slots) {
+    if (slots <= ws_slots) return;
+    free_ws();
+    const size_t n = (size_t)ix.n;
+    HIP_OK(hipMalloc(&ws.d, n * slots * 4));
+    HIP_OK(hipMalloc(&ws.sigma, n * slots * 8));
+    HIP_OK(hipMalloc(&ws.hop, n * slots * 4));
+    HIP_OK(hipMalloc(&ws.order, n * slots * 4));
+    HIP_OK(hipMalloc(&ws.lcnt, (n + 2) * slots * 4));
+    HIP_OK(hipMalloc(&ws.delta, n * slots * 8));
+    ws_slots = slots;
+  }
+};
+
+namespace bcx_dev {
+// Bag-sweep full BC over `count` sources (vertex ids), added into bc_inout.
+void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_inout,
+               uint32_t *overflow);
+}  // namespace bcx_dev

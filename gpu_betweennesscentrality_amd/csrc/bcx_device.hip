@@ -30,66 +30,13 @@
 #include "bcx.h"
 #include "bcx_host.hpp"
 #include "bcx_internal.hpp"
+#include "bcx_dev.hpp"
 
-#define HIP_OK(x)                                                                  \
-  do {                                                                             \
-    hipError_t e_ = (x);                                                           \
-    if (e_ != hipSuccess)                                                          \
-      throw bcx::Error{BCX_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)}; \
-  } while (0)
+using namespace bcx_dev;
 
 namespace {
 
-constexpr int kInfDev = 999999999;
-constexpr int kWave = 64;
 
-struct DevIndex {
-  int n;
-  int levels;
-  int64_t arcs;
-  const int *depth;       // n, preorder
-  const int64_t *row_off; // n + 1
-  const int *dis;         // entries
-  const uint64_t *cnt;    // entries
-  const uint32_t *hop;    // entries
-  const int *rmq;         // levels * n: rmq[k*n + i] = min depth over [i, i + 2^k)
-  const int *grow;        // n + 1, CSR in preorder ids
-  const int *gcol;        // arcs
-  const int *gw;          // arcs
-};
-
-// Batch bookkeeping lives in device memory so that one captured graph can be
-// replayed for every batch: src list, cursor (first list index of the batch),
-// total list length.
-struct DevBatch {
-  const int *srcs;
-  int *cursor;
-  const int *total;
-};
-
-struct DevWork {
-  int *d;          // slots * n
-  uint64_t *sigma; // slots * n
-  uint32_t *hop;   // slots * n
-  int *order;      // slots * n
-  int *lcnt;       // slots * (n + 2): layer bins when they exceed LDS
-  double *delta;   // slots * n
-};
-
-// Saturating count arithmetic: ~0 marks "overflowed" and is absorbing under
-// products with nonzero counts and under sums, so a result overflowed iff its
-// final count is ~0 (candidates that lose the minimum do not matter).
-__device__ __forceinline__ uint64_t mul_sat(uint64_t a, uint64_t b) {
-  if (((a | b) >> 32) == 0) return a * b;
-  if (a == 0 || b == 0) return 0;
-  if (a == ~0ull || b == ~0ull || __umul64hi(a, b) != 0) return ~0ull;
-  return a * b;
-}
-
-__device__ __forceinline__ uint64_t add_sat(uint64_t a, uint64_t b) {
-  const uint64_t r = a + b;
-  return r < a ? ~0ull : r;
-}
 
 // Depth of LCA(u, v) for preorder ids: for u < v it is min(depth[u+1..v]) - 1.
 __device__ __forceinline__ int lca_depth_dev(const DevIndex &ix, int u, int v) {
@@ -141,14 +88,6 @@ __global__ __launch_bounds__(256) void scan_plain_kernel(DevIndex ix, DevBatch b
   if (c == ~0ull) atomicOr(ovf_flag, 1);
 }
 
-__device__ __forceinline__ int wave_min(int x) {
-  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, kWave));
-  return x;
-}
-__device__ __forceinline__ int wave_max(int x) {
-  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, kWave));
-  return x;
-}
 
 // Batched form.  A block owns a tile of 64 sources (lane j <-> source j) and a
 // chunk of targets; its waves share the tile's distance rows, staged in LDS
@@ -396,76 +335,9 @@ __global__ void fill_f64_kernel(double *p, int64_t n, double v) {
   if (i < n) p[i] = v;
 }
 
-template <class T>
-T *dev_upload(const std::vector<T> &h) {
-  T *p = nullptr;
-  const size_t bytes = std::max<size_t>(1, h.size() * sizeof(T));
-  HIP_OK(hipMalloc(&p, bytes));
-  if (!h.empty()) HIP_OK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-  return p;
-}
 
 }  // namespace
 
-struct bcx_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  DevIndex ix{};
-  std::vector<void *> owned;          // index + graph buffers
-  std::vector<int32_t> perm, iperm;   // preorder <-> vertex
-  bcx_tuning tn{64, 32, 256, 4096};
-  // batch workspace
-  int ws_slots = 0;
-  DevWork ws{};
-  int *d_srcs = nullptr;
-  int *d_cursor = nullptr;
-  int *d_total = nullptr;
-  int *d_ovf = nullptr;
-  double *d_bc = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  int graph_batch = 0;
-  bcx_tuning graph_tn{};
-  bool profiling = false;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  bcx_timing timing{};
-
-  ~bcx_ctx() {
-    if (hipSetDevice(device) != hipSuccess) return;
-    if (stream) (void)hipStreamSynchronize(stream);
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    free_ws();
-    for (void *p : owned) (void)hipFree(p);
-    for (auto &e : ev)
-      if (e) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-  void free_ws() {
-    if (graph_exec) {
-      (void)hipGraphExecDestroy(graph_exec);
-      graph_exec = nullptr;
-    }
-    void *bufs[] = {ws.d, ws.sigma, ws.hop, ws.order, ws.lcnt, ws.delta};
-    for (void *p : bufs)
-      if (p) (void)hipFree(p);
-    ws = DevWork{};
-    ws_slots = 0;
-  }
-  static size_t slot_bytes(int n) {
-    return (size_t)n * (4 + 8 + 4 + 4 + 8) + ((size_t)n + 2) * 4;
-  }
-  void ensure_ws(int slots) {
-    if (slots <= ws_slots) return;
-    free_ws();
-    const size_t n = (size_t)ix.n;
-    HIP_OK(hipMalloc(&ws.d, n * slots * 4));
-    HIP_OK(hipMalloc(&ws.sigma, n * slots * 8));
-    HIP_OK(hipMalloc(&ws.hop, n * slots * 4));
-    HIP_OK(hipMalloc(&ws.order, n * slots * 4));
-    HIP_OK(hipMalloc(&ws.lcnt, (n + 2) * slots * 4));
-    HIP_OK(hipMalloc(&ws.delta, n * slots * 8));
-    ws_slots = slots;
-  }
-};
 
 namespace {
 
@@ -651,7 +523,113 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
   d.gcol = (const int *)keep(dev_upload(gcol));
   d.gw = (const int *)keep(dev_upload(gw));
   c->ix = d;
-  c->d_srcs = (int *)keep(dev_upload(std::vector<int32_t>((size_t)n, 0)));
+  // Tree view for the bag sweep: bags, inverse bags and depth levels.
+  {
+    const int64_t B = ix.bag_off[(size_t)n];
+    if (B > 0x7fffffff) return bcx::fail(BCX_EGRAPH, "more than 2^31 bag entries");
+    std::vector<int32_t> boff((size_t)n + 1), ioff((size_t)n + 1, 0), ix_x((size_t)B),
+        ix_e((size_t)B), lvl_off((size_t)ix.tree_height + 1, 0), lvl_v((size_t)n);
+    for (int32_t p = 0; p <= n; ++p) boff[(size_t)p] = (int32_t)ix.bag_off[(size_t)p];
+    for (int64_t e = 0; e < B; ++e) ioff[(size_t)ix.bag[(size_t)e] + 1]++;
+    for (int32_t p = 0; p < n; ++p) ioff[(size_t)p + 1] += ioff[(size_t)p];
+    {
+      std::vector<int32_t> f(ioff.begin(), ioff.end() - 1);
+      for (int32_t x = 0; x < n; ++x)  // descendants in increasing id order
+        for (int32_t e = boff[(size_t)x]; e < boff[(size_t)x + 1]; ++e) {
+          const int32_t k = f[(size_t)ix.bag[(size_t)e]]++;
+          ix_x[(size_t)k] = x;
+          ix_e[(size_t)k] = e;
+        }
+    }
+    for (int32_t p = 0; p < n; ++p) lvl_off[(size_t)ix.depth[(size_t)p] + 1]++;
+    for (int32_t k = 0; k < ix.tree_height; ++k) lvl_off[(size_t)k + 1] += lvl_off[(size_t)k];
+    {
+      std::vector<int32_t> f(lvl_off.begin(), lvl_off.end() - 1);
+      for (int32_t p = 0; p < n; ++p) lvl_v[(size_t)f[(size_t)ix.depth[(size_t)p]]++] = p;
+    }
+    DevTree t{};
+    t.n = n;
+    t.height = ix.tree_height;
+    t.bags = B;
+    t.subtree = (const int *)keep(dev_upload(ix.subtree));
+    t.bag_off = (const int *)keep(dev_upload(boff));
+    t.bag_v = (const int *)keep(dev_upload(ix.bag));
+    t.bag_len = (const int *)keep(dev_upload(ix.bag_len));
+    t.bag_cnt = (const uint64_t *)keep(dev_upload(ix.bag_cnt));
+    t.ibag_off = (const int *)keep(dev_upload(ioff));
+    t.ibag_x = (const int *)keep(dev_upload(ix_x));
+    t.ibag_e = (const int *)keep(dev_upload(ix_e));
+    t.lvl_off = (const int *)keep(dev_upload(lvl_off));
+    t.lvl_v = (const int *)keep(dev_upload(lvl_v));
+    // Tight pairs of each bag, found by binary search of a in bag(b) (bags are
+    // ordered by depth and every pair of a bag is joined by a fill shortcut).
+    std::vector<int32_t> tz, tab;
+    std::vector<double> tfr;
+    for (int32_t k = 0; k < ix.tree_height; ++k)
+      for (int32_t q = lvl_off[(size_t)k]; q < lvl_off[(size_t)k + 1]; ++q) {
+        const int32_t z = lvl_v[(size_t)q];
+        for (int32_t i = boff[(size_t)z]; i < boff[(size_t)z + 1]; ++i)
+          for (int32_t j = i + 1; j < boff[(size_t)z + 1]; ++j) {
+            const int32_t a = ix.bag[(size_t)i], b = ix.bag[(size_t)j];
+            const int32_t da = ix.depth[(size_t)a];
+            int32_t lo = boff[(size_t)b], hi = boff[(size_t)b + 1];
+            while (lo < hi) {
+              const int32_t mid = (lo + hi) / 2;
+              if (ix.depth[(size_t)ix.bag[(size_t)mid]] < da)
+                lo = mid + 1;
+              else
+                hi = mid;
+            }
+            if (lo == boff[(size_t)b + 1] || ix.bag[(size_t)lo] != a)
+              return bcx::fail(BCX_EGRAPH, "index bag is not closed under its shortcuts");
+            if ((int64_t)ix.bag_len[(size_t)i] + ix.bag_len[(size_t)j] != ix.bag_len[(size_t)lo])
+              continue;
+            tz.push_back(z);
+            tab.push_back(lo);
+            tfr.push_back((double)ix.bag_cnt[(size_t)i] * (double)ix.bag_cnt[(size_t)j] /
+                          (double)ix.bag_cnt[(size_t)lo]);
+            // remember the two entries through the entry -> pair lists below
+            tab.push_back(i);
+            tab.push_back(j);
+          }
+      }
+    const int64_t P = (int64_t)tz.size();
+    std::vector<int32_t> lpo((size_t)ix.tree_height + 1, 0), tp_ab((size_t)P), zpo((size_t)n + 1, 0),
+        zpp((size_t)P), epo((size_t)B + 1, 0), epp((size_t)(2 * P));
+    for (int64_t q = 0; q < P; ++q) {
+      tp_ab[(size_t)q] = tab[(size_t)(3 * q)];
+      lpo[(size_t)ix.depth[(size_t)tz[(size_t)q]] + 1]++;
+      zpo[(size_t)tz[(size_t)q] + 1]++;
+      epo[(size_t)tab[(size_t)(3 * q + 1)] + 1]++;
+      epo[(size_t)tab[(size_t)(3 * q + 2)] + 1]++;
+    }
+    for (int32_t k = 0; k < ix.tree_height; ++k) lpo[(size_t)k + 1] += lpo[(size_t)k];
+    for (int32_t p = 0; p < n; ++p) zpo[(size_t)p + 1] += zpo[(size_t)p];
+    for (int64_t e = 0; e < B; ++e) epo[(size_t)e + 1] += epo[(size_t)e];
+    {
+      std::vector<int32_t> fz(zpo.begin(), zpo.end() - 1), fe(epo.begin(), epo.end() - 1);
+      for (int64_t q = 0; q < P; ++q) {  // increasing q: fixed summation order
+        zpp[(size_t)fz[(size_t)tz[(size_t)q]]++] = (int32_t)q;
+        epp[(size_t)fe[(size_t)tab[(size_t)(3 * q + 1)]]++] = (int32_t)q;
+        epp[(size_t)fe[(size_t)tab[(size_t)(3 * q + 2)]]++] = (int32_t)q;
+      }
+    }
+    t.pairs = P;
+    t.lvl_pair_off = (const int *)keep(dev_upload(lpo));
+    t.tp_ab = (const int *)keep(dev_upload(tp_ab));
+    t.tp_frac = (const double *)keep(dev_upload(tfr));
+    t.zp_off = (const int *)keep(dev_upload(zpo));
+    t.zp_pair = (const int *)keep(dev_upload(zpp));
+    t.ep_off = (const int *)keep(dev_upload(epo));
+    t.ep_pair = (const int *)keep(dev_upload(epp));
+    c->h_lvl_pair_off = lpo;
+    c->tree = t;
+    c->h_parent = ix.parent;
+    c->h_depth = ix.depth;
+    c->h_subtree = ix.subtree;
+    c->h_lvl_off = lvl_off;
+  }
+  c->d_srcs =(int *)keep(dev_upload(std::vector<int32_t>((size_t)n, 0)));
   c->d_cursor = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
   c->d_total = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
   c->d_ovf = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
@@ -720,10 +698,14 @@ int bcx_bc_sources(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, 
   BCX_GUARD_BEGIN
   if (!c || !bc_inout || (count > 0 && !sources) || count < 0)
     return bcx::fail(BCX_EINVAL, "bad argument");
-  if (mode != BCX_MODE_PLAIN && mode != BCX_MODE_BATCHED) return bcx::fail(BCX_EINVAL, "bad mode");
+  if (mode != BCX_MODE_PLAIN && mode != BCX_MODE_BATCHED && mode != BCX_MODE_SWEEP)
+    return bcx::fail(BCX_EINVAL, "bad mode");
   if (count > c->ix.n) return bcx::fail(BCX_EINVAL, "more sources than vertices");
   HIP_OK(hipSetDevice(c->device));
-  run_sources(c, mode, sources, count, bc_inout, overflow);
+  if (mode == BCX_MODE_SWEEP)
+    run_sweep(c, sources, count, bc_inout, overflow);
+  else
+    run_sources(c, mode, sources, count, bc_inout, overflow);
   return BCX_OK;
   BCX_GUARD_END
 }

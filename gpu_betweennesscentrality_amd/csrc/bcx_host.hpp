@@ -39,9 +39,14 @@ struct Index {
   std::vector<int32_t> dis;     // restricted (up-path) distance
   std::vector<uint64_t> cnt;    // number of shortest up-paths
   std::vector<uint32_t> hop;    // max hops among those paths
-  // Bags (upward shortcut neighbours) per node, preorder ids, for tools.
+  // Bags: the upward shortcut neighbours of each node (all ancestors), ordered
+  // by ancestor depth, with the shortcut's restricted length, number of
+  // restricted shortest paths and max hop count at elimination time.
   std::vector<int64_t> bag_off;
-  std::vector<int32_t> bag;
+  std::vector<int32_t> bag;       // preorder id of the ancestor
+  std::vector<int32_t> bag_len;
+  std::vector<uint64_t> bag_cnt;
+  std::vector<uint32_t> bag_hop;
   int32_t tree_height = 0;  // max depth + 1
   int32_t tree_width = 0;   // max bag size
   int32_t count_overflow = 0;

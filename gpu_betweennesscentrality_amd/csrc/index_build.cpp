@@ -248,11 +248,24 @@ void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix) {
   ix.bag_off.assign((size_t)n + 1, 0);
   for (int32_t p = 0; p < n; ++p)
     ix.bag_off[(size_t)p + 1] = ix.bag_off[(size_t)p] + (int64_t)bags[(size_t)ix.perm[(size_t)p]].size();
-  ix.bag.resize((size_t)ix.bag_off[(size_t)n]);
+  const size_t B = (size_t)ix.bag_off[(size_t)n];
+  ix.bag.resize(B);
+  ix.bag_len.resize(B);
+  ix.bag_cnt.resize(B);
+  ix.bag_hop.resize(B);
   for (int32_t p = 0; p < n; ++p) {
-    const auto &b = bags[(size_t)ix.perm[(size_t)p]];
-    for (size_t i = 0; i < b.size(); ++i)
-      ix.bag[(size_t)ix.bag_off[(size_t)p] + i] = ix.iperm[(size_t)b[i].nb];
+    // Bag entries ordered by ancestor depth (root side first).
+    std::vector<Arc> b = bags[(size_t)ix.perm[(size_t)p]];
+    std::sort(b.begin(), b.end(), [&](const Arc &x, const Arc &y) {
+      return ix.depth[(size_t)ix.iperm[(size_t)x.nb]] < ix.depth[(size_t)ix.iperm[(size_t)y.nb]];
+    });
+    for (size_t i = 0; i < b.size(); ++i) {
+      const size_t e = (size_t)ix.bag_off[(size_t)p] + i;
+      ix.bag[e] = ix.iperm[(size_t)b[i].nb];
+      ix.bag_len[e] = b[i].len;
+      ix.bag_cnt[e] = b[i].cnt;
+      ix.bag_hop[e] = b[i].hop;
+    }
   }
 
   // Label rows.
@@ -371,7 +384,7 @@ void index_query(const Index &ix, int32_t s, int32_t t, int32_t &d, uint64_t &si
 
 // ---- binary image ---------------------------------------------------------
 namespace {
-const char kMagic[8] = {'B', 'C', 'X', 'I', 'D', 'X', '0', '1'};
+const char kMagic[8] = {'B', 'C', 'X', 'I', 'D', 'X', '0', '2'};
 template <class T>
 void wvec(FILE *f, const std::vector<T> &v) {
   int64_t k = (int64_t)v.size();
@@ -406,6 +419,9 @@ void save_index(const Index &ix, const char *path) {
     wvec(f, ix.hop);
     wvec(f, ix.bag_off);
     wvec(f, ix.bag);
+    wvec(f, ix.bag_len);
+    wvec(f, ix.bag_cnt);
+    wvec(f, ix.bag_hop);
   } catch (...) {
     std::fclose(f);
     throw;
@@ -436,6 +452,9 @@ void load_index(const char *path, Index &ix) {
     rvec(f, ix.hop);
     rvec(f, ix.bag_off);
     rvec(f, ix.bag);
+    rvec(f, ix.bag_len);
+    rvec(f, ix.bag_cnt);
+    rvec(f, ix.bag_hop);
   } catch (...) {
     std::fclose(f);
     throw;

@@ -49,6 +49,7 @@ enum {
 enum {
   BCX_MODE_PLAIN = 0,   /* one source per launch sequence                 */
   BCX_MODE_BATCHED = 1, /* source tiles, many sources per launch, graphs  */
+  BCX_MODE_SWEEP = 2,   /* bag sweep over the tree, 64-source tiles        */
 };
 
 typedef struct bcx_graph bcx_graph; /* parsed graph, host memory           */

@@ -181,3 +181,40 @@ def test_cli_mhlqbc_and_qbc(tmp_path, grid):
     assert out.returncode == 0, out.stdout + out.stderr
     assert "BC value is correct" in out.stdout
     assert close_bc(np.fromfile(tmp_path / "bc.bin", np.float64), ref)
+
+
+# ---- bag-sweep form ---------------------------------------------------------
+
+@pytest.mark.parametrize("maker", [
+    lambda: gg.make("grid1k"),
+    lambda: gg.grid_graph(k=20, seed=3, wmax=2),   # many ties
+    lambda: gg.unit_grid(12),                       # counts up to C(22, 11)
+    lambda: gg.road_graph(3000, 3750, seed=21),
+])
+def test_sweep_full_bc(maker):
+    g = maker()
+    G, ix, ctx = build(g)
+    ref, _ = oracle.brandes(oracle.CSR.from_arcgraph(g))
+    for batch in (64, 4096):
+        ctx.set_tuning(batch=batch)
+        out, ovf = ctx.full_bc(bcx.MODE_SWEEP)
+        assert not ovf
+        assert close_bc(out, ref), f"batch {batch}: max rel err " \
+            f"{np.max(np.abs(out - ref) / np.maximum(np.abs(ref), 1)):.3e}"
+
+
+def test_sweep_scattered_sources(road):
+    g, csr, ctx = road
+    src = gg.permutation(6, 2, g.n)[:700].astype(np.int32)  # not tree-consecutive
+    ctx.set_tuning(batch=256)
+    out, ovf = ctx.bc_sources(src, bcx.MODE_SWEEP)
+    ref, _ = oracle.brandes(csr, src)
+    assert not ovf
+    assert close_bc(out, ref)
+
+
+def test_sweep_overflow_flag():
+    g = gg.unit_grid(35)
+    G, ix, ctx = build(g)
+    _, ovf = ctx.bc_sources(np.array([0], np.int32), bcx.MODE_SWEEP)
+    assert ovf
