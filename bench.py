@@ -43,6 +43,8 @@ def main() -> int:
     ap.add_argument("--no-rq", action="store_true")
     ap.add_argument("--rq-requests", type=int, default=6)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="one label-scan batch, one sweep batch and a 1 GiB copy (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -77,6 +79,15 @@ def main() -> int:
         return tree_perm[start:start + size]
 
     total = args.warmup + args.steps
+    if args.profile_only:
+        ctx.set_tuning(batch=B)
+        ctx.bc_sources(tree_batch(1, 3, B), bcx.MODE_BATCHED)
+        Bs = min(args.sweep_batch, n)
+        ctx.set_tuning(batch=Bs)
+        ctx.bc_sources(tree_batch(1, 3, Bs), bcx.MODE_SWEEP)
+        ms = ctx.copy_probe(1 << 30)
+        log(f"[bench] profile-only done; 1 GiB device copy {ms:.3f} ms")
+        return 0
 
     def time_mode(mode: int, label: str, size: int):
         ctx.set_tuning(batch=size)
@@ -98,6 +109,9 @@ def main() -> int:
         ctx.bc_sources(tree_batch(total // 2, total, size), mode)
         tm = ctx.timing()
         ctx.set_profiling(False)
+        if tm.get("sweep_fwd_ms"):
+            log(f"[bench]   sweep phases per tile: fwd {tm['sweep_fwd_ms']:.1f} bwd {tm['sweep_bwd_ms']:.1f} "
+                f"asc {tm['sweep_asc_ms']:.1f} ms")
         log(f"[bench] {label}: {done} sources in {el:.3f}s -> {rate:.1f} sources/s; "
             f"phases {tm['scan_ms']:.1f} / {tm['dep_ms']:.1f} / {tm['reduce_ms']:.1f} ms")
         return rate, el, tm, ovf

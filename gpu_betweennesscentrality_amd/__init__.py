@@ -49,7 +49,9 @@ class Tuning(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("scan_ms", C.c_double), ("dep_ms", C.c_double), ("reduce_ms", C.c_double),
-                ("scan_launches", C.c_int64), ("dep_launches", C.c_int64), ("sources", C.c_int64)]
+                ("scan_launches", C.c_int64), ("dep_launches", C.c_int64), ("sources", C.c_int64),
+                ("sweep_fwd_ms", C.c_double), ("sweep_bwd_ms", C.c_double),
+                ("sweep_asc_ms", C.c_double)]
 
 
 _P = C.c_void_p
@@ -88,6 +90,7 @@ _SIGS = {
     "bcx_ctx_timing": (C.c_int, [_P, C.POINTER(Timing)]),
     "bcx_ctx_timing_reset": (None, [_P]),
     "bcx_ctx_set_profiling": (C.c_int, [_P, C.c_int]),
+    "bcx_ctx_copy_probe": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_double)]),
     "bcx_full_bc": (C.c_int, [_P, C.c_int, _F64P, C.POINTER(C.c_uint32)]),
     "bcx_bc_sources": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, _F64P, C.POINTER(C.c_uint32)]),
     "bcx_query_dist_count": (C.c_int, [_P, C.c_int32, _I32P, _U64P, _U32P]),
@@ -276,6 +279,12 @@ class Context:
 
     def set_profiling(self, on: bool) -> None:
         _check(lib().bcx_ctx_set_profiling(self._h, int(on)))
+
+    def copy_probe(self, nbytes: int) -> float:
+        """Milliseconds for one device-to-device copy of nbytes."""
+        ms = C.c_double()
+        _check(lib().bcx_ctx_copy_probe(self._h, nbytes, C.byref(ms)))
+        return ms.value
 
     def timing(self) -> dict:
         t = Timing()

@@ -86,11 +86,11 @@ struct DevTree {
   const int *ep_pair;      // 2P
 };
 
-// Per-tile state of the sweep, [tile][node][lane] for the lane arrays.
+// Per-tile state of the sweep: one record per (tile, node) with the lanes'
+// values, per-tile sums [tile][node] and [tile][entry].
 struct SweepWork {
-  int *D;          // distance
-  uint64_t *S;     // path count
-  double *dD;      // dependency of the descending state (ascending, for
+  char *state;     // per (tile, node) record: distance[64], path count[64],
+                   // dependency of the descending state[64] (ascending, for
                    // ancestors, after the ascending pass)
   double *node_t;  // [tile][n]: node flow summed over the tile's lanes
   double *phi_t;   // [tile][B]: shortcut flow summed over the tile's lanes
@@ -102,6 +102,7 @@ struct SweepWork {
   int *cand;       // candidate lists of the ascending pass
   int *cand_off;   // [tiles + 1]
   int cand_cap;
+  int tstride;     // tiles the state arrays are laid out for
 };
 
 // Saturating count arithmetic: ~0 marks "overflowed" and is absorbing under

@@ -1020,3 +1020,32 @@ extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t coun
   return BCX_OK;
   BCX_GUARD_END
 }
+
+// Measurement hook: one device-to-device copy of `bytes` on the context
+// stream, timed with HIP events.  Used to calibrate HBM traffic counters
+// against a transfer of known size; no reference counterpart.
+extern "C" int bcx_ctx_copy_probe(bcx_ctx *c, size_t bytes, double *ms) {
+  BCX_GUARD_BEGIN
+  if (!c || bytes == 0) return bcx::fail(BCX_EINVAL, "bad argument");
+  HIP_OK(hipSetDevice(c->device));
+  void *a = nullptr, *b = nullptr;
+  HIP_OK(hipMalloc(&a, bytes));
+  hipError_t e = hipMalloc(&b, bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    HIP_OK(e);
+  }
+  float t = 0;
+  e = hipMemsetAsync(a, 1, bytes, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(c->ev[0], c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  HIP_OK(e);
+  if (ms) *ms = t;
+  return BCX_OK;
+  BCX_GUARD_END
+}

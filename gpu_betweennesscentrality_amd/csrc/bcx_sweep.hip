@@ -35,8 +35,21 @@ using namespace bcx_dev;
 
 namespace {
 
-__device__ __forceinline__ int64_t lix(int t, int n, int v, int lane) {
-  return ((int64_t)t * n + v) * kWave + lane;
+// Lane state is one record per (tile, node): 64 distances, then 64 path
+// counts, then 64 dependencies (1280 bytes), so the rows a node reads of an
+// ancestor or descendant are one contiguous region of memory.
+constexpr int64_t kRec = kWave * (4 + 8 + 8);
+__device__ __forceinline__ int64_t lix_(int nn, int t, int v, int /*lane*/) {
+  return ((int64_t)t * nn + v) * kRec;
+}
+__device__ __forceinline__ int &SD(const SweepWork &sw, int64_t r, int lane) {
+  return ((int *)(sw.state + r))[lane];
+}
+__device__ __forceinline__ uint64_t &SS(const SweepWork &sw, int64_t r, int lane) {
+  return ((uint64_t *)(sw.state + r + kWave * 4))[lane];
+}
+__device__ __forceinline__ double &SDD(const SweepWork &sw, int64_t r, int lane) {
+  return ((double *)(sw.state + r + kWave * 12))[lane];
 }
 
 struct Lane {
@@ -86,42 +99,249 @@ constexpr int kSweepThreads = 1024;
 constexpr int kSweepWaves = kSweepThreads / kWave;
 
 // Forward candidates of node v from bag entries [e, e_end) with a stride.
+// Entries are taken four at a time so their row loads are in flight together.
 __device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, const Lane &L,
                                         int e, int e_end, int stride, int &best, uint64_t &c) {
   const int n = tr.n;
-  for (; e < e_end; e += stride) {
-    const int64_t a = lix(L.t, n, tr.bag_v[e], L.lane);
-    const int cand = sw.D[a] + tr.bag_len[e];
-    const uint64_t su = sw.S[a];
-    if (cand < best) {
-      best = cand;
-      c = mul_sat(su, tr.bag_cnt[e]);
-    } else if (cand == best) {
-      c = add_sat(c, mul_sat(su, tr.bag_cnt[e]));
+  for (; e < e_end; e += 4 * stride) {
+    int cand[4];
+    uint64_t m[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int eq = e + q * stride;
+      const bool ok = eq < e_end;
+      const int ec = ok ? eq : e;
+      const int64_t a = lix_(tr.n, L.t, tr.bag_v[ec], L.lane);
+      cand[q] = ok ? SD(sw, a, L.lane) + tr.bag_len[ec] : 0x7fffffff;
+      m[q] = ok ? SS(sw, a, L.lane) : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int eq = e + q * stride;
+      if (eq >= e_end) break;
+      if (cand[q] < best) {
+        best = cand[q];
+        c = mul_sat(m[q], tr.bag_cnt[eq]);
+      } else if (cand[q] == best) {
+        c = add_sat(c, mul_sat(m[q], tr.bag_cnt[eq]));
+      }
     }
   }
 }
 
-// Backward pulls of node v over inverse-bag entries [k, k_end) with a stride;
-// writes the tile's flow on each entry and returns the lane's dependency part.
+// Backward pulls of node v over inverse-bag entries [k, k_end) with a stride,
+// four at a time; writes the tile's flow on each entry and returns the lane's
+// dependency part.
 __device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork &sw,
                                               const Lane &L, int k, int k_end, int stride,
                                               int dv, double sv) {
   const int n = tr.n;
   double acc = 0.0;
-  for (; k < k_end; k += stride) {
-    const int x = tr.ibag_x[k], e = tr.ibag_e[k];
-    const int64_t ax = lix(L.t, n, x, L.lane);
-    const int dx = sw.D[ax];
-    const uint64_t sx = sw.S[ax];
-    const double ddx = sw.dD[ax];
-    const double f =
-        dv + tr.bag_len[e] == dx ? (double)tr.bag_cnt[e] * sv * (1.0 + ddx) / (double)sx : 0.0;
-    acc += f;
-    const double fs = wave_sum_dpp(L.valid ? f : 0.0);
-    if (L.lane == 0) sw.phi_t[(int64_t)L.t * tr.bags + e] = fs;
+  for (; k < k_end; k += 4 * stride) {
+    int dx[4], ee[4], len[4];
+    uint64_t sx[4];
+    double ddx[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kq = k + q * stride;
+      const int kc = kq < k_end ? kq : k;
+      const int x = tr.ibag_x[kc];
+      ee[q] = tr.ibag_e[kc];
+      len[q] = tr.bag_len[ee[q]];
+      const int64_t ax = lix_(tr.n, L.t, x, L.lane);
+      dx[q] = SD(sw, ax, L.lane);
+      sx[q] = SS(sw, ax, L.lane);
+      ddx[q] = SDD(sw, ax, L.lane);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kq = k + q * stride;
+      if (kq >= k_end) break;
+      const double f = dv + len[q] == dx[q]
+                           ? (double)tr.bag_cnt[ee[q]] * sv * (1.0 + ddx[q]) / (double)sx[q]
+                           : 0.0;
+      acc += f;
+      const double fs = wave_sum_dpp(L.valid ? f : 0.0);
+      if (L.lane == 0) sw.phi_t[(int64_t)L.t * tr.bags + ee[q]] = fs;
+    }
   }
   return acc;
+}
+
+
+__device__ __forceinline__ int rl(int x, int j) { return __builtin_amdgcn_readlane(x, j); }
+__device__ __forceinline__ uint64_t rl64(uint64_t x, int j) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, j);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), j);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+constexpr int kStage = 4;  // bag entries per node staged in registers
+
+constexpr int kNodes = 2;  // nodes of a level processed together by one wavefront
+
+// Forward over the nodes of one level assigned to this wavefront (nodes
+// wave, wave + 16, ...).  Lane j first loads node j's metadata and its first
+// kStage bag entries; the wavefront then handles kNodes nodes per step with
+// all their row loads in flight together.
+__device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree &tr,
+                                               const SweepWork &sw, const Lane &L, int wave,
+                                               int lo, int cnt, int &ovf) {
+  const int n = tr.n;
+  const int mine = (cnt - wave + kSweepWaves - 1) / kSweepWaves;
+  for (int base = 0; base < mine; base += kWave) {
+    const int jn = min(kWave, mine - base);
+    int sv = 0, se0 = 0, se1 = 0, ssub = 0, sdep = 0;
+    int su[kStage], slen[kStage];
+    uint64_t sc[kStage];
+#pragma unroll
+    for (int r = 0; r < kStage; ++r) {
+      su[r] = 0;
+      slen[r] = 0;
+      sc[r] = 0;
+    }
+    if (L.lane < jn) {
+      sv = tr.lvl_v[lo + wave + kSweepWaves * (base + L.lane)];
+      se0 = tr.bag_off[sv];
+      se1 = tr.bag_off[sv + 1];
+      ssub = tr.subtree[sv];
+      sdep = ix.depth[sv];
+#pragma unroll
+      for (int r = 0; r < kStage; ++r)
+        if (se0 + r < se1) {
+          su[r] = tr.bag_v[se0 + r];
+          slen[r] = tr.bag_len[se0 + r];
+          sc[r] = tr.bag_cnt[se0 + r];
+        }
+    }
+    for (int j0 = 0; j0 < jn; j0 += kNodes) {
+      int cand[kNodes][kStage];
+      uint64_t m[kNodes][kStage];
+#pragma unroll
+      for (int u = 0; u < kNodes; ++u) {
+        const int j = min(j0 + u, jn - 1);
+        const int nb = min(kStage, rl(se1, j) - rl(se0, j));
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) {
+          const int64_t a = lix_(n, L.t, rl(su[r], j), L.lane);
+          cand[u][r] = r < nb ? SD(sw, a, L.lane) + rl(slen[r], j) : 0x7fffffff;
+          m[u][r] = r < nb ? SS(sw, a, L.lane) : 0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kNodes; ++u) {
+        const int j = j0 + u;
+        if (j >= jn) break;
+        const int v = rl(sv, j), e0 = rl(se0, j), e1 = rl(se1, j);
+        const int nb = min(kStage, e1 - e0);
+        int best = 0x7fffffff;
+        uint64_t c = 0;
+        if (v <= L.s && L.s < v + rl(ssub, j)) {  // v is an ancestor of s: up-label
+          const int64_t o = L.rs + rl(sdep, j);
+          best = ix.dis[o];
+          c = ix.cnt[o];
+        }
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) {
+          if (r >= nb) break;
+          const uint64_t cr = rl64(sc[r], j);
+          if (cand[u][r] < best) {
+            best = cand[u][r];
+            c = mul_sat(m[u][r], cr);
+          } else if (cand[u][r] == best) {
+            c = add_sat(c, mul_sat(m[u][r], cr));
+          }
+        }
+        if (e1 - e0 > kStage) fwd_bag(tr, sw, L, e0 + kStage, e1, 1, best, c);
+        const int64_t a = lix_(n, L.t, v, L.lane);
+        SD(sw, a, L.lane) = best;
+        SS(sw, a, L.lane) = c;
+        ovf |= L.valid && c == ~0ull;
+      }
+    }
+  }
+}
+
+// Backward over the nodes of one level assigned to this wavefront, with the
+// same staging and kNodes nodes per step.
+__device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWork &sw,
+                                               const Lane &L, int wave, int lo, int cnt) {
+  const int n = tr.n;
+  const int mine = (cnt - wave + kSweepWaves - 1) / kSweepWaves;
+  for (int base = 0; base < mine; base += kWave) {
+    const int jn = min(kWave, mine - base);
+    int sv = 0, sk0 = 0, sk1 = 0;
+    int sx[kStage], se[kStage], slen[kStage];
+    uint64_t sc[kStage];
+#pragma unroll
+    for (int r = 0; r < kStage; ++r) {
+      sx[r] = 0;
+      se[r] = 0;
+      slen[r] = 0;
+      sc[r] = 0;
+    }
+    if (L.lane < jn) {
+      sv = tr.lvl_v[lo + wave + kSweepWaves * (base + L.lane)];
+      sk0 = tr.ibag_off[sv];
+      sk1 = tr.ibag_off[sv + 1];
+#pragma unroll
+      for (int r = 0; r < kStage; ++r)
+        if (sk0 + r < sk1) {
+          sx[r] = tr.ibag_x[sk0 + r];
+          se[r] = tr.ibag_e[sk0 + r];
+          slen[r] = tr.bag_len[se[r]];
+          sc[r] = tr.bag_cnt[se[r]];
+        }
+    }
+    for (int j0 = 0; j0 < jn; j0 += kNodes) {
+      int dv[kNodes];
+      uint64_t svu[kNodes];
+      int dx[kNodes][kStage];
+      uint64_t sxx[kNodes][kStage];
+      double ddx[kNodes][kStage];
+#pragma unroll
+      for (int u = 0; u < kNodes; ++u) {
+        const int j = min(j0 + u, jn - 1);
+        const int nb = min(kStage, rl(sk1, j) - rl(sk0, j));
+        const int64_t av = lix_(n, L.t, rl(sv, j), L.lane);
+        dv[u] = SD(sw, av, L.lane);
+        svu[u] = SS(sw, av, L.lane);
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) {
+          const int64_t ax = lix_(n, L.t, rl(sx[r], j), L.lane);
+          dx[u][r] = r < nb ? SD(sw, ax, L.lane) : -1;
+          sxx[u][r] = r < nb ? SS(sw, ax, L.lane) : 1;
+          ddx[u][r] = r < nb ? SDD(sw, ax, L.lane) : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kNodes; ++u) {
+        const int j = j0 + u;
+        if (j >= jn) break;
+        const int v = rl(sv, j), k0 = rl(sk0, j), k1 = rl(sk1, j);
+        const int nb = min(kStage, k1 - k0);
+        const double svv = (double)svu[u];
+        double acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) {
+          if (r >= nb) break;
+          const int e = rl(se[r], j);
+          const double f = dv[u] + rl(slen[r], j) == dx[u][r]
+                               ? (double)rl64(sc[r], j) * svv * (1.0 + ddx[u][r]) /
+                                     (double)sxx[u][r]
+                               : 0.0;
+          acc += f;
+          const double fs = wave_sum_dpp(L.valid ? f : 0.0);
+          if (L.lane == 0) sw.phi_t[(int64_t)L.t * tr.bags + e] = fs;
+        }
+        if (k1 - k0 > kStage) acc += bwd_entries(tr, sw, L, k0 + kStage, k1, 1, dv[u], svv);
+        const int64_t av = lix_(n, L.t, v, L.lane);
+        SDD(sw, av, L.lane) = acc;
+        const double ns = wave_sum_dpp((L.valid && L.s != v) ? acc : 0.0);
+        if (L.lane == 0) sw.node_t[(int64_t)L.t * n + v] = ns;
+      }
+    }
+  }
 }
 
 // One workgroup per tile of 64 sources runs the whole sweep for its tile:
@@ -131,7 +351,8 @@ __device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork
 // partial results merged in wavefront order, so results are deterministic.
 __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, DevTree tr,
                                                                    SweepWork sw, int nsrc,
-                                                                   int *ovf_flag) {
+                                                                   int *ovf_flag,
+                                                                   uint64_t *stamps) {
   __shared__ double sh_acc[kSweepWaves][kWave];
   __shared__ int sh_best[kSweepWaves][kWave];
   __shared__ uint64_t sh_cnt[kSweepWaves][kWave];
@@ -139,26 +360,14 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
   const int wave = threadIdx.x / kWave;
   const int n = tr.n, H = tr.height;
   int ovf = 0;
+  // Optional phase timestamps (constant 100 MHz clock), one row per tile.
+  if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x] = __builtin_amdgcn_s_memrealtime();
 
   // ---- forward: distance and path count ----
   for (int k = 0; k < H; ++k) {
     const int lo = tr.lvl_off[k], cnt = tr.lvl_off[k + 1] - lo;
     if (cnt >= kSweepWaves) {
-      for (int q = wave; q < cnt; q += kSweepWaves) {
-        const int v = __builtin_amdgcn_readfirstlane(tr.lvl_v[lo + q]);
-        int best = 0x7fffffff;
-        uint64_t c = 0;
-        if (v <= L.s && L.s < v + tr.subtree[v]) {  // v is an ancestor of s: up-label
-          const int64_t o = L.rs + ix.depth[v];
-          best = ix.dis[o];
-          c = ix.cnt[o];
-        }
-        fwd_bag(tr, sw, L, tr.bag_off[v], tr.bag_off[v + 1], 1, best, c);
-        const int64_t a = lix(L.t, n, v, L.lane);
-        sw.D[a] = best;
-        sw.S[a] = c;
-        ovf |= L.valid && c == ~0ull;
-      }
+      fwd_level_wave(ix, tr, sw, L, wave, lo, cnt, ovf);
     } else {
       for (int q = 0; q < cnt; ++q) {
         const int v = __builtin_amdgcn_readfirstlane(tr.lvl_v[lo + q]);
@@ -185,9 +394,9 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
               cc = add_sat(cc, sh_cnt[w][L.lane]);
             }
           }
-          const int64_t a = lix(L.t, n, v, L.lane);
-          sw.D[a] = b;
-          sw.S[a] = cc;
+          const int64_t a = lix_(tr.n, L.t, v, L.lane);
+          SD(sw, a, L.lane) = b;
+          SS(sw, a, L.lane) = cc;
           ovf |= L.valid && cc == ~0ull;
         }
         __syncthreads();
@@ -196,32 +405,26 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
     __syncthreads();
   }
 
+  if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
+
   // ---- backward: dependencies, shortcut flows, node flows ----
   for (int k = H - 1; k >= 0; --k) {
     const int lo = tr.lvl_off[k], cnt = tr.lvl_off[k + 1] - lo;
     if (cnt >= kSweepWaves) {
-      for (int q = wave; q < cnt; q += kSweepWaves) {
-        const int v = __builtin_amdgcn_readfirstlane(tr.lvl_v[lo + q]);
-        const int64_t av = lix(L.t, n, v, L.lane);
-        const double acc = bwd_entries(tr, sw, L, tr.ibag_off[v], tr.ibag_off[v + 1], 1,
-                                       sw.D[av], (double)sw.S[av]);
-        sw.dD[av] = acc;
-        const double ns = wave_sum_dpp((L.valid && L.s != v) ? acc : 0.0);
-        if (L.lane == 0) sw.node_t[(int64_t)L.t * n + v] = ns;
-      }
+      bwd_level_wave(tr, sw, L, wave, lo, cnt);
     } else {
       for (int q = 0; q < cnt; ++q) {
         const int v = __builtin_amdgcn_readfirstlane(tr.lvl_v[lo + q]);
-        const int64_t av = lix(L.t, n, v, L.lane);
+        const int64_t av = lix_(tr.n, L.t, v, L.lane);
         sh_acc[wave][L.lane] = bwd_entries(tr, sw, L, tr.ibag_off[v] + wave, tr.ibag_off[v + 1],
-                                           kSweepWaves, sw.D[av], (double)sw.S[av]);
+                                           kSweepWaves, SD(sw, av, L.lane), (double)SS(sw, av, L.lane));
         __syncthreads();
         if (wave == 0) {
           double acc = 0.0;
           for (int w = 0; w < kSweepWaves; ++w) acc += sh_acc[w][L.lane];
-          sw.dD[av] = acc;
+          SDD(sw, av, L.lane) = acc;
           const double ns = wave_sum_dpp((L.valid && L.s != v) ? acc : 0.0);
-          if (L.lane == 0) sw.node_t[(int64_t)L.t * n + v] = ns;
+          if (L.lane == 0) sw.node_t[(int64_t)L.t * tr.n + v] = ns;
         }
         __syncthreads();
       }
@@ -229,21 +432,24 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
     __syncthreads();
   }
 
-  // ---- ascending states (one wavefront; the union of the tile's root paths
-  // in top-down order).  dA(x) replaces dD(x) in place once dD(x) is read:
-  // later nodes read dA of their ancestors, and a lane whose source is not
-  // below x never reads x again.
+  if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x + 2] = __builtin_amdgcn_s_memrealtime();
+
+  // ---- ascending states: the union of the tile's root paths in top-down
+  // order, each node's bag entries split over all wavefronts and merged in
+  // wavefront order.  Once node x is done, its slots hold the ascending state:
+  // S(x) := sigma_A(x) (the up-label count where the up-label distance is
+  // tight, else 0) and dD(x) := dA(x), with
   //   dA(x) = sigma_A(x) * (1 + dD(x)) / sigma(x)
-  //         + sum over tight bag entries (x -> y) c * sigma_A(x) * dA(y) / sigma_A(y),
-  // sigma_A = the up-label count where the up-label distance is tight.
-  if (wave == 0) {
+  //         + sum over tight bag entries (x -> y) c * sigma_A(x) * dA(y) / sigma_A(y).
+  // Later nodes read only their ancestors' ascending state, and a lane whose
+  // source is not below x never reads x again.
+  {
     const int c0 = sw.cand_off[L.t], c1 = sw.cand_off[L.t + 1];
     for (int kk = c0; kk < c1; ++kk) {
       const int x = __builtin_amdgcn_readfirstlane(sw.cand[kk]);
       const bool anc = x <= L.s && L.s < x + tr.subtree[x];
-      const int64_t ax = lix(L.t, n, x, L.lane);
-      const int dx = sw.D[ax];
-      const double ddx = sw.dD[ax];
+      const int64_t ax = lix_(n, L.t, x, L.lane);
+      const int dx = SD(sw, ax, L.lane);
       uint64_t sa = 0;
       if (anc) {
         const int64_t o = L.rs + ix.depth[x];
@@ -252,27 +458,47 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
       const double sax = (double)sa;
       double up = 0.0;
       if (__any(sa != 0)) {
-        for (int e = tr.bag_off[x]; e < tr.bag_off[x + 1]; ++e) {
-          double f = 0.0;
-          if (sa != 0) {
-            const int y = tr.bag_v[e];
-            const int64_t ay = lix(L.t, n, y, L.lane);
-            const int dy = sw.D[ay];
-            if (dx + tr.bag_len[e] == dy) {
-              const int64_t oy = L.rs + ix.depth[y];
-              if (ix.dis[oy] == dy) f = (double)tr.bag_cnt[e] * sax * sw.dD[ay] / (double)ix.cnt[oy];
-            }
+        const int e1 = tr.bag_off[x + 1];
+        for (int e = tr.bag_off[x] + wave; e < e1; e += 4 * kSweepWaves) {
+          double f[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int eq = e + q * kSweepWaves;
+            const int ec = eq < e1 ? eq : e;
+            const int64_t ay = lix_(n, L.t, tr.bag_v[ec], L.lane);
+            const int dy = SD(sw, ay, L.lane);
+            const uint64_t say = SS(sw, ay, L.lane);  // sigma_A(y), y already processed
+            const double day = SDD(sw, ay, L.lane);   // dA(y)
+            f[q] = (eq < e1 && sa != 0 && say != 0 && dx + tr.bag_len[ec] == dy)
+                       ? (double)tr.bag_cnt[ec] * sax * day / (double)say
+                       : 0.0;
           }
-          up += f;
-          const double fs = wave_sum_dpp(L.valid ? f : 0.0);
-          if (L.lane == 0) sw.phi_t[(int64_t)L.t * tr.bags + e] += fs;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int eq = e + q * kSweepWaves;
+            if (eq >= e1) break;
+            up += f[q];
+            const double fs = wave_sum_dpp(L.valid ? f[q] : 0.0);
+            if (L.lane == 0) sw.phi_t[(int64_t)L.t * tr.bags + eq] += fs;
+          }
         }
       }
-      sw.dD[ax] = sa != 0 ? sax * (1.0 + ddx) / (double)sw.S[ax] + up : 0.0;
-      const double ns = wave_sum_dpp((L.valid && x != L.s) ? up : 0.0);
-      if (L.lane == 0) sw.node_t[(int64_t)L.t * n + x] += ns;
+      sh_acc[wave][L.lane] = up;
+      __syncthreads();
+      if (wave == 0) {
+        double u = 0.0;
+        for (int w = 0; w < kSweepWaves; ++w) u += sh_acc[w][L.lane];
+        const double ddx = SDD(sw, ax, L.lane);
+        const double sx = (double)SS(sw, ax, L.lane);
+        SDD(sw, ax, L.lane) = sa != 0 ? sax * (1.0 + ddx) / sx + u : 0.0;
+        SS(sw, ax, L.lane) = sa;
+        const double ns = wave_sum_dpp((L.valid && x != L.s) ? u : 0.0);
+        if (L.lane == 0) sw.node_t[(int64_t)L.t * n + x] += ns;
+      }
+      __syncthreads();
     }
   }
+  if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x + 3] = __builtin_amdgcn_s_memrealtime();
   if (ovf) atomicOr(ovf_flag, 1);
 }
 
@@ -341,7 +567,7 @@ inline unsigned blocks_for(int64_t items, int per) { return (unsigned)((items + 
 }  // namespace
 
 void bcx_ctx::free_sweep() {
-  void *bufs[] = {sw.D,   sw.S,        sw.dD,       sw.node_t, sw.phi_t,
+  void *bufs[] = {sw.state, sw.node_t, sw.phi_t,
                   sw.node, sw.phi,     sw.interior, sw.tile_src, sw.cand, sw.cand_off,
                   sw.pf};
   for (void *p : bufs)
@@ -354,9 +580,7 @@ void bcx_ctx::ensure_sweep(int tiles) {
   if (tiles <= sweep_tiles) return;
   free_sweep();
   const size_t n = (size_t)ix.n, B = (size_t)tree.bags, T = (size_t)tiles, W = kWave;
-  HIP_OK(hipMalloc(&sw.D, T * n * W * 4));
-  HIP_OK(hipMalloc(&sw.S, T * n * W * 8));
-  HIP_OK(hipMalloc(&sw.dD, T * n * W * 8));
+  HIP_OK(hipMalloc(&sw.state, T * n * W * 20));
   HIP_OK(hipMalloc(&sw.node_t, T * n * 8));
   HIP_OK(hipMalloc(&sw.phi_t, T * std::max<size_t>(B, 1) * 8));
   HIP_OK(hipMalloc(&sw.node, n * 8));
@@ -367,6 +591,7 @@ void bcx_ctx::ensure_sweep(int tiles) {
   HIP_OK(hipMalloc(&sw.cand_off, (T + 1) * 4));
   sw.cand_cap = (int)std::min<size_t>(T * ((size_t)tree.height + W), (size_t)1 << 30);
   HIP_OK(hipMalloc(&sw.cand, (size_t)sw.cand_cap * 4));
+  sw.tstride = tiles;
   sweep_tiles = tiles;
 }
 
@@ -451,7 +676,10 @@ void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_ino
       HIP_OK(hipMemcpyAsync(sw.cand, cand.data(), cand.size() * 4, hipMemcpyHostToDevice,
                             c->stream));
     if (timed) HIP_OK(hipEventRecord(c->ev[0], c->stream));
-    sweep_tile_kernel<<<bt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf);
+    uint64_t *stamps = nullptr;
+    if (timed) HIP_OK(hipMalloc(&stamps, (size_t)bt * 4 * 8));
+    sweep_tile_kernel<<<bt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf,
+                                                           stamps);
     if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
     if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
     sweep_tile_reduce_kernel<<<blocks_for(n + B, 256), 256, 0, c->stream>>>(n, B, bt, sw);
@@ -466,6 +694,18 @@ void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_ino
       c->timing.scan_ms += a;
       c->timing.dep_ms += b;
       c->timing.reduce_ms += r;
+      std::vector<uint64_t> st((size_t)bt * 4);
+      HIP_OK(hipMemcpy(st.data(), stamps, st.size() * 8, hipMemcpyDeviceToHost));
+      HIP_OK(hipFree(stamps));
+      double f = 0, b2 = 0, as = 0;
+      for (int t = 0; t < bt; ++t) {
+        f += (double)(st[4 * t + 1] - st[4 * t]);
+        b2 += (double)(st[4 * t + 2] - st[4 * t + 1]);
+        as += (double)(st[4 * t + 3] - st[4 * t + 2]);
+      }
+      c->timing.sweep_fwd_ms += f / bt * 1e-5;  // 100 MHz ticks
+      c->timing.sweep_bwd_ms += b2 / bt * 1e-5;
+      c->timing.sweep_asc_ms += as / bt * 1e-5;
     }
     c->timing.scan_launches += 1;
     c->timing.dep_launches += 1;

@@ -83,6 +83,9 @@ typedef struct bcx_timing {
   int64_t scan_launches;
   int64_t dep_launches;
   int64_t sources;
+  double sweep_fwd_ms;  /* bag sweep: mean per-tile phase times (profiling) */
+  double sweep_bwd_ms;
+  double sweep_asc_ms;
 } bcx_timing;
 
 const char *bcx_last_error(void);
@@ -151,6 +154,9 @@ void bcx_ctx_timing_reset(bcx_ctx *ctx);
 /* on != 0: launch batches kernel by kernel with HIP events between phases
  * (accumulated into bcx_timing) instead of replaying a captured graph. */
 int bcx_ctx_set_profiling(bcx_ctx *ctx, int on);
+/* Measurement hook (no reference counterpart): one device-to-device copy of
+ * `bytes`, timed; used to calibrate HBM traffic counters. */
+int bcx_ctx_copy_probe(bcx_ctx *ctx, size_t bytes, double *ms);
 
 /* ---- hot path --------------------------------------------------------- */
 /* Full BC over all sources: bc_out[n] (undirected, unnormalised, each
