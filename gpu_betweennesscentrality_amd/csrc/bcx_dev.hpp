@@ -84,14 +84,31 @@ struct DevTree {
   const int *zp_pair;      // P
   const int *ep_off;       // B + 1: pairs touching each bag entry
   const int *ep_pair;      // 2P
+  // Staging slots: the node's fields and its first kStageEntries bag /
+  // inverse-bag entries in fixed places, so a wavefront stages 64 nodes with
+  // one round of coalesced loads.  Level k's nodes are dealt round-robin to
+  // the 16 wavefronts of a workgroup (position lo + w + 16 i to wavefront w)
+  // and wavefront w's i-th node sits in slot st_off[k] + w * ceil(count / 16)
+  // + i.  Unused entry places name node 0 with length 0 and count 0.
+  const int *st_off;          // height + 1
+  const int4 *fs_node;        // {v, bag_off[v], bag_off[v + 1], subtree[v]}
+  const int4 *fs_u;           // ancestors of bag entries 0..3
+  const int4 *fs_len;         // their shortcut lengths
+  const ulonglong2 *fs_cnt;   // 2 per node: counts of entries 0..1, 2..3
+  const int4 *bs_node;        // {v, ibag_off[v], ibag_off[v + 1], 0}
+  const int4 *bs_x;           // descendants of inverse-bag entries 0..3
+  const int4 *bs_e;           // their entry indices
+  const int4 *bs_len;         // their shortcut lengths
+  const ulonglong2 *bs_cnt;   // 2 per node: counts of entries 0..1, 2..3
 };
+
+constexpr int kStageEntries = 4;
 
 // Per-tile state of the sweep: one record per (tile, node) with the lanes'
 // values, per-tile sums [tile][node] and [tile][entry].
 struct SweepWork {
-  char *state;     // per (tile, node) record: distance[64], path count[64],
-                   // dependency of the descending state[64] (ascending, for
-                   // ancestors, after the ascending pass)
+  char *state;     // per (tile, node) record: distance[64], then 64 8-byte
+                   // slots (path count, then dependency quotient)
   double *node_t;  // [tile][n]: node flow summed over the tile's lanes
   double *phi_t;   // [tile][B]: shortcut flow summed over the tile's lanes
   double *node;    // [n]: node flow summed over tiles

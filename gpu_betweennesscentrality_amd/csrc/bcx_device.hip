@@ -623,6 +623,59 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     t.ep_off = (const int *)keep(dev_upload(epo));
     t.ep_pair = (const int *)keep(dev_upload(epp));
     c->h_lvl_pair_off = lpo;
+    {
+      // Staging arrays: level k's node at position lo + w + 16 i (the i-th
+      // node of wavefront w) goes to slot st_off[k] + w * chunk + i, with
+      // chunk = ceil(count / 16), so each wavefront's nodes are contiguous.
+      const int H = ix.tree_height, kW = 16;
+      std::vector<int32_t> st_off((size_t)H + 1, 0), slot_of((size_t)n);
+      for (int k = 0; k < H; ++k) {
+        const int32_t lo = lvl_off[(size_t)k], cnt = lvl_off[(size_t)k + 1] - lo;
+        const int32_t chunk = (cnt + kW - 1) / kW;
+        st_off[(size_t)k + 1] = st_off[(size_t)k] + kW * chunk;
+        for (int32_t i = 0; i < cnt; ++i)
+          slot_of[(size_t)(lo + i)] = st_off[(size_t)k] + (i % kW) * chunk + i / kW;
+      }
+      const size_t S = (size_t)st_off[(size_t)H];
+      std::vector<int32_t> fnode(S * 4, 0), fu(S * 4, 0), flen(S * 4, 0), bnode(S * 4, 0),
+          bx(S * 4, 0), be(S * 4, 0), blen(S * 4, 0);
+      std::vector<uint64_t> fcnt(S * 4, 0), bcnt(S * 4, 0);
+      for (int32_t q = 0; q < n; ++q) {
+        const int32_t v = lvl_v[(size_t)q];
+        const size_t o = (size_t)slot_of[(size_t)q] * 4;
+        fnode[o] = v;
+        fnode[o + 1] = boff[(size_t)v];
+        fnode[o + 2] = boff[(size_t)v + 1];
+        fnode[o + 3] = ix.subtree[(size_t)v];
+        for (int r = 0; r < kStageEntries && boff[(size_t)v] + r < boff[(size_t)v + 1]; ++r) {
+          const size_t e = (size_t)boff[(size_t)v] + (size_t)r;
+          fu[o + r] = ix.bag[e];
+          flen[o + r] = ix.bag_len[e];
+          fcnt[o + r] = ix.bag_cnt[e];
+        }
+        bnode[o] = v;
+        bnode[o + 1] = ioff[(size_t)v];
+        bnode[o + 2] = ioff[(size_t)v + 1];
+        bnode[o + 3] = 0;
+        for (int r = 0; r < kStageEntries && ioff[(size_t)v] + r < ioff[(size_t)v + 1]; ++r) {
+          const size_t k = (size_t)ioff[(size_t)v] + (size_t)r;
+          bx[o + r] = ix_x[k];
+          be[o + r] = ix_e[k];
+          blen[o + r] = ix.bag_len[(size_t)ix_e[k]];
+          bcnt[o + r] = ix.bag_cnt[(size_t)ix_e[k]];
+        }
+      }
+      t.st_off = (const int *)keep(dev_upload(st_off));
+      t.fs_node = (const int4 *)keep(dev_upload(fnode));
+      t.fs_u = (const int4 *)keep(dev_upload(fu));
+      t.fs_len = (const int4 *)keep(dev_upload(flen));
+      t.fs_cnt = (const ulonglong2 *)keep(dev_upload(fcnt));
+      t.bs_node = (const int4 *)keep(dev_upload(bnode));
+      t.bs_x = (const int4 *)keep(dev_upload(bx));
+      t.bs_e = (const int4 *)keep(dev_upload(be));
+      t.bs_len = (const int4 *)keep(dev_upload(blen));
+      t.bs_cnt = (const ulonglong2 *)keep(dev_upload(bcnt));
+    }
     c->tree = t;
     c->h_parent = ix.parent;
     c->h_depth = ix.depth;

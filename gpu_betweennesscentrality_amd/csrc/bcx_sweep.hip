@@ -35,10 +35,14 @@ using namespace bcx_dev;
 
 namespace {
 
-// Lane state is one record per (tile, node): 64 distances, then 64 path
-// counts, then 64 dependencies (1280 bytes), so the rows a node reads of an
-// ancestor or descendant are one contiguous region of memory.
-constexpr int64_t kRec = kWave * (4 + 8 + 8);
+// Lane state is one record per (tile, node): 64 distances, then 64 8-byte
+// slots (768 bytes), so the rows a node reads of an ancestor or descendant
+// are one contiguous region of memory.  The slot holds the path count sigma
+// through the forward sweep; a node's backward step is the last reader of
+// its own count and replaces it with the dependency quotient
+// Q = (1 + dD) / sigma, which its ancestors then read.
+constexpr int64_t kSlotOff = kWave * 4;
+constexpr int64_t kRec = kWave * (4 + 8);
 __device__ __forceinline__ int64_t lix_(int nn, int t, int v, int /*lane*/) {
   return ((int64_t)t * nn + v) * kRec;
 }
@@ -46,10 +50,13 @@ __device__ __forceinline__ int &SD(const SweepWork &sw, int64_t r, int lane) {
   return ((int *)(sw.state + r))[lane];
 }
 __device__ __forceinline__ uint64_t &SS(const SweepWork &sw, int64_t r, int lane) {
-  return ((uint64_t *)(sw.state + r + kWave * 4))[lane];
+  return ((uint64_t *)(sw.state + r + kSlotOff))[lane];
 }
-__device__ __forceinline__ double &SDD(const SweepWork &sw, int64_t r, int lane) {
-  return ((double *)(sw.state + r + kWave * 12))[lane];
+__device__ __forceinline__ double getQ(const SweepWork &sw, int64_t r, int lane) {
+  return __builtin_bit_cast(double, SS(sw, r, lane));
+}
+__device__ __forceinline__ void setQ(const SweepWork &sw, int64_t r, int lane, double q) {
+  SS(sw, r, lane) = __builtin_bit_cast(uint64_t, q);
 }
 
 struct Lane {
@@ -70,10 +77,16 @@ __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &
   return L;
 }
 
-// Sum of a double over the 64 lanes of a wavefront with DPP lane moves
-// (quad swaps, row rotations, row broadcasts), in one fixed order; the total
-// is taken from lane 63.  All 64 lanes must be active.
-template <int kCtrl, int kRowMask = 0xf>
+// Lane moves of a double.  dpp_row: a DPP move inside rows of 16 lanes that
+// writes every lane; dpp_move: one with a row mask (rows left out keep 0).
+template <int kCtrl>
+__device__ __forceinline__ double dpp_row(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, kCtrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), kCtrl, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+template <int kCtrl, int kRowMask>
 __device__ __forceinline__ double dpp_move(double v) {
   const uint64_t b = __builtin_bit_cast(uint64_t, v);
   int lo = (int)(uint32_t)b, hi = (int)(uint32_t)(b >> 32);
@@ -82,11 +95,19 @@ __device__ __forceinline__ double dpp_move(double v) {
   return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+// Sum over each row of 16 lanes, left in every lane of the row.
+__device__ __forceinline__ double row_sum(double v) {
+  v += dpp_row<0xb1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_row<0x4e>(v);   // quad_perm [2,3,0,1]: quad sums
+  v += dpp_row<0x124>(v);  // row_ror:4
+  v += dpp_row<0x128>(v);  // row_ror:8
+  return v;
+}
+
+// Sum of a double over the 64 lanes of a wavefront, in one fixed order,
+// returned in every lane.  All 64 lanes must be active.
 __device__ __forceinline__ double wave_sum_dpp(double v) {
-  v += dpp_move<0xb1>(v);        // quad_perm [1,0,3,2]
-  v += dpp_move<0x4e>(v);        // quad_perm [2,3,0,1]: quad sums
-  v += dpp_move<0x124>(v);       // row_ror:4
-  v += dpp_move<0x128>(v);       // row_ror:8: row sums in every lane
+  v = row_sum(v);
   v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
   v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
   const uint64_t b = __builtin_bit_cast(uint64_t, v);
@@ -95,79 +116,130 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+// Cross-half and cross-row pair sums with the gfx950 lane swaps:
+//   half_pair_sum(a, b): lanes 0..31 get a[i] + a[i + 32], lanes 32..63 get
+//                        b[i - 32] + b[i];
+//   row_pair_sum(a, b):  rows 0, 2 get a[i] + a[i + 16], rows 1, 3 get
+//                        b[i - 16] + b[i].
+__device__ __forceinline__ double half_pair_sum(double a, double b) {
+  const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)ua, (uint32_t)ub, false, false);
+  const auto hi =
+      __builtin_amdgcn_permlane32_swap((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), false, false);
+  const double x = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+  const double y = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+  return x + y;
+}
+__device__ __forceinline__ double row_pair_sum(double a, double b) {
+  const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)ua, (uint32_t)ub, false, false);
+  const auto hi =
+      __builtin_amdgcn_permlane16_swap((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), false, false);
+  const double x = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+  const double y = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+  return x + y;
+}
+
+// Four 64-lane sums at once (a transposed reduction): every lane of row r
+// (lanes 16r .. 16r + 15) receives the sum over all lanes of a_r.  Six lane
+// swaps and four row steps instead of four full reductions.  All 64 lanes
+// must be active.
+__device__ __forceinline__ double wave_sum4(double a0, double a1, double a2, double a3) {
+  const double b0 = half_pair_sum(a0, a2);  // a0 in lanes 0..31, a2 in 32..63
+  const double b1 = half_pair_sum(a1, a3);  // a1 in lanes 0..31, a3 in 32..63
+  return row_sum(row_pair_sum(b0, b1));     // rows 0..3: a0, a1, a2, a3
+}
+
+// Of four uniform values, the one for this lane's row.
+__device__ __forceinline__ int row_pick(int lane, int x0, int x1, int x2, int x3) {
+  const int r = lane >> 4;
+  return r == 0 ? x0 : r == 1 ? x1 : r == 2 ? x2 : x3;
+}
+
 constexpr int kSweepThreads = 1024;
 constexpr int kSweepWaves = kSweepThreads / kWave;
 
-// Forward candidates of node v from bag entries [e, e_end) with a stride.
-// Entries are taken four at a time so their row loads are in flight together.
+// Forward candidates of node v from bag entries [e, e_end) with a stride,
+// merged into the running (best, c).  Entries are taken kGroup at a time so
+// their distance rows are in flight together; a path count is read only in
+// the lanes where its candidate ties the new minimum.
+constexpr int kGroup = 8;  // bag entries whose rows are loaded together
+
 __device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, const Lane &L,
                                         int e, int e_end, int stride, int &best, uint64_t &c) {
-  const int n = tr.n;
-  for (; e < e_end; e += 4 * stride) {
-    int cand[4];
-    uint64_t m[4];
+  for (; e < e_end; e += kGroup * stride) {
+    int cand[kGroup];
+    int64_t a[kGroup];
+    int b = best;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kGroup; ++q) {
       const int eq = e + q * stride;
       const bool ok = eq < e_end;
       const int ec = ok ? eq : e;
-      const int64_t a = lix_(tr.n, L.t, tr.bag_v[ec], L.lane);
-      cand[q] = ok ? SD(sw, a, L.lane) + tr.bag_len[ec] : 0x7fffffff;
-      m[q] = ok ? SS(sw, a, L.lane) : 0;
+      a[q] = lix_(tr.n, L.t, tr.bag_v[ec], L.lane);
+      const int dl = SD(sw, a[q], L.lane) + tr.bag_len[ec];  // loaded in every case
+      cand[q] = ok ? dl : 0x7fffffff;
+      b = min(b, cand[q]);
     }
+    uint64_t m[kGroup];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kGroup; ++q) {
+      m[q] = 0;
+      if (cand[q] == b) m[q] = SS(sw, a[q], L.lane);
+    }
+    if (b < best) c = 0;
+    best = b;
+#pragma unroll
+    for (int q = 0; q < kGroup; ++q) {
       const int eq = e + q * stride;
       if (eq >= e_end) break;
-      if (cand[q] < best) {
-        best = cand[q];
-        c = mul_sat(m[q], tr.bag_cnt[eq]);
-      } else if (cand[q] == best) {
-        c = add_sat(c, mul_sat(m[q], tr.bag_cnt[eq]));
-      }
+      if (cand[q] == b) c = add_sat(c, mul_sat(m[q], tr.bag_cnt[eq]));
     }
   }
 }
 
 // Backward pulls of node v over inverse-bag entries [k, k_end) with a stride,
-// four at a time; writes the tile's flow on each entry and returns the lane's
-// dependency part.
+// kGroup at a time; writes the tile's flow on each entry and returns the lane's
+// dependency part.  A descendant x that is done holds, in its 8-byte slot,
+// the quotient Q(x) = (1 + dD(x)) / sigma(x), so an entry costs the distance
+// row of x plus, only in lanes where the entry is tight, one row of Q(x).
+// Must be called with all 64 lanes active.
 __device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork &sw,
                                               const Lane &L, int k, int k_end, int stride,
                                               int dv, double sv) {
-  const int n = tr.n;
   double acc = 0.0;
-  for (; k < k_end; k += 4 * stride) {
-    int dx[4], ee[4], len[4];
-    uint64_t sx[4];
-    double ddx[4];
+  for (; k < k_end; k += kGroup * stride) {
+    int dx[kGroup], ee[kGroup], len[kGroup];
+    int64_t ax[kGroup];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kGroup; ++q) {
       const int kq = k + q * stride;
       const int kc = kq < k_end ? kq : k;
       const int x = tr.ibag_x[kc];
       ee[q] = tr.ibag_e[kc];
       len[q] = tr.bag_len[ee[q]];
-      const int64_t ax = lix_(tr.n, L.t, x, L.lane);
-      dx[q] = SD(sw, ax, L.lane);
-      sx[q] = SS(sw, ax, L.lane);
-      ddx[q] = SDD(sw, ax, L.lane);
+      ax[q] = lix_(tr.n, L.t, x, L.lane);
+      dx[q] = SD(sw, ax[q], L.lane);
+    }
+    double f[kGroup];
+#pragma unroll
+    for (int q = 0; q < kGroup; ++q) {
+      double qx = 0.0;
+      if (k + q * stride < k_end && dv + len[q] == dx[q]) qx = getQ(sw, ax[q], L.lane);
+      f[q] = (double)tr.bag_cnt[ee[q]] * sv * qx;
+      acc += f[q];
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int kq = k + q * stride;
-      if (kq >= k_end) break;
-      const double f = dv + len[q] == dx[q]
-                           ? (double)tr.bag_cnt[ee[q]] * sv * (1.0 + ddx[q]) / (double)sx[q]
-                           : 0.0;
-      acc += f;
-      const double fs = wave_sum_dpp(L.valid ? f : 0.0);
-      if (L.lane == 0) sw.phi_t[(int64_t)L.t * tr.bags + ee[q]] = fs;
+    for (int h = 0; h < kGroup; h += 4) {
+      const double fs = wave_sum4(L.valid ? f[h] : 0.0, L.valid ? f[h + 1] : 0.0,
+                                  L.valid ? f[h + 2] : 0.0, L.valid ? f[h + 3] : 0.0);
+      const int q = h + (L.lane >> 4);
+      const int e = row_pick(L.lane, ee[h], ee[h + 1], ee[h + 2], ee[h + 3]);
+      if ((L.lane & 15) == 0 && k + q * stride < k_end) sw.phi_t[(int64_t)L.t * tr.bags + e] = fs;
     }
   }
   return acc;
 }
-
 
 __device__ __forceinline__ int rl(int x, int j) { return __builtin_amdgcn_readlane(x, j); }
 __device__ __forceinline__ uint64_t rl64(uint64_t x, int j) {
@@ -176,85 +248,94 @@ __device__ __forceinline__ uint64_t rl64(uint64_t x, int j) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-constexpr int kStage = 4;  // bag entries per node staged in registers
+constexpr int kStage = kStageEntries;  // bag entries per node staged in registers
 
-constexpr int kNodes = 2;  // nodes of a level processed together by one wavefront
+// Row loads below are issued unconditionally, from in-bounds indices (staged
+// entries beyond a node's bag name node 0), and masked with a select
+// afterwards: a load under a condition becomes a branch that waits for it,
+// which serializes the loads of a step.
 
-// Forward over the nodes of one level assigned to this wavefront (nodes
-// wave, wave + 16, ...).  Lane j first loads node j's metadata and its first
-// kStage bag entries; the wavefront then handles kNodes nodes per step with
-// all their row loads in flight together.
+constexpr int kNodes = 4;  // nodes of a level processed together by one wavefront
+
+// Level k's nodes are dealt round-robin to the 16 wavefronts (neighbouring
+// nodes share ancestors, so the wavefronts of a workgroup touch the same rows
+// at about the same time); this wavefront's staging slots are
+// [first, first + count).
+__device__ __forceinline__ void level_block(const DevTree &tr, int k, int cnt, int wave,
+                                            int &first, int &count) {
+  const int chunk = (cnt + kSweepWaves - 1) / kSweepWaves;
+  first = tr.st_off[k] + wave * chunk;
+  count = max(0, (cnt - wave + kSweepWaves - 1) / kSweepWaves);
+}
+
+// Forward over this wavefront's block of a level (all at tree depth `depth`).
+// Lane j stages node j's fields and first kStage bag entries with one round
+// of loads; the wavefront then handles kNodes nodes per step: all their
+// distance rows in flight together, then the path-count rows of the tight
+// candidates only.  Saturating sums of saturating products do not depend on
+// the order of their terms, so the count equals that of a running
+// min-and-count.
 __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree &tr,
                                                const SweepWork &sw, const Lane &L, int wave,
-                                               int lo, int cnt, int &ovf) {
+                                               int cnt, int depth, int &ovf) {
   const int n = tr.n;
-  const int mine = (cnt - wave + kSweepWaves - 1) / kSweepWaves;
+  int first, mine;
+  level_block(tr, depth, cnt, wave, first, mine);
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
-    int sv = 0, se0 = 0, se1 = 0, ssub = 0, sdep = 0;
-    int su[kStage], slen[kStage];
-    uint64_t sc[kStage];
-#pragma unroll
-    for (int r = 0; r < kStage; ++r) {
-      su[r] = 0;
-      slen[r] = 0;
-      sc[r] = 0;
-    }
-    if (L.lane < jn) {
-      sv = tr.lvl_v[lo + wave + kSweepWaves * (base + L.lane)];
-      se0 = tr.bag_off[sv];
-      se1 = tr.bag_off[sv + 1];
-      ssub = tr.subtree[sv];
-      sdep = ix.depth[sv];
-#pragma unroll
-      for (int r = 0; r < kStage; ++r)
-        if (se0 + r < se1) {
-          su[r] = tr.bag_v[se0 + r];
-          slen[r] = tr.bag_len[se0 + r];
-          sc[r] = tr.bag_cnt[se0 + r];
-        }
-    }
+    const int p = first + base + min(L.lane, jn - 1);
+    const int4 nd = tr.fs_node[p], su = tr.fs_u[p], sl = tr.fs_len[p];
+    const ulonglong2 c01 = tr.fs_cnt[2 * p], c23 = tr.fs_cnt[2 * p + 1];
+    const int sur[kStage] = {su.x, su.y, su.z, su.w};
+    const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
+    const uint64_t scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
     for (int j0 = 0; j0 < jn; j0 += kNodes) {
-      int cand[kNodes][kStage];
+      int cand[kNodes][kStage], best[kNodes], upd[kNodes];
+      uint64_t upc[kNodes];
+#pragma unroll
+      for (int u = 0; u < kNodes; ++u) {
+        const int j = min(j0 + u, jn - 1);
+        const int v = rl(nd.x, j);
+        const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) {
+          const int dl = SD(sw, lix_(n, L.t, rl(sur[r], j), L.lane), L.lane) + rl(slr[r], j);
+          cand[u][r] = r < nb ? dl : 0x7fffffff;
+        }
+        upd[u] = 0x7fffffff;
+        upc[u] = 0;
+        if (v <= L.s && L.s < v + rl(nd.w, j)) {  // v is an ancestor of s: up-label
+          const int64_t o = L.rs + depth;
+          upd[u] = ix.dis[o];
+          upc[u] = ix.cnt[o];
+        }
+        best[u] = upd[u];
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) best[u] = min(best[u], cand[u][r]);
+      }
       uint64_t m[kNodes][kStage];
 #pragma unroll
       for (int u = 0; u < kNodes; ++u) {
         const int j = min(j0 + u, jn - 1);
-        const int nb = min(kStage, rl(se1, j) - rl(se0, j));
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          const int64_t a = lix_(n, L.t, rl(su[r], j), L.lane);
-          cand[u][r] = r < nb ? SD(sw, a, L.lane) + rl(slen[r], j) : 0x7fffffff;
-          m[u][r] = r < nb ? SS(sw, a, L.lane) : 0;
+          m[u][r] = 0;
+          if (cand[u][r] == best[u]) m[u][r] = SS(sw, lix_(n, L.t, rl(sur[r], j), L.lane), L.lane);
         }
       }
 #pragma unroll
       for (int u = 0; u < kNodes; ++u) {
         const int j = j0 + u;
         if (j >= jn) break;
-        const int v = rl(sv, j), e0 = rl(se0, j), e1 = rl(se1, j);
-        const int nb = min(kStage, e1 - e0);
-        int best = 0x7fffffff;
-        uint64_t c = 0;
-        if (v <= L.s && L.s < v + rl(ssub, j)) {  // v is an ancestor of s: up-label
-          const int64_t o = L.rs + rl(sdep, j);
-          best = ix.dis[o];
-          c = ix.cnt[o];
-        }
+        const int v = rl(nd.x, j), e0 = rl(nd.y, j), e1 = rl(nd.z, j);
+        int b = best[u];
+        uint64_t c = upd[u] == b ? upc[u] : 0;
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) {
-          if (r >= nb) break;
-          const uint64_t cr = rl64(sc[r], j);
-          if (cand[u][r] < best) {
-            best = cand[u][r];
-            c = mul_sat(m[u][r], cr);
-          } else if (cand[u][r] == best) {
-            c = add_sat(c, mul_sat(m[u][r], cr));
-          }
-        }
-        if (e1 - e0 > kStage) fwd_bag(tr, sw, L, e0 + kStage, e1, 1, best, c);
+        for (int r = 0; r < kStage; ++r)
+          if (cand[u][r] == b) c = add_sat(c, mul_sat(m[u][r], rl64(scr[r], j)));
+        if (e1 - e0 > kStage) fwd_bag(tr, sw, L, e0 + kStage, e1, 1, b, c);
         const int64_t a = lix_(n, L.t, v, L.lane);
-        SD(sw, a, L.lane) = best;
+        SD(sw, a, L.lane) = b;
         SS(sw, a, L.lane) = c;
         ovf |= L.valid && c == ~0ull;
       }
@@ -262,84 +343,79 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
   }
 }
 
-// Backward over the nodes of one level assigned to this wavefront, with the
-// same staging and kNodes nodes per step.
+// Backward over this wavefront's block of a level, with the same staging and
+// kNodes nodes per step.  The flows of a node's staged entries and the node
+// flows of the step's nodes are each one four-way transposed lane sum.
 __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWork &sw,
-                                               const Lane &L, int wave, int lo, int cnt) {
+                                               const Lane &L, int wave, int cnt, int depth) {
+  static_assert(kNodes == 4 && kStage == 4, "four-way lane sums");
   const int n = tr.n;
-  const int mine = (cnt - wave + kSweepWaves - 1) / kSweepWaves;
+  int first, mine;
+  level_block(tr, depth, cnt, wave, first, mine);
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
-    int sv = 0, sk0 = 0, sk1 = 0;
-    int sx[kStage], se[kStage], slen[kStage];
-    uint64_t sc[kStage];
-#pragma unroll
-    for (int r = 0; r < kStage; ++r) {
-      sx[r] = 0;
-      se[r] = 0;
-      slen[r] = 0;
-      sc[r] = 0;
-    }
-    if (L.lane < jn) {
-      sv = tr.lvl_v[lo + wave + kSweepWaves * (base + L.lane)];
-      sk0 = tr.ibag_off[sv];
-      sk1 = tr.ibag_off[sv + 1];
-#pragma unroll
-      for (int r = 0; r < kStage; ++r)
-        if (sk0 + r < sk1) {
-          sx[r] = tr.ibag_x[sk0 + r];
-          se[r] = tr.ibag_e[sk0 + r];
-          slen[r] = tr.bag_len[se[r]];
-          sc[r] = tr.bag_cnt[se[r]];
-        }
-    }
+    const int p = first + base + min(L.lane, jn - 1);
+    const int4 nd = tr.bs_node[p], sx = tr.bs_x[p], se = tr.bs_e[p], sl = tr.bs_len[p];
+    const ulonglong2 c01 = tr.bs_cnt[2 * p], c23 = tr.bs_cnt[2 * p + 1];
+    const int sxr[kStage] = {sx.x, sx.y, sx.z, sx.w};
+    const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
+    const uint64_t scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
     for (int j0 = 0; j0 < jn; j0 += kNodes) {
       int dv[kNodes];
       uint64_t svu[kNodes];
       int dx[kNodes][kStage];
-      uint64_t sxx[kNodes][kStage];
-      double ddx[kNodes][kStage];
 #pragma unroll
       for (int u = 0; u < kNodes; ++u) {
         const int j = min(j0 + u, jn - 1);
-        const int nb = min(kStage, rl(sk1, j) - rl(sk0, j));
-        const int64_t av = lix_(n, L.t, rl(sv, j), L.lane);
+        const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
+        const int64_t av = lix_(n, L.t, rl(nd.x, j), L.lane);
         dv[u] = SD(sw, av, L.lane);
         svu[u] = SS(sw, av, L.lane);
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          const int64_t ax = lix_(n, L.t, rl(sx[r], j), L.lane);
-          dx[u][r] = r < nb ? SD(sw, ax, L.lane) : -1;
-          sxx[u][r] = r < nb ? SS(sw, ax, L.lane) : 1;
-          ddx[u][r] = r < nb ? SDD(sw, ax, L.lane) : 0.0;
+          const int dl = SD(sw, lix_(n, L.t, rl(sxr[r], j), L.lane), L.lane);
+          dx[u][r] = r < nb ? dl : -1;
         }
       }
+      double qx[kNodes][kStage];  // Q(x) in the lanes where entry r is tight
 #pragma unroll
       for (int u = 0; u < kNodes; ++u) {
-        const int j = j0 + u;
-        if (j >= jn) break;
-        const int v = rl(sv, j), k0 = rl(sk0, j), k1 = rl(sk1, j);
-        const int nb = min(kStage, k1 - k0);
-        const double svv = (double)svu[u];
-        double acc = 0.0;
+        const int j = min(j0 + u, jn - 1);
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          if (r >= nb) break;
-          const int e = rl(se[r], j);
-          const double f = dv[u] + rl(slen[r], j) == dx[u][r]
-                               ? (double)rl64(sc[r], j) * svv * (1.0 + ddx[u][r]) /
-                                     (double)sxx[u][r]
-                               : 0.0;
-          acc += f;
-          const double fs = wave_sum_dpp(L.valid ? f : 0.0);
-          if (L.lane == 0) sw.phi_t[(int64_t)L.t * tr.bags + e] = fs;
+          qx[u][r] = 0.0;
+          if (dv[u] + rl(slr[r], j) == dx[u][r])
+            qx[u][r] = getQ(sw, lix_(n, L.t, rl(sxr[r], j), L.lane), L.lane);
         }
-        if (k1 - k0 > kStage) acc += bwd_entries(tr, sw, L, k0 + kStage, k1, 1, dv[u], svv);
-        const int64_t av = lix_(n, L.t, v, L.lane);
-        SDD(sw, av, L.lane) = acc;
-        const double ns = wave_sum_dpp((L.valid && L.s != v) ? acc : 0.0);
-        if (L.lane == 0) sw.node_t[(int64_t)L.t * n + v] = ns;
       }
+      double ns_in[kNodes];
+#pragma unroll
+      for (int u = 0; u < kNodes; ++u) {
+        const int j = min(j0 + u, jn - 1);
+        const bool live = j0 + u < jn;  // false for the clamped repeats of the last node
+        const int v = rl(nd.x, j), k0 = rl(nd.y, j), k1 = rl(nd.z, j);
+        const int nb = min(kStage, k1 - k0);
+        const double svv = (double)svu[u];
+        double f[kStage], acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) {
+          f[r] = (double)rl64(scr[r], j) * svv * qx[u][r];
+          acc += f[r];
+        }
+        const double fs = wave_sum4(L.valid ? f[0] : 0.0, L.valid ? f[1] : 0.0,
+                                    L.valid ? f[2] : 0.0, L.valid ? f[3] : 0.0);
+        const int e = row_pick(L.lane, rl(se.x, j), rl(se.y, j), rl(se.z, j), rl(se.w, j));
+        if (live && (L.lane & 15) == 0 && (L.lane >> 4) < nb)
+          sw.phi_t[(int64_t)L.t * tr.bags + e] = fs;
+        if (live && k1 - k0 > kStage) acc += bwd_entries(tr, sw, L, k0 + kStage, k1, 1, dv[u], svv);
+        if (live) setQ(sw, lix_(n, L.t, v, L.lane), L.lane, (1.0 + acc) / svv);
+        ns_in[u] = (live && L.valid && L.s != v) ? acc : 0.0;
+      }
+      const double ns = wave_sum4(ns_in[0], ns_in[1], ns_in[2], ns_in[3]);
+      const int u = L.lane >> 4;
+      const int v = row_pick(L.lane, rl(nd.x, min(j0, jn - 1)), rl(nd.x, min(j0 + 1, jn - 1)),
+                             rl(nd.x, min(j0 + 2, jn - 1)), rl(nd.x, min(j0 + 3, jn - 1)));
+      if ((L.lane & 15) == 0 && j0 + u < jn) sw.node_t[(int64_t)L.t * n + v] = ns;
     }
   }
 }
@@ -357,7 +433,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
   __shared__ int sh_best[kSweepWaves][kWave];
   __shared__ uint64_t sh_cnt[kSweepWaves][kWave];
   const Lane L = lane_setup(ix, sw, blockIdx.x, nsrc);
-  const int wave = threadIdx.x / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int n = tr.n, H = tr.height;
   int ovf = 0;
   // Optional phase timestamps (constant 100 MHz clock), one row per tile.
@@ -367,7 +443,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
   for (int k = 0; k < H; ++k) {
     const int lo = tr.lvl_off[k], cnt = tr.lvl_off[k + 1] - lo;
     if (cnt >= kSweepWaves) {
-      fwd_level_wave(ix, tr, sw, L, wave, lo, cnt, ovf);
+      fwd_level_wave(ix, tr, sw, L, wave, cnt, k, ovf);
     } else {
       for (int q = 0; q < cnt; ++q) {
         const int v = __builtin_amdgcn_readfirstlane(tr.lvl_v[lo + q]);
@@ -411,18 +487,19 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
   for (int k = H - 1; k >= 0; --k) {
     const int lo = tr.lvl_off[k], cnt = tr.lvl_off[k + 1] - lo;
     if (cnt >= kSweepWaves) {
-      bwd_level_wave(tr, sw, L, wave, lo, cnt);
+      bwd_level_wave(tr, sw, L, wave, cnt, k);
     } else {
       for (int q = 0; q < cnt; ++q) {
         const int v = __builtin_amdgcn_readfirstlane(tr.lvl_v[lo + q]);
         const int64_t av = lix_(tr.n, L.t, v, L.lane);
+        const double svv = (double)SS(sw, av, L.lane);
         sh_acc[wave][L.lane] = bwd_entries(tr, sw, L, tr.ibag_off[v] + wave, tr.ibag_off[v + 1],
-                                           kSweepWaves, SD(sw, av, L.lane), (double)SS(sw, av, L.lane));
+                                           kSweepWaves, SD(sw, av, L.lane), svv);
         __syncthreads();
         if (wave == 0) {
           double acc = 0.0;
           for (int w = 0; w < kSweepWaves; ++w) acc += sh_acc[w][L.lane];
-          SDD(sw, av, L.lane) = acc;
+          setQ(sw, av, L.lane, (1.0 + acc) / svv);
           const double ns = wave_sum_dpp((L.valid && L.s != v) ? acc : 0.0);
           if (L.lane == 0) sw.node_t[(int64_t)L.t * tr.n + v] = ns;
         }
@@ -436,13 +513,17 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
 
   // ---- ascending states: the union of the tile's root paths in top-down
   // order, each node's bag entries split over all wavefronts and merged in
-  // wavefront order.  Once node x is done, its slots hold the ascending state:
-  // S(x) := sigma_A(x) (the up-label count where the up-label distance is
-  // tight, else 0) and dD(x) := dA(x), with
+  // wavefront order.  With sigma_A(x) the up-label count where the up-label
+  // distance is tight (else 0), the ascending dependency is
   //   dA(x) = sigma_A(x) * (1 + dD(x)) / sigma(x)
-  //         + sum over tight bag entries (x -> y) c * sigma_A(x) * dA(y) / sigma_A(y).
-  // Later nodes read only their ancestors' ascending state, and a lane whose
-  // source is not below x never reads x again.
+  //         + sum over tight bag entries (x -> y) c * sigma_A(x) * dA(y) / sigma_A(y),
+  // and x's dependency slot, which holds Q(x) = (1 + dD(x)) / sigma(x) after
+  // the backward sweep, is replaced by the quotient
+  //   QA(x) = dA(x) / sigma_A(x) = Q(x) + sum over tight entries c * QA(y)
+  // (0 where sigma_A(x) = 0).  Entry (x -> y) carries sigma_A(x) * c * QA(y)
+  // and x's node flow gains sigma_A(x) times the sum.  Later nodes read only
+  // their ancestors' quotients, and a lane whose source is not below x never
+  // reads x again.
   {
     const int c0 = sw.cand_off[L.t], c1 = sw.cand_off[L.t + 1];
     for (int kk = c0; kk < c1; ++kk) {
@@ -460,27 +541,30 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
       if (__any(sa != 0)) {
         const int e1 = tr.bag_off[x + 1];
         for (int e = tr.bag_off[x] + wave; e < e1; e += 4 * kSweepWaves) {
-          double f[4];
+          int dy[4];
+          int64_t ay[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int eq = e + q * kSweepWaves;
             const int ec = eq < e1 ? eq : e;
-            const int64_t ay = lix_(n, L.t, tr.bag_v[ec], L.lane);
-            const int dy = SD(sw, ay, L.lane);
-            const uint64_t say = SS(sw, ay, L.lane);  // sigma_A(y), y already processed
-            const double day = SDD(sw, ay, L.lane);   // dA(y)
-            f[q] = (eq < e1 && sa != 0 && say != 0 && dx + tr.bag_len[ec] == dy)
-                       ? (double)tr.bag_cnt[ec] * sax * day / (double)say
-                       : 0.0;
+            ay[q] = lix_(n, L.t, tr.bag_v[ec], L.lane);
+            dy[q] = SD(sw, ay[q], L.lane);
           }
+          double g[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int eq = e + q * kSweepWaves;
-            if (eq >= e1) break;
-            up += f[q];
-            const double fs = wave_sum_dpp(L.valid ? f[q] : 0.0);
-            if (L.lane == 0) sw.phi_t[(int64_t)L.t * tr.bags + eq] += fs;
+            const int ec = eq < e1 ? eq : e;
+            g[q] = 0.0;
+            if (eq < e1 && sa != 0 && dx + tr.bag_len[ec] == dy[q])
+              g[q] = (double)tr.bag_cnt[ec] * getQ(sw, ay[q], L.lane);  // c * QA(y)
           }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) up += g[q];
+          const double fs = wave_sum4(L.valid ? sax * g[0] : 0.0, L.valid ? sax * g[1] : 0.0,
+                                      L.valid ? sax * g[2] : 0.0, L.valid ? sax * g[3] : 0.0);
+          const int eq = e + (L.lane >> 4) * kSweepWaves;
+          if ((L.lane & 15) == 0 && eq < e1) sw.phi_t[(int64_t)L.t * tr.bags + eq] += fs;
         }
       }
       sh_acc[wave][L.lane] = up;
@@ -488,11 +572,9 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
       if (wave == 0) {
         double u = 0.0;
         for (int w = 0; w < kSweepWaves; ++w) u += sh_acc[w][L.lane];
-        const double ddx = SDD(sw, ax, L.lane);
-        const double sx = (double)SS(sw, ax, L.lane);
-        SDD(sw, ax, L.lane) = sa != 0 ? sax * (1.0 + ddx) / sx + u : 0.0;
-        SS(sw, ax, L.lane) = sa;
-        const double ns = wave_sum_dpp((L.valid && x != L.s) ? u : 0.0);
+        const double qx = getQ(sw, ax, L.lane);
+        setQ(sw, ax, L.lane, sa != 0 ? qx + u : 0.0);
+        const double ns = wave_sum_dpp((L.valid && x != L.s) ? sax * u : 0.0);
         if (L.lane == 0) sw.node_t[(int64_t)L.t * n + x] += ns;
       }
       __syncthreads();
@@ -580,7 +662,7 @@ void bcx_ctx::ensure_sweep(int tiles) {
   if (tiles <= sweep_tiles) return;
   free_sweep();
   const size_t n = (size_t)ix.n, B = (size_t)tree.bags, T = (size_t)tiles, W = kWave;
-  HIP_OK(hipMalloc(&sw.state, T * n * W * 20));
+  HIP_OK(hipMalloc(&sw.state, T * n * kRec));
   HIP_OK(hipMalloc(&sw.node_t, T * n * 8));
   HIP_OK(hipMalloc(&sw.phi_t, T * std::max<size_t>(B, 1) * 8));
   HIP_OK(hipMalloc(&sw.node, n * 8));
@@ -615,7 +697,7 @@ void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_ino
     // held by this context's sweep buffers.
     size_t fr = 0, tot = 0;
     HIP_OK(hipMemGetInfo(&fr, &tot));
-    const size_t per_tile = (size_t)n * kWave * 20 + (size_t)n * 8 + (size_t)B * 8;
+    const size_t per_tile = (size_t)n * kRec + (size_t)n * 8 + (size_t)B * 8;
     const size_t held = (size_t)c->sweep_tiles * per_tile;
     const size_t cap = (size_t)((double)(fr + held) * 0.7) / std::max<size_t>(per_tile, 1);
     if (cap < 1) throw bcx::Error{BCX_ENOMEM, "not enough device memory for one sweep tile"};
