@@ -1,12 +1,17 @@
 """Benchmark of the betweenness hot path on one MI355X.
 
-A step is one batched launch sequence over one batch of source vertices
-(label scan -> layered dependencies -> BC accumulation) on the seeded
-road-like 320,000-vertex graph with its MHL index resident in HBM.  The value
-is whole-job full-BC throughput in sources/s; the plain one-source-per-launch
-form, the RQ serving latencies and the CPU baseline are reported beside it.
+A step is one batched launch sequence over one batch of source vertices on
+the seeded road-like 320,000-vertex graph with its MHL index resident in
+HBM: the bag-sweep form (16,384 sources: per-tile sweep kernel, tile
+reduction; the shortcut unpacking runs once per full BC and is amortized).
+The value is whole-job full-BC throughput in sources/s; the label-scan form,
+the plain one-source-per-launch form, the RQ serving latencies and the CPU
+baseline are reported beside it.
 
   python bench.py [--steps K] [--warmup W] [--workload road320k] [--gpus 1]
+  python bench.py --profile-only            # one batch of each form + a 1 GiB copy
+  python bench.py --summarize-trace CSV     # rocprofv3 kernel_stats.csv -> markdown
+  python bench.py --summarize-pmc FETCH_CSV WRITE_CSV   # HBM traffic per launch
 """
 from __future__ import annotations
 
@@ -23,6 +28,91 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+SWEEP_KERNEL = "sweep_tile_kernel"
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+COPY_BYTES = 1 << 30  # the --profile-only device copy
+PMC_SUMMARY = ROOT / "profiles" / "sweep_tile_road320k_pmc.md"
+
+
+def sweep_algorithmic_bytes(n: int, bags: int, sources: int) -> float:
+    """Bytes one sweep launch must move for `sources` sources (64 per tile).
+
+    Per tile, lanes = 64 sources, so a distance row is 256 B and a count or
+    quotient row 512 B: both sweeps read the distance row of every bag entry
+    (2 * 256 * bags); per node the forward writes distance + count (768 B), the
+    backward reads them back (768 B) and writes the quotient (512 B).  Count
+    and quotient reads of tight entries come on top and are not counted, so
+    this is a lower bound: 8 * bags + 32 * n bytes per source.
+    """
+    return (sources / 64.0) * (512.0 * bags + 2048.0 * n)
+
+
+def _kernel_rows(csv_path: str, counter: str):
+    import csv
+    rows = {}
+    for r in csv.DictReader(open(csv_path)):
+        if r.get("Counter_Name") != counter:
+            continue
+        name = r["Kernel_Name"]
+        rows.setdefault(name, []).append(float(r["Counter_Value"]))
+    return rows
+
+
+def summarize_pmc(fetch_csv: str, write_csv: str) -> str:
+    """HBM traffic of the sweep kernel from FETCH_SIZE and WRITE_SIZE runs.
+
+    Each counter is scaled by what it reads for the 1 GiB device copy of the
+    same run (a copy fetches and writes exactly 1 GiB).
+    """
+    f = _kernel_rows(fetch_csv, "FETCH_SIZE")
+    w = _kernel_rows(write_csv, "WRITE_SIZE")
+
+    def pick(rows, sub):
+        for k, v in rows.items():
+            if sub in k:
+                return max(v)
+        raise SystemExit(f"no {sub} row")
+
+    fk, wk = pick(f, SWEEP_KERNEL), pick(w, SWEEP_KERNEL)
+    fc, wc = pick(f, COPY_KERNEL), pick(w, COPY_KERNEL)
+    ff = (COPY_BYTES / 1024.0) / fc
+    wf = (COPY_BYTES / 1024.0) / wc
+    fetch = fk * 1024.0 * ff
+    write = wk * 1024.0 * wf
+    return (f"| counter | sweep kernel (KiB) | 1 GiB copy (KiB) | scale | bytes per launch |\n"
+            f"|---|---|---|---|---|\n"
+            f"| FETCH_SIZE | {fk:.0f} | {fc:.0f} | {ff:.3f} | {fetch:.4e} |\n"
+            f"| WRITE_SIZE | {wk:.0f} | {wc:.0f} | {wf:.3f} | {write:.4e} |\n\n"
+            f"traffic_bytes_per_launch: {fetch + write:.6e}\n")
+
+
+def summarize_trace(csv_path: str) -> str:
+    import csv
+    out = ["| kernel | calls | avg (us) | min (us) | max (us) | share |", "|---|---|---|---|---|---|"]
+    for r in csv.DictReader(open(csv_path)):
+        name = r["Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+        out.append(f"| {name} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
+                   f"{float(r['Percentage']):.2f}% |")
+    return "\n".join(out) + "\n"
+
+
+def committed_traffic(sources: int, workload: str):
+    """Traffic per launch from the committed PMC summary, if it covers this
+    workload and launch size (summary lines `workload:` and `sources:`)."""
+    if not PMC_SUMMARY.exists():
+        return None
+    meta = {}
+    for line in PMC_SUMMARY.read_text().splitlines():
+        if ":" in line and not line.startswith("|"):
+            k, _, v = line.partition(":")
+            meta[k.strip()] = v.strip()
+    try:
+        if meta.get("workload") != workload or int(meta.get("sources", -1)) != sources:
+            return None
+        return float(meta["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def log(*a):
@@ -45,7 +135,17 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="one label-scan batch, one sweep batch and a 1 GiB copy (for rocprofv3)")
+    ap.add_argument("--summarize-trace", metavar="CSV",
+                    help="print a markdown table of a rocprofv3 kernel_stats.csv and exit")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print the sweep kernel's HBM traffic from two counter_collection.csv")
     args = ap.parse_args()
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace), end="")
+        return 0
+    if args.summarize_pmc:
+        print(summarize_pmc(*args.summarize_pmc), end="")
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the path shards by source ranges, see DESIGN.md)", file=sys.stderr)
@@ -130,8 +230,13 @@ def main() -> int:
     dep_ms = scan_tm["dep_ms"]
     red_ms = scan_tm["reduce_ms"]
     S = 64
-    scan_bytes = B * (16.0 * n * lbar / S + 16.0 * n)
-    achieved = scan_bytes / (scan_ms * 1e-3) / 1e9
+    # Roofline of the dominant kernel, the sweep tile kernel: algorithmic bytes
+    # of one launch over its duration (HIP events on the context's stream).
+    bags = int(st["bag_entries"])
+    sweep_ms = tm["scan_ms"]
+    algo = sweep_algorithmic_bytes(n, bags, Bs)
+    achieved = algo / (sweep_ms * 1e-3) / 1e9
+    traffic = committed_traffic(Bs, args.workload)
 
     # Plain form: one source per launch sequence.
     ps = tree_perm[: args.plain_sources]
@@ -168,7 +273,7 @@ def main() -> int:
             "n": n, "arcs": int(g.arcs), "sweep_batch": Bs, "labelscan_batch": B, "tile": S,
             "tchunk": tuning["tchunk"], "dep_threads": tuning["bp_threads"],
             "tree_height": st["tree_height"], "tree_width": st["tree_width"],
-            "label_entries": st["label_entries"], "Lbar": round(lbar, 2),
+            "label_entries": st["label_entries"], "bag_entries": bags, "Lbar": round(lbar, 2),
             "labelscan_sources_per_s": round(scan_rate, 2),
             "sweep_phase_ms": [round(tm["scan_ms"], 3), round(tm["dep_ms"], 3), round(tm["reduce_ms"], 3)],
             "plain_sources_per_s": round(plain, 2),
@@ -180,8 +285,11 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "scan_tile_kernel",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None if traffic is None else round(traffic, 0),
+            "kernel": SWEEP_KERNEL, "kernel_ms": round(sweep_ms, 3),
+            "algorithmic_bytes": round(algo, 0),
+            "traffic_GBs": None if traffic is None else round(traffic / (sweep_ms * 1e-3) / 1e9, 2),
         },
         "cpu_baseline": cpu,
         "rq": rq,

@@ -39,7 +39,8 @@ class BcxError(RuntimeError):
 class IndexStats(C.Structure):
     _fields_ = [("n", C.c_int32), ("tree_height", C.c_int32), ("tree_width", C.c_int32),
                 ("count_overflow", C.c_int32), ("label_entries", C.c_int64),
-                ("mean_depth", C.c_double), ("build_seconds", C.c_double)]
+                ("mean_depth", C.c_double), ("build_seconds", C.c_double),
+                ("bag_entries", C.c_int64)]
 
 
 class Tuning(C.Structure):

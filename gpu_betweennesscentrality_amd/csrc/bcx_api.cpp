@@ -129,6 +129,7 @@ int bcx_index_stats_get(const bcx_index *ix, bcx_index_stats *st) {
   st->label_entries = x.entries();
   st->mean_depth = x.n ? (double)x.entries() / x.n : 0.0;
   st->build_seconds = x.build_seconds;
+  st->bag_entries = x.bag_off.empty() ? 0 : x.bag_off.back();
   return BCX_OK;
 }
 

@@ -68,6 +68,7 @@ struct DevTree {
   const int *bag_v;        // B: ancestor
   const int *bag_len;      // B: restricted shortcut length
   const uint64_t *bag_cnt; // B: restricted shortest-path count
+  const double *bag_cntf;  // B: the same count as a double
   const int *ibag_off;     // n + 1: entries whose ancestor is this node
   const int *ibag_x;       // B: the descendant owning the entry
   const int *ibag_e;       // B: the entry index
@@ -99,7 +100,7 @@ struct DevTree {
   const int4 *bs_x;           // descendants of inverse-bag entries 0..3
   const int4 *bs_e;           // their entry indices
   const int4 *bs_len;         // their shortcut lengths
-  const ulonglong2 *bs_cnt;   // 2 per node: counts of entries 0..1, 2..3
+  const double2 *bs_cntf;     // 2 per node: counts of entries 0..1, 2..3
 };
 
 constexpr int kStageEntries = 4;
@@ -192,8 +193,7 @@ struct bcx_ctx {
   ~bcx_ctx() {
     if (hipSetDevice(device) != hipSuccess) return;
     if (stream) (void)hipStreamSynchronize(stream);
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    free_ws();
+    free_ws();  // destroys graph_exec once and clears it
     free_sweep();
     for (void *p : owned) (void)hipFree(p);
     for (auto &e : ev)

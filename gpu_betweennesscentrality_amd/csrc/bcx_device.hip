@@ -556,6 +556,11 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     t.bag_v = (const int *)keep(dev_upload(ix.bag));
     t.bag_len = (const int *)keep(dev_upload(ix.bag_len));
     t.bag_cnt = (const uint64_t *)keep(dev_upload(ix.bag_cnt));
+    {
+      std::vector<double> cf((size_t)B);
+      for (int64_t e = 0; e < B; ++e) cf[(size_t)e] = (double)ix.bag_cnt[(size_t)e];
+      t.bag_cntf = (const double *)keep(dev_upload(cf));
+    }
     t.ibag_off = (const int *)keep(dev_upload(ioff));
     t.ibag_x = (const int *)keep(dev_upload(ix_x));
     t.ibag_e = (const int *)keep(dev_upload(ix_e));
@@ -639,7 +644,8 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       const size_t S = (size_t)st_off[(size_t)H];
       std::vector<int32_t> fnode(S * 4, 0), fu(S * 4, 0), flen(S * 4, 0), bnode(S * 4, 0),
           bx(S * 4, 0), be(S * 4, 0), blen(S * 4, 0);
-      std::vector<uint64_t> fcnt(S * 4, 0), bcnt(S * 4, 0);
+      std::vector<uint64_t> fcnt(S * 4, 0);
+      std::vector<double> bcnt(S * 4, 0.0);
       for (int32_t q = 0; q < n; ++q) {
         const int32_t v = lvl_v[(size_t)q];
         const size_t o = (size_t)slot_of[(size_t)q] * 4;
@@ -662,7 +668,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
           bx[o + r] = ix_x[k];
           be[o + r] = ix_e[k];
           blen[o + r] = ix.bag_len[(size_t)ix_e[k]];
-          bcnt[o + r] = ix.bag_cnt[(size_t)ix_e[k]];
+          bcnt[o + r] = (double)ix.bag_cnt[(size_t)ix_e[k]];
         }
       }
       t.st_off = (const int *)keep(dev_upload(st_off));
@@ -674,7 +680,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       t.bs_x = (const int4 *)keep(dev_upload(bx));
       t.bs_e = (const int4 *)keep(dev_upload(be));
       t.bs_len = (const int4 *)keep(dev_upload(blen));
-      t.bs_cnt = (const ulonglong2 *)keep(dev_upload(bcnt));
+      t.bs_cntf = (const double2 *)keep(dev_upload(bcnt));
     }
     c->tree = t;
     c->h_parent = ix.parent;

@@ -43,30 +43,26 @@ namespace {
 // Q = (1 + dD) / sigma, which its ancestors then read.
 constexpr int64_t kSlotOff = kWave * 4;
 constexpr int64_t kRec = kWave * (4 + 8);
-__device__ __forceinline__ int64_t lix_(int nn, int t, int v, int /*lane*/) {
-  return ((int64_t)t * nn + v) * kRec;
+__device__ __forceinline__ int &SD(char *row, int lane) { return ((int *)row)[lane]; }
+__device__ __forceinline__ uint64_t &SS(char *row, int lane) {
+  return ((uint64_t *)(row + kSlotOff))[lane];
 }
-__device__ __forceinline__ int &SD(const SweepWork &sw, int64_t r, int lane) {
-  return ((int *)(sw.state + r))[lane];
+__device__ __forceinline__ double getQ(char *row, int lane) {
+  return __builtin_bit_cast(double, SS(row, lane));
 }
-__device__ __forceinline__ uint64_t &SS(const SweepWork &sw, int64_t r, int lane) {
-  return ((uint64_t *)(sw.state + r + kSlotOff))[lane];
-}
-__device__ __forceinline__ double getQ(const SweepWork &sw, int64_t r, int lane) {
-  return __builtin_bit_cast(double, SS(sw, r, lane));
-}
-__device__ __forceinline__ void setQ(const SweepWork &sw, int64_t r, int lane, double q) {
-  SS(sw, r, lane) = __builtin_bit_cast(uint64_t, q);
+__device__ __forceinline__ void setQ(char *row, int lane, double q) {
+  SS(row, lane) = __builtin_bit_cast(uint64_t, q);
 }
 
 struct Lane {
   int t, lane, s;
   bool valid;
   int64_t rs;
+  char *tb;  // the tile's records (uniform)
 };
 
 __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &sw, int t,
-                                           int nsrc) {
+                                           int nsrc, int n) {
   Lane L;
   L.t = t;
   L.lane = threadIdx.x & (kWave - 1);
@@ -74,7 +70,14 @@ __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &
   L.valid = slot < nsrc;
   L.s = sw.tile_src[slot];  // padded with a valid source beyond nsrc
   L.rs = ix.row_off[L.s];
+  L.tb = sw.state + (int64_t)t * n * kRec;
   return L;
+}
+
+// Record of node v in the tile: a uniform address, so a row access is that
+// scalar base plus the lane's offset.
+__device__ __forceinline__ char *rowp(const Lane &L, int v) {
+  return L.tb + (uint64_t)(uint32_t)v * kRec;
 }
 
 // Lane moves of a double.  dpp_row: a DPP move inside rows of 16 lanes that
@@ -169,15 +172,15 @@ __device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, 
                                         int e, int e_end, int stride, int &best, uint64_t &c) {
   for (; e < e_end; e += kGroup * stride) {
     int cand[kGroup];
-    int64_t a[kGroup];
+    char *a[kGroup];
     int b = best;
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
       const int eq = e + q * stride;
       const bool ok = eq < e_end;
       const int ec = ok ? eq : e;
-      a[q] = lix_(tr.n, L.t, tr.bag_v[ec], L.lane);
-      const int dl = SD(sw, a[q], L.lane) + tr.bag_len[ec];  // loaded in every case
+      a[q] = rowp(L, tr.bag_v[ec]);
+      const int dl = SD(a[q], L.lane) + tr.bag_len[ec];  // loaded in every case
       cand[q] = ok ? dl : 0x7fffffff;
       b = min(b, cand[q]);
     }
@@ -185,7 +188,7 @@ __device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, 
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
       m[q] = 0;
-      if (cand[q] == b) m[q] = SS(sw, a[q], L.lane);
+      if (cand[q] == b) m[q] = SS(a[q], L.lane);
     }
     if (b < best) c = 0;
     best = b;
@@ -210,7 +213,7 @@ __device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork
   double acc = 0.0;
   for (; k < k_end; k += kGroup * stride) {
     int dx[kGroup], ee[kGroup], len[kGroup];
-    int64_t ax[kGroup];
+    char *ax[kGroup];
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
       const int kq = k + q * stride;
@@ -218,15 +221,15 @@ __device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork
       const int x = tr.ibag_x[kc];
       ee[q] = tr.ibag_e[kc];
       len[q] = tr.bag_len[ee[q]];
-      ax[q] = lix_(tr.n, L.t, x, L.lane);
-      dx[q] = SD(sw, ax[q], L.lane);
+      ax[q] = rowp(L, x);
+      dx[q] = SD(ax[q], L.lane);
     }
     double f[kGroup];
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
       double qx = 0.0;
-      if (k + q * stride < k_end && dv + len[q] == dx[q]) qx = getQ(sw, ax[q], L.lane);
-      f[q] = (double)tr.bag_cnt[ee[q]] * sv * qx;
+      if (k + q * stride < k_end && dv + len[q] == dx[q]) qx = getQ(ax[q], L.lane);
+      f[q] = tr.bag_cntf[ee[q]] * sv * qx;
       acc += f[q];
     }
 #pragma unroll
@@ -242,6 +245,12 @@ __device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork
 }
 
 __device__ __forceinline__ int rl(int x, int j) { return __builtin_amdgcn_readlane(x, j); }
+__device__ __forceinline__ double rlf(double x, int j) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, j);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), j);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 __device__ __forceinline__ uint64_t rl64(uint64_t x, int j) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, j);
   const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), j);
@@ -255,7 +264,8 @@ constexpr int kStage = kStageEntries;  // bag entries per node staged in registe
 // afterwards: a load under a condition becomes a branch that waits for it,
 // which serializes the loads of a step.
 
-constexpr int kNodes = 4;  // nodes of a level processed together by one wavefront
+// Nodes of a level processed together by one wavefront (forward kNodesF,
+// backward kNodesB) are template parameters of the tile kernel.
 
 // Level k's nodes are dealt round-robin to the 16 wavefronts (neighbouring
 // nodes share ancestors, so the wavefronts of a workgroup touch the same rows
@@ -270,11 +280,12 @@ __device__ __forceinline__ void level_block(const DevTree &tr, int k, int cnt, i
 
 // Forward over this wavefront's block of a level (all at tree depth `depth`).
 // Lane j stages node j's fields and first kStage bag entries with one round
-// of loads; the wavefront then handles kNodes nodes per step: all their
+// of loads; the wavefront then handles kNodesF nodes per step: all their
 // distance rows in flight together, then the path-count rows of the tight
 // candidates only.  Saturating sums of saturating products do not depend on
 // the order of their terms, so the count equals that of a running
 // min-and-count.
+template <int kNodesF>
 __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree &tr,
                                                const SweepWork &sw, const Lane &L, int wave,
                                                int cnt, int depth, int &ovf) {
@@ -289,17 +300,17 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
     const int sur[kStage] = {su.x, su.y, su.z, su.w};
     const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
     const uint64_t scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
-    for (int j0 = 0; j0 < jn; j0 += kNodes) {
-      int cand[kNodes][kStage], best[kNodes], upd[kNodes];
-      uint64_t upc[kNodes];
+    for (int j0 = 0; j0 < jn; j0 += kNodesF) {
+      int cand[kNodesF][kStage], best[kNodesF], upd[kNodesF];
+      uint64_t upc[kNodesF];
 #pragma unroll
-      for (int u = 0; u < kNodes; ++u) {
+      for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
         const int v = rl(nd.x, j);
         const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          const int dl = SD(sw, lix_(n, L.t, rl(sur[r], j), L.lane), L.lane) + rl(slr[r], j);
+          const int dl = SD(rowp(L, rl(sur[r], j)), L.lane) + rl(slr[r], j);
           cand[u][r] = r < nb ? dl : 0x7fffffff;
         }
         upd[u] = 0x7fffffff;
@@ -313,18 +324,18 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 #pragma unroll
         for (int r = 0; r < kStage; ++r) best[u] = min(best[u], cand[u][r]);
       }
-      uint64_t m[kNodes][kStage];
+      uint64_t m[kNodesF][kStage];
 #pragma unroll
-      for (int u = 0; u < kNodes; ++u) {
+      for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
           m[u][r] = 0;
-          if (cand[u][r] == best[u]) m[u][r] = SS(sw, lix_(n, L.t, rl(sur[r], j), L.lane), L.lane);
+          if (cand[u][r] == best[u]) m[u][r] = SS(rowp(L, rl(sur[r], j)), L.lane);
         }
       }
 #pragma unroll
-      for (int u = 0; u < kNodes; ++u) {
+      for (int u = 0; u < kNodesF; ++u) {
         const int j = j0 + u;
         if (j >= jn) break;
         const int v = rl(nd.x, j), e0 = rl(nd.y, j), e1 = rl(nd.z, j);
@@ -332,11 +343,14 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
         uint64_t c = upd[u] == b ? upc[u] : 0;
 #pragma unroll
         for (int r = 0; r < kStage; ++r)
-          if (cand[u][r] == b) c = add_sat(c, mul_sat(m[u][r], rl64(scr[r], j)));
+          if (cand[u][r] == b) {
+            const uint64_t cr = rl64(scr[r], j);  // uniform; 1 for most shortcuts
+            c = add_sat(c, cr == 1 ? m[u][r] : mul_sat(m[u][r], cr));
+          }
         if (e1 - e0 > kStage) fwd_bag(tr, sw, L, e0 + kStage, e1, 1, b, c);
-        const int64_t a = lix_(n, L.t, v, L.lane);
-        SD(sw, a, L.lane) = b;
-        SS(sw, a, L.lane) = c;
+        char *a = rowp(L, v);
+        SD(a, L.lane) = b;
+        SS(a, L.lane) = c;
         ovf |= L.valid && c == ~0ull;
       }
     }
@@ -344,11 +358,12 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 }
 
 // Backward over this wavefront's block of a level, with the same staging and
-// kNodes nodes per step.  The flows of a node's staged entries and the node
+// kNodesB nodes per step.  The flows of a node's staged entries and the node
 // flows of the step's nodes are each one four-way transposed lane sum.
+template <int kNodesB>
 __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWork &sw,
                                                const Lane &L, int wave, int cnt, int depth) {
-  static_assert(kNodes == 4 && kStage == 4, "four-way lane sums");
+  static_assert(kNodesB <= 4 && kStage == 4, "four-way lane sums");
   const int n = tr.n;
   int first, mine;
   level_block(tr, depth, cnt, wave, first, mine);
@@ -356,41 +371,41 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
     const int jn = min(kWave, mine - base);
     const int p = first + base + min(L.lane, jn - 1);
     const int4 nd = tr.bs_node[p], sx = tr.bs_x[p], se = tr.bs_e[p], sl = tr.bs_len[p];
-    const ulonglong2 c01 = tr.bs_cnt[2 * p], c23 = tr.bs_cnt[2 * p + 1];
+    const double2 c01 = tr.bs_cntf[2 * p], c23 = tr.bs_cntf[2 * p + 1];
     const int sxr[kStage] = {sx.x, sx.y, sx.z, sx.w};
     const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
-    const uint64_t scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
-    for (int j0 = 0; j0 < jn; j0 += kNodes) {
-      int dv[kNodes];
-      uint64_t svu[kNodes];
-      int dx[kNodes][kStage];
+    const double scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
+    for (int j0 = 0; j0 < jn; j0 += kNodesB) {
+      int dv[kNodesB];
+      uint64_t svu[kNodesB];
+      int dx[kNodesB][kStage];
 #pragma unroll
-      for (int u = 0; u < kNodes; ++u) {
+      for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
         const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
-        const int64_t av = lix_(n, L.t, rl(nd.x, j), L.lane);
-        dv[u] = SD(sw, av, L.lane);
-        svu[u] = SS(sw, av, L.lane);
+        char *av = rowp(L, rl(nd.x, j));
+        dv[u] = SD(av, L.lane);
+        svu[u] = SS(av, L.lane);
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          const int dl = SD(sw, lix_(n, L.t, rl(sxr[r], j), L.lane), L.lane);
+          const int dl = SD(rowp(L, rl(sxr[r], j)), L.lane);
           dx[u][r] = r < nb ? dl : -1;
         }
       }
-      double qx[kNodes][kStage];  // Q(x) in the lanes where entry r is tight
+      double qx[kNodesB][kStage];  // Q(x) in the lanes where entry r is tight
 #pragma unroll
-      for (int u = 0; u < kNodes; ++u) {
+      for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
           qx[u][r] = 0.0;
           if (dv[u] + rl(slr[r], j) == dx[u][r])
-            qx[u][r] = getQ(sw, lix_(n, L.t, rl(sxr[r], j), L.lane), L.lane);
+            qx[u][r] = getQ(rowp(L, rl(sxr[r], j)), L.lane);
         }
       }
-      double ns_in[kNodes];
+      double ns_in[kNodesB];
 #pragma unroll
-      for (int u = 0; u < kNodes; ++u) {
+      for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
         const bool live = j0 + u < jn;  // false for the clamped repeats of the last node
         const int v = rl(nd.x, j), k0 = rl(nd.y, j), k1 = rl(nd.z, j);
@@ -399,7 +414,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         double f[kStage], acc = 0.0;
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          f[r] = (double)rl64(scr[r], j) * svv * qx[u][r];
+          f[r] = rlf(scr[r], j) * svv * qx[u][r];
           acc += f[r];
         }
         const double fs = wave_sum4(L.valid ? f[0] : 0.0, L.valid ? f[1] : 0.0,
@@ -408,14 +423,16 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         if (live && (L.lane & 15) == 0 && (L.lane >> 4) < nb)
           sw.phi_t[(int64_t)L.t * tr.bags + e] = fs;
         if (live && k1 - k0 > kStage) acc += bwd_entries(tr, sw, L, k0 + kStage, k1, 1, dv[u], svv);
-        if (live) setQ(sw, lix_(n, L.t, v, L.lane), L.lane, (1.0 + acc) / svv);
+        if (live) setQ(rowp(L, v), L.lane, (1.0 + acc) / svv);
         ns_in[u] = (live && L.valid && L.s != v) ? acc : 0.0;
       }
-      const double ns = wave_sum4(ns_in[0], ns_in[1], ns_in[2], ns_in[3]);
+      const double ns = wave_sum4(ns_in[0], kNodesB > 1 ? ns_in[1 % kNodesB] : 0.0,
+                                  kNodesB > 2 ? ns_in[2 % kNodesB] : 0.0,
+                                  kNodesB > 3 ? ns_in[3 % kNodesB] : 0.0);
       const int u = L.lane >> 4;
       const int v = row_pick(L.lane, rl(nd.x, min(j0, jn - 1)), rl(nd.x, min(j0 + 1, jn - 1)),
                              rl(nd.x, min(j0 + 2, jn - 1)), rl(nd.x, min(j0 + 3, jn - 1)));
-      if ((L.lane & 15) == 0 && j0 + u < jn) sw.node_t[(int64_t)L.t * n + v] = ns;
+      if ((L.lane & 15) == 0 && u < kNodesB && j0 + u < jn) sw.node_t[(int64_t)L.t * n + v] = ns;
     }
   }
 }
@@ -425,6 +442,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
 // states.  Levels narrower than the workgroup's 16 wavefronts are processed
 // node by node with the node's entries split over all wavefronts and the
 // partial results merged in wavefront order, so results are deterministic.
+template <int kNodesF, int kNodesB>
 __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, DevTree tr,
                                                                    SweepWork sw, int nsrc,
                                                                    int *ovf_flag,
@@ -432,7 +450,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
   __shared__ double sh_acc[kSweepWaves][kWave];
   __shared__ int sh_best[kSweepWaves][kWave];
   __shared__ uint64_t sh_cnt[kSweepWaves][kWave];
-  const Lane L = lane_setup(ix, sw, blockIdx.x, nsrc);
+  const Lane L = lane_setup(ix, sw, blockIdx.x, nsrc, tr.n);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int n = tr.n, H = tr.height;
   int ovf = 0;
@@ -443,7 +461,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
   for (int k = 0; k < H; ++k) {
     const int lo = tr.lvl_off[k], cnt = tr.lvl_off[k + 1] - lo;
     if (cnt >= kSweepWaves) {
-      fwd_level_wave(ix, tr, sw, L, wave, cnt, k, ovf);
+      fwd_level_wave<kNodesF>(ix, tr, sw, L, wave, cnt, k, ovf);
     } else {
       for (int q = 0; q < cnt; ++q) {
         const int v = __builtin_amdgcn_readfirstlane(tr.lvl_v[lo + q]);
@@ -470,9 +488,9 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
               cc = add_sat(cc, sh_cnt[w][L.lane]);
             }
           }
-          const int64_t a = lix_(tr.n, L.t, v, L.lane);
-          SD(sw, a, L.lane) = b;
-          SS(sw, a, L.lane) = cc;
+          char *a = rowp(L, v);
+          SD(a, L.lane) = b;
+          SS(a, L.lane) = cc;
           ovf |= L.valid && cc == ~0ull;
         }
         __syncthreads();
@@ -487,19 +505,19 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
   for (int k = H - 1; k >= 0; --k) {
     const int lo = tr.lvl_off[k], cnt = tr.lvl_off[k + 1] - lo;
     if (cnt >= kSweepWaves) {
-      bwd_level_wave(tr, sw, L, wave, cnt, k);
+      bwd_level_wave<kNodesB>(tr, sw, L, wave, cnt, k);
     } else {
       for (int q = 0; q < cnt; ++q) {
         const int v = __builtin_amdgcn_readfirstlane(tr.lvl_v[lo + q]);
-        const int64_t av = lix_(tr.n, L.t, v, L.lane);
-        const double svv = (double)SS(sw, av, L.lane);
+        char *av = rowp(L, v);
+        const double svv = (double)SS(av, L.lane);
         sh_acc[wave][L.lane] = bwd_entries(tr, sw, L, tr.ibag_off[v] + wave, tr.ibag_off[v + 1],
-                                           kSweepWaves, SD(sw, av, L.lane), svv);
+                                           kSweepWaves, SD(av, L.lane), svv);
         __syncthreads();
         if (wave == 0) {
           double acc = 0.0;
           for (int w = 0; w < kSweepWaves; ++w) acc += sh_acc[w][L.lane];
-          setQ(sw, av, L.lane, (1.0 + acc) / svv);
+          setQ(av, L.lane, (1.0 + acc) / svv);
           const double ns = wave_sum_dpp((L.valid && L.s != v) ? acc : 0.0);
           if (L.lane == 0) sw.node_t[(int64_t)L.t * tr.n + v] = ns;
         }
@@ -529,8 +547,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
     for (int kk = c0; kk < c1; ++kk) {
       const int x = __builtin_amdgcn_readfirstlane(sw.cand[kk]);
       const bool anc = x <= L.s && L.s < x + tr.subtree[x];
-      const int64_t ax = lix_(n, L.t, x, L.lane);
-      const int dx = SD(sw, ax, L.lane);
+      char *ax = rowp(L, x);
+      const int dx = SD(ax, L.lane);
       uint64_t sa = 0;
       if (anc) {
         const int64_t o = L.rs + ix.depth[x];
@@ -542,13 +560,13 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
         const int e1 = tr.bag_off[x + 1];
         for (int e = tr.bag_off[x] + wave; e < e1; e += 4 * kSweepWaves) {
           int dy[4];
-          int64_t ay[4];
+          char *ay[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int eq = e + q * kSweepWaves;
             const int ec = eq < e1 ? eq : e;
-            ay[q] = lix_(n, L.t, tr.bag_v[ec], L.lane);
-            dy[q] = SD(sw, ay[q], L.lane);
+            ay[q] = rowp(L, tr.bag_v[ec]);
+            dy[q] = SD(ay[q], L.lane);
           }
           double g[4];
 #pragma unroll
@@ -557,7 +575,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
             const int ec = eq < e1 ? eq : e;
             g[q] = 0.0;
             if (eq < e1 && sa != 0 && dx + tr.bag_len[ec] == dy[q])
-              g[q] = (double)tr.bag_cnt[ec] * getQ(sw, ay[q], L.lane);  // c * QA(y)
+              g[q] = tr.bag_cntf[ec] * getQ(ay[q], L.lane);  // c * QA(y)
           }
 #pragma unroll
           for (int q = 0; q < 4; ++q) up += g[q];
@@ -572,8 +590,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
       if (wave == 0) {
         double u = 0.0;
         for (int w = 0; w < kSweepWaves; ++w) u += sh_acc[w][L.lane];
-        const double qx = getQ(sw, ax, L.lane);
-        setQ(sw, ax, L.lane, sa != 0 ? qx + u : 0.0);
+        const double qx = getQ(ax, L.lane);
+        setQ(ax, L.lane, sa != 0 ? qx + u : 0.0);
         const double ns = wave_sum_dpp((L.valid && x != L.s) ? sax * u : 0.0);
         if (L.lane == 0) sw.node_t[(int64_t)L.t * n + x] += ns;
       }
@@ -586,6 +604,22 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
 
 // node[v] += sum over tiles of node_t, phi[e] += sum over tiles of phi_t,
 // tiles in increasing order.
+using TileKernel = void (*)(DevIndex, DevTree, SweepWork, int, int *, uint64_t *);
+
+// Nodes per wavefront step (forward, backward): 2 and 2 by default, the
+// fastest on road graphs (more nodes per step cost scalar-register spills and
+// do not shorten the forward sweep, which is bandwidth-bound); the
+// environment variable BCX_SWEEP_NODES=42 selects 4 forward nodes per step.
+TileKernel tile_kernel() {
+  static TileKernel k = [] {
+    const char *e = getenv("BCX_SWEEP_NODES");
+    const std::string v = e ? e : "";
+    if (v == "42") return (TileKernel)sweep_tile_kernel<4, 2>;
+    return (TileKernel)sweep_tile_kernel<2, 2>;
+  }();
+  return k;
+}
+
 __global__ __launch_bounds__(256) void sweep_tile_reduce_kernel(int n, int64_t B, int tiles,
                                                                 SweepWork sw) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -760,8 +794,7 @@ void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_ino
     if (timed) HIP_OK(hipEventRecord(c->ev[0], c->stream));
     uint64_t *stamps = nullptr;
     if (timed) HIP_OK(hipMalloc(&stamps, (size_t)bt * 4 * 8));
-    sweep_tile_kernel<<<bt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf,
-                                                           stamps);
+    tile_kernel()<<<bt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf, stamps);
     if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
     if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
     sweep_tile_reduce_kernel<<<blocks_for(n + B, 256), 256, 0, c->stream>>>(n, B, bt, sw);

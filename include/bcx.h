@@ -67,6 +67,8 @@ typedef struct bcx_index_stats {
   int64_t label_entries;  /* sum over nodes of (depth + 1)                  */
   double mean_depth;      /* label_entries / n                              */
   double build_seconds;   /* elimination + tree + labels, wall clock        */
+  int64_t bag_entries;    /* sum over nodes of the bag size (shortcuts to
+                             ancestors): the bag-sweep's work per source    */
 } bcx_index_stats;
 
 typedef struct bcx_tuning {
