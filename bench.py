@@ -190,19 +190,17 @@ def main() -> int:
         return 0
 
     def time_mode(mode: int, label: str, size: int):
+        # One call per phase, as a full BC runs: the call launches one batch
+        # per step and finishes the job once (BC accumulation, and for the
+        # bag-sweep form the shortcut unpacking).
         ctx.set_tuning(batch=size)
-        for k in range(args.warmup):
-            ctx.bc_sources(tree_batch(k, total, size), mode)
-        bc = np.zeros(n)
-        ovf = False
-        done = 0
+        if args.warmup:
+            ctx.bc_sources(np.concatenate([tree_batch(k, total, size) for k in range(args.warmup)]), mode)
+        src = np.concatenate([tree_batch(k, total, size) for k in range(args.warmup, total)])
         t = time.perf_counter()
-        for k in range(args.warmup, total):
-            src = tree_batch(k, total, size)
-            bc, o = ctx.bc_sources(src, mode, bc)
-            done += len(src)
-            ovf |= o
+        bc, ovf = ctx.bc_sources(src, mode)
         el = time.perf_counter() - t
+        done = len(src)
         rate = done / el
         ctx.set_profiling(True)
         ctx.reset_timing()
@@ -248,7 +246,7 @@ def main() -> int:
 
     rq = None
     if not args.no_rq:
-        rq = _rq_bench(bcx, gg, ctx, g, args.rq_requests)
+        rq = _rq_bench(bcx, gg, ctx, g, args.rq_requests, Bs)
 
     cpu = None
     if not args.no_cpu:
@@ -298,9 +296,12 @@ def main() -> int:
     return 0
 
 
-def _rq_bench(bcx, gg, ctx, g, nreq: int) -> dict:
-    """Continuous-batching RQ serving: Poisson arrivals, admission by budget."""
+def _rq_bench(bcx, gg, ctx, g, nreq: int, batch: int) -> dict:
+    """Continuous-batching RQ serving on the bag-sweep form: Poisson arrivals,
+    admission by budget; every request rides the shared cyclic sweep for one
+    full cycle of batches."""
     n = g.n
+    ctx.set_tuning(batch=batch)
     row = np.zeros(n + 1, np.int64)
     np.add.at(row, g.u.astype(np.int64) + 1, 1)
     row = np.cumsum(row)
@@ -345,7 +346,8 @@ def _rq_bench(bcx, gg, ctx, g, nreq: int) -> dict:
            "p99_s": round(float(np.percentile(lat, 99)), 4),
            "p99_over_p50": round(float(np.percentile(lat, 99) / np.percentile(lat, 50)), 3),
            "requests_per_s": round(nreq / wall, 4), "queries_per_s": round(float(qs.sum()) / wall, 1),
-           "q_sizes": qs.tolist(), "batch_seconds": round(t_batch or 0.0, 4)}
+           "q_sizes": qs.tolist(), "batch_seconds": round(t_batch or 0.0, 4),
+           "form": "bag-sweep", "sources_per_batch": batch}
     log(f"[bench] rq: {res}")
     return res
 

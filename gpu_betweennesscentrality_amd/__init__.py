@@ -97,6 +97,7 @@ _SIGS = {
     "bcx_query_dist_count": (C.c_int, [_P, C.c_int32, _I32P, _U64P, _U32P]),
     "bcx_subset_bc": (C.c_int, [_P, _I32P, C.c_int32, _F64P]),
     "bcx_rq_create": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "bcx_rq_create_mode": (C.c_int, [_P, C.c_int, C.c_size_t, C.POINTER(_P)]),
     "bcx_rq_free": (None, [_P]),
     "bcx_rq_submit": (C.c_int, [_P, _I32P, C.c_int32, C.POINTER(C.c_int64)]),
     "bcx_rq_cancel": (C.c_int, [_P, C.c_int64]),
@@ -346,9 +347,9 @@ class Context:
 class RQServer:
     """Continuous-batching subset-BC engine over one Context (bcx_rq_*)."""
 
-    def __init__(self, ctx: Context, hbm_budget_bytes: int):
+    def __init__(self, ctx: Context, hbm_budget_bytes: int, mode: int = MODE_SWEEP):
         h = _P()
-        _check(lib().bcx_rq_create(ctx._h, hbm_budget_bytes, C.byref(h)))
+        _check(lib().bcx_rq_create_mode(ctx._h, mode, hbm_budget_bytes, C.byref(h)))
         self._h = h
         self._ctx = ctx
         self._q: dict[int, int] = {}

@@ -234,4 +234,19 @@ namespace bcx_dev {
 // Bag-sweep full BC over `count` sources (vertex ids), added into bc_inout.
 void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_inout,
                uint32_t *overflow);
+// Pieces of the bag sweep, for callers that keep flows per batch.
+// Tiles per launch for `count` sources (tuning batch, capped by device
+// memory); allocates the sweep buffers for that many tiles.
+int sweep_tiles_for(bcx_ctx *c, int32_t count);
+// One launch over the sorted tree ids pre[0 .. nsrc) (at most sweep_tiles
+// tiles of 64): node flows into node_out[n], shortcut flows into phi_out[B],
+// added when `accumulate`, else stored.  Sets ctx d_ovf on count overflow.
+void sweep_launch(bcx_ctx *c, const int32_t *pre, int32_t nsrc, double *node_out,
+                  double *phi_out, bool accumulate);
+// ctx sw.node / sw.phi = sum of nb per-batch slices ([nb][n], [nb][B]) in order.
+void sweep_sum_batches(bcx_ctx *c, const double *nodeb, const double *phi_b, int nb);
+// Unpack sw.node / sw.phi and add 0.5 * (node flow + interior flow) to bc_dev[n]
+// (tree ids).
+void sweep_finish(bcx_ctx *c, double *bc_dev);
+void sweep_zero(bcx_ctx *c, double *p, int64_t cnt);
 }  // namespace bcx_dev

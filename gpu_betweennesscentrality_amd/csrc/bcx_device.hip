@@ -820,7 +820,7 @@ int bcx_subset_bc(bcx_ctx *c, const int32_t *points, int32_t q, double *bc_out) 
     if (points[i] < 0 || points[i] >= c->ix.n) return bcx::fail(BCX_EINVAL, "query vertex out of range");
   std::vector<double> bc((size_t)c->ix.n);
   uint32_t ovf = 0;
-  int rc = bcx_full_bc(c, BCX_MODE_BATCHED, bc.data(), &ovf);
+  int rc = bcx_full_bc(c, BCX_MODE_SWEEP, bc.data(), &ovf);
   if (rc != BCX_OK) return rc;
   for (int32_t i = 0; i < q; ++i) bc_out[i] = bc[(size_t)points[i]];
   return ovf ? 1 : BCX_OK;
@@ -830,20 +830,25 @@ int bcx_subset_bc(bcx_ctx *c, const int32_t *points, int32_t q, double *bc_out) 
 }  // extern "C"
 
 // ---- RQ serving engine ------------------------------------------------------
-// One cyclic sweep over all sources (tree order, batches of tn.batch) serves
-// every admitted request: after each batch, each request gathers the batch's
-// dependency sums at its query vertices into its own [batch][q] table; after
-// one full cycle the table is summed in batch order 0..nb-1.  A request thus
-// sees exactly the contributions of the n sources computed while it was in
-// flight, and its result bits do not depend on its admission batch or on the
-// other requests.  The engine shares the context's workspace: calls on one
-// context must not overlap.
+// One cyclic pass over all sources (tree order, in batches) serves every
+// admitted request, and a request is answered after one full cycle of
+// batches computed while it was in flight.  Its result does not depend on its
+// admission batch or on the other requests:
+//   label-scan form: after each batch, each request gathers the batch's
+//     dependency sums at its query vertices into its own [batch][q] table;
+//     after the cycle the table is summed in batch order 0..nb-1;
+//   bag-sweep form: each batch's node and shortcut flows are kept in a
+//     per-batch slot shared by all requests; when a request has seen every
+//     batch, the slots are summed in batch order, unpacked once, and the
+//     request gathers its vertices (requests finishing together share it).
+// The engine shares the context's workspace: calls on one context must not
+// overlap.
 struct bcx_rq {
   struct Req {
     int64_t id;
     int32_t q;
     int *points = nullptr;      // preorder ids
-    double *contrib = nullptr;  // nb * q
+    double *contrib = nullptr;  // nb * q (label-scan form)
     double *out = nullptr;      // q
     int32_t done_batches = 0;
     bool finished = false;
@@ -851,18 +856,22 @@ struct bcx_rq {
     size_t bytes = 0;
   };
   bcx_ctx *ctx = nullptr;
+  int mode = BCX_MODE_SWEEP;
   size_t budget = 0, used = 0;
   int32_t batch = 0, nb = 0, cur = 0;
   int64_t next_id = 1;
   double *bsum = nullptr;
   int *d_srcs = nullptr, *d_cursor = nullptr, *d_total = nullptr;  // own sweep list
+  double *nodeb = nullptr, *phib = nullptr;  // bag-sweep form: [nb][n], [nb][B]
+  std::vector<int32_t> ident;                // bag-sweep form: tree ids 0 .. n-1
   std::vector<Req> reqs;
 
   ~bcx_rq() {
     if (!ctx || hipSetDevice(ctx->device) != hipSuccess) return;
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &r : reqs) release(r);
-    for (void *p : {(void *)bsum, (void *)d_srcs, (void *)d_cursor, (void *)d_total})
+    for (void *p : {(void *)bsum, (void *)d_srcs, (void *)d_cursor, (void *)d_total,
+                    (void *)nodeb, (void *)phib})
       if (p) (void)hipFree(p);
   }
   void release(Req &r) {
@@ -874,33 +883,55 @@ struct bcx_rq {
     used -= r.bytes;
     r.bytes = 0;
   }
-  size_t need(int32_t q) const { return (size_t)q * (4 + 8) + (size_t)nb * q * 8; }
+  size_t need(int32_t q) const {
+    return (size_t)q * (4 + 8) + (mode == BCX_MODE_SWEEP ? 0 : (size_t)nb * q * 8);
+  }
 };
 
 extern "C" {
 
-int bcx_rq_create(bcx_ctx *c, size_t hbm_budget_bytes, bcx_rq **out) {
+int bcx_rq_create_mode(bcx_ctx *c, int mode, size_t hbm_budget_bytes, bcx_rq **out) {
   BCX_GUARD_BEGIN
   if (!c || !out) return bcx::fail(BCX_EINVAL, "null argument");
+  if (mode != BCX_MODE_BATCHED && mode != BCX_MODE_SWEEP)
+    return bcx::fail(BCX_EINVAL, "RQ mode must be BCX_MODE_BATCHED or BCX_MODE_SWEEP");
   HIP_OK(hipSetDevice(c->device));
   std::unique_ptr<bcx_rq> r(new bcx_rq());
   r->ctx = c;
+  r->mode = mode;
   r->budget = hbm_budget_bytes;
-  r->batch = std::max(1, std::min(c->tn.batch, c->ix.n));
-  r->nb = (c->ix.n + r->batch - 1) / r->batch;
+  const int32_t n = c->ix.n;
+  if (mode == BCX_MODE_SWEEP) {
+    r->batch = bcx_dev::sweep_tiles_for(c, n) * bcx_dev::kWave;
+    r->nb = (n + r->batch - 1) / r->batch;
+    const size_t B = (size_t)std::max<int64_t>(c->tree.bags, 1);
+    HIP_OK(hipMalloc(&r->nodeb, (size_t)r->nb * n * 8));
+    HIP_OK(hipMalloc(&r->phib, (size_t)r->nb * B * 8));
+    HIP_OK(hipMalloc(&r->bsum, (size_t)n * 8));
+    r->ident.resize((size_t)n);
+    for (int32_t i = 0; i < n; ++i) r->ident[(size_t)i] = i;
+    *out = r.release();
+    return BCX_OK;
+  }
+  r->batch = std::max(1, std::min(c->tn.batch, n));
+  r->nb = (n + r->batch - 1) / r->batch;
   c->ensure_ws(r->batch);
-  HIP_OK(hipMalloc(&r->bsum, (size_t)c->ix.n * 8));
+  HIP_OK(hipMalloc(&r->bsum, (size_t)n * 8));
   // The sweep's source list is the identity over tree order.
-  std::vector<int32_t> ident((size_t)c->ix.n);
-  for (int32_t i = 0; i < c->ix.n; ++i) ident[(size_t)i] = i;
+  std::vector<int32_t> ident((size_t)n);
+  for (int32_t i = 0; i < n; ++i) ident[(size_t)i] = i;
   HIP_OK(hipMalloc(&r->d_srcs, ident.size() * 4));
   HIP_OK(hipMalloc(&r->d_cursor, 16));
   HIP_OK(hipMalloc(&r->d_total, 16));
   HIP_OK(hipMemcpy(r->d_srcs, ident.data(), ident.size() * 4, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(r->d_total, &c->ix.n, 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(r->d_total, &n, 4, hipMemcpyHostToDevice));
   *out = r.release();
   return BCX_OK;
   BCX_GUARD_END
+}
+
+int bcx_rq_create(bcx_ctx *c, size_t hbm_budget_bytes, bcx_rq **out) {
+  return bcx_rq_create_mode(c, BCX_MODE_SWEEP, hbm_budget_bytes, out);
 }
 
 void bcx_rq_free(bcx_rq *rq) { delete rq; }
@@ -929,7 +960,7 @@ int bcx_rq_submit(bcx_rq *rq, const int32_t *points, int32_t q, int64_t *id) {
   r.q = q;
   try {
     HIP_OK(hipMalloc(&r.points, (size_t)q * 4));
-    HIP_OK(hipMalloc(&r.contrib, (size_t)rq->nb * q * 8));
+    if (rq->mode != BCX_MODE_SWEEP) HIP_OK(hipMalloc(&r.contrib, (size_t)rq->nb * q * 8));
     HIP_OK(hipMalloc(&r.out, (size_t)q * 8));
     HIP_OK(hipMemcpyAsync(r.points, pre.data(), (size_t)q * 4, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
@@ -970,7 +1001,36 @@ int bcx_rq_step(bcx_rq *rq, int32_t *active) {
   int32_t live = 0;
   for (auto &r : rq->reqs)
     if (!r.finished) ++live;
-  if (live > 0) {
+  if (live > 0 && rq->mode == BCX_MODE_SWEEP) {
+    const int32_t n = c->ix.n;
+    const int64_t B = c->tree.bags;
+    const int32_t base = rq->cur * rq->batch;
+    const int32_t cnt = std::min(rq->batch, n - base);
+    bcx_dev::sweep_launch(c, rq->ident.data() + base, cnt, rq->nodeb + (int64_t)rq->cur * n,
+                          rq->phib + (int64_t)rq->cur * std::max<int64_t>(B, 1), false);
+    bool done_any = false;
+    for (auto &r : rq->reqs)
+      if (!r.finished && ++r.done_batches == rq->nb) done_any = true;
+    if (done_any) {
+      bcx_dev::sweep_sum_batches(c, rq->nodeb, rq->phib, rq->nb);
+      bcx_dev::sweep_zero(c, rq->bsum, n);
+      bcx_dev::sweep_finish(c, rq->bsum);
+      for (auto &r : rq->reqs) {
+        if (r.finished || r.done_batches != rq->nb) continue;
+        gather_kernel<<<(r.q + 255) / 256, 256, 0, c->stream>>>(rq->bsum, r.points, r.q, r.out);
+        r.result.resize((size_t)r.q);
+        HIP_OK(hipMemcpyAsync(r.result.data(), r.out, (size_t)r.q * 8, hipMemcpyDeviceToHost,
+                              c->stream));
+        r.finished = true;
+      }
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->timing.sources += cnt;
+    rq->cur = (rq->cur + 1) % rq->nb;
+    for (auto &r : rq->reqs)
+      if (r.finished && r.bytes) rq->release(r);
+  } else if (live > 0) {
     const int32_t base = rq->cur * rq->batch;
     HIP_OK(hipMemcpyAsync(rq->d_cursor, &base, 4, hipMemcpyHostToDevice, c->stream));
     const DevBatch bt{rq->d_srcs, rq->d_cursor, rq->d_total};

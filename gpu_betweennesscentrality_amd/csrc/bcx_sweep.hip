@@ -289,7 +289,6 @@ template <int kNodesF>
 __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree &tr,
                                                const SweepWork &sw, const Lane &L, int wave,
                                                int cnt, int depth, int &ovf) {
-  const int n = tr.n;
   int first, mine;
   level_block(tr, depth, cnt, wave, first, mine);
   for (int base = 0; base < mine; base += kWave) {
@@ -621,48 +620,114 @@ TileKernel tile_kernel() {
 }
 
 __global__ __launch_bounds__(256) void sweep_tile_reduce_kernel(int n, int64_t B, int tiles,
-                                                                SweepWork sw) {
+                                                                SweepWork sw, double *node_out,
+                                                                double *phi_out, int accumulate) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    double acc = sw.node[i];
+    double acc = accumulate ? node_out[i] : 0.0;
     for (int t = 0; t < tiles; ++t) acc += sw.node_t[(int64_t)t * n + i];
+    node_out[i] = acc;
+  } else if (i < n + B) {
+    const int64_t e = i - n;
+    double acc = accumulate ? phi_out[e] : 0.0;
+    for (int t = 0; t < tiles; ++t) acc += sw.phi_t[(int64_t)t * B + e];
+    phi_out[e] = acc;
+  }
+}
+
+// sw.node / sw.phi = the sum over nb per-batch slices, in slice order.
+__global__ __launch_bounds__(256) void sweep_sum_batches_kernel(int n, int64_t B, int nb,
+                                                                const double *nodeb,
+                                                                const double *phib, SweepWork sw) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) acc += nodeb[(int64_t)b * n + i];
     sw.node[i] = acc;
   } else if (i < n + B) {
     const int64_t e = i - n;
-    double acc = sw.phi[e];
-    for (int t = 0; t < tiles; ++t) acc += sw.phi_t[(int64_t)t * B + e];
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) acc += phib[(int64_t)b * B + e];
     sw.phi[e] = acc;
   }
 }
 
-// Unpack one depth level, top-down.  For node z and each tight pair (a, b) of
-// its bag (a path a-z-b as short as shortcut (a, b)), z carries the share
-// c(a,z) c(z,b) / c(a,b) of that shortcut's flow, and the same amount moves
-// onto z's shortcuts (a, z) and (z, b) for deeper levels: the
-// restricted-path counting of Reduce (TreeDecomp.cpp:157-193), reversed.
-// Step 1: the share of every tight pair of the level.
-__global__ __launch_bounds__(256) void sweep_pair_flow_kernel(DevTree tr, SweepWork sw,
-                                                              int q_lo, int q_cnt) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= q_cnt) return;
-  const int q = q_lo + k;
-  sw.pf[q] = sw.phi[tr.tp_ab[q]] * tr.tp_frac[q];
-}
+// Unpacking of the shortcut flows, top-down over the depth levels.  For node
+// z and each tight pair (a, b) of its bag (a path a-z-b as short as shortcut
+// (a, b)), z carries the share c(a,z) c(z,b) / c(a,b) of that shortcut's flow,
+// and the same amount moves onto z's shortcuts (a, z) and (z, b) for deeper
+// levels: the restricted-path counting of Reduce (TreeDecomp.cpp:157-193),
+// reversed.  Per level, step 1 computes the share of every tight pair of the
+// level, step 2 gives each node of the level its interior flow and adds to
+// its entries' flows, each sum over pairs in increasing pair order.
+//
+// Wide levels (at least kUnpackWide nodes) take two grid-wide launches, one
+// thread per pair and one thread per node.  Runs of narrower levels go to one
+// resident workgroup that walks them with a barrier after each step, so the
+// long chain of small top levels costs no launches; there a node gets a
+// thread on levels of at least kWave nodes and a whole wavefront otherwise
+// (the large bags near the root carry most pairs; the lanes take the pairs in
+// strides and a fixed lane tree sums them).  Every sum has a fixed order, so
+// results are deterministic.
+constexpr int kUnpackThreads = 1024;
+constexpr int kUnpackWide = 1024;
 
-// Step 2: per node of the level, its interior flow and its entries' new flow,
-// summed over pairs in increasing pair order.
-__global__ __launch_bounds__(256) void sweep_pair_apply_kernel(DevTree tr, SweepWork sw,
-                                                               int lvl_lo, int lvl_cnt) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= lvl_cnt) return;
-  const int z = tr.lvl_v[lvl_lo + k];
+__device__ __forceinline__ void unpack_node_serial(const DevTree &tr, const SweepWork &sw, int z) {
   double acc = 0.0;
-  for (int i = tr.zp_off[z]; i < tr.zp_off[z + 1]; ++i) acc += sw.pf[tr.zp_pair[i]];
+  for (int p = tr.zp_off[z]; p < tr.zp_off[z + 1]; ++p) acc += sw.pf[tr.zp_pair[p]];
   sw.interior[z] = acc;
   for (int e = tr.bag_off[z]; e < tr.bag_off[z + 1]; ++e) {
-    double f = 0.0;
-    for (int i = tr.ep_off[e]; i < tr.ep_off[e + 1]; ++i) f += sw.pf[tr.ep_pair[i]];
-    if (tr.ep_off[e + 1] > tr.ep_off[e]) sw.phi[e] += f;
+    const int a0 = tr.ep_off[e], a1 = tr.ep_off[e + 1];
+    if (a1 > a0) {
+      double f = 0.0;
+      for (int a = a0; a < a1; ++a) f += sw.pf[tr.ep_pair[a]];
+      sw.phi[e] += f;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void sweep_pair_flow_kernel(DevTree tr, SweepWork sw, int k) {
+  const int q = tr.lvl_pair_off[k] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < tr.lvl_pair_off[k + 1]) sw.pf[q] = sw.phi[tr.tp_ab[q]] * tr.tp_frac[q];
+}
+
+__global__ __launch_bounds__(256) void sweep_pair_apply_kernel(DevTree tr, SweepWork sw, int k) {
+  const int i = tr.lvl_off[k] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < tr.lvl_off[k + 1]) unpack_node_serial(tr, sw, tr.lvl_v[i]);
+}
+
+__global__ __launch_bounds__(kUnpackThreads) void sweep_unpack_kernel(DevTree tr, SweepWork sw,
+                                                                      int k0, int k1) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+  constexpr int kWaves = kUnpackThreads / kWave;
+  for (int k = k0; k < k1; ++k) {
+    const int q0 = tr.lvl_pair_off[k], q1 = tr.lvl_pair_off[k + 1];
+    for (int q = q0 + tid; q < q1; q += kUnpackThreads)
+      sw.pf[q] = sw.phi[tr.tp_ab[q]] * tr.tp_frac[q];
+    __syncthreads();
+    const int lo = tr.lvl_off[k], hi = tr.lvl_off[k + 1];
+    if (hi - lo >= kWave) {
+      for (int i = lo + tid; i < hi; i += kUnpackThreads) unpack_node_serial(tr, sw, tr.lvl_v[i]);
+    } else {
+      for (int i = lo + wave; i < hi; i += kWaves) {
+        const int z = __builtin_amdgcn_readfirstlane(tr.lvl_v[i]);
+        const int p0 = tr.zp_off[z], p1 = tr.zp_off[z + 1];
+        double acc = 0.0;
+        for (int p = p0 + lane; p < p1; p += kWave) acc += sw.pf[tr.zp_pair[p]];
+        acc = wave_sum_dpp(acc);
+        if (lane == 0) sw.interior[z] = acc;
+        for (int e = tr.bag_off[z] + lane; e < tr.bag_off[z + 1]; e += kWave) {
+          const int a0 = tr.ep_off[e], a1 = tr.ep_off[e + 1];
+          if (a1 > a0) {
+            double f = 0.0;
+            for (int a = a0; a < a1; ++a) f += sw.pf[tr.ep_pair[a]];
+            sw.phi[e] += f;
+          }
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -713,133 +778,168 @@ void bcx_ctx::ensure_sweep(int tiles) {
 
 namespace bcx_dev {
 
+int sweep_tiles_for(bcx_ctx *c, int32_t count) {
+  const int n = c->ix.n;
+  const int64_t B = c->tree.bags;
+  const int total_tiles = (count + kWave - 1) / kWave;
+  int tiles = std::max(1, std::min(c->tn.batch / kWave, std::max(1, total_tiles)));
+  // Keep the per-tile state within ~70% of the device memory not already held
+  // by this context's sweep buffers.
+  size_t fr = 0, tot = 0;
+  HIP_OK(hipMemGetInfo(&fr, &tot));
+  const size_t per_tile = (size_t)n * kRec + (size_t)n * 8 + (size_t)B * 8;
+  const size_t held = (size_t)c->sweep_tiles * per_tile;
+  const size_t cap = (size_t)((double)(fr + held) * 0.7) / std::max<size_t>(per_tile, 1);
+  if (cap < 1) throw bcx::Error{BCX_ENOMEM, "not enough device memory for one sweep tile"};
+  tiles = (int)std::min<size_t>((size_t)tiles, cap);
+  c->ensure_sweep(tiles);
+  return tiles;
+}
+
+void sweep_launch(bcx_ctx *c, const int32_t *pre, int32_t nsrc, double *node_out,
+                  double *phi_out, bool accumulate) {
+  const int n = c->ix.n;
+  const int64_t B = c->tree.bags;
+  const int bt = (nsrc + kWave - 1) / kWave;
+  if (bt < 1 || bt > c->sweep_tiles) throw bcx::Error{BCX_EINVAL, "sweep launch exceeds its tiles"};
+  SweepWork sw = c->sw;
+  const auto &par = c->h_parent;
+  const auto &sub = c->h_subtree;
+  // Tile sources (padded with the tile's first source) and the union of their
+  // root paths in an order where every ancestor comes first.
+  std::vector<int32_t> tsrc((size_t)bt * kWave, 0), cand, cand_off((size_t)bt + 1, 0), path;
+  for (int t = 0; t < bt; ++t) {
+    const int t0 = t * kWave;
+    const int tn = std::min(kWave, nsrc - t0);
+    for (int j = 0; j < kWave; ++j) tsrc[(size_t)t * kWave + j] = pre[(size_t)(t0 + (j < tn ? j : 0))];
+    int prev = -1;
+    for (int j = 0; j < tn; ++j) {
+      const int s = pre[(size_t)(t0 + j)];
+      path.clear();
+      for (int x = s; x >= 0; x = par[(size_t)x]) {
+        if (prev >= 0 && x <= prev && prev < x + sub[(size_t)x]) break;  // on prev's path
+        path.push_back(x);
+      }
+      for (size_t q = path.size(); q-- > 0;) cand.push_back(path[q]);
+      prev = s;
+    }
+    cand_off[(size_t)t + 1] = (int32_t)cand.size();
+  }
+  if ((int64_t)cand.size() > c->sw.cand_cap) {
+    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipFree(c->sw.cand));
+    c->sw.cand = nullptr;
+    HIP_OK(hipMalloc(&c->sw.cand, cand.size() * 4));
+    c->sw.cand_cap = (int)cand.size();
+    sw.cand = c->sw.cand;
+  }
+  HIP_OK(hipMemcpyAsync(sw.tile_src, tsrc.data(), tsrc.size() * 4, hipMemcpyHostToDevice,
+                        c->stream));
+  HIP_OK(hipMemcpyAsync(sw.cand_off, cand_off.data(), cand_off.size() * 4,
+                        hipMemcpyHostToDevice, c->stream));
+  if (!cand.empty())
+    HIP_OK(hipMemcpyAsync(sw.cand, cand.data(), cand.size() * 4, hipMemcpyHostToDevice,
+                          c->stream));
+  const bool timed = c->profiling;
+  if (timed) HIP_OK(hipEventRecord(c->ev[0], c->stream));
+  uint64_t *stamps = nullptr;
+  if (timed) HIP_OK(hipMalloc(&stamps, (size_t)bt * 4 * 8));
+  tile_kernel()<<<bt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf, stamps);
+  if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
+  if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
+  sweep_tile_reduce_kernel<<<blocks_for(n + B, 256), 256, 0, c->stream>>>(
+      n, B, bt, sw, node_out, phi_out, accumulate ? 1 : 0);
+  HIP_OK(hipGetLastError());
+  if (timed) {
+    HIP_OK(hipEventRecord(c->ev[3], c->stream));
+    HIP_OK(hipEventSynchronize(c->ev[3]));
+    float a = 0, b = 0, r = 0;
+    HIP_OK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIP_OK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    HIP_OK(hipEventElapsedTime(&r, c->ev[2], c->ev[3]));
+    c->timing.scan_ms += a;
+    c->timing.dep_ms += b;
+    c->timing.reduce_ms += r;
+    std::vector<uint64_t> st((size_t)bt * 4);
+    HIP_OK(hipMemcpy(st.data(), stamps, st.size() * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipFree(stamps));
+    double f = 0, b2 = 0, as = 0;
+    for (int t = 0; t < bt; ++t) {
+      f += (double)(st[4 * t + 1] - st[4 * t]);
+      b2 += (double)(st[4 * t + 2] - st[4 * t + 1]);
+      as += (double)(st[4 * t + 3] - st[4 * t + 2]);
+    }
+    c->timing.sweep_fwd_ms += f / bt * 1e-5;  // 100 MHz ticks
+    c->timing.sweep_bwd_ms += b2 / bt * 1e-5;
+    c->timing.sweep_asc_ms += as / bt * 1e-5;
+  }
+  c->timing.scan_launches += 1;
+  c->timing.dep_launches += 1;
+  // The host vectors above go out of scope: wait for the copies.
+  HIP_OK(hipStreamSynchronize(c->stream));
+}
+
+void sweep_sum_batches(bcx_ctx *c, const double *nodeb, const double *phib, int nb) {
+  const int n = c->ix.n;
+  const int64_t B = c->tree.bags;
+  sweep_sum_batches_kernel<<<blocks_for(n + B, 256), 256, 0, c->stream>>>(n, B, nb, nodeb, phib,
+                                                                          c->sw);
+  HIP_OK(hipGetLastError());
+}
+
+void sweep_finish(bcx_ctx *c, double *bc_dev) {
+  const int n = c->ix.n;
+  const SweepWork sw = c->sw;
+  // Unpack: runs of narrow levels in one workgroup, wide levels grid-wide.
+  const auto &lvl = c->h_lvl_off;
+  const auto &lpo = c->h_lvl_pair_off;
+  const int H = c->tree.height;
+  for (int k = 0; k < H;) {
+    const int cnt = lvl[(size_t)k + 1] - lvl[(size_t)k];
+    if (cnt >= kUnpackWide) {
+      const int qcnt = lpo[(size_t)k + 1] - lpo[(size_t)k];
+      if (qcnt > 0)
+        sweep_pair_flow_kernel<<<blocks_for(qcnt, 256), 256, 0, c->stream>>>(c->tree, sw, k);
+      sweep_pair_apply_kernel<<<blocks_for(cnt, 256), 256, 0, c->stream>>>(c->tree, sw, k);
+      ++k;
+      continue;
+    }
+    int k1 = k + 1;
+    while (k1 < H && lvl[(size_t)k1 + 1] - lvl[(size_t)k1] < kUnpackWide) ++k1;
+    sweep_unpack_kernel<<<1, kUnpackThreads, 0, c->stream>>>(c->tree, sw, k, k1);
+    k = k1;
+  }
+  sweep_combine_kernel<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, sw, bc_dev);
+  HIP_OK(hipGetLastError());
+}
+
+void sweep_zero(bcx_ctx *c, double *p, int64_t cnt) {
+  if (cnt > 0) zero_f64_kernel<<<blocks_for(cnt, 256), 256, 0, c->stream>>>(p, cnt);
+  HIP_OK(hipGetLastError());
+}
+
 void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_inout,
                uint32_t *overflow) {
   const int n = c->ix.n;
   const int64_t B = c->tree.bags;
-  const int H = c->tree.height;
   std::vector<int32_t> pre((size_t)count);
   for (int32_t k = 0; k < count; ++k) {
     if (sources[k] < 0 || sources[k] >= n) throw bcx::Error{BCX_EINVAL, "source out of range"};
     pre[(size_t)k] = c->iperm[(size_t)sources[k]];
   }
   std::sort(pre.begin(), pre.end());
-  const int total_tiles = (count + kWave - 1) / kWave;
-  int tiles_per_batch = std::max(1, std::min(c->tn.batch / kWave, std::max(1, total_tiles)));
-  {
-    // Keep the per-tile state within ~70% of the device memory not already
-    // held by this context's sweep buffers.
-    size_t fr = 0, tot = 0;
-    HIP_OK(hipMemGetInfo(&fr, &tot));
-    const size_t per_tile = (size_t)n * kRec + (size_t)n * 8 + (size_t)B * 8;
-    const size_t held = (size_t)c->sweep_tiles * per_tile;
-    const size_t cap = (size_t)((double)(fr + held) * 0.7) / std::max<size_t>(per_tile, 1);
-    if (cap < 1) throw bcx::Error{BCX_ENOMEM, "not enough device memory for one sweep tile"};
-    tiles_per_batch = (int)std::min<size_t>((size_t)tiles_per_batch, cap);
-  }
-  c->ensure_sweep(tiles_per_batch);
-  SweepWork sw = c->sw;
+  const int tiles = sweep_tiles_for(c, count);
   std::vector<double> bc_pre((size_t)n);
   for (int32_t v = 0; v < n; ++v) bc_pre[(size_t)c->iperm[(size_t)v]] = bc_inout[v];
   const int zero = 0;
   HIP_OK(hipMemcpyAsync(c->d_ovf, &zero, 4, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(c->d_bc, bc_pre.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
-  zero_f64_kernel<<<blocks_for(n, 256), 256, 0, c->stream>>>(sw.node, n);
-  if (B > 0) zero_f64_kernel<<<blocks_for(B, 256), 256, 0, c->stream>>>(sw.phi, B);
-  const auto &lvl = c->h_lvl_off;
-  const auto &par = c->h_parent;
-  const auto &sub = c->h_subtree;
-  std::vector<int32_t> tsrc, cand, cand_off, path;
-  const bool timed = c->profiling;
-  for (int tb = 0; tb < total_tiles; tb += tiles_per_batch) {
-    const int bt = std::min(tiles_per_batch, total_tiles - tb);
-    const int first = tb * kWave;
-    const int nsrc = std::min(bt * kWave, count - first);
-    // Tile sources (padded with the tile's first source) and the union of
-    // their root paths in an order where every ancestor comes first.
-    tsrc.assign((size_t)bt * kWave, 0);
-    cand.clear();
-    cand_off.assign((size_t)bt + 1, 0);
-    for (int t = 0; t < bt; ++t) {
-      const int t0 = first + t * kWave;
-      const int tn = std::min(kWave, count - t0);
-      for (int j = 0; j < kWave; ++j) tsrc[(size_t)t * kWave + j] = pre[(size_t)(t0 + (j < tn ? j : 0))];
-      int prev = -1;
-      for (int j = 0; j < tn; ++j) {
-        const int s = pre[(size_t)(t0 + j)];
-        path.clear();
-        for (int x = s; x >= 0; x = par[(size_t)x]) {
-          if (prev >= 0 && x <= prev && prev < x + sub[(size_t)x]) break;  // on prev's path
-          path.push_back(x);
-        }
-        for (size_t q = path.size(); q-- > 0;) cand.push_back(path[q]);
-        prev = s;
-      }
-      cand_off[(size_t)t + 1] = (int32_t)cand.size();
-    }
-    if ((int64_t)cand.size() > c->sw.cand_cap) {
-      HIP_OK(hipStreamSynchronize(c->stream));
-      HIP_OK(hipFree(c->sw.cand));
-      c->sw.cand_cap = (int)cand.size();
-      HIP_OK(hipMalloc(&c->sw.cand, cand.size() * 4));
-      sw.cand = c->sw.cand;
-    }
-    HIP_OK(hipMemcpyAsync(sw.tile_src, tsrc.data(), tsrc.size() * 4, hipMemcpyHostToDevice,
-                          c->stream));
-    HIP_OK(hipMemcpyAsync(sw.cand_off, cand_off.data(), cand_off.size() * 4,
-                          hipMemcpyHostToDevice, c->stream));
-    if (!cand.empty())
-      HIP_OK(hipMemcpyAsync(sw.cand, cand.data(), cand.size() * 4, hipMemcpyHostToDevice,
-                            c->stream));
-    if (timed) HIP_OK(hipEventRecord(c->ev[0], c->stream));
-    uint64_t *stamps = nullptr;
-    if (timed) HIP_OK(hipMalloc(&stamps, (size_t)bt * 4 * 8));
-    tile_kernel()<<<bt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf, stamps);
-    if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
-    if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
-    sweep_tile_reduce_kernel<<<blocks_for(n + B, 256), 256, 0, c->stream>>>(n, B, bt, sw);
-    HIP_OK(hipGetLastError());
-    if (timed) {
-      HIP_OK(hipEventRecord(c->ev[3], c->stream));
-      HIP_OK(hipEventSynchronize(c->ev[3]));
-      float a = 0, b = 0, r = 0;
-      HIP_OK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-      HIP_OK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
-      HIP_OK(hipEventElapsedTime(&r, c->ev[2], c->ev[3]));
-      c->timing.scan_ms += a;
-      c->timing.dep_ms += b;
-      c->timing.reduce_ms += r;
-      std::vector<uint64_t> st((size_t)bt * 4);
-      HIP_OK(hipMemcpy(st.data(), stamps, st.size() * 8, hipMemcpyDeviceToHost));
-      HIP_OK(hipFree(stamps));
-      double f = 0, b2 = 0, as = 0;
-      for (int t = 0; t < bt; ++t) {
-        f += (double)(st[4 * t + 1] - st[4 * t]);
-        b2 += (double)(st[4 * t + 2] - st[4 * t + 1]);
-        as += (double)(st[4 * t + 3] - st[4 * t + 2]);
-      }
-      c->timing.sweep_fwd_ms += f / bt * 1e-5;  // 100 MHz ticks
-      c->timing.sweep_bwd_ms += b2 / bt * 1e-5;
-      c->timing.sweep_asc_ms += as / bt * 1e-5;
-    }
-    c->timing.scan_launches += 1;
-    c->timing.dep_launches += 1;
-    // The host vectors are reused by the next batch: wait for the copies.
-    HIP_OK(hipStreamSynchronize(c->stream));
-  }
-  const auto &lpo = c->h_lvl_pair_off;
-  for (int k = 0; k < H; ++k) {
-    const int lo = lvl[(size_t)k], cnt = lvl[(size_t)k + 1] - lo;
-    const int qlo = lpo[(size_t)k], qcnt = lpo[(size_t)k + 1] - qlo;
-    if (qcnt == 0) {
-      sweep_pair_apply_kernel<<<blocks_for(cnt, 256), 256, 0, c->stream>>>(c->tree, sw, lo, cnt);
-      continue;
-    }
-    sweep_pair_flow_kernel<<<blocks_for(qcnt, 256), 256, 0, c->stream>>>(c->tree, sw, qlo, qcnt);
-    sweep_pair_apply_kernel<<<blocks_for(cnt, 256), 256, 0, c->stream>>>(c->tree, sw, lo, cnt);
-  }
-  sweep_combine_kernel<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, sw, c->d_bc);
-  HIP_OK(hipGetLastError());
+  sweep_zero(c, c->sw.node, n);
+  sweep_zero(c, c->sw.phi, B);
+  for (int32_t first = 0; first < count; first += tiles * kWave)
+    sweep_launch(c, pre.data() + first, std::min(tiles * kWave, count - first), c->sw.node,
+                 c->sw.phi, true);
+  sweep_finish(c, c->d_bc);
   c->timing.sources += count;
   int ovf = 0;
   HIP_OK(hipMemcpyAsync(bc_pre.data(), c->d_bc, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));

@@ -181,8 +181,16 @@ int bcx_subset_bc(bcx_ctx *ctx, const int32_t *points, int32_t q,
 /* Continuous-batching subset-BC server on one context: requests are admitted
  * against a device-memory budget and ride one cyclic source sweep; each
  * completes after a full cycle and its result is summed in a fixed order, so
- * it is bitwise the same alone or co-batched.  Driven by bcx_rq_step(). */
+ * it is bitwise the same alone or co-batched.  Driven by bcx_rq_step().
+ * Replaces the reference's one-shot brandesBCIndexGPU_PointBatch
+ * (RQ.cu:465-531): the result equals its BC_Kernel_merge sum (RQ.cu:451-459).
+ * bcx_rq_create_mode picks the form of the cyclic sweep: BCX_MODE_SWEEP (per
+ * batch node and shortcut flows kept in shared slots; a finished request is
+ * unpacked once) or BCX_MODE_BATCHED (label scan; per request a [batch][q]
+ * table).  bcx_rq_create is bcx_rq_create_mode with BCX_MODE_SWEEP. */
 int bcx_rq_create(bcx_ctx *ctx, size_t hbm_budget_bytes, bcx_rq **out);
+int bcx_rq_create_mode(bcx_ctx *ctx, int mode, size_t hbm_budget_bytes,
+                       bcx_rq **out);
 void bcx_rq_free(bcx_rq *rq);
 /* Admit a request; *id receives its handle.  BCX_EBUSY if it does not fit now. */
 int bcx_rq_submit(bcx_rq *rq, const int32_t *points, int32_t q, int64_t *id);

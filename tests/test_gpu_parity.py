@@ -125,19 +125,20 @@ def test_subset_bc_matches_full(grid):
     assert close_bc(out, ref[pts])
 
 
-def test_rq_engine_alone_and_mixed(grid):
+@pytest.mark.parametrize("mode", [bcx.MODE_SWEEP, bcx.MODE_BATCHED])
+def test_rq_engine_alone_and_mixed(grid, mode):
     g, csr, ctx = grid
     ref, _ = oracle.brandes(csr)
     ctx.set_tuning(batch=128)
     a = np.array([3, 9, 400, 1000], np.int32)
     b = np.arange(0, 1024, 7, dtype=np.int32)
-    srv = bcx.RQServer(ctx, 1 << 30)
+    srv = bcx.RQServer(ctx, 1 << 30, mode)
     ra = srv.submit(a)
     alone = None
     while alone is None:
         srv.step()
         alone = srv.poll(ra)
-    srv2 = bcx.RQServer(ctx, 1 << 30)
+    srv2 = bcx.RQServer(ctx, 1 << 30, mode)
     rb = srv2.submit(b)
     for _ in range(3):
         srv2.step()
