@@ -183,7 +183,7 @@ def main() -> int:
         ctx.set_tuning(batch=B)
         ctx.bc_sources(tree_batch(1, 3, B), bcx.MODE_BATCHED)
         Bs = min(args.sweep_batch, n)
-        ctx.set_tuning(batch=Bs)
+        ctx.set_tuning(sweep_batch=Bs)
         ctx.bc_sources(tree_batch(1, 3, Bs), bcx.MODE_SWEEP)
         ms = ctx.copy_probe(1 << 30)
         log(f"[bench] profile-only done; 1 GiB device copy {ms:.3f} ms")
@@ -193,7 +193,7 @@ def main() -> int:
         # One call per phase, as a full BC runs: the call launches one batch
         # per step and finishes the job once (BC accumulation, and for the
         # bag-sweep form the shortcut unpacking).
-        ctx.set_tuning(batch=size)
+        ctx.set_tuning(**({"sweep_batch": size} if mode == bcx.MODE_SWEEP else {"batch": size}))
         if args.warmup:
             ctx.bc_sources(np.concatenate([tree_batch(k, total, size) for k in range(args.warmup)]), mode)
         src = np.concatenate([tree_batch(k, total, size) for k in range(args.warmup, total)])
@@ -301,7 +301,7 @@ def _rq_bench(bcx, gg, ctx, g, nreq: int, batch: int) -> dict:
     admission by budget; every request rides the shared cyclic sweep for one
     full cycle of batches."""
     n = g.n
-    ctx.set_tuning(batch=batch)
+    ctx.set_tuning(sweep_batch=batch)
     row = np.zeros(n + 1, np.int64)
     np.add.at(row, g.u.astype(np.int64) + 1, 1)
     row = np.cumsum(row)

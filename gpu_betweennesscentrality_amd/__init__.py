@@ -45,7 +45,7 @@ class IndexStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile", C.c_int32), ("tchunk", C.c_int32), ("bp_threads", C.c_int32),
-                ("batch", C.c_int32)]
+                ("batch", C.c_int32), ("sweep_batch", C.c_int32)]
 
 
 class Timing(C.Structure):

@@ -169,7 +169,7 @@ struct bcx_ctx {
   bcx_dev::DevIndex ix{};
   std::vector<void *> owned;          // index + graph buffers
   std::vector<int32_t> perm, iperm;   // preorder <-> vertex
-  bcx_tuning tn{64, 32, 256, 4096};
+  bcx_tuning tn{64, 32, 256, 4096, 16384};
   // batch workspace
   int ws_slots = 0;
   bcx_dev::DevWork ws{};

@@ -476,6 +476,12 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
   c->device = device;
   HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto &e : c->ev) HIP_OK(hipEventCreate(&e));
+  {
+    // One sweep tile per compute unit per launch.
+    int cus = 0;
+    HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    c->tn.sweep_batch = std::max(1, cus) * kWave;
+  }
   const int n = g.n;
   c->perm = ix.perm;
   c->iperm = ix.iperm;
@@ -708,6 +714,8 @@ int bcx_ctx_set_tuning(bcx_ctx *c, const bcx_tuning *tn) {
       tn->bp_threads != 512)
     return bcx::fail(BCX_EINVAL, "bp_threads must be 64, 128, 256 or 512");
   if (tn->batch < 1 || tn->batch > (1 << 20)) return bcx::fail(BCX_EINVAL, "batch out of range");
+  if (tn->sweep_batch < 1 || tn->sweep_batch > (1 << 22))
+    return bcx::fail(BCX_EINVAL, "sweep_batch out of range");
   c->tn = *tn;
   return BCX_OK;
 }

@@ -782,7 +782,7 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   const int n = c->ix.n;
   const int64_t B = c->tree.bags;
   const int total_tiles = (count + kWave - 1) / kWave;
-  int tiles = std::max(1, std::min(c->tn.batch / kWave, std::max(1, total_tiles)));
+  int tiles = std::max(1, std::min(c->tn.sweep_batch / kWave, std::max(1, total_tiles)));
   // Keep the per-tile state within ~70% of the device memory not already held
   // by this context's sweep buffers.
   size_t fr = 0, tot = 0;

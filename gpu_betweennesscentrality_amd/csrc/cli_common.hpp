@@ -37,17 +37,20 @@ inline int parse_device(const char *s) {
 }
 
 // Knobs that the reference fixes at compile time come from the environment so
-// the positional command line stays identical: BCX_MODE=plain|batched,
-// BCX_TCHUNK, BCX_BP_THREADS, BCX_BATCH.
+// the positional command line stays identical: BCX_MODE=plain|batched|sweep
+// (default sweep), BCX_TCHUNK, BCX_BP_THREADS, BCX_BATCH, BCX_SWEEP_BATCH.
 inline int apply_env_tuning(bcx_ctx *ctx) {
   bcx_tuning tn;
   check(bcx_ctx_get_tuning(ctx, &tn), "bcx_ctx_get_tuning");
   tn.tchunk = env_int("BCX_TCHUNK", tn.tchunk);
   tn.bp_threads = env_int("BCX_BP_THREADS", tn.bp_threads);
   tn.batch = env_int("BCX_BATCH", tn.batch);
+  tn.sweep_batch = env_int("BCX_SWEEP_BATCH", tn.sweep_batch);
   check(bcx_ctx_set_tuning(ctx, &tn), "bcx_ctx_set_tuning");
-  const char *m = std::getenv("BCX_MODE");
-  return (m && std::string(m) == "plain") ? BCX_MODE_PLAIN : BCX_MODE_BATCHED;
+  const std::string m = std::getenv("BCX_MODE") ? std::getenv("BCX_MODE") : "sweep";
+  if (m == "plain") return BCX_MODE_PLAIN;
+  if (m == "batched") return BCX_MODE_BATCHED;
+  return BCX_MODE_SWEEP;
 }
 
 // "vertexID  BCValue" then "id value" with 2 decimals (QBC.cu:614-626).

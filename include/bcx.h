@@ -76,6 +76,8 @@ typedef struct bcx_tuning {
   int32_t tchunk;        /* targets per wavefront work item                 */
   int32_t bp_threads;    /* threads per dependency group: 64..512, pow2   */
   int32_t batch;         /* sources per batched launch sequence             */
+  int32_t sweep_batch;   /* sources per bag-sweep launch (64 per tile, one
+                            tile per workgroup); default 64 x compute units */
 } bcx_tuning;
 
 typedef struct bcx_timing {

@@ -129,7 +129,7 @@ def test_subset_bc_matches_full(grid):
 def test_rq_engine_alone_and_mixed(grid, mode):
     g, csr, ctx = grid
     ref, _ = oracle.brandes(csr)
-    ctx.set_tuning(batch=128)
+    ctx.set_tuning(batch=128, sweep_batch=128)
     a = np.array([3, 9, 400, 1000], np.int32)
     b = np.arange(0, 1024, 7, dtype=np.int32)
     srv = bcx.RQServer(ctx, 1 << 30, mode)
@@ -184,6 +184,28 @@ def test_cli_mhlqbc_and_qbc(tmp_path, grid):
     assert close_bc(np.fromfile(tmp_path / "bc.bin", np.float64), ref)
 
 
+def test_cli_rq_table(tmp_path, grid):
+    """RQ executable: query-file contract and stdout table (RQ.cu:542-610)."""
+    g, csr, ctx = grid
+    ref, _ = oracle.brandes(csr)
+    gp = tmp_path / "g.txt"
+    gg.write_graph(g, gp)
+    ids = [1, 2, 17, 300, 512, 1024]  # 1-based, as in the reference's query files
+    qp = tmp_path / "q.txt"
+    qp.write_text(f"{len(ids)}\n" + "\n".join(map(str, ids)) + "\n")
+    exe = ROOT / "gpu_betweennesscentrality_amd" / "bin" / "RQ"
+    out = subprocess.run([exe, gp, qp, "0"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert f"The number of vertices to query: {len(ids)}" in out.stdout
+    lines = out.stdout.splitlines()
+    k = lines.index("-" * 20)
+    rows = [ln.split() for ln in lines[k + 1:k + 1 + len(ids)]]
+    assert [int(r[0]) for r in rows] == ids
+    got = np.array([float(r[1]) for r in rows])
+    want = ref[np.array(ids) - 1]
+    assert np.all(np.abs(got - want) <= 1e-9 * np.maximum(np.abs(want), 1.0) + 5e-9)
+
+
 # ---- bag-sweep form ---------------------------------------------------------
 
 @pytest.mark.parametrize("maker", [
@@ -197,7 +219,7 @@ def test_sweep_full_bc(maker):
     G, ix, ctx = build(g)
     ref, _ = oracle.brandes(oracle.CSR.from_arcgraph(g))
     for batch in (64, 4096):
-        ctx.set_tuning(batch=batch)
+        ctx.set_tuning(sweep_batch=batch)
         out, ovf = ctx.full_bc(bcx.MODE_SWEEP)
         assert not ovf
         assert close_bc(out, ref), f"batch {batch}: max rel err " \
@@ -207,7 +229,7 @@ def test_sweep_full_bc(maker):
 def test_sweep_scattered_sources(road):
     g, csr, ctx = road
     src = gg.permutation(6, 2, g.n)[:700].astype(np.int32)  # not tree-consecutive
-    ctx.set_tuning(batch=256)
+    ctx.set_tuning(sweep_batch=256)
     out, ovf = ctx.bc_sources(src, bcx.MODE_SWEEP)
     ref, _ = oracle.brandes(csr, src)
     assert not ovf
