@@ -70,10 +70,13 @@ struct DevTree {
   const uint64_t *bag_cnt; // B: restricted shortest-path count
   const double *bag_cntf;  // B: the same count as a double
   const int *ibag_off;     // n + 1: entries whose ancestor is this node
-  const int *ibag_x;       // B: the descendant owning the entry
+  const int *ibag_x;       // B: the descendant owning the entry; sign bit set
+                           // when that descendant has no descendants
   const int *ibag_e;       // B: the entry index
   const int *lvl_off;      // height + 1
   const int *lvl_v;        // n: nodes grouped by depth
+  const int *blvl_off;     // height + 1: the same for nodes with descendants
+  const int *blvl_v;
   // Tight bag pairs (a, b of bag(z) with l(a,z) + l(z,b) == l(a,b)), grouped
   // by the depth of z: pair q moves share frac[q] of shortcut (a,b)'s flow
   // (entry tp_ab[q] of bag(b)) onto z and onto z's entries for a and b.
@@ -87,16 +90,21 @@ struct DevTree {
   const int *ep_pair;      // 2P
   // Staging slots: the node's fields and its first kStageEntries bag /
   // inverse-bag entries in fixed places, so a wavefront stages 64 nodes with
-  // one round of coalesced loads.  Level k's nodes are dealt round-robin to
-  // the 16 wavefronts of a workgroup (position lo + w + 16 i to wavefront w)
-  // and wavefront w's i-th node sits in slot st_off[k] + w * ceil(count / 16)
-  // + i.  Unused entry places name node 0 with length 0 and count 0.
-  const int *st_off;          // height + 1
+  // one round of coalesced loads.  Level k's nodes are split over the 16
+  // wavefronts of a workgroup by balanced work; wavefront w's i-th node sits
+  // in slot st_off[k] + w * chunk + i, chunk = (st_off[k+1] - st_off[k]) / 16,
+  // and the wavefront has st_cnt[16 k + w] nodes.  Unused entry places name
+  // node 0 with length 0 and count 0.
+  const int *st_off;          // height + 1 (forward: all nodes)
+  const int *st_cnt;          // 16 * height
+  const int *bst_off;         // height + 1 (backward: nodes with descendants)
+  const int *bst_cnt;         // 16 * height
   const int4 *fs_node;        // {v, bag_off[v], bag_off[v + 1], subtree[v]}
   const int4 *fs_u;           // ancestors of bag entries 0..3
   const int4 *fs_len;         // their shortcut lengths
   const ulonglong2 *fs_cnt;   // 2 per node: counts of entries 0..1, 2..3
-  const int4 *bs_node;        // {v, ibag_off[v], ibag_off[v + 1], 0}
+  const int4 *bs_node;        // {v, ibag_off[v], ibag_off[v + 1], leaf mask}:
+                              // bit r set when entry r's descendant has none
   const int4 *bs_x;           // descendants of inverse-bag entries 0..3
   const int4 *bs_e;           // their entry indices
   const int4 *bs_len;         // their shortcut lengths

@@ -568,7 +568,6 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       t.bag_cntf = (const double *)keep(dev_upload(cf));
     }
     t.ibag_off = (const int *)keep(dev_upload(ioff));
-    t.ibag_x = (const int *)keep(dev_upload(ix_x));
     t.ibag_e = (const int *)keep(dev_upload(ix_e));
     t.lvl_off = (const int *)keep(dev_upload(lvl_off));
     t.lvl_v = (const int *)keep(dev_upload(lvl_v));
@@ -635,23 +634,79 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     t.ep_pair = (const int *)keep(dev_upload(epp));
     c->h_lvl_pair_off = lpo;
     {
-      // Staging arrays: level k's node at position lo + w + 16 i (the i-th
-      // node of wavefront w) goes to slot st_off[k] + w * chunk + i, with
-      // chunk = ceil(count / 16), so each wavefront's nodes are contiguous.
+      // Staging slots.  A level's nodes are split over the 16 wavefronts of a
+      // workgroup so that their work is balanced: longest first, each node to
+      // the least loaded wavefront, with cost 1 per node plus 1.8 per group of
+      // 8 entries past the 4 staged ones (measured: a two-node step and an
+      // overflow group cost about 4 us each).  Each wavefront's nodes keep
+      // their tree order, so the wavefronts of a workgroup still touch
+      // neighbouring rows at about the same time.  Wavefront w's i-th node of
+      // level k sits in slot off[k] + w * chunk + i (chunk = the largest
+      // wavefront count of the level) and its count is cnt[16 k + w].  The
+      // forward list holds every node of the level, the backward list the
+      // nodes with descendants (non-empty inverse bag).
       const int H = ix.tree_height, kW = 16;
-      std::vector<int32_t> st_off((size_t)H + 1, 0), slot_of((size_t)n);
+      auto leaf = [&](int32_t v) { return ioff[(size_t)v] == ioff[(size_t)v + 1]; };
+      std::vector<int32_t> blvl_off((size_t)H + 1, 0), blvl_v;
+      blvl_v.reserve((size_t)n);
       for (int k = 0; k < H; ++k) {
-        const int32_t lo = lvl_off[(size_t)k], cnt = lvl_off[(size_t)k + 1] - lo;
-        const int32_t chunk = (cnt + kW - 1) / kW;
-        st_off[(size_t)k + 1] = st_off[(size_t)k] + kW * chunk;
-        for (int32_t i = 0; i < cnt; ++i)
-          slot_of[(size_t)(lo + i)] = st_off[(size_t)k] + (i % kW) * chunk + i / kW;
+        for (int32_t q = lvl_off[(size_t)k]; q < lvl_off[(size_t)k + 1]; ++q)
+          if (!leaf(lvl_v[(size_t)q])) blvl_v.push_back(lvl_v[(size_t)q]);
+        blvl_off[(size_t)k + 1] = (int32_t)blvl_v.size();
       }
-      const size_t S = (size_t)st_off[(size_t)H];
-      std::vector<int32_t> fnode(S * 4, 0), fu(S * 4, 0), flen(S * 4, 0), bnode(S * 4, 0),
-          bx(S * 4, 0), be(S * 4, 0), blen(S * 4, 0);
+      auto slots = [&](const std::vector<int32_t> &off, const std::vector<int32_t> &list,
+                       auto entries, std::vector<int32_t> &st_off, std::vector<int32_t> &st_cnt,
+                       std::vector<int32_t> &slot_of) {
+        st_off.assign((size_t)H + 1, 0);
+        st_cnt.assign((size_t)H * kW, 0);
+        slot_of.assign((size_t)off[(size_t)H], 0);
+        std::vector<int32_t> order, wave_of;
+        for (int k = 0; k < H; ++k) {
+          const int32_t lo = off[(size_t)k], cnt = off[(size_t)k + 1] - lo;
+          order.resize((size_t)cnt);
+          wave_of.assign((size_t)cnt, 0);
+          std::vector<double> cost((size_t)cnt);
+          for (int32_t i = 0; i < cnt; ++i) {
+            const int64_t e = entries(list[(size_t)(lo + i)]);
+            cost[(size_t)i] = 1.0 + (e > kStageEntries ? 1.8 * (double)((e - kStageEntries + 7) / 8) : 0.0);
+            order[(size_t)i] = i;
+          }
+          std::stable_sort(order.begin(), order.end(),
+                           [&](int32_t a, int32_t b) { return cost[(size_t)a] > cost[(size_t)b]; });
+          double load[16] = {0};
+          for (int32_t i : order) {
+            int w = 0;
+            for (int x = 1; x < kW; ++x)
+              if (load[x] < load[w]) w = x;
+            load[w] += cost[(size_t)i];
+            wave_of[(size_t)i] = w;
+          }
+          int32_t chunk = 0;
+          for (int32_t i = 0; i < cnt; ++i) {
+            const int w = wave_of[(size_t)i];
+            chunk = std::max(chunk, ++st_cnt[(size_t)k * kW + (size_t)w]);
+          }
+          st_off[(size_t)k + 1] = st_off[(size_t)k] + kW * chunk;
+          int32_t pos[16] = {0};
+          for (int32_t i = 0; i < cnt; ++i) {  // tree order within each wavefront
+            const int w = wave_of[(size_t)i];
+            slot_of[(size_t)(lo + i)] = st_off[(size_t)k] + w * chunk + pos[w]++;
+          }
+        }
+        return (size_t)st_off[(size_t)H];
+      };
+      std::vector<int32_t> st_off, st_cnt, slot_of, bst_off, bst_cnt, bslot_of;
+      const size_t S = slots(lvl_off, lvl_v,
+                             [&](int32_t v) { return (int64_t)(boff[(size_t)v + 1] - boff[(size_t)v]); },
+                             st_off, st_cnt, slot_of);
+      const size_t Sb = slots(blvl_off, blvl_v,
+                              [&](int32_t v) { return (int64_t)(ioff[(size_t)v + 1] - ioff[(size_t)v]); },
+                              bst_off, bst_cnt, bslot_of);
+      std::vector<int32_t> fnode(S * 4, 0), fu(S * 4, 0), flen(S * 4, 0);
       std::vector<uint64_t> fcnt(S * 4, 0);
-      std::vector<double> bcnt(S * 4, 0.0);
+      std::vector<int32_t> bnode(std::max<size_t>(Sb, 1) * 4, 0), bx(std::max<size_t>(Sb, 1) * 4, 0),
+          be(std::max<size_t>(Sb, 1) * 4, 0), blen(std::max<size_t>(Sb, 1) * 4, 0);
+      std::vector<double> bcnt(std::max<size_t>(Sb, 1) * 4, 0.0);
       for (int32_t q = 0; q < n; ++q) {
         const int32_t v = lvl_v[(size_t)q];
         const size_t o = (size_t)slot_of[(size_t)q] * 4;
@@ -665,19 +720,36 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
           flen[o + r] = ix.bag_len[e];
           fcnt[o + r] = ix.bag_cnt[e];
         }
-        bnode[o] = v;
-        bnode[o + 1] = ioff[(size_t)v];
-        bnode[o + 2] = ioff[(size_t)v + 1];
-        bnode[o + 3] = 0;
+      }
+      for (size_t q = 0; q < blvl_v.size(); ++q) {
+        const int32_t v = blvl_v[q];
+        const size_t o = (size_t)bslot_of[q] * 4;
+        int32_t leafm = 0;
         for (int r = 0; r < kStageEntries && ioff[(size_t)v] + r < ioff[(size_t)v + 1]; ++r) {
           const size_t k = (size_t)ioff[(size_t)v] + (size_t)r;
           bx[o + r] = ix_x[k];
           be[o + r] = ix_e[k];
           blen[o + r] = ix.bag_len[(size_t)ix_e[k]];
           bcnt[o + r] = (double)ix.bag_cnt[(size_t)ix_e[k]];
+          if (leaf(ix_x[k])) leafm |= 1 << r;
         }
+        bnode[o] = v;
+        bnode[o + 1] = ioff[(size_t)v];
+        bnode[o + 2] = ioff[(size_t)v + 1];
+        bnode[o + 3] = leafm;
       }
+      // Inverse-bag descendants with the sign bit set for those without
+      // descendants of their own.
+      std::vector<int32_t> ix_xs(ix_x);
+      for (auto &x : ix_xs)
+        if (leaf(x)) x = (int32_t)((uint32_t)x | 0x80000000u);
+      t.ibag_x = (const int *)keep(dev_upload(ix_xs));
+      t.blvl_off = (const int *)keep(dev_upload(blvl_off));
+      t.blvl_v = (const int *)keep(dev_upload(blvl_v));
+      t.bst_off = (const int *)keep(dev_upload(bst_off));
       t.st_off = (const int *)keep(dev_upload(st_off));
+      t.st_cnt = (const int *)keep(dev_upload(st_cnt));
+      t.bst_cnt = (const int *)keep(dev_upload(bst_cnt));
       t.fs_node = (const int4 *)keep(dev_upload(fnode));
       t.fs_u = (const int4 *)keep(dev_upload(fu));
       t.fs_len = (const int4 *)keep(dev_upload(flen));

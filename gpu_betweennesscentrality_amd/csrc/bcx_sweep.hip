@@ -47,11 +47,34 @@ __device__ __forceinline__ int &SD(char *row, int lane) { return ((int *)row)[la
 __device__ __forceinline__ uint64_t &SS(char *row, int lane) {
   return ((uint64_t *)(row + kSlotOff))[lane];
 }
-__device__ __forceinline__ double getQ(char *row, int lane) {
-  return __builtin_bit_cast(double, SS(row, lane));
-}
 __device__ __forceinline__ void setQ(char *row, int lane, double q) {
   SS(row, lane) = __builtin_bit_cast(uint64_t, q);
+}
+
+// A distance word carries a flag in bit 30, set when the lane's path count is
+// 1 (distances stay below 2^30: the reader caps the weight sum at 999999999).
+// A flagged lane's count slot is neither written nor read, which on road
+// graphs, where ties are rare, removes nearly all count-row traffic.
+constexpr int kOneFlag = 1 << 30;
+// Length given to entries past the end of a bag: a candidate distance plus it
+// exceeds every true distance (below 999999999) without overflowing, so such
+// an entry is never minimal or tight.  Selecting it on the scalar length keeps
+// the vector arithmetic on loaded rows free of branches (a uniform select of a
+// loaded value becomes a branch that waits for the load).
+constexpr int kNoLen = kOneFlag - 1;
+__device__ __forceinline__ int dist_of(int w) { return w & (kOneFlag - 1); }
+__device__ __forceinline__ bool one_of(int w) { return (w & kOneFlag) != 0; }
+__device__ __forceinline__ void put_dist_count(char *row, int lane, int d, uint64_t c) {
+  SD(row, lane) = d | (c == 1 ? kOneFlag : 0);
+  if (c != 1) SS(row, lane) = c;
+}
+// Quotient Q = (1 + dD) / sigma from a raw slot word.  A node without
+// descendants has dD = 0 and its backward step is skipped, so its slot still
+// holds sigma (not read when flagged 1); other nodes hold Q itself.
+__device__ __forceinline__ bool slot_needed(bool leaf, int w) { return !(leaf && one_of(w)); }
+__device__ __forceinline__ double q_from_slot(bool leaf, int w, uint64_t raw) {
+  const double inv = 1.0 / (double)(one_of(w) ? 1 : raw);
+  return leaf ? inv : __builtin_bit_cast(double, raw);
 }
 
 struct Lane {
@@ -171,7 +194,7 @@ constexpr int kGroup = 8;  // bag entries whose rows are loaded together
 __device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, const Lane &L,
                                         int e, int e_end, int stride, int &best, uint64_t &c) {
   for (; e < e_end; e += kGroup * stride) {
-    int cand[kGroup];
+    int cand[kGroup], w[kGroup];
     char *a[kGroup];
     int b = best;
 #pragma unroll
@@ -180,15 +203,16 @@ __device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, 
       const bool ok = eq < e_end;
       const int ec = ok ? eq : e;
       a[q] = rowp(L, tr.bag_v[ec]);
-      const int dl = SD(a[q], L.lane) + tr.bag_len[ec];  // loaded in every case
-      cand[q] = ok ? dl : 0x7fffffff;
+      w[q] = SD(a[q], L.lane);  // loaded in every case
+      const int len = ok ? tr.bag_len[ec] : kNoLen;
+      cand[q] = dist_of(w[q]) + len;
       b = min(b, cand[q]);
     }
     uint64_t m[kGroup];
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
-      m[q] = 0;
-      if (cand[q] == b) m[q] = SS(a[q], L.lane);
+      m[q] = 1;
+      if (cand[q] == b && !one_of(w[q])) m[q] = SS(a[q], L.lane);
     }
     if (b < best) c = 0;
     best = b;
@@ -205,30 +229,41 @@ __device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, 
 // kGroup at a time; writes the tile's flow on each entry and returns the lane's
 // dependency part.  A descendant x that is done holds, in its 8-byte slot,
 // the quotient Q(x) = (1 + dD(x)) / sigma(x), so an entry costs the distance
-// row of x plus, only in lanes where the entry is tight, one row of Q(x).
+// row of x plus, only in lanes where the entry is tight, one row of Q(x).  A
+// descendant without descendants of its own (marked by the sign bit of its
+// id in ibag_x) has Q = 1 / sigma, read from its count (or its flag).
 // Must be called with all 64 lanes active.
 __device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork &sw,
                                               const Lane &L, int k, int k_end, int stride,
                                               int dv, double sv) {
   double acc = 0.0;
   for (; k < k_end; k += kGroup * stride) {
-    int dx[kGroup], ee[kGroup], len[kGroup];
+    int wx[kGroup], ee[kGroup], len[kGroup];
+    bool leaf[kGroup];
     char *ax[kGroup];
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
       const int kq = k + q * stride;
       const int kc = kq < k_end ? kq : k;
-      const int x = tr.ibag_x[kc];
+      const int xr = tr.ibag_x[kc];
+      leaf[q] = xr < 0;
       ee[q] = tr.ibag_e[kc];
-      len[q] = tr.bag_len[ee[q]];
-      ax[q] = rowp(L, x);
-      dx[q] = SD(ax[q], L.lane);
+      len[q] = kq < k_end ? tr.bag_len[ee[q]] : kNoLen;
+      ax[q] = rowp(L, xr & 0x7fffffff);
+      wx[q] = SD(ax[q], L.lane);
+    }
+    uint64_t raw[kGroup];
+    bool tight[kGroup];
+#pragma unroll
+    for (int q = 0; q < kGroup; ++q) {
+      tight[q] = dv + len[q] == dist_of(wx[q]);
+      raw[q] = 0;
+      if (tight[q] && slot_needed(leaf[q], wx[q])) raw[q] = SS(ax[q], L.lane);
     }
     double f[kGroup];
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
-      double qx = 0.0;
-      if (k + q * stride < k_end && dv + len[q] == dx[q]) qx = getQ(ax[q], L.lane);
+      const double qx = tight[q] ? q_from_slot(leaf[q], wx[q], raw[q]) : 0.0;
       f[q] = tr.bag_cntf[ee[q]] * sv * qx;
       acc += f[q];
     }
@@ -267,15 +302,15 @@ constexpr int kStage = kStageEntries;  // bag entries per node staged in registe
 // Nodes of a level processed together by one wavefront (forward kNodesF,
 // backward kNodesB) are template parameters of the tile kernel.
 
-// Level k's nodes are dealt round-robin to the 16 wavefronts (neighbouring
-// nodes share ancestors, so the wavefronts of a workgroup touch the same rows
-// at about the same time); this wavefront's staging slots are
-// [first, first + count).
-__device__ __forceinline__ void level_block(const DevTree &tr, int k, int cnt, int wave,
+// This wavefront's staging slots of level k: [first, first + count).  The
+// host splits each level's nodes over the 16 wavefronts by balanced work and
+// keeps tree order within a wavefront, so the wavefronts of a workgroup touch
+// neighbouring (ancestor-sharing) rows at about the same time.
+__device__ __forceinline__ void level_block(const int *st_off, const int *st_cnt, int k, int wave,
                                             int &first, int &count) {
-  const int chunk = (cnt + kSweepWaves - 1) / kSweepWaves;
-  first = tr.st_off[k] + wave * chunk;
-  count = max(0, (cnt - wave + kSweepWaves - 1) / kSweepWaves);
+  const int chunk = (st_off[k + 1] - st_off[k]) / kSweepWaves;
+  first = st_off[k] + wave * chunk;
+  count = st_cnt[k * kSweepWaves + wave];
 }
 
 // Forward over this wavefront's block of a level (all at tree depth `depth`).
@@ -290,7 +325,7 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
                                                const SweepWork &sw, const Lane &L, int wave,
                                                int cnt, int depth, int &ovf) {
   int first, mine;
-  level_block(tr, depth, cnt, wave, first, mine);
+  level_block(tr.st_off, tr.st_cnt, depth, wave, first, mine);
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
     const int p = first + base + min(L.lane, jn - 1);
@@ -300,17 +335,20 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
     const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
     const uint64_t scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
     for (int j0 = 0; j0 < jn; j0 += kNodesF) {
-      int cand[kNodesF][kStage], best[kNodesF], upd[kNodesF];
+      int cand[kNodesF][kStage], best[kNodesF], upd[kNodesF], onem[kNodesF];
       uint64_t upc[kNodesF];
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
         const int v = rl(nd.x, j);
         const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
+        onem[u] = 0;
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          const int dl = SD(rowp(L, rl(sur[r], j)), L.lane) + rl(slr[r], j);
-          cand[u][r] = r < nb ? dl : 0x7fffffff;
+          const int w = SD(rowp(L, rl(sur[r], j)), L.lane);
+          const int len = r < nb ? rl(slr[r], j) : kNoLen;
+          cand[u][r] = dist_of(w) + len;
+          onem[u] |= (int)one_of(w) << r;
         }
         upd[u] = 0x7fffffff;
         upc[u] = 0;
@@ -329,8 +367,9 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
         const int j = min(j0 + u, jn - 1);
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          m[u][r] = 0;
-          if (cand[u][r] == best[u]) m[u][r] = SS(rowp(L, rl(sur[r], j)), L.lane);
+          m[u][r] = 1;
+          if (cand[u][r] == best[u] && !((onem[u] >> r) & 1))
+            m[u][r] = SS(rowp(L, rl(sur[r], j)), L.lane);
         }
       }
 #pragma unroll
@@ -347,25 +386,27 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
             c = add_sat(c, cr == 1 ? m[u][r] : mul_sat(m[u][r], cr));
           }
         if (e1 - e0 > kStage) fwd_bag(tr, sw, L, e0 + kStage, e1, 1, b, c);
-        char *a = rowp(L, v);
-        SD(a, L.lane) = b;
-        SS(a, L.lane) = c;
+        put_dist_count(rowp(L, v), L.lane, b, c);
         ovf |= L.valid && c == ~0ull;
       }
     }
   }
 }
 
-// Backward over this wavefront's block of a level, with the same staging and
-// kNodesB nodes per step.  The flows of a node's staged entries and the node
-// flows of the step's nodes are each one four-way transposed lane sum.
+// Backward over this wavefront's block of a level's nodes that have
+// descendants (a node without descendants has dD = 0 and no flows; its Q is
+// derived from its count by the readers), with the same staging and kNodesB
+// nodes per step.  Round 1 loads the nodes' distance words and their staged
+// entries' distance words; round 2 the nodes' counts (lanes not flagged 1) and
+// the tight entries' quotients.  The flows of a node's staged entries and the
+// node flows of the step's nodes are each one four-way transposed lane sum.
 template <int kNodesB>
 __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWork &sw,
                                                const Lane &L, int wave, int cnt, int depth) {
   static_assert(kNodesB <= 4 && kStage == 4, "four-way lane sums");
   const int n = tr.n;
   int first, mine;
-  level_block(tr, depth, cnt, wave, first, mine);
+  level_block(tr.bst_off, tr.bst_cnt, depth, wave, first, mine);
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
     const int p = first + base + min(L.lane, jn - 1);
@@ -375,32 +416,49 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
     const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
     const double scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
     for (int j0 = 0; j0 < jn; j0 += kNodesB) {
-      int dv[kNodesB];
-      uint64_t svu[kNodesB];
-      int dx[kNodesB][kStage];
+      int wv[kNodesB];
+      int wx[kNodesB][kStage];
+#pragma unroll
+      for (int u = 0; u < kNodesB; ++u) {
+        const int j = min(j0 + u, jn - 1);
+        wv[u] = SD(rowp(L, rl(nd.x, j)), L.lane);
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) wx[u][r] = SD(rowp(L, rl(sxr[r], j)), L.lane);
+      }
+      // Round 2: raw slot words only (counts of nodes not flagged 1, and the
+      // tight entries' slots); the arithmetic follows once all are issued.
+      uint64_t svr[kNodesB];
+      uint64_t qr[kNodesB][kStage];
+      int tm[kNodesB];
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
         const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
-        char *av = rowp(L, rl(nd.x, j));
-        dv[u] = SD(av, L.lane);
-        svu[u] = SS(av, L.lane);
+        svr[u] = 1;
+        if (!one_of(wv[u])) svr[u] = SS(rowp(L, rl(nd.x, j)), L.lane);
+        const int dv = dist_of(wv[u]);
+        const int leafm = rl(nd.w, j);
+        tm[u] = 0;
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          const int dl = SD(rowp(L, rl(sxr[r], j)), L.lane);
-          dx[u][r] = r < nb ? dl : -1;
+          const int len = r < nb ? rl(slr[r], j) : kNoLen;
+          const bool t = dv + len == dist_of(wx[u][r]);
+          tm[u] |= (int)t << r;
+          qr[u][r] = 0;
+          if (t && slot_needed((leafm >> r) & 1, wx[u][r]))
+            qr[u][r] = SS(rowp(L, rl(sxr[r], j)), L.lane);
         }
       }
+      double svu[kNodesB];
       double qx[kNodesB][kStage];  // Q(x) in the lanes where entry r is tight
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
+        const int leafm = rl(nd.w, j);
+        svu[u] = (double)svr[u];
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) {
-          qx[u][r] = 0.0;
-          if (dv[u] + rl(slr[r], j) == dx[u][r])
-            qx[u][r] = getQ(rowp(L, rl(sxr[r], j)), L.lane);
-        }
+        for (int r = 0; r < kStage; ++r)
+          qx[u][r] = ((tm[u] >> r) & 1) ? q_from_slot((leafm >> r) & 1, wx[u][r], qr[u][r]) : 0.0;
       }
       double ns_in[kNodesB];
 #pragma unroll
@@ -409,7 +467,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         const bool live = j0 + u < jn;  // false for the clamped repeats of the last node
         const int v = rl(nd.x, j), k0 = rl(nd.y, j), k1 = rl(nd.z, j);
         const int nb = min(kStage, k1 - k0);
-        const double svv = (double)svu[u];
+        const double svv = svu[u];
         double f[kStage], acc = 0.0;
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
@@ -421,7 +479,8 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         const int e = row_pick(L.lane, rl(se.x, j), rl(se.y, j), rl(se.z, j), rl(se.w, j));
         if (live && (L.lane & 15) == 0 && (L.lane >> 4) < nb)
           sw.phi_t[(int64_t)L.t * tr.bags + e] = fs;
-        if (live && k1 - k0 > kStage) acc += bwd_entries(tr, sw, L, k0 + kStage, k1, 1, dv[u], svv);
+        if (live && k1 - k0 > kStage)
+          acc += bwd_entries(tr, sw, L, k0 + kStage, k1, 1, dist_of(wv[u]), svv);
         if (live) setQ(rowp(L, v), L.lane, (1.0 + acc) / svv);
         ns_in[u] = (live && L.valid && L.s != v) ? acc : 0.0;
       }
@@ -487,9 +546,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
               cc = add_sat(cc, sh_cnt[w][L.lane]);
             }
           }
-          char *a = rowp(L, v);
-          SD(a, L.lane) = b;
-          SS(a, L.lane) = cc;
+          put_dist_count(rowp(L, v), L.lane, b, cc);
           ovf |= L.valid && cc == ~0ull;
         }
         __syncthreads();
@@ -502,16 +559,19 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
 
   // ---- backward: dependencies, shortcut flows, node flows ----
   for (int k = H - 1; k >= 0; --k) {
-    const int lo = tr.lvl_off[k], cnt = tr.lvl_off[k + 1] - lo;
+    const int lo = tr.blvl_off[k], cnt = tr.blvl_off[k + 1] - lo;
     if (cnt >= kSweepWaves) {
       bwd_level_wave<kNodesB>(tr, sw, L, wave, cnt, k);
     } else {
       for (int q = 0; q < cnt; ++q) {
-        const int v = __builtin_amdgcn_readfirstlane(tr.lvl_v[lo + q]);
+        const int v = __builtin_amdgcn_readfirstlane(tr.blvl_v[lo + q]);
         char *av = rowp(L, v);
-        const double svv = (double)SS(av, L.lane);
+        const int wv = SD(av, L.lane);
+        uint64_t sv = 1;
+        if (!one_of(wv)) sv = SS(av, L.lane);
+        const double svv = (double)sv;
         sh_acc[wave][L.lane] = bwd_entries(tr, sw, L, tr.ibag_off[v] + wave, tr.ibag_off[v + 1],
-                                           kSweepWaves, SD(av, L.lane), svv);
+                                           kSweepWaves, dist_of(wv), svv);
         __syncthreads();
         if (wave == 0) {
           double acc = 0.0;
@@ -547,7 +607,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
       const int x = __builtin_amdgcn_readfirstlane(sw.cand[kk]);
       const bool anc = x <= L.s && L.s < x + tr.subtree[x];
       char *ax = rowp(L, x);
-      const int dx = SD(ax, L.lane);
+      const int wx = SD(ax, L.lane);
+      const int dx = dist_of(wx);
       uint64_t sa = 0;
       if (anc) {
         const int64_t o = L.rs + ix.depth[x];
@@ -565,16 +626,25 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
             const int eq = e + q * kSweepWaves;
             const int ec = eq < e1 ? eq : e;
             ay[q] = rowp(L, tr.bag_v[ec]);
-            dy[q] = SD(ay[q], L.lane);
+            dy[q] = dist_of(SD(ay[q], L.lane));
+          }
+          uint64_t qa[4];
+          bool t[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int eq = e + q * kSweepWaves;
+            const int ec = eq < e1 ? eq : e;
+            const int len = eq < e1 ? tr.bag_len[ec] : kNoLen;
+            t[q] = sa != 0 && dx + len == dy[q];
+            qa[q] = 0;
+            if (t[q]) qa[q] = SS(ay[q], L.lane);  // QA(y): y has descendants
           }
           double g[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int eq = e + q * kSweepWaves;
             const int ec = eq < e1 ? eq : e;
-            g[q] = 0.0;
-            if (eq < e1 && sa != 0 && dx + tr.bag_len[ec] == dy[q])
-              g[q] = tr.bag_cntf[ec] * getQ(ay[q], L.lane);  // c * QA(y)
+            g[q] = t[q] ? tr.bag_cntf[ec] * __builtin_bit_cast(double, qa[q]) : 0.0;  // c * QA(y)
           }
 #pragma unroll
           for (int q = 0; q < 4; ++q) up += g[q];
@@ -589,7 +659,10 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
       if (wave == 0) {
         double u = 0.0;
         for (int w = 0; w < kSweepWaves; ++w) u += sh_acc[w][L.lane];
-        const double qx = getQ(ax, L.lane);
+        const bool xleaf = tr.ibag_off[x] == tr.ibag_off[x + 1];
+        uint64_t raw = 0;
+        if (slot_needed(xleaf, wx)) raw = SS(ax, L.lane);
+        const double qx = q_from_slot(xleaf, wx, raw);
         setQ(ax, L.lane, sa != 0 ? qx + u : 0.0);
         const double ns = wave_sum_dpp((L.valid && x != L.s) ? sax * u : 0.0);
         if (L.lane == 0) sw.node_t[(int64_t)L.t * n + x] += ns;
@@ -601,8 +674,6 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
   if (ovf) atomicOr(ovf_flag, 1);
 }
 
-// node[v] += sum over tiles of node_t, phi[e] += sum over tiles of phi_t,
-// tiles in increasing order.
 using TileKernel = void (*)(DevIndex, DevTree, SweepWork, int, int *, uint64_t *);
 
 // Nodes per wavefront step (forward, backward): 2 and 2 by default, the
@@ -619,6 +690,8 @@ TileKernel tile_kernel() {
   return k;
 }
 
+// node_out[v] (+)= sum over tiles of node_t, phi_out[e] (+)= sum over tiles
+// of phi_t, tiles in increasing order.
 __global__ __launch_bounds__(256) void sweep_tile_reduce_kernel(int n, int64_t B, int tiles,
                                                                 SweepWork sw, double *node_out,
                                                                 double *phi_out, int accumulate) {
@@ -844,6 +917,9 @@ void sweep_launch(bcx_ctx *c, const int32_t *pre, int32_t nsrc, double *node_out
   if (timed) HIP_OK(hipEventRecord(c->ev[0], c->stream));
   uint64_t *stamps = nullptr;
   if (timed) HIP_OK(hipMalloc(&stamps, (size_t)bt * 4 * 8));
+  // Node sums start at zero: nodes without descendants get only ascending flow.
+  zero_f64_kernel<<<blocks_for((int64_t)bt * n, 256), 256, 0, c->stream>>>(sw.node_t,
+                                                                          (int64_t)bt * n);
   tile_kernel()<<<bt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf, stamps);
   if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
   if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
