@@ -43,45 +43,13 @@ namespace {
 // Q = (1 + dD) / sigma, which its ancestors then read.
 constexpr int64_t kSlotOff = kWave * 4;
 constexpr int64_t kRec = kWave * (4 + 8);
-__device__ __forceinline__ int &SD(char *row, int lane) { return ((int *)row)[lane]; }
-__device__ __forceinline__ uint64_t &SS(char *row, int lane) {
-  return ((uint64_t *)(row + kSlotOff))[lane];
-}
-__device__ __forceinline__ void setQ(char *row, int lane, double q) {
-  SS(row, lane) = __builtin_bit_cast(uint64_t, q);
-}
-
-// A distance word carries a flag in bit 30, set when the lane's path count is
-// 1 (distances stay below 2^30: the reader caps the weight sum at 999999999).
-// A flagged lane's count slot is neither written nor read, which on road
-// graphs, where ties are rare, removes nearly all count-row traffic.
-constexpr int kOneFlag = 1 << 30;
-// Length given to entries past the end of a bag: a candidate distance plus it
-// exceeds every true distance (below 999999999) without overflowing, so such
-// an entry is never minimal or tight.  Selecting it on the scalar length keeps
-// the vector arithmetic on loaded rows free of branches (a uniform select of a
-// loaded value becomes a branch that waits for the load).
-constexpr int kNoLen = kOneFlag - 1;
-__device__ __forceinline__ int dist_of(int w) { return w & (kOneFlag - 1); }
-__device__ __forceinline__ bool one_of(int w) { return (w & kOneFlag) != 0; }
-__device__ __forceinline__ void put_dist_count(char *row, int lane, int d, uint64_t c) {
-  SD(row, lane) = d | (c == 1 ? kOneFlag : 0);
-  if (c != 1) SS(row, lane) = c;
-}
-// Quotient Q = (1 + dD) / sigma from a raw slot word.  A node without
-// descendants has dD = 0 and its backward step is skipped, so its slot still
-// holds sigma (not read when flagged 1); other nodes hold Q itself.
-__device__ __forceinline__ bool slot_needed(bool leaf, int w) { return !(leaf && one_of(w)); }
-__device__ __forceinline__ double q_from_slot(bool leaf, int w, uint64_t raw) {
-  const double inv = 1.0 / (double)(one_of(w) ? 1 : raw);
-  return leaf ? inv : __builtin_bit_cast(double, raw);
-}
 
 struct Lane {
   int t, lane, s;
   bool valid;
   int64_t rs;
-  char *tb;  // the tile's records (uniform)
+  char *tb;  // the tile's records (uniform); a record is addressed by its
+             // 32-bit offset (node * kRec), the tile spanning under 4 GB
 };
 
 __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &sw, int t,
@@ -97,10 +65,52 @@ __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &
   return L;
 }
 
-// Record of node v in the tile: a uniform address, so a row access is that
-// scalar base plus the lane's offset.
-__device__ __forceinline__ char *rowp(const Lane &L, int v) {
-  return L.tb + (uint64_t)(uint32_t)v * kRec;
+// Record offset of node v (uniform).  A row access is the uniform tile base
+// plus one 32-bit per-lane offset, so each costs one full-rate add.
+__device__ __forceinline__ uint32_t roff(int v) { return (uint32_t)v * (uint32_t)kRec; }
+__device__ __forceinline__ int &SD(const Lane &L, uint32_t r) {
+  return *(int *)(L.tb + (r + (uint32_t)L.lane * 4u));
+}
+__device__ __forceinline__ uint64_t &SS(const Lane &L, uint32_t r) {
+  return *(uint64_t *)(L.tb + (r + (uint32_t)kSlotOff + (uint32_t)L.lane * 8u));
+}
+__device__ __forceinline__ void setQ(const Lane &L, uint32_t r, double q) {
+  SS(L, r) = __builtin_bit_cast(uint64_t, q);
+}
+
+// A distance word carries a flag in bit 30, set when the lane's path count is
+// 1 (distances stay below 2^30: the reader caps the weight sum at 999999999).
+// A flagged lane's count slot is neither written nor read, which on road
+// graphs, where ties are rare, removes nearly all count-row traffic.
+constexpr int kOneFlag = 1 << 30;
+// Length given to entries past the end of a bag: a candidate distance plus it
+// exceeds every true distance (below 999999999) without overflowing, so such
+// an entry is never minimal or tight.  Selecting it on the scalar length keeps
+// the vector arithmetic on loaded rows free of branches (a uniform select of a
+// loaded value becomes a branch that waits for the load).
+constexpr int kNoLen = kOneFlag - 1;
+__device__ __forceinline__ int dist_of(int w) { return w & (kOneFlag - 1); }
+__device__ __forceinline__ bool one_of(int w) { return (w & kOneFlag) != 0; }
+__device__ __forceinline__ void put_dist_count(const Lane &L, uint32_t r, int d, uint64_t c) {
+  SD(L, r) = d | (c == 1 ? kOneFlag : 0);
+  if (c != 1) SS(L, r) = c;
+}
+// Quotient Q = (1 + dD) / sigma from a raw slot word.  A node without
+// descendants has dD = 0 and its backward step is skipped, so its slot still
+// holds sigma (not read when flagged 1); other nodes hold Q itself.
+__device__ __forceinline__ bool slot_needed(bool leaf, int w) { return !(leaf && one_of(w)); }
+__device__ __forceinline__ double q_from_slot(bool leaf, int w, uint64_t raw) {
+  const double inv = 1.0 / (double)(one_of(w) ? 1 : raw);
+  return leaf ? inv : __builtin_bit_cast(double, raw);
+}
+
+// Slot word of record r in lanes where `need` holds; other lanes read the
+// tile's first record (one hot cache line) and their value is not used.
+// Loads are issued unconditionally so that the compiler can count them
+// exactly: a load under a lane mask may not issue, and each later wait would
+// then also wait for every store and load issued since.
+__device__ __forceinline__ uint64_t slot_if(bool need, uint32_t r, const Lane &L) {
+  return SS(L, need ? r : 0u);
 }
 
 // Lane moves of a double.  dpp_row: a DPP move inside rows of 16 lanes that
@@ -195,15 +205,15 @@ __device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, 
                                         int e, int e_end, int stride, int &best, uint64_t &c) {
   for (; e < e_end; e += kGroup * stride) {
     int cand[kGroup], w[kGroup];
-    char *a[kGroup];
+    uint32_t a[kGroup];
     int b = best;
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
       const int eq = e + q * stride;
       const bool ok = eq < e_end;
       const int ec = ok ? eq : e;
-      a[q] = rowp(L, tr.bag_v[ec]);
-      w[q] = SD(a[q], L.lane);  // loaded in every case
+      a[q] = roff(tr.bag_v[ec]);
+      w[q] = SD(L, a[q]);  // loaded in every case
       const int len = ok ? tr.bag_len[ec] : kNoLen;
       cand[q] = dist_of(w[q]) + len;
       b = min(b, cand[q]);
@@ -211,8 +221,9 @@ __device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, 
     uint64_t m[kGroup];
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
-      m[q] = 1;
-      if (cand[q] == b && !one_of(w[q])) m[q] = SS(a[q], L.lane);
+      const bool need = cand[q] == b && !one_of(w[q]);
+      const uint64_t raw = slot_if(need, a[q], L);
+      m[q] = need ? raw : 1;
     }
     if (b < best) c = 0;
     best = b;
@@ -240,7 +251,7 @@ __device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork
   for (; k < k_end; k += kGroup * stride) {
     int wx[kGroup], ee[kGroup], len[kGroup];
     bool leaf[kGroup];
-    char *ax[kGroup];
+    uint32_t ax[kGroup];
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
       const int kq = k + q * stride;
@@ -249,16 +260,15 @@ __device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork
       leaf[q] = xr < 0;
       ee[q] = tr.ibag_e[kc];
       len[q] = kq < k_end ? tr.bag_len[ee[q]] : kNoLen;
-      ax[q] = rowp(L, xr & 0x7fffffff);
-      wx[q] = SD(ax[q], L.lane);
+      ax[q] = roff(xr & 0x7fffffff);
+      wx[q] = SD(L, ax[q]);
     }
     uint64_t raw[kGroup];
     bool tight[kGroup];
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
       tight[q] = dv + len[q] == dist_of(wx[q]);
-      raw[q] = 0;
-      if (tight[q] && slot_needed(leaf[q], wx[q])) raw[q] = SS(ax[q], L.lane);
+      raw[q] = slot_if(tight[q] && slot_needed(leaf[q], wx[q]), ax[q], L);
     }
     double f[kGroup];
 #pragma unroll
@@ -315,15 +325,19 @@ __device__ __forceinline__ void level_block(const int *st_off, const int *st_cnt
 
 // Forward over this wavefront's block of a level (all at tree depth `depth`).
 // Lane j stages node j's fields and first kStage bag entries with one round
-// of loads; the wavefront then handles kNodesF nodes per step: all their
-// distance rows in flight together, then the path-count rows of the tight
-// candidates only.  Saturating sums of saturating products do not depend on
-// the order of their terms, so the count equals that of a running
-// min-and-count.
+// of loads; the wavefront then handles kNodesF nodes per step.  Round 1 of a
+// step loads the distance words of the nodes' staged entries (and the
+// up-label where the node is an ancestor of the lane's source); round 2 the
+// path-count rows of tight candidates not flagged 1.  The steps are
+// pipelined: a step issues the next step's round 1 after its own round 2 and
+// before its stores, so every wait is for loads that are already old (stores
+// and loads share one completion counter).  Saturating sums of saturating
+// products do not depend on the order of their terms, so the count equals
+// that of a running min-and-count.
 template <int kNodesF>
 __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree &tr,
                                                const SweepWork &sw, const Lane &L, int wave,
-                                               int cnt, int depth, int &ovf) {
+                                               int depth, int &ovf) {
   int first, mine;
   level_block(tr.st_off, tr.st_cnt, depth, wave, first, mine);
   for (int base = 0; base < mine; base += kWave) {
@@ -334,51 +348,74 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
     const int sur[kStage] = {su.x, su.y, su.z, su.w};
     const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
     const uint64_t scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
-    for (int j0 = 0; j0 < jn; j0 += kNodesF) {
-      int cand[kNodesF][kStage], best[kNodesF], upd[kNodesF], onem[kNodesF];
-      uint64_t upc[kNodesF];
+    // Round 1 of the step at j0 (nodes past the block repeat its last node).
+    auto round1 = [&](int j0, int (&w)[kNodesF][kStage], int (&upd)[kNodesF],
+                      uint64_t (&upc)[kNodesF]) {
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
         const int v = rl(nd.x, j);
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) w[u][r] = SD(L, roff(rl(sur[r], j)));
+        // Up-label where v is an ancestor of s (else entry 0, not used).
+        const bool anc = v <= L.s && L.s < v + rl(nd.w, j);
+        const int64_t o = anc ? L.rs + depth : 0;
+        const int ud = ix.dis[o];
+        const uint64_t uc = ix.cnt[o];
+        upd[u] = anc ? ud : 0x7fffffff;
+        upc[u] = anc ? uc : 0;
+      }
+    };
+    int w[kNodesF][kStage], upd[kNodesF];
+    uint64_t upc[kNodesF];
+    round1(0, w, upd, upc);
+    for (int j0 = 0; j0 < jn; j0 += kNodesF) {
+      int cand[kNodesF][kStage], best[kNodesF], onem[kNodesF];
+#pragma unroll
+      for (int u = 0; u < kNodesF; ++u) {
+        const int j = min(j0 + u, jn - 1);
         const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
         onem[u] = 0;
-#pragma unroll
-        for (int r = 0; r < kStage; ++r) {
-          const int w = SD(rowp(L, rl(sur[r], j)), L.lane);
-          const int len = r < nb ? rl(slr[r], j) : kNoLen;
-          cand[u][r] = dist_of(w) + len;
-          onem[u] |= (int)one_of(w) << r;
-        }
-        upd[u] = 0x7fffffff;
-        upc[u] = 0;
-        if (v <= L.s && L.s < v + rl(nd.w, j)) {  // v is an ancestor of s: up-label
-          const int64_t o = L.rs + depth;
-          upd[u] = ix.dis[o];
-          upc[u] = ix.cnt[o];
-        }
         best[u] = upd[u];
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) best[u] = min(best[u], cand[u][r]);
+        for (int r = 0; r < kStage; ++r) {
+          const int len = r < nb ? rl(slr[r], j) : kNoLen;
+          cand[u][r] = dist_of(w[u][r]) + len;
+          onem[u] |= (int)one_of(w[u][r]) << r;
+          best[u] = min(best[u], cand[u][r]);
+        }
       }
+      // Phase order is kept as written: all candidates, then every load of
+      // the step (a load under a lane mask may not issue, so the one wait
+      // that follows covers all of them), then the arithmetic and stores.
+      __builtin_amdgcn_sched_barrier(0);
       uint64_t m[kNodesF][kStage];
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          m[u][r] = 1;
-          if (cand[u][r] == best[u] && !((onem[u] >> r) & 1))
-            m[u][r] = SS(rowp(L, rl(sur[r], j)), L.lane);
+          const bool need = cand[u][r] == best[u] && !((onem[u] >> r) & 1);
+          const uint64_t raw = slot_if(need, roff(rl(sur[r], j)), L);
+          m[u][r] = need ? raw : 1;
         }
       }
+      int un[kNodesF];
+      uint64_t uc[kNodesF];
+#pragma unroll
+      for (int u = 0; u < kNodesF; ++u) {
+        un[u] = upd[u];
+        uc[u] = upc[u];
+      }
+      round1(j0 + kNodesF, w, upd, upc);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = j0 + u;
         if (j >= jn) break;
         const int v = rl(nd.x, j), e0 = rl(nd.y, j), e1 = rl(nd.z, j);
         int b = best[u];
-        uint64_t c = upd[u] == b ? upc[u] : 0;
+        uint64_t c = un[u] == b ? uc[u] : 0;
 #pragma unroll
         for (int r = 0; r < kStage; ++r)
           if (cand[u][r] == b) {
@@ -386,7 +423,7 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
             c = add_sat(c, cr == 1 ? m[u][r] : mul_sat(m[u][r], cr));
           }
         if (e1 - e0 > kStage) fwd_bag(tr, sw, L, e0 + kStage, e1, 1, b, c);
-        put_dist_count(rowp(L, v), L.lane, b, c);
+        put_dist_count(L, roff(v), b, c);
         ovf |= L.valid && c == ~0ull;
       }
     }
@@ -398,11 +435,12 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 // derived from its count by the readers), with the same staging and kNodesB
 // nodes per step.  Round 1 loads the nodes' distance words and their staged
 // entries' distance words; round 2 the nodes' counts (lanes not flagged 1) and
-// the tight entries' quotients.  The flows of a node's staged entries and the
-// node flows of the step's nodes are each one four-way transposed lane sum.
+// the tight entries' quotients; steps are pipelined as in the forward pass.
+// The flows of a node's staged entries and the node flows of the step's nodes
+// are each one four-way transposed lane sum.
 template <int kNodesB>
 __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWork &sw,
-                                               const Lane &L, int wave, int cnt, int depth) {
+                                               const Lane &L, int wave, int depth) {
   static_assert(kNodesB <= 4 && kStage == 4, "four-way lane sums");
   const int n = tr.n;
   int first, mine;
@@ -415,18 +453,30 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
     const int sxr[kStage] = {sx.x, sx.y, sx.z, sx.w};
     const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
     const double scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
-    for (int j0 = 0; j0 < jn; j0 += kNodesB) {
-      int wv[kNodesB];
-      int wx[kNodesB][kStage];
+    // Round 1 of the step at j0 (nodes past the block repeat its last node).
+    auto round1 = [&](int j0, int (&wv)[kNodesB], int (&wx)[kNodesB][kStage]) {
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
-        wv[u] = SD(rowp(L, rl(nd.x, j)), L.lane);
+        wv[u] = SD(L, roff(rl(nd.x, j)));
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) wx[u][r] = SD(rowp(L, rl(sxr[r], j)), L.lane);
+        for (int r = 0; r < kStage; ++r) wx[u][r] = SD(L, roff(rl(sxr[r], j)));
+      }
+    };
+    int wvn[kNodesB], wxn[kNodesB][kStage];
+    round1(0, wvn, wxn);
+    for (int j0 = 0; j0 < jn; j0 += kNodesB) {
+      int wv[kNodesB], wx[kNodesB][kStage];
+#pragma unroll
+      for (int u = 0; u < kNodesB; ++u) {
+        wv[u] = wvn[u];
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) wx[u][r] = wxn[u][r];
       }
       // Round 2: raw slot words only (counts of nodes not flagged 1, and the
-      // tight entries' slots); the arithmetic follows once all are issued.
+      // tight entries' slots); the arithmetic follows once all are issued,
+      // and the phase order is kept as written (see the forward pass).
+      __builtin_amdgcn_sched_barrier(0);
       uint64_t svr[kNodesB];
       uint64_t qr[kNodesB][kStage];
       int tm[kNodesB];
@@ -434,8 +484,10 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
         const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
-        svr[u] = 1;
-        if (!one_of(wv[u])) svr[u] = SS(rowp(L, rl(nd.x, j)), L.lane);
+        {
+          const uint64_t raw = slot_if(!one_of(wv[u]), roff(rl(nd.x, j)), L);
+          svr[u] = one_of(wv[u]) ? 1 : raw;
+        }
         const int dv = dist_of(wv[u]);
         const int leafm = rl(nd.w, j);
         tm[u] = 0;
@@ -444,11 +496,11 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
           const int len = r < nb ? rl(slr[r], j) : kNoLen;
           const bool t = dv + len == dist_of(wx[u][r]);
           tm[u] |= (int)t << r;
-          qr[u][r] = 0;
-          if (t && slot_needed((leafm >> r) & 1, wx[u][r]))
-            qr[u][r] = SS(rowp(L, rl(sxr[r], j)), L.lane);
+          qr[u][r] = slot_if(t && slot_needed((leafm >> r) & 1, wx[u][r]), roff(rl(sxr[r], j)), L);
         }
       }
+      round1(j0 + kNodesB, wvn, wxn);
+      __builtin_amdgcn_sched_barrier(0);
       double svu[kNodesB];
       double qx[kNodesB][kStage];  // Q(x) in the lanes where entry r is tight
 #pragma unroll
@@ -481,7 +533,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
           sw.phi_t[(int64_t)L.t * tr.bags + e] = fs;
         if (live && k1 - k0 > kStage)
           acc += bwd_entries(tr, sw, L, k0 + kStage, k1, 1, dist_of(wv[u]), svv);
-        if (live) setQ(rowp(L, v), L.lane, (1.0 + acc) / svv);
+        if (live) setQ(L, roff(v), (1.0 + acc) / svv);
         ns_in[u] = (live && L.valid && L.s != v) ? acc : 0.0;
       }
       const double ns = wave_sum4(ns_in[0], kNodesB > 1 ? ns_in[1 % kNodesB] : 0.0,
@@ -519,7 +571,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
   for (int k = 0; k < H; ++k) {
     const int lo = tr.lvl_off[k], cnt = tr.lvl_off[k + 1] - lo;
     if (cnt >= kSweepWaves) {
-      fwd_level_wave<kNodesF>(ix, tr, sw, L, wave, cnt, k, ovf);
+      fwd_level_wave<kNodesF>(ix, tr, sw, L, wave, k, ovf);
     } else {
       for (int q = 0; q < cnt; ++q) {
         const int v = __builtin_amdgcn_readfirstlane(tr.lvl_v[lo + q]);
@@ -546,7 +598,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
               cc = add_sat(cc, sh_cnt[w][L.lane]);
             }
           }
-          put_dist_count(rowp(L, v), L.lane, b, cc);
+          put_dist_count(L, roff(v), b, cc);
           ovf |= L.valid && cc == ~0ull;
         }
         __syncthreads();
@@ -561,14 +613,14 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
   for (int k = H - 1; k >= 0; --k) {
     const int lo = tr.blvl_off[k], cnt = tr.blvl_off[k + 1] - lo;
     if (cnt >= kSweepWaves) {
-      bwd_level_wave<kNodesB>(tr, sw, L, wave, cnt, k);
+      bwd_level_wave<kNodesB>(tr, sw, L, wave, k);
     } else {
       for (int q = 0; q < cnt; ++q) {
         const int v = __builtin_amdgcn_readfirstlane(tr.blvl_v[lo + q]);
-        char *av = rowp(L, v);
-        const int wv = SD(av, L.lane);
+        const uint32_t av = roff(v);
+        const int wv = SD(L, av);
         uint64_t sv = 1;
-        if (!one_of(wv)) sv = SS(av, L.lane);
+        if (!one_of(wv)) sv = SS(L, av);
         const double svv = (double)sv;
         sh_acc[wave][L.lane] = bwd_entries(tr, sw, L, tr.ibag_off[v] + wave, tr.ibag_off[v + 1],
                                            kSweepWaves, dist_of(wv), svv);
@@ -576,7 +628,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
         if (wave == 0) {
           double acc = 0.0;
           for (int w = 0; w < kSweepWaves; ++w) acc += sh_acc[w][L.lane];
-          setQ(av, L.lane, (1.0 + acc) / svv);
+          setQ(L, av, (1.0 + acc) / svv);
           const double ns = wave_sum_dpp((L.valid && L.s != v) ? acc : 0.0);
           if (L.lane == 0) sw.node_t[(int64_t)L.t * tr.n + v] = ns;
         }
@@ -606,8 +658,8 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
     for (int kk = c0; kk < c1; ++kk) {
       const int x = __builtin_amdgcn_readfirstlane(sw.cand[kk]);
       const bool anc = x <= L.s && L.s < x + tr.subtree[x];
-      char *ax = rowp(L, x);
-      const int wx = SD(ax, L.lane);
+      const uint32_t ax = roff(x);
+      const int wx = SD(L, ax);
       const int dx = dist_of(wx);
       uint64_t sa = 0;
       if (anc) {
@@ -620,13 +672,13 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
         const int e1 = tr.bag_off[x + 1];
         for (int e = tr.bag_off[x] + wave; e < e1; e += 4 * kSweepWaves) {
           int dy[4];
-          char *ay[4];
+          uint32_t ay[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int eq = e + q * kSweepWaves;
             const int ec = eq < e1 ? eq : e;
-            ay[q] = rowp(L, tr.bag_v[ec]);
-            dy[q] = dist_of(SD(ay[q], L.lane));
+            ay[q] = roff(tr.bag_v[ec]);
+            dy[q] = dist_of(SD(L, ay[q]));
           }
           uint64_t qa[4];
           bool t[4];
@@ -637,7 +689,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
             const int len = eq < e1 ? tr.bag_len[ec] : kNoLen;
             t[q] = sa != 0 && dx + len == dy[q];
             qa[q] = 0;
-            if (t[q]) qa[q] = SS(ay[q], L.lane);  // QA(y): y has descendants
+            if (t[q]) qa[q] = SS(L, ay[q]);  // QA(y): y has descendants
           }
           double g[4];
 #pragma unroll
@@ -661,9 +713,9 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
         for (int w = 0; w < kSweepWaves; ++w) u += sh_acc[w][L.lane];
         const bool xleaf = tr.ibag_off[x] == tr.ibag_off[x + 1];
         uint64_t raw = 0;
-        if (slot_needed(xleaf, wx)) raw = SS(ax, L.lane);
+        if (slot_needed(xleaf, wx)) raw = SS(L, ax);
         const double qx = q_from_slot(xleaf, wx, raw);
-        setQ(ax, L.lane, sa != 0 ? qx + u : 0.0);
+        setQ(L, ax, sa != 0 ? qx + u : 0.0);
         const double ns = wave_sum_dpp((L.valid && x != L.s) ? sax * u : 0.0);
         if (L.lane == 0) sw.node_t[(int64_t)L.t * n + x] += ns;
       }
@@ -676,19 +728,9 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
 
 using TileKernel = void (*)(DevIndex, DevTree, SweepWork, int, int *, uint64_t *);
 
-// Nodes per wavefront step (forward, backward): 2 and 2 by default, the
-// fastest on road graphs (more nodes per step cost scalar-register spills and
-// do not shorten the forward sweep, which is bandwidth-bound); the
-// environment variable BCX_SWEEP_NODES=42 selects 4 forward nodes per step.
-TileKernel tile_kernel() {
-  static TileKernel k = [] {
-    const char *e = getenv("BCX_SWEEP_NODES");
-    const std::string v = e ? e : "";
-    if (v == "42") return (TileKernel)sweep_tile_kernel<4, 2>;
-    return (TileKernel)sweep_tile_kernel<2, 2>;
-  }();
-  return k;
-}
+// Nodes per wavefront step (forward, backward): 2 and 2, the fastest on road
+// graphs (with the steps pipelined, more nodes per step only spill registers).
+TileKernel tile_kernel() { return (TileKernel)sweep_tile_kernel<2, 2>; }
 
 // node_out[v] (+)= sum over tiles of node_t, phi_out[e] (+)= sum over tiles
 // of phi_t, tiles in increasing order.
@@ -854,6 +896,8 @@ namespace bcx_dev {
 int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   const int n = c->ix.n;
   const int64_t B = c->tree.bags;
+  if ((uint64_t)n * (uint64_t)kRec > 0xffffffffull)
+    throw bcx::Error{BCX_EGRAPH, "bag-sweep records of one tile exceed 4 GiB (more than 5592405 vertices)"};
   const int total_tiles = (count + kWave - 1) / kWave;
   int tiles = std::max(1, std::min(c->tn.sweep_batch / kWave, std::max(1, total_tiles)));
   // Keep the per-tile state within ~70% of the device memory not already held
