@@ -65,6 +65,13 @@ __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &
   return L;
 }
 
+// Read-only data at a wave-uniform index, through the constant address space
+// so that it becomes a scalar load (scalar cache, no vector instructions).
+template <class T>
+__device__ __forceinline__ T cld(const T *p, int64_t i) {
+  return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
 // Record offset of node v (uniform).  A row access is the uniform tile base
 // plus one 32-bit per-lane offset, so each costs one full-rate add.
 __device__ __forceinline__ uint32_t roff(int v) { return (uint32_t)v * (uint32_t)kRec; }
@@ -212,9 +219,9 @@ __device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, 
       const int eq = e + q * stride;
       const bool ok = eq < e_end;
       const int ec = ok ? eq : e;
-      a[q] = roff(tr.bag_v[ec]);
+      a[q] = roff(cld(tr.bag_v, ec));
       w[q] = SD(L, a[q]);  // loaded in every case
-      const int len = ok ? tr.bag_len[ec] : kNoLen;
+      const int len = ok ? cld(tr.bag_len, ec) : kNoLen;
       cand[q] = dist_of(w[q]) + len;
       b = min(b, cand[q]);
     }
@@ -231,7 +238,7 @@ __device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, 
     for (int q = 0; q < kGroup; ++q) {
       const int eq = e + q * stride;
       if (eq >= e_end) break;
-      if (cand[q] == b) c = add_sat(c, mul_sat(m[q], tr.bag_cnt[eq]));
+      if (cand[q] == b) c = add_sat(c, mul_sat(m[q], cld(tr.bag_cnt, eq)));
     }
   }
 }
@@ -256,10 +263,10 @@ __device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork
     for (int q = 0; q < kGroup; ++q) {
       const int kq = k + q * stride;
       const int kc = kq < k_end ? kq : k;
-      const int xr = tr.ibag_x[kc];
+      const int xr = cld(tr.ibag_x, kc);
       leaf[q] = xr < 0;
-      ee[q] = tr.ibag_e[kc];
-      len[q] = kq < k_end ? tr.bag_len[ee[q]] : kNoLen;
+      ee[q] = cld(tr.ibag_e, kc);
+      len[q] = kq < k_end ? cld(tr.bag_len, ee[q]) : kNoLen;
       ax[q] = roff(xr & 0x7fffffff);
       wx[q] = SD(L, ax[q]);
     }
@@ -274,7 +281,7 @@ __device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) {
       const double qx = tight[q] ? q_from_slot(leaf[q], wx[q], raw[q]) : 0.0;
-      f[q] = tr.bag_cntf[ee[q]] * sv * qx;
+      f[q] = cld(tr.bag_cntf, ee[q]) * sv * qx;
       acc += f[q];
     }
 #pragma unroll
@@ -318,9 +325,10 @@ constexpr int kStage = kStageEntries;  // bag entries per node staged in registe
 // neighbouring (ancestor-sharing) rows at about the same time.
 __device__ __forceinline__ void level_block(const int *st_off, const int *st_cnt, int k, int wave,
                                             int &first, int &count) {
-  const int chunk = (st_off[k + 1] - st_off[k]) / kSweepWaves;
-  first = st_off[k] + wave * chunk;
-  count = st_cnt[k * kSweepWaves + wave];
+  const int o0 = cld(st_off, k), o1 = cld(st_off, k + 1);
+  const int chunk = (o1 - o0) / kSweepWaves;
+  first = o0 + wave * chunk;
+  count = cld(st_cnt, k * kSweepWaves + wave);
 }
 
 // Forward over this wavefront's block of a level (all at tree depth `depth`).
@@ -569,20 +577,20 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
 
   // ---- forward: distance and path count ----
   for (int k = 0; k < H; ++k) {
-    const int lo = tr.lvl_off[k], cnt = tr.lvl_off[k + 1] - lo;
+    const int lo = cld(tr.lvl_off, k), cnt = cld(tr.lvl_off, k + 1) - lo;
     if (cnt >= kSweepWaves) {
       fwd_level_wave<kNodesF>(ix, tr, sw, L, wave, k, ovf);
     } else {
       for (int q = 0; q < cnt; ++q) {
-        const int v = __builtin_amdgcn_readfirstlane(tr.lvl_v[lo + q]);
+        const int v = cld(tr.lvl_v, lo + q);
         int best = 0x7fffffff;
         uint64_t c = 0;
-        if (wave == 0 && v <= L.s && L.s < v + tr.subtree[v]) {
-          const int64_t o = L.rs + ix.depth[v];
+        if (wave == 0 && v <= L.s && L.s < v + cld(tr.subtree, v)) {
+          const int64_t o = L.rs + k;
           best = ix.dis[o];
           c = ix.cnt[o];
         }
-        fwd_bag(tr, sw, L, tr.bag_off[v] + wave, tr.bag_off[v + 1], kSweepWaves, best, c);
+        fwd_bag(tr, sw, L, cld(tr.bag_off, v) + wave, cld(tr.bag_off, v + 1), kSweepWaves, best, c);
         sh_best[wave][L.lane] = best;
         sh_cnt[wave][L.lane] = c;
         __syncthreads();
@@ -611,18 +619,18 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
 
   // ---- backward: dependencies, shortcut flows, node flows ----
   for (int k = H - 1; k >= 0; --k) {
-    const int lo = tr.blvl_off[k], cnt = tr.blvl_off[k + 1] - lo;
+    const int lo = cld(tr.blvl_off, k), cnt = cld(tr.blvl_off, k + 1) - lo;
     if (cnt >= kSweepWaves) {
       bwd_level_wave<kNodesB>(tr, sw, L, wave, k);
     } else {
       for (int q = 0; q < cnt; ++q) {
-        const int v = __builtin_amdgcn_readfirstlane(tr.blvl_v[lo + q]);
+        const int v = cld(tr.blvl_v, lo + q);
         const uint32_t av = roff(v);
         const int wv = SD(L, av);
         uint64_t sv = 1;
         if (!one_of(wv)) sv = SS(L, av);
         const double svv = (double)sv;
-        sh_acc[wave][L.lane] = bwd_entries(tr, sw, L, tr.ibag_off[v] + wave, tr.ibag_off[v + 1],
+        sh_acc[wave][L.lane] = bwd_entries(tr, sw, L, cld(tr.ibag_off, v) + wave, cld(tr.ibag_off, v + 1),
                                            kSweepWaves, dist_of(wv), svv);
         __syncthreads();
         if (wave == 0) {
@@ -654,30 +662,30 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
   // their ancestors' quotients, and a lane whose source is not below x never
   // reads x again.
   {
-    const int c0 = sw.cand_off[L.t], c1 = sw.cand_off[L.t + 1];
+    const int c0 = cld(sw.cand_off, L.t), c1 = cld(sw.cand_off, L.t + 1);
     for (int kk = c0; kk < c1; ++kk) {
-      const int x = __builtin_amdgcn_readfirstlane(sw.cand[kk]);
-      const bool anc = x <= L.s && L.s < x + tr.subtree[x];
+      const int x = cld(sw.cand, kk);
+      const bool anc = x <= L.s && L.s < x + cld(tr.subtree, x);
       const uint32_t ax = roff(x);
       const int wx = SD(L, ax);
       const int dx = dist_of(wx);
       uint64_t sa = 0;
       if (anc) {
-        const int64_t o = L.rs + ix.depth[x];
+        const int64_t o = L.rs + cld(ix.depth, x);
         if (ix.dis[o] == dx) sa = ix.cnt[o];
       }
       const double sax = (double)sa;
       double up = 0.0;
       if (__any(sa != 0)) {
-        const int e1 = tr.bag_off[x + 1];
-        for (int e = tr.bag_off[x] + wave; e < e1; e += 4 * kSweepWaves) {
+        const int e1 = cld(tr.bag_off, x + 1);
+        for (int e = cld(tr.bag_off, x) + wave; e < e1; e += 4 * kSweepWaves) {
           int dy[4];
           uint32_t ay[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int eq = e + q * kSweepWaves;
             const int ec = eq < e1 ? eq : e;
-            ay[q] = roff(tr.bag_v[ec]);
+            ay[q] = roff(cld(tr.bag_v, ec));
             dy[q] = dist_of(SD(L, ay[q]));
           }
           uint64_t qa[4];
@@ -686,7 +694,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
           for (int q = 0; q < 4; ++q) {
             const int eq = e + q * kSweepWaves;
             const int ec = eq < e1 ? eq : e;
-            const int len = eq < e1 ? tr.bag_len[ec] : kNoLen;
+            const int len = eq < e1 ? cld(tr.bag_len, ec) : kNoLen;
             t[q] = sa != 0 && dx + len == dy[q];
             qa[q] = 0;
             if (t[q]) qa[q] = SS(L, ay[q]);  // QA(y): y has descendants
@@ -696,7 +704,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
           for (int q = 0; q < 4; ++q) {
             const int eq = e + q * kSweepWaves;
             const int ec = eq < e1 ? eq : e;
-            g[q] = t[q] ? tr.bag_cntf[ec] * __builtin_bit_cast(double, qa[q]) : 0.0;  // c * QA(y)
+            g[q] = t[q] ? cld(tr.bag_cntf, ec) * __builtin_bit_cast(double, qa[q]) : 0.0;  // c * QA(y)
           }
 #pragma unroll
           for (int q = 0; q < 4; ++q) up += g[q];
@@ -711,7 +719,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
       if (wave == 0) {
         double u = 0.0;
         for (int w = 0; w < kSweepWaves; ++w) u += sh_acc[w][L.lane];
-        const bool xleaf = tr.ibag_off[x] == tr.ibag_off[x + 1];
+        const bool xleaf = cld(tr.ibag_off, x) == cld(tr.ibag_off, x + 1);
         uint64_t raw = 0;
         if (slot_needed(xleaf, wx)) raw = SS(L, ax);
         const double qx = q_from_slot(xleaf, wx, raw);
