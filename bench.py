@@ -31,20 +31,21 @@ HBM_PEAK_GBS = 8000.0
 SWEEP_KERNEL = "sweep_tile_kernel"
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 COPY_BYTES = 1 << 30  # the --profile-only device copy
-PMC_SUMMARY = ROOT / "profiles" / "sweep_tile_road320k_pmc.md"
+PROFILES = ROOT / "profiles"
 
 
-def sweep_algorithmic_bytes(n: int, bags: int, sources: int) -> float:
-    """Bytes one sweep launch must move for `sources` sources (64 per tile).
+def sweep_algorithmic_bytes(n: int, bags: int, inner: int, sources: int) -> float:
+    """Bytes one sweep launch must move for `sources` sources, a lower bound.
 
-    Per tile, lanes = 64 sources, so a distance row is 256 B and a count or
-    quotient row 512 B: both sweeps read the distance row of every bag entry
-    (2 * 256 * bags); per node the forward writes distance + count (768 B), the
-    backward reads them back (768 B) and writes the quotient (512 B).  Count
-    and quotient reads of tight entries come on top and are not counted, so
-    this is a lower bound: 8 * bags + 32 * n bytes per source.
+    Per source: both passes read one 4-byte distance word per bag entry
+    (8 * bags); the forward pass writes one distance word per node (4 * n);
+    the backward pass, which runs only on the `inner` nodes (nodes with
+    descendants), reads each one's distance word back and writes its 8-byte
+    quotient (12 * inner).  Count words (written and read only where a path
+    count is not 1) and the quotients of tight entries are data-dependent and
+    not counted.
     """
-    return (sources / 64.0) * (512.0 * bags + 2048.0 * n)
+    return float(sources) * (8.0 * bags + 4.0 * n + 12.0 * inner)
 
 
 def _kernel_rows(csv_path: str, counter: str):
@@ -98,12 +99,14 @@ def summarize_trace(csv_path: str) -> str:
 
 
 def committed_traffic(sources: int, workload: str):
-    """Traffic per launch from the committed PMC summary, if it covers this
-    workload and launch size (summary lines `workload:` and `sources:`)."""
-    if not PMC_SUMMARY.exists():
+    """Traffic per launch from the committed PMC summary of this workload
+    (profiles/sweep_tile_<workload>_pmc.md), if it covers this launch size
+    (summary lines `workload:` and `sources:`)."""
+    path = PROFILES / f"sweep_tile_{workload}_pmc.md"
+    if not path.exists():
         return None
     meta = {}
-    for line in PMC_SUMMARY.read_text().splitlines():
+    for line in path.read_text().splitlines():
         if ":" in line and not line.startswith("|"):
             k, _, v = line.partition(":")
             meta[k.strip()] = v.strip()
@@ -231,8 +234,10 @@ def main() -> int:
     # Roofline of the dominant kernel, the sweep tile kernel: algorithmic bytes
     # of one launch over its duration (HIP events on the context's stream).
     bags = int(st["bag_entries"])
+    parent, _ = ix.tree()
+    inner = int(np.unique(parent[parent >= 0]).size)  # nodes with descendants
     sweep_ms = tm["scan_ms"]
-    algo = sweep_algorithmic_bytes(n, bags, Bs)
+    algo = sweep_algorithmic_bytes(n, bags, inner, Bs)
     achieved = algo / (sweep_ms * 1e-3) / 1e9
     traffic = committed_traffic(Bs, args.workload)
 
@@ -271,7 +276,8 @@ def main() -> int:
             "n": n, "arcs": int(g.arcs), "sweep_batch": Bs, "labelscan_batch": B, "tile": S,
             "tchunk": tuning["tchunk"], "dep_threads": tuning["bp_threads"],
             "tree_height": st["tree_height"], "tree_width": st["tree_width"],
-            "label_entries": st["label_entries"], "bag_entries": bags, "Lbar": round(lbar, 2),
+            "label_entries": st["label_entries"], "bag_entries": bags, "inner_nodes": inner,
+            "Lbar": round(lbar, 2),
             "labelscan_sources_per_s": round(scan_rate, 2),
             "sweep_phase_ms": [round(tm["scan_ms"], 3), round(tm["dep_ms"], 3), round(tm["reduce_ms"], 3)],
             "plain_sources_per_s": round(plain, 2),
