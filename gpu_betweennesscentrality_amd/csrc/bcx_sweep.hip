@@ -908,16 +908,18 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
     throw bcx::Error{BCX_EGRAPH, "bag-sweep records of one tile exceed 4 GiB (more than 5592405 vertices)"};
   const int total_tiles = (count + kWave - 1) / kWave;
   int tiles = std::max(1, std::min(c->tn.sweep_batch / kWave, std::max(1, total_tiles)));
-  // Keep the per-tile state within ~70% of the device memory not already held
+  // Keep the per-tile state within 85% of the device memory not already held
   // by this context's sweep buffers.  When that cuts the launch short and the
   // label-scan workspace is allocated, release it (it is re-allocated on its
-  // next use) and measure again.
+  // next use) and measure again.  A tile's sweep is a latency chain through
+  // the levels, so a launch of few tiles costs nearly as much as a full one:
+  // fitting the whole batch in one launch matters.
   const size_t per_tile = (size_t)n * kRec + (size_t)n * 8 + (size_t)B * 8;
   auto cap_now = [&]() {
     size_t fr = 0, tot = 0;
     HIP_OK(hipMemGetInfo(&fr, &tot));
     const size_t held = (size_t)c->sweep_tiles * per_tile;
-    return (size_t)((double)(fr + held) * 0.7) / std::max<size_t>(per_tile, 1);
+    return (size_t)((double)(fr + held) * 0.85) / std::max<size_t>(per_tile, 1);
   };
   size_t cap = cap_now();
   if (cap < (size_t)tiles && c->ws_slots > 0) {
