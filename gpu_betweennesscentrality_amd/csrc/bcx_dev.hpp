@@ -31,9 +31,11 @@ struct DevIndex {
   int64_t arcs;
   const int *depth;       // n, preorder
   const int64_t *row_off; // n + 1
-  const int *dis;         // entries
-  const uint64_t *cnt;    // entries
-  const uint32_t *hop;    // entries
+  const int *dis;         // entries + 1
+  const uint64_t *cnt;    // entries + 1
+  const uint32_t *hop;    // entries + 1
+  int64_t none;           // the entry past the labels: distance 0x7fffffff,
+                          // count 0, hop 0 (a load target that matches nothing)
   const int *rmq;         // levels * n: rmq[k*n + i] = min depth over [i, i + 2^k)
   const int *grow;        // n + 1, CSR in preorder ids
   const int *gcol;        // arcs

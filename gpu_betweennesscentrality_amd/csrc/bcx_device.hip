@@ -521,9 +521,20 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
   d.arcs = g.arcs();
   d.depth = (const int *)keep(dev_upload(ix.depth));
   d.row_off = (const int64_t *)keep(dev_upload(ix.row_off));
-  d.dis = (const int *)keep(dev_upload(ix.dis));
-  d.cnt = (const uint64_t *)keep(dev_upload(ix.cnt));
-  d.hop = (const uint32_t *)keep(dev_upload(ix.hop));
+  // Label arrays with one extra entry that matches nothing.
+  auto upload_tail = [&](const auto &h, auto tail) {
+    using T = decltype(tail);
+    T *p = nullptr;
+    HIP_OK(hipMalloc(&p, (h.size() + 1) * sizeof(T)));
+    keep(p);
+    if (!h.empty()) HIP_OK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(p + h.size(), &tail, sizeof(T), hipMemcpyHostToDevice));
+    return (const T *)p;
+  };
+  d.dis = upload_tail(ix.dis, (int)0x7fffffff);
+  d.cnt = upload_tail(ix.cnt, (uint64_t)0);
+  d.hop = upload_tail(ix.hop, (uint32_t)0);
+  d.none = (int64_t)ix.dis.size();
   d.rmq = (const int *)keep(dev_upload(rmq));
   d.grow = (const int *)keep(dev_upload(grow));
   d.gcol = (const int *)keep(dev_upload(gcol));

@@ -48,8 +48,10 @@ struct Lane {
   int t, lane, s;
   bool valid;
   int64_t rs;
-  char *tb;  // the tile's records (uniform); a record is addressed by its
-             // 32-bit offset (node * kRec), the tile spanning under 4 GB
+  char *tb;      // the tile's records (uniform); a record is addressed by its
+                 // 32-bit offset (node * kRec), the tile spanning under 4 GB
+  uint32_t sink; // offset of record n, which no node owns: stores that must
+                 // not land anywhere go there (see store_all)
 };
 
 __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &sw, int t,
@@ -61,7 +63,8 @@ __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &
   L.valid = slot < nsrc;
   L.s = sw.tile_src[slot];  // padded with a valid source beyond nsrc
   L.rs = ix.row_off[L.s];
-  L.tb = sw.state + (int64_t)t * n * kRec;
+  L.tb = sw.state + (int64_t)t * (n + 1) * kRec;
+  L.sink = (uint32_t)n * (uint32_t)kRec;
   return L;
 }
 
@@ -101,6 +104,55 @@ __device__ __forceinline__ bool one_of(int w) { return (w & kOneFlag) != 0; }
 __device__ __forceinline__ void put_dist_count(const Lane &L, uint32_t r, int d, uint64_t c) {
   SD(L, r) = d | (c == 1 ? kOneFlag : 0);
   if (c != 1) SS(L, r) = c;
+}
+
+// Stores inside the pipelined steps are issued unconditionally, with lanes
+// (or whole steps) that must not write sent to the tile's sink record.  Loads
+// and stores share one completion counter, so the wait for the next step's
+// rows (issued before these stores) can then leave exactly these stores
+// outstanding; a store under a condition would make that wait drain them.
+__device__ __forceinline__ void put_dist_count_all(const Lane &L, uint32_t r, int d, uint64_t c) {
+  SD(L, r) = d | (c == 1 ? kOneFlag : 0);
+  SS(L, c == 1 ? L.sink : r) = c;
+}
+__device__ __forceinline__ void store_all(const Lane &L, bool put, double *p, double x) {
+  double *sink = (double *)&SS(L, L.sink);
+  *(put ? p : sink) = x;
+}
+// kCount stores to the sink after a pipelined loop's first round of row
+// loads: the loop body's first wait then sees as many stores after the rows
+// on entry as on every later iteration, and leaves them outstanding.
+// The compiler barrier keeps the repeated stores from being merged.
+template <int kCount>
+__device__ __forceinline__ void sink_stores(const Lane &L) {
+#pragma unroll
+  for (int i = 0; i < kCount; ++i) {
+    SD(L, L.sink) = 0;
+    __asm__ volatile("" ::: "memory");
+  }
+}
+
+// Waits until at most kLeft loads and stores are outstanding.  At the end of
+// a pipelined step, kLeft is the next step's row loads plus this step's
+// stores: everything older, including loads whose values went unused on some
+// path, is then known complete, so no register of this step counts as pending
+// in the next (a pending register is waited for before it is overwritten).
+template <int kLeft>
+__device__ __forceinline__ void wait_vm_left() {
+  static_assert(kLeft < 64, "vmcnt field");
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_waitcnt(0x0f70 | (kLeft & 15) | ((kLeft >> 4) << 14));
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Waits for every load and store issued so far.  Used once per staged batch
+// of 64 nodes: the staged fields are then known to be in registers, so the
+// steps that read them need no wait of their own (which the compiler would
+// otherwise place, counted conservatively, inside every step).
+__device__ __forceinline__ void wait_all_vm() {
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0), expcnt and lgkmcnt not waited
+  __builtin_amdgcn_sched_barrier(0);
 }
 // Quotient Q = (1 + dD) / sigma from a raw slot word.  A node without
 // descendants has dD = 0 and its backward step is skipped, so its slot still
@@ -353,38 +405,34 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
     const int p = first + base + min(L.lane, jn - 1);
     const int4 nd = tr.fs_node[p], su = tr.fs_u[p], sl = tr.fs_len[p];
     const ulonglong2 c01 = tr.fs_cnt[2 * p], c23 = tr.fs_cnt[2 * p + 1];
+    wait_all_vm();
     const int sur[kStage] = {su.x, su.y, su.z, su.w};
     const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
     const uint64_t scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
-    // Round 1 of the step at j0 (nodes past the block repeat its last node).
-    auto round1 = [&](int j0, int (&w)[kNodesF][kStage], int (&upd)[kNodesF],
-                      uint64_t (&upc)[kNodesF]) {
+    // Round 1 of the step at j0: the staged entries' distance rows (nodes past
+    // the block repeat its last node).
+    auto round1 = [&](int j0, int (&w)[kNodesF][kStage]) {
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
-        const int v = rl(nd.x, j);
 #pragma unroll
         for (int r = 0; r < kStage; ++r) w[u][r] = SD(L, roff(rl(sur[r], j)));
-        // Up-label where v is an ancestor of s (else entry 0, not used).
-        const bool anc = v <= L.s && L.s < v + rl(nd.w, j);
-        const int64_t o = anc ? L.rs + depth : 0;
-        const int ud = ix.dis[o];
-        const uint64_t uc = ix.cnt[o];
-        upd[u] = anc ? ud : 0x7fffffff;
-        upc[u] = anc ? uc : 0;
       }
     };
-    int w[kNodesF][kStage], upd[kNodesF];
-    uint64_t upc[kNodesF];
-    round1(0, w, upd, upc);
+    int w[kNodesF][kStage];
+    round1(0, w);
+    sink_stores<2 * kNodesF>(L);  // a step stores a distance and a count row per node
     for (int j0 = 0; j0 < jn; j0 += kNodesF) {
+      // This step's rows have arrived once only the stores issued after them
+      // (the last step's, or the sink stores) are outstanding.
+      wait_vm_left<2 * kNodesF>();
       int cand[kNodesF][kStage], best[kNodesF], onem[kNodesF];
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
         const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
         onem[u] = 0;
-        best[u] = upd[u];
+        best[u] = 0x7fffffff;
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
           const int len = r < nb ? rl(slr[r], j) : kNoLen;
@@ -397,6 +445,30 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
       // the step (a load under a lane mask may not issue, so the one wait
       // that follows covers all of them), then the arithmetic and stores.
       __builtin_amdgcn_sched_barrier(0);
+      // Round 2, first the up-label where v is an ancestor of the lane's
+      // source.  A tile's sources are neighbours in tree order, so for nearly
+      // every node no lane needs it and the wavefront skips both loads; other
+      // lanes read the entry that matches nothing, so the values need no
+      // select (which would wait for them here).  The bag minimum above leaves
+      // the up-label out: where the up-label is smaller no candidate is tight
+      // and the count rows loaded below go unused; where it ties, the tight
+      // candidates are the same.
+      int upd[kNodesF];
+      uint64_t upc[kNodesF];
+#pragma unroll
+      for (int u = 0; u < kNodesF; ++u) {
+        const int j = min(j0 + u, jn - 1);
+        const int v = rl(nd.x, j), size = rl(nd.w, j);
+        const bool anc = (v <= L.s) & (L.s < v + size);
+        upd[u] = 0x7fffffff;
+        upc[u] = 0;
+        if (__builtin_amdgcn_ballot_w64(anc)) {
+          const int64_t o = anc ? L.rs + depth : ix.none;
+          upd[u] = ix.dis[o];
+          upc[u] = ix.cnt[o];
+        }
+      }
+      // Then the count rows of tight candidates not flagged 1.
       uint64_t m[kNodesF][kStage];
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
@@ -408,32 +480,29 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
           m[u][r] = need ? raw : 1;
         }
       }
-      int un[kNodesF];
-      uint64_t uc[kNodesF];
-#pragma unroll
-      for (int u = 0; u < kNodesF; ++u) {
-        un[u] = upd[u];
-        uc[u] = upc[u];
-      }
-      round1(j0 + kNodesF, w, upd, upc);
+      round1(j0 + kNodesF, w);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
-        const int j = j0 + u;
-        if (j >= jn) break;
+        const int j = min(j0 + u, jn - 1);
+        const bool live = j0 + u < jn;  // false for the clamped repeats of the last node
         const int v = rl(nd.x, j), e0 = rl(nd.y, j), e1 = rl(nd.z, j);
-        int b = best[u];
-        uint64_t c = un[u] == b ? uc[u] : 0;
+        int b = min(best[u], upd[u]);
+        uint64_t c = upd[u] == b ? upc[u] : 0;
 #pragma unroll
         for (int r = 0; r < kStage; ++r)
           if (cand[u][r] == b) {
             const uint64_t cr = rl64(scr[r], j);  // uniform; 1 for most shortcuts
             c = add_sat(c, cr == 1 ? m[u][r] : mul_sat(m[u][r], cr));
           }
-        if (e1 - e0 > kStage) fwd_bag(tr, sw, L, e0 + kStage, e1, 1, b, c);
-        put_dist_count(L, roff(v), b, c);
-        ovf |= L.valid && c == ~0ull;
+        if (live && e1 - e0 > kStage) {
+          fwd_bag(tr, sw, L, e0 + kStage, e1, 1, b, c);
+          wait_all_vm();  // leaves no load of this rare path pending in the next step
+        }
+        put_dist_count_all(L, live ? roff(v) : L.sink, b, c);
+        ovf |= live && L.valid && c == ~0ull;
       }
+      wait_vm_left<kNodesF * kStage + 2 * kNodesF>();
     }
   }
 }
@@ -458,6 +527,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
     const int p = first + base + min(L.lane, jn - 1);
     const int4 nd = tr.bs_node[p], sx = tr.bs_x[p], se = tr.bs_e[p], sl = tr.bs_len[p];
     const double2 c01 = tr.bs_cntf[2 * p], c23 = tr.bs_cntf[2 * p + 1];
+    wait_all_vm();
     const int sxr[kStage] = {sx.x, sx.y, sx.z, sx.w};
     const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
     const double scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
@@ -473,7 +543,11 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
     };
     int wvn[kNodesB], wxn[kNodesB][kStage];
     round1(0, wvn, wxn);
+    sink_stores<2 * kNodesB + 1>(L);  // a step stores flows and a quotient per node, then node flows
     for (int j0 = 0; j0 < jn; j0 += kNodesB) {
+      // This step's rows have arrived once only the stores issued after them
+      // are outstanding (as in the forward pass).
+      wait_vm_left<2 * kNodesB + 1>();
       int wv[kNodesB], wx[kNodesB][kStage];
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
@@ -537,11 +611,13 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         const double fs = wave_sum4(L.valid ? f[0] : 0.0, L.valid ? f[1] : 0.0,
                                     L.valid ? f[2] : 0.0, L.valid ? f[3] : 0.0);
         const int e = row_pick(L.lane, rl(se.x, j), rl(se.y, j), rl(se.z, j), rl(se.w, j));
-        if (live && (L.lane & 15) == 0 && (L.lane >> 4) < nb)
-          sw.phi_t[(int64_t)L.t * tr.bags + e] = fs;
-        if (live && k1 - k0 > kStage)
+        store_all(L, live && (L.lane & 15) == 0 && (L.lane >> 4) < nb,
+                  sw.phi_t + (int64_t)L.t * tr.bags + e, fs);
+        if (live && k1 - k0 > kStage) {
           acc += bwd_entries(tr, sw, L, k0 + kStage, k1, 1, dist_of(wv[u]), svv);
-        if (live) setQ(L, roff(v), (1.0 + acc) / svv);
+          wait_all_vm();  // leaves no load of this rare path pending in the next step
+        }
+        setQ(L, live ? roff(v) : L.sink, (1.0 + acc) / svv);
         ns_in[u] = (live && L.valid && L.s != v) ? acc : 0.0;
       }
       const double ns = wave_sum4(ns_in[0], kNodesB > 1 ? ns_in[1 % kNodesB] : 0.0,
@@ -550,7 +626,9 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
       const int u = L.lane >> 4;
       const int v = row_pick(L.lane, rl(nd.x, min(j0, jn - 1)), rl(nd.x, min(j0 + 1, jn - 1)),
                              rl(nd.x, min(j0 + 2, jn - 1)), rl(nd.x, min(j0 + 3, jn - 1)));
-      if ((L.lane & 15) == 0 && u < kNodesB && j0 + u < jn) sw.node_t[(int64_t)L.t * n + v] = ns;
+      store_all(L, (L.lane & 15) == 0 && u < kNodesB && j0 + u < jn,
+                sw.node_t + (int64_t)L.t * n + v, ns);
+      wait_vm_left<kNodesB * (1 + kStage) + 2 * kNodesB + 1>();
     }
   }
 }
@@ -884,7 +962,7 @@ void bcx_ctx::ensure_sweep(int tiles) {
   if (tiles <= sweep_tiles) return;
   free_sweep();
   const size_t n = (size_t)ix.n, B = (size_t)tree.bags, T = (size_t)tiles, W = kWave;
-  HIP_OK(hipMalloc(&sw.state, T * n * kRec));
+  HIP_OK(hipMalloc(&sw.state, T * (n + 1) * kRec));
   HIP_OK(hipMalloc(&sw.node_t, T * n * 8));
   HIP_OK(hipMalloc(&sw.phi_t, T * std::max<size_t>(B, 1) * 8));
   HIP_OK(hipMalloc(&sw.node, n * 8));
@@ -904,8 +982,8 @@ namespace bcx_dev {
 int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   const int n = c->ix.n;
   const int64_t B = c->tree.bags;
-  if ((uint64_t)n * (uint64_t)kRec > 0xffffffffull)
-    throw bcx::Error{BCX_EGRAPH, "bag-sweep records of one tile exceed 4 GiB (more than 5592405 vertices)"};
+  if (((uint64_t)n + 1) * (uint64_t)kRec > 0x100000000ull)
+    throw bcx::Error{BCX_EGRAPH, "bag-sweep records of one tile exceed 4 GiB (more than 5592404 vertices)"};
   const int total_tiles = (count + kWave - 1) / kWave;
   int tiles = std::max(1, std::min(c->tn.sweep_batch / kWave, std::max(1, total_tiles)));
   // Keep the per-tile state within 85% of the device memory not already held
@@ -914,7 +992,7 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   // next use) and measure again.  A tile's sweep is a latency chain through
   // the levels, so a launch of few tiles costs nearly as much as a full one:
   // fitting the whole batch in one launch matters.
-  const size_t per_tile = (size_t)n * kRec + (size_t)n * 8 + (size_t)B * 8;
+  const size_t per_tile = ((size_t)n + 1) * kRec + (size_t)n * 8 + (size_t)B * 8;
   auto cap_now = [&]() {
     size_t fr = 0, tot = 0;
     HIP_OK(hipMemGetInfo(&fr, &tot));
