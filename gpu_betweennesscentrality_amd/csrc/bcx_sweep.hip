@@ -163,12 +163,21 @@ __device__ __forceinline__ double q_from_slot(bool leaf, int w, uint64_t raw) {
   return leaf ? inv : __builtin_bit_cast(double, raw);
 }
 
-// Slot word of record r in lanes where `need` holds; other lanes read the
-// tile's first record (one hot cache line) and their value is not used.
-// Loads are issued unconditionally so that the compiler can count them
-// exactly: a load under a lane mask may not issue, and each later wait would
-// then also wait for every store and load issued since.
+// Slot word of record r in lanes where `need` holds (other lanes get a value
+// that their callers do not use).  The load runs under the lane mask, so lanes
+// that need nothing move no data through the memory pipeline, and a step
+// where no lane needs it skips it.  The pipelined steps wait for such loads
+// with explicit waits (wait_vm_left), which hold whether or not they issued.
 __device__ __forceinline__ uint64_t slot_if(bool need, uint32_t r, const Lane &L) {
+  uint64_t x = 1;
+  if (need) x = SS(L, r);
+  return x;
+}
+// The same with the load issued in every lane (lanes that need nothing read
+// the tile's first record), for a value converted right after: the compiler
+// would move a conversion of slot_if's result into its load's branch, where
+// it waits for everything outstanding.
+__device__ __forceinline__ uint64_t slot_all(bool need, uint32_t r, const Lane &L) {
   return SS(L, need ? r : 0u);
 }
 
@@ -411,6 +420,9 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
     const uint64_t scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
     // Round 1 of the step at j0: the staged entries' distance rows (nodes past
     // the block repeat its last node).
+    // Every staged place is loaded, also past the node's bag (node 0's row,
+    // a hot line): a load that may not issue would shorten every count-based
+    // wait after it, and skipping them measured slower.
     auto round1 = [&](int j0, int (&w)[kNodesF][kStage]) {
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
@@ -567,7 +579,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         const int j = min(j0 + u, jn - 1);
         const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
         {
-          const uint64_t raw = slot_if(!one_of(wv[u]), roff(rl(nd.x, j)), L);
+          const uint64_t raw = slot_all(!one_of(wv[u]), roff(rl(nd.x, j)), L);
           svr[u] = one_of(wv[u]) ? 1 : raw;
         }
         const int dv = dist_of(wv[u]);
