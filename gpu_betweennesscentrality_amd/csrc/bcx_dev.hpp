@@ -90,27 +90,30 @@ struct DevTree {
   const int *zp_pair;      // P
   const int *ep_off;       // B + 1: pairs touching each bag entry
   const int *ep_pair;      // 2P
-  // Staging slots: the node's fields and its first kStageEntries bag /
-  // inverse-bag entries in fixed places, so a wavefront stages 64 nodes with
-  // one round of coalesced loads.  Level k's nodes are split over the 16
-  // wavefronts of a workgroup by balanced work; wavefront w's i-th node sits
-  // in slot st_off[k] + w * chunk + i, chunk = (st_off[k+1] - st_off[k]) / 16,
-  // and the wavefront has st_cnt[16 k + w] nodes.  Unused entry places name
-  // node 0 with length 0 and count 0.
+  // Staging slots: a node's bag / inverse-bag entries cut into pieces of at
+  // most kStageEntries, one slot per piece holding the node's fields and the
+  // piece's entries in fixed places, so a wavefront stages 64 pieces with one
+  // round of coalesced loads.  Level k's nodes are split over the 16
+  // wavefronts of a workgroup by balanced work, a node's pieces consecutive;
+  // wavefront w's i-th slot is st_off[k] + w * chunk + i, chunk =
+  // (st_off[k+1] - st_off[k]) / 16, and the wavefront has st_cnt[16 k + w]
+  // slots.  Unused entry places name node 0 with length 0 and count 0.
   const int *st_off;          // height + 1 (forward: all nodes)
   const int *st_cnt;          // 16 * height
   const int *bst_off;         // height + 1 (backward: nodes with descendants)
   const int *bst_cnt;         // 16 * height
-  const int4 *fs_node;        // {v, bag_off[v], bag_off[v + 1], subtree[v]}
-  const int4 *fs_u;           // ancestors of bag entries 0..3
+  const int4 *fs_node;        // {v, first entry, end, subtree[v] | first piece
+                              // << 30 | last piece << 29}
+  const int4 *fs_u;           // ancestors of the piece's entries 0..3
   const int4 *fs_len;         // their shortcut lengths
-  const ulonglong2 *fs_cnt;   // 2 per node: counts of entries 0..1, 2..3
-  const int4 *bs_node;        // {v, ibag_off[v], ibag_off[v + 1], leaf mask}:
-                              // bit r set when entry r's descendant has none
-  const int4 *bs_x;           // descendants of inverse-bag entries 0..3
+  const ulonglong2 *fs_cnt;   // 2 per slot: counts of entries 0..1, 2..3
+  const int4 *bs_node;        // {v, first entry, end, leaf mask | first piece
+                              // << 4 | last piece << 5}: leaf-mask bit r set
+                              // when entry r's descendant has none
+  const int4 *bs_x;           // descendants of the piece's entries 0..3
   const int4 *bs_e;           // their entry indices
   const int4 *bs_len;         // their shortcut lengths
-  const double2 *bs_cntf;     // 2 per node: counts of entries 0..1, 2..3
+  const double2 *bs_cntf;     // 2 per slot: counts of entries 0..1, 2..3
 };
 
 constexpr int kStageEntries = 4;

@@ -645,17 +645,19 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     t.ep_pair = (const int *)keep(dev_upload(epp));
     c->h_lvl_pair_off = lpo;
     {
-      // Staging slots.  A level's nodes are split over the 16 wavefronts of a
-      // workgroup so that their work is balanced: longest first, each node to
-      // the least loaded wavefront, with cost 1 per node plus 1.8 per group of
-      // 8 entries past the 4 staged ones (measured: a two-node step and an
-      // overflow group cost about 4 us each).  Each wavefront's nodes keep
-      // their tree order, so the wavefronts of a workgroup still touch
-      // neighbouring rows at about the same time.  Wavefront w's i-th node of
-      // level k sits in slot off[k] + w * chunk + i (chunk = the largest
-      // wavefront count of the level) and its count is cnt[16 k + w].  The
-      // forward list holds every node of the level, the backward list the
-      // nodes with descendants (non-empty inverse bag).
+      // Staging slots.  A node's (inverse-)bag entries are cut into pieces of
+      // at most kStageEntries, one slot per piece, so that every entry of a
+      // wide level goes through the same pipelined step.  A level's nodes are
+      // split over the 16 wavefronts of a workgroup so that their work is
+      // balanced: longest first, each node to the least loaded wavefront, with
+      // cost 1 per piece.  A node's pieces stay together and in order (the
+      // wavefront carries a node's partial result from piece to piece), and
+      // each wavefront's nodes keep their tree order, so the wavefronts of a
+      // workgroup still touch neighbouring rows at about the same time.
+      // Wavefront w's i-th slot of level k is off[k] + w * chunk + i (chunk =
+      // the largest wavefront count of the level) and its count of slots is
+      // cnt[16 k + w].  The forward list holds every node of the level, the
+      // backward list the nodes with descendants (non-empty inverse bag).
       const int H = ix.tree_height, kW = 16;
       auto leaf = [&](int32_t v) { return ioff[(size_t)v] == ioff[(size_t)v + 1]; };
       std::vector<int32_t> blvl_off((size_t)H + 1, 0), blvl_v;
@@ -665,6 +667,9 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
           if (!leaf(lvl_v[(size_t)q])) blvl_v.push_back(lvl_v[(size_t)q]);
         blvl_off[(size_t)k + 1] = (int32_t)blvl_v.size();
       }
+      auto pieces = [](int64_t e) {
+        return (int32_t)std::max<int64_t>(1, (e + kStageEntries - 1) / kStageEntries);
+      };
       auto slots = [&](const std::vector<int32_t> &off, const std::vector<int32_t> &list,
                        auto entries, std::vector<int32_t> &st_off, std::vector<int32_t> &st_cnt,
                        std::vector<int32_t> &slot_of) {
@@ -676,15 +681,14 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
           const int32_t lo = off[(size_t)k], cnt = off[(size_t)k + 1] - lo;
           order.resize((size_t)cnt);
           wave_of.assign((size_t)cnt, 0);
-          std::vector<double> cost((size_t)cnt);
+          std::vector<int32_t> cost((size_t)cnt);
           for (int32_t i = 0; i < cnt; ++i) {
-            const int64_t e = entries(list[(size_t)(lo + i)]);
-            cost[(size_t)i] = 1.0 + (e > kStageEntries ? 1.8 * (double)((e - kStageEntries + 7) / 8) : 0.0);
+            cost[(size_t)i] = pieces(entries(list[(size_t)(lo + i)]));
             order[(size_t)i] = i;
           }
           std::stable_sort(order.begin(), order.end(),
                            [&](int32_t a, int32_t b) { return cost[(size_t)a] > cost[(size_t)b]; });
-          double load[16] = {0};
+          int64_t load[16] = {0};
           for (int32_t i : order) {
             int w = 0;
             for (int x = 1; x < kW; ++x)
@@ -695,13 +699,15 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
           int32_t chunk = 0;
           for (int32_t i = 0; i < cnt; ++i) {
             const int w = wave_of[(size_t)i];
-            chunk = std::max(chunk, ++st_cnt[(size_t)k * kW + (size_t)w]);
+            st_cnt[(size_t)k * kW + (size_t)w] += cost[(size_t)i];
+            chunk = std::max(chunk, st_cnt[(size_t)k * kW + (size_t)w]);
           }
           st_off[(size_t)k + 1] = st_off[(size_t)k] + kW * chunk;
           int32_t pos[16] = {0};
           for (int32_t i = 0; i < cnt; ++i) {  // tree order within each wavefront
             const int w = wave_of[(size_t)i];
-            slot_of[(size_t)(lo + i)] = st_off[(size_t)k] + w * chunk + pos[w]++;
+            slot_of[(size_t)(lo + i)] = st_off[(size_t)k] + w * chunk + pos[w];
+            pos[w] += cost[(size_t)i];
           }
         }
         return (size_t)st_off[(size_t)H];
@@ -718,36 +724,49 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       std::vector<int32_t> bnode(std::max<size_t>(Sb, 1) * 4, 0), bx(std::max<size_t>(Sb, 1) * 4, 0),
           be(std::max<size_t>(Sb, 1) * 4, 0), blen(std::max<size_t>(Sb, 1) * 4, 0);
       std::vector<double> bcnt(std::max<size_t>(Sb, 1) * 4, 0.0);
+      // Forward piece: {v, first entry, end, subtree size | first piece << 30
+      // | last piece << 29}; backward piece: {v, first entry, end, leaf mask
+      // | first piece << 4 | last piece << 5}.
       for (int32_t q = 0; q < n; ++q) {
         const int32_t v = lvl_v[(size_t)q];
-        const size_t o = (size_t)slot_of[(size_t)q] * 4;
-        fnode[o] = v;
-        fnode[o + 1] = boff[(size_t)v];
-        fnode[o + 2] = boff[(size_t)v + 1];
-        fnode[o + 3] = ix.subtree[(size_t)v];
-        for (int r = 0; r < kStageEntries && boff[(size_t)v] + r < boff[(size_t)v + 1]; ++r) {
-          const size_t e = (size_t)boff[(size_t)v] + (size_t)r;
-          fu[o + r] = ix.bag[e];
-          flen[o + r] = ix.bag_len[e];
-          fcnt[o + r] = ix.bag_cnt[e];
+        const int32_t np = pieces(boff[(size_t)v + 1] - boff[(size_t)v]);
+        for (int32_t pc = 0; pc < np; ++pc) {
+          const size_t o = (size_t)(slot_of[(size_t)q] + pc) * 4;
+          const int32_t e0 = boff[(size_t)v] + pc * kStageEntries;
+          const int32_t e1 = std::min(e0 + kStageEntries, boff[(size_t)v + 1]);
+          fnode[o] = v;
+          fnode[o + 1] = e0;
+          fnode[o + 2] = e1;
+          fnode[o + 3] = ix.subtree[(size_t)v] | (pc == 0 ? 1 << 30 : 0) | (pc == np - 1 ? 1 << 29 : 0);
+          for (int32_t r = 0; e0 + r < e1; ++r) {
+            const size_t e = (size_t)(e0 + r);
+            fu[o + (size_t)r] = ix.bag[e];
+            flen[o + (size_t)r] = ix.bag_len[e];
+            fcnt[o + (size_t)r] = ix.bag_cnt[e];
+          }
         }
       }
       for (size_t q = 0; q < blvl_v.size(); ++q) {
         const int32_t v = blvl_v[q];
-        const size_t o = (size_t)bslot_of[q] * 4;
-        int32_t leafm = 0;
-        for (int r = 0; r < kStageEntries && ioff[(size_t)v] + r < ioff[(size_t)v + 1]; ++r) {
-          const size_t k = (size_t)ioff[(size_t)v] + (size_t)r;
-          bx[o + r] = ix_x[k];
-          be[o + r] = ix_e[k];
-          blen[o + r] = ix.bag_len[(size_t)ix_e[k]];
-          bcnt[o + r] = (double)ix.bag_cnt[(size_t)ix_e[k]];
-          if (leaf(ix_x[k])) leafm |= 1 << r;
+        const int32_t np = pieces(ioff[(size_t)v + 1] - ioff[(size_t)v]);
+        for (int32_t pc = 0; pc < np; ++pc) {
+          const size_t o = (size_t)(bslot_of[q] + pc) * 4;
+          const int32_t k0 = ioff[(size_t)v] + pc * kStageEntries;
+          const int32_t k1 = std::min(k0 + kStageEntries, ioff[(size_t)v + 1]);
+          int32_t leafm = 0;
+          for (int32_t r = 0; k0 + r < k1; ++r) {
+            const size_t k = (size_t)(k0 + r);
+            bx[o + (size_t)r] = ix_x[k];
+            be[o + (size_t)r] = ix_e[k];
+            blen[o + (size_t)r] = ix.bag_len[(size_t)ix_e[k]];
+            bcnt[o + (size_t)r] = (double)ix.bag_cnt[(size_t)ix_e[k]];
+            if (leaf(ix_x[k])) leafm |= 1 << r;
+          }
+          bnode[o] = v;
+          bnode[o + 1] = k0;
+          bnode[o + 2] = k1;
+          bnode[o + 3] = leafm | (pc == 0 ? 1 << 4 : 0) | (pc == np - 1 ? 1 << 5 : 0);
         }
-        bnode[o] = v;
-        bnode[o + 1] = ioff[(size_t)v];
-        bnode[o + 2] = ioff[(size_t)v + 1];
-        bnode[o + 3] = leafm;
       }
       // Inverse-bag descendants with the sign bit set for those without
       // descendants of their own.

@@ -393,22 +393,25 @@ __device__ __forceinline__ void level_block(const int *st_off, const int *st_cnt
 }
 
 // Forward over this wavefront's block of a level (all at tree depth `depth`).
-// Lane j stages node j's fields and first kStage bag entries with one round
-// of loads; the wavefront then handles kNodesF nodes per step.  Round 1 of a
-// step loads the distance words of the nodes' staged entries (and the
-// up-label where the node is an ancestor of the lane's source); round 2 the
-// path-count rows of tight candidates not flagged 1.  The steps are
-// pipelined: a step issues the next step's round 1 after its own round 2 and
-// before its stores, so every wait is for loads that are already old (stores
-// and loads share one completion counter).  Saturating sums of saturating
-// products do not depend on the order of their terms, so the count equals
-// that of a running min-and-count.
+// Lane j stages piece j (a node's fields and up to kStage of its bag entries)
+// with one round of loads; the wavefront then handles kNodesF pieces per
+// step.  Round 1 of a step loads the distance words of the pieces' entries;
+// round 2 the up-label (first piece, where the node is an ancestor of the
+// lane's source) and the path-count rows of tight candidates not flagged 1.
+// The steps are pipelined: a step issues the next step's round 1 after its
+// own round 2 and before its stores, so every wait is for loads that are
+// already old (stores and loads share one completion counter).  A node's
+// minimum and count are carried from piece to piece and stored by its last
+// piece.  Saturating sums of saturating products do not depend on the order
+// of their terms, so the count equals that of a running min-and-count.
 template <int kNodesF>
 __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree &tr,
                                                const SweepWork &sw, const Lane &L, int wave,
                                                int depth, int &ovf) {
   int first, mine;
   level_block(tr.st_off, tr.st_cnt, depth, wave, first, mine);
+  int cb = 0x7fffffff;  // the node's minimum and count over its earlier pieces
+  uint64_t cc = 0;
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
     const int p = first + base + min(L.lane, jn - 1);
@@ -470,11 +473,11 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
-        const int v = rl(nd.x, j), size = rl(nd.w, j);
+        const int v = rl(nd.x, j), fl = rl(nd.w, j), size = fl & ((1 << 29) - 1);
         const bool anc = (v <= L.s) & (L.s < v + size);
         upd[u] = 0x7fffffff;
         upc[u] = 0;
-        if (__builtin_amdgcn_ballot_w64(anc)) {
+        if (((fl >> 30) & 1) && __builtin_amdgcn_ballot_w64(anc)) {  // first piece only
           const int64_t o = anc ? L.rs + depth : ix.none;
           upd[u] = ix.dis[o];
           upc[u] = ix.cnt[o];
@@ -497,8 +500,9 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
-        const bool live = j0 + u < jn;  // false for the clamped repeats of the last node
-        const int v = rl(nd.x, j), e0 = rl(nd.y, j), e1 = rl(nd.z, j);
+        const bool live = j0 + u < jn;  // false for the clamped repeats of the last piece
+        const int v = rl(nd.x, j), fl = rl(nd.w, j);
+        const bool pfirst = (fl >> 30) & 1, fin = live && ((fl >> 29) & 1);
         int b = min(best[u], upd[u]);
         uint64_t c = upd[u] == b ? upc[u] : 0;
 #pragma unroll
@@ -507,12 +511,18 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
             const uint64_t cr = rl64(scr[r], j);  // uniform; 1 for most shortcuts
             c = add_sat(c, cr == 1 ? m[u][r] : mul_sat(m[u][r], cr));
           }
-        if (live && e1 - e0 > kStage) {
-          fwd_bag(tr, sw, L, e0 + kStage, e1, 1, b, c);
-          wait_all_vm();  // leaves no load of this rare path pending in the next step
+        // Merge with the node's earlier pieces (saturating sums do not
+        // depend on how their terms are grouped).
+        if (!pfirst) {
+          c = cb < b ? cc : cb == b ? add_sat(c, cc) : c;
+          b = min(b, cb);
         }
-        put_dist_count_all(L, live ? roff(v) : L.sink, b, c);
-        ovf |= live && L.valid && c == ~0ull;
+        if (live) {
+          cb = b;
+          cc = c;
+        }
+        put_dist_count_all(L, fin ? roff(v) : L.sink, b, c);
+        ovf |= fin && L.valid && c == ~0ull;
       }
       wait_vm_left<kNodesF * kStage + 2 * kNodesF>();
     }
@@ -521,12 +531,14 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 
 // Backward over this wavefront's block of a level's nodes that have
 // descendants (a node without descendants has dD = 0 and no flows; its Q is
-// derived from its count by the readers), with the same staging and kNodesB
-// nodes per step.  Round 1 loads the nodes' distance words and their staged
-// entries' distance words; round 2 the nodes' counts (lanes not flagged 1) and
-// the tight entries' quotients; steps are pipelined as in the forward pass.
-// The flows of a node's staged entries and the node flows of the step's nodes
-// are each one four-way transposed lane sum.
+// derived from its count by the readers), with the same staging by pieces
+// (of the inverse bag) and kNodesB pieces per step.  Round 1 loads the node's
+// distance word and the piece's entries' distance words; round 2 the node's
+// count (lanes not flagged 1) and the tight entries' quotients; steps are
+// pipelined as in the forward pass.  The node's dependency part is summed
+// piece by piece; its last piece writes Q and the node flow.  The flows of a
+// piece's entries and the node flows of the step are each one four-way
+// transposed lane sum.
 template <int kNodesB>
 __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWork &sw,
                                                const Lane &L, int wave, int depth) {
@@ -534,6 +546,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
   const int n = tr.n;
   int first, mine;
   level_block(tr.bst_off, tr.bst_cnt, depth, wave, first, mine);
+  double carry = 0.0;  // the node's flow sum over its earlier pieces
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
     const int p = first + base + min(L.lane, jn - 1);
@@ -607,12 +620,14 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
           qx[u][r] = ((tm[u] >> r) & 1) ? q_from_slot((leafm >> r) & 1, wx[u][r], qr[u][r]) : 0.0;
       }
       double ns_in[kNodesB];
+      int fin_mask = 0;  // the step's pieces that end their node
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
-        const bool live = j0 + u < jn;  // false for the clamped repeats of the last node
-        const int v = rl(nd.x, j), k0 = rl(nd.y, j), k1 = rl(nd.z, j);
-        const int nb = min(kStage, k1 - k0);
+        const bool live = j0 + u < jn;  // false for the clamped repeats of the last piece
+        const int v = rl(nd.x, j), k0 = rl(nd.y, j), k1 = rl(nd.z, j), fl = rl(nd.w, j);
+        const int nb = k1 - k0;  // at most kStage
+        const bool pfirst = (fl >> 4) & 1, fin = live && ((fl >> 5) & 1);
         const double svv = svu[u];
         double f[kStage], acc = 0.0;
 #pragma unroll
@@ -625,12 +640,12 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         const int e = row_pick(L.lane, rl(se.x, j), rl(se.y, j), rl(se.z, j), rl(se.w, j));
         store_all(L, live && (L.lane & 15) == 0 && (L.lane >> 4) < nb,
                   sw.phi_t + (int64_t)L.t * tr.bags + e, fs);
-        if (live && k1 - k0 > kStage) {
-          acc += bwd_entries(tr, sw, L, k0 + kStage, k1, 1, dist_of(wv[u]), svv);
-          wait_all_vm();  // leaves no load of this rare path pending in the next step
-        }
-        setQ(L, live ? roff(v) : L.sink, (1.0 + acc) / svv);
-        ns_in[u] = (live && L.valid && L.s != v) ? acc : 0.0;
+        // The node's dependency part, summed piece by piece in entry order.
+        const double tot = pfirst ? acc : carry + acc;
+        if (live) carry = tot;
+        setQ(L, fin ? roff(v) : L.sink, (1.0 + tot) / svv);
+        ns_in[u] = (fin && L.valid && L.s != v) ? tot : 0.0;
+        fin_mask |= (int)fin << u;
       }
       const double ns = wave_sum4(ns_in[0], kNodesB > 1 ? ns_in[1 % kNodesB] : 0.0,
                                   kNodesB > 2 ? ns_in[2 % kNodesB] : 0.0,
@@ -638,7 +653,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
       const int u = L.lane >> 4;
       const int v = row_pick(L.lane, rl(nd.x, min(j0, jn - 1)), rl(nd.x, min(j0 + 1, jn - 1)),
                              rl(nd.x, min(j0 + 2, jn - 1)), rl(nd.x, min(j0 + 3, jn - 1)));
-      store_all(L, (L.lane & 15) == 0 && u < kNodesB && j0 + u < jn,
+      store_all(L, (L.lane & 15) == 0 && u < kNodesB && ((fin_mask >> u) & 1),
                 sw.node_t + (int64_t)L.t * n + v, ns);
       wait_vm_left<kNodesB * (1 + kStage) + 2 * kNodesB + 1>();
     }
