@@ -98,10 +98,17 @@ struct DevTree {
   // wavefront w's i-th slot is st_off[k] + w * chunk + i, chunk =
   // (st_off[k+1] - st_off[k]) / 16, and the wavefront has st_cnt[16 k + w]
   // slots.  Unused entry places name node 0 with length 0 and count 0.
+  // A level whose pieces are split over the wavefronts in contiguous runs
+  // (lvl_nseg[k] > 0) leaves per-node partial results (segments) in LDS,
+  // wavefront w's first one numbered st_seg[16 k + w], merged after the level.
   const int *st_off;          // height + 1 (forward: all nodes)
   const int *st_cnt;          // 16 * height
+  const int *st_seg;          // 16 * height
+  const int *lvl_nseg;        // height: segments of a level split in runs, or 0
   const int *bst_off;         // height + 1 (backward: nodes with descendants)
   const int *bst_cnt;         // 16 * height
+  const int *bst_seg;         // 16 * height
+  const int *blvl_nseg;       // height
   const int4 *fs_node;        // {v, first entry, end, subtree[v] | first piece
                               // << 30 | last piece << 29}
   const int4 *fs_u;           // ancestors of the piece's entries 0..3
@@ -117,6 +124,8 @@ struct DevTree {
 };
 
 constexpr int kStageEntries = 4;
+// Per-node partial results a level split in runs may leave (LDS space).
+constexpr int kSweepMaxSegments = 48;
 
 // Per-tile state of the sweep: one record per (tile, node) with the lanes'
 // values, per-tile sums [tile][node] and [tile][entry].

@@ -646,18 +646,27 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     c->h_lvl_pair_off = lpo;
     {
       // Staging slots.  A node's (inverse-)bag entries are cut into pieces of
-      // at most kStageEntries, one slot per piece, so that every entry of a
-      // wide level goes through the same pipelined step.  A level's nodes are
-      // split over the 16 wavefronts of a workgroup so that their work is
-      // balanced: longest first, each node to the least loaded wavefront, with
-      // cost 1 per piece.  A node's pieces stay together and in order (the
-      // wavefront carries a node's partial result from piece to piece), and
-      // each wavefront's nodes keep their tree order, so the wavefronts of a
-      // workgroup still touch neighbouring rows at about the same time.
+      // at most kStageEntries, one slot per piece, so that every entry goes
+      // through the same pipelined step.  A level's pieces are split over the
+      // 16 wavefronts of a workgroup in one of two ways, whichever finishes
+      // first by piece count:
+      //  * by node: longest first, each node to the least loaded wavefront
+      //    (a node's pieces stay together and in order: the wavefront carries
+      //    its partial result from piece to piece and stores the result);
+      //  * in contiguous runs of the level's piece list (levels of few nodes
+      //    with many pieces).  A node may then span wavefronts: each wavefront
+      //    leaves one partial result per node it touches (a segment) in LDS,
+      //    and after the level the segments of a node are merged in order.
+      //    Costed at 2 pieces for the merge; at most kSweepMaxSegments
+      //    segments, so only for levels of up to kSweepMaxSegments - 16 nodes.
+      // Either way each wavefront's pieces keep tree order, so the wavefronts
+      // of a workgroup touch neighbouring rows at about the same time.
       // Wavefront w's i-th slot of level k is off[k] + w * chunk + i (chunk =
-      // the largest wavefront count of the level) and its count of slots is
-      // cnt[16 k + w].  The forward list holds every node of the level, the
-      // backward list the nodes with descendants (non-empty inverse bag).
+      // the largest wavefront count of the level), its count of slots is
+      // cnt[16 k + w] and, when split in runs, its first segment is
+      // seg[16 k + w] out of nseg[k] (0 for levels split by node).  The
+      // forward list holds every node of the level, the backward list the
+      // nodes with descendants (non-empty inverse bag).
       const int H = ix.tree_height, kW = 16;
       auto leaf = [&](int32_t v) { return ioff[(size_t)v] == ioff[(size_t)v + 1]; };
       std::vector<int32_t> blvl_off((size_t)H + 1, 0), blvl_v;
@@ -670,20 +679,29 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       auto pieces = [](int64_t e) {
         return (int32_t)std::max<int64_t>(1, (e + kStageEntries - 1) / kStageEntries);
       };
+      struct Staging {
+        std::vector<int32_t> off, cnt, seg, nseg, slot;  // slot: per piece, list order
+        std::vector<int64_t> pp;                         // first piece of each list node
+      };
       auto slots = [&](const std::vector<int32_t> &off, const std::vector<int32_t> &list,
-                       auto entries, std::vector<int32_t> &st_off, std::vector<int32_t> &st_cnt,
-                       std::vector<int32_t> &slot_of) {
-        st_off.assign((size_t)H + 1, 0);
-        st_cnt.assign((size_t)H * kW, 0);
-        slot_of.assign((size_t)off[(size_t)H], 0);
-        std::vector<int32_t> order, wave_of;
+                       auto entries, Staging &st) {
+        const size_t N = (size_t)off[(size_t)H];
+        st.pp.assign(N + 1, 0);
+        for (size_t i = 0; i < N; ++i) st.pp[i + 1] = st.pp[i] + pieces(entries(list[i]));
+        st.slot.assign((size_t)st.pp[N], 0);
+        st.off.assign((size_t)H + 1, 0);
+        st.cnt.assign((size_t)H * kW, 0);
+        st.seg.assign((size_t)H * kW, 0);
+        st.nseg.assign((size_t)H, 0);
+        std::vector<int32_t> order, wave_of, cost;
         for (int k = 0; k < H; ++k) {
           const int32_t lo = off[(size_t)k], cnt = off[(size_t)k + 1] - lo;
+          const int64_t p0 = st.pp[(size_t)lo], P = st.pp[(size_t)(lo + cnt)] - p0;
           order.resize((size_t)cnt);
           wave_of.assign((size_t)cnt, 0);
-          std::vector<int32_t> cost((size_t)cnt);
+          cost.resize((size_t)cnt);
           for (int32_t i = 0; i < cnt; ++i) {
-            cost[(size_t)i] = pieces(entries(list[(size_t)(lo + i)]));
+            cost[(size_t)i] = (int32_t)(st.pp[(size_t)(lo + i) + 1] - st.pp[(size_t)(lo + i)]);
             order[(size_t)i] = i;
           }
           std::stable_sort(order.begin(), order.end(),
@@ -696,29 +714,52 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
             load[w] += cost[(size_t)i];
             wave_of[(size_t)i] = w;
           }
+          const int64_t by_node = *std::max_element(load, load + kW);
+          const bool runs = cnt <= kSweepMaxSegments - kW && (P + kW - 1) / kW + 2 < by_node;
           int32_t chunk = 0;
-          for (int32_t i = 0; i < cnt; ++i) {
-            const int w = wave_of[(size_t)i];
-            st_cnt[(size_t)k * kW + (size_t)w] += cost[(size_t)i];
-            chunk = std::max(chunk, st_cnt[(size_t)k * kW + (size_t)w]);
+          if (!runs) {
+            for (int32_t i = 0; i < cnt; ++i) {
+              const int w = wave_of[(size_t)i];
+              st.cnt[(size_t)k * kW + (size_t)w] += cost[(size_t)i];
+              chunk = std::max(chunk, st.cnt[(size_t)k * kW + (size_t)w]);
+            }
+            int32_t pos[16] = {0};
+            for (int32_t i = 0; i < cnt; ++i) {  // tree order within each wavefront
+              const int w = wave_of[(size_t)i];
+              for (int32_t pc = 0; pc < cost[(size_t)i]; ++pc)
+                st.slot[(size_t)(st.pp[(size_t)(lo + i)] + pc)] = st.off[(size_t)k] + w * chunk + pos[w]++;
+            }
+          } else {
+            // Wavefront w takes the level's pieces [P w / 16, P (w + 1) / 16).
+            std::vector<char> first((size_t)P, 0);
+            for (int32_t i = 0; i < cnt; ++i) first[(size_t)(st.pp[(size_t)(lo + i)] - p0)] = 1;
+            for (int w = 0; w < kW; ++w)
+              chunk = std::max<int32_t>(chunk, (int32_t)(P * (w + 1) / kW - P * w / kW));
+            int32_t seg = 0;
+            for (int w = 0; w < kW; ++w) {
+              const int64_t a = P * w / kW, b = P * (w + 1) / kW;
+              st.cnt[(size_t)k * kW + (size_t)w] = (int32_t)(b - a);
+              st.seg[(size_t)k * kW + (size_t)w] = seg;
+              for (int64_t l = a; l < b; ++l) {
+                st.slot[(size_t)(p0 + l)] = st.off[(size_t)k] + w * chunk + (int32_t)(l - a);
+                if (l == a || first[(size_t)l]) ++seg;
+              }
+            }
+            if (seg > kSweepMaxSegments)  // cannot happen: seg <= cnt + 15
+              throw bcx::Error{BCX_EGRAPH, "bag-sweep staging: too many segments"};
+            st.nseg[(size_t)k] = seg;
           }
-          st_off[(size_t)k + 1] = st_off[(size_t)k] + kW * chunk;
-          int32_t pos[16] = {0};
-          for (int32_t i = 0; i < cnt; ++i) {  // tree order within each wavefront
-            const int w = wave_of[(size_t)i];
-            slot_of[(size_t)(lo + i)] = st_off[(size_t)k] + w * chunk + pos[w];
-            pos[w] += cost[(size_t)i];
-          }
+          st.off[(size_t)k + 1] = st.off[(size_t)k] + kW * chunk;
         }
-        return (size_t)st_off[(size_t)H];
+        return (size_t)st.off[(size_t)H];
       };
-      std::vector<int32_t> st_off, st_cnt, slot_of, bst_off, bst_cnt, bslot_of;
+      Staging fst, bst;
       const size_t S = slots(lvl_off, lvl_v,
                              [&](int32_t v) { return (int64_t)(boff[(size_t)v + 1] - boff[(size_t)v]); },
-                             st_off, st_cnt, slot_of);
+                             fst);
       const size_t Sb = slots(blvl_off, blvl_v,
                               [&](int32_t v) { return (int64_t)(ioff[(size_t)v + 1] - ioff[(size_t)v]); },
-                              bst_off, bst_cnt, bslot_of);
+                              bst);
       std::vector<int32_t> fnode(S * 4, 0), fu(S * 4, 0), flen(S * 4, 0);
       std::vector<uint64_t> fcnt(S * 4, 0);
       std::vector<int32_t> bnode(std::max<size_t>(Sb, 1) * 4, 0), bx(std::max<size_t>(Sb, 1) * 4, 0),
@@ -726,12 +767,13 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       std::vector<double> bcnt(std::max<size_t>(Sb, 1) * 4, 0.0);
       // Forward piece: {v, first entry, end, subtree size | first piece << 30
       // | last piece << 29}; backward piece: {v, first entry, end, leaf mask
-      // | first piece << 4 | last piece << 5}.
+      // | first piece << 4 | last piece << 5}.  (The bag sweep refuses graphs
+      // of 2^29 vertices and more long before.)
       for (int32_t q = 0; q < n; ++q) {
         const int32_t v = lvl_v[(size_t)q];
         const int32_t np = pieces(boff[(size_t)v + 1] - boff[(size_t)v]);
         for (int32_t pc = 0; pc < np; ++pc) {
-          const size_t o = (size_t)(slot_of[(size_t)q] + pc) * 4;
+          const size_t o = (size_t)fst.slot[(size_t)(fst.pp[(size_t)q] + pc)] * 4;
           const int32_t e0 = boff[(size_t)v] + pc * kStageEntries;
           const int32_t e1 = std::min(e0 + kStageEntries, boff[(size_t)v + 1]);
           fnode[o] = v;
@@ -750,7 +792,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
         const int32_t v = blvl_v[q];
         const int32_t np = pieces(ioff[(size_t)v + 1] - ioff[(size_t)v]);
         for (int32_t pc = 0; pc < np; ++pc) {
-          const size_t o = (size_t)(bslot_of[q] + pc) * 4;
+          const size_t o = (size_t)bst.slot[(size_t)(bst.pp[q] + pc)] * 4;
           const int32_t k0 = ioff[(size_t)v] + pc * kStageEntries;
           const int32_t k1 = std::min(k0 + kStageEntries, ioff[(size_t)v + 1]);
           int32_t leafm = 0;
@@ -776,10 +818,14 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       t.ibag_x = (const int *)keep(dev_upload(ix_xs));
       t.blvl_off = (const int *)keep(dev_upload(blvl_off));
       t.blvl_v = (const int *)keep(dev_upload(blvl_v));
-      t.bst_off = (const int *)keep(dev_upload(bst_off));
-      t.st_off = (const int *)keep(dev_upload(st_off));
-      t.st_cnt = (const int *)keep(dev_upload(st_cnt));
-      t.bst_cnt = (const int *)keep(dev_upload(bst_cnt));
+      t.st_off = (const int *)keep(dev_upload(fst.off));
+      t.st_cnt = (const int *)keep(dev_upload(fst.cnt));
+      t.st_seg = (const int *)keep(dev_upload(fst.seg));
+      t.lvl_nseg = (const int *)keep(dev_upload(fst.nseg));
+      t.bst_off = (const int *)keep(dev_upload(bst.off));
+      t.bst_cnt = (const int *)keep(dev_upload(bst.cnt));
+      t.bst_seg = (const int *)keep(dev_upload(bst.seg));
+      t.blvl_nseg = (const int *)keep(dev_upload(bst.nseg));
       t.fs_node = (const int4 *)keep(dev_upload(fnode));
       t.fs_u = (const int4 *)keep(dev_upload(fu));
       t.fs_len = (const int4 *)keep(dev_upload(flen));

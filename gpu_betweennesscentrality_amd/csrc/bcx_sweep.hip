@@ -156,7 +156,9 @@ __device__ __forceinline__ void wait_all_vm() {
 }
 // Quotient Q = (1 + dD) / sigma from a raw slot word.  A node without
 // descendants has dD = 0 and its backward step is skipped, so its slot still
-// holds sigma (not read when flagged 1); other nodes hold Q itself.
+// holds sigma (not read when flagged 1); other nodes hold Q itself.  (Storing
+// 1 / sigma from the forward pass instead moves the divisions there, and
+// measured no faster.)
 __device__ __forceinline__ bool slot_needed(bool leaf, int w) { return !(leaf && one_of(w)); }
 __device__ __forceinline__ double q_from_slot(bool leaf, int w, uint64_t raw) {
   const double inv = 1.0 / (double)(one_of(w) ? 1 : raw);
@@ -263,100 +265,6 @@ __device__ __forceinline__ int row_pick(int lane, int x0, int x1, int x2, int x3
 constexpr int kSweepThreads = 1024;
 constexpr int kSweepWaves = kSweepThreads / kWave;
 
-// Forward candidates of node v from bag entries [e, e_end) with a stride,
-// merged into the running (best, c).  Entries are taken kGroup at a time so
-// their distance rows are in flight together; a path count is read only in
-// the lanes where its candidate ties the new minimum.
-constexpr int kGroup = 8;  // bag entries whose rows are loaded together
-
-__device__ __forceinline__ void fwd_bag(const DevTree &tr, const SweepWork &sw, const Lane &L,
-                                        int e, int e_end, int stride, int &best, uint64_t &c) {
-  for (; e < e_end; e += kGroup * stride) {
-    int cand[kGroup], w[kGroup];
-    uint32_t a[kGroup];
-    int b = best;
-#pragma unroll
-    for (int q = 0; q < kGroup; ++q) {
-      const int eq = e + q * stride;
-      const bool ok = eq < e_end;
-      const int ec = ok ? eq : e;
-      a[q] = roff(cld(tr.bag_v, ec));
-      w[q] = SD(L, a[q]);  // loaded in every case
-      const int len = ok ? cld(tr.bag_len, ec) : kNoLen;
-      cand[q] = dist_of(w[q]) + len;
-      b = min(b, cand[q]);
-    }
-    uint64_t m[kGroup];
-#pragma unroll
-    for (int q = 0; q < kGroup; ++q) {
-      const bool need = cand[q] == b && !one_of(w[q]);
-      const uint64_t raw = slot_if(need, a[q], L);
-      m[q] = need ? raw : 1;
-    }
-    if (b < best) c = 0;
-    best = b;
-#pragma unroll
-    for (int q = 0; q < kGroup; ++q) {
-      const int eq = e + q * stride;
-      if (eq >= e_end) break;
-      if (cand[q] == b) c = add_sat(c, mul_sat(m[q], cld(tr.bag_cnt, eq)));
-    }
-  }
-}
-
-// Backward pulls of node v over inverse-bag entries [k, k_end) with a stride,
-// kGroup at a time; writes the tile's flow on each entry and returns the lane's
-// dependency part.  A descendant x that is done holds, in its 8-byte slot,
-// the quotient Q(x) = (1 + dD(x)) / sigma(x), so an entry costs the distance
-// row of x plus, only in lanes where the entry is tight, one row of Q(x).  A
-// descendant without descendants of its own (marked by the sign bit of its
-// id in ibag_x) has Q = 1 / sigma, read from its count (or its flag).
-// Must be called with all 64 lanes active.
-__device__ __forceinline__ double bwd_entries(const DevTree &tr, const SweepWork &sw,
-                                              const Lane &L, int k, int k_end, int stride,
-                                              int dv, double sv) {
-  double acc = 0.0;
-  for (; k < k_end; k += kGroup * stride) {
-    int wx[kGroup], ee[kGroup], len[kGroup];
-    bool leaf[kGroup];
-    uint32_t ax[kGroup];
-#pragma unroll
-    for (int q = 0; q < kGroup; ++q) {
-      const int kq = k + q * stride;
-      const int kc = kq < k_end ? kq : k;
-      const int xr = cld(tr.ibag_x, kc);
-      leaf[q] = xr < 0;
-      ee[q] = cld(tr.ibag_e, kc);
-      len[q] = kq < k_end ? cld(tr.bag_len, ee[q]) : kNoLen;
-      ax[q] = roff(xr & 0x7fffffff);
-      wx[q] = SD(L, ax[q]);
-    }
-    uint64_t raw[kGroup];
-    bool tight[kGroup];
-#pragma unroll
-    for (int q = 0; q < kGroup; ++q) {
-      tight[q] = dv + len[q] == dist_of(wx[q]);
-      raw[q] = slot_if(tight[q] && slot_needed(leaf[q], wx[q]), ax[q], L);
-    }
-    double f[kGroup];
-#pragma unroll
-    for (int q = 0; q < kGroup; ++q) {
-      const double qx = tight[q] ? q_from_slot(leaf[q], wx[q], raw[q]) : 0.0;
-      f[q] = cld(tr.bag_cntf, ee[q]) * sv * qx;
-      acc += f[q];
-    }
-#pragma unroll
-    for (int h = 0; h < kGroup; h += 4) {
-      const double fs = wave_sum4(L.valid ? f[h] : 0.0, L.valid ? f[h + 1] : 0.0,
-                                  L.valid ? f[h + 2] : 0.0, L.valid ? f[h + 3] : 0.0);
-      const int q = h + (L.lane >> 4);
-      const int e = row_pick(L.lane, ee[h], ee[h + 1], ee[h + 2], ee[h + 3]);
-      if ((L.lane & 15) == 0 && k + q * stride < k_end) sw.phi_t[(int64_t)L.t * tr.bags + e] = fs;
-    }
-  }
-  return acc;
-}
-
 __device__ __forceinline__ int rl(int x, int j) { return __builtin_amdgcn_readlane(x, j); }
 __device__ __forceinline__ double rlf(double x, int j) {
   const uint64_t b = __builtin_bit_cast(uint64_t, x);
@@ -392,6 +300,14 @@ __device__ __forceinline__ void level_block(const int *st_off, const int *st_cnt
   count = cld(st_cnt, k * kSweepWaves + wave);
 }
 
+// LDS segments of a level split in runs (see the staging): per segment, two
+// 64-bit words per lane and the node it belongs to.
+struct Segs {
+  uint64_t (*x)[kWave];
+  uint64_t (*y)[kWave];
+  int *node;
+};
+
 // Forward over this wavefront's block of a level (all at tree depth `depth`).
 // Lane j stages piece j (a node's fields and up to kStage of its bag entries)
 // with one round of loads; the wavefront then handles kNodesF pieces per
@@ -404,12 +320,13 @@ __device__ __forceinline__ void level_block(const int *st_off, const int *st_cnt
 // minimum and count are carried from piece to piece and stored by its last
 // piece.  Saturating sums of saturating products do not depend on the order
 // of their terms, so the count equals that of a running min-and-count.
-template <int kNodesF>
+template <int kNodesF, bool runs>  // runs: split in runs, results go to LDS segments
 __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree &tr,
                                                const SweepWork &sw, const Lane &L, int wave,
-                                               int depth, int &ovf) {
+                                               int depth, int &ovf, const Segs &sg) {
   int first, mine;
   level_block(tr.st_off, tr.st_cnt, depth, wave, first, mine);
+  int seg = runs ? cld(tr.st_seg, depth * kSweepWaves + wave) : 0;
   int cb = 0x7fffffff;  // the node's minimum and count over its earlier pieces
   uint64_t cc = 0;
   for (int base = 0; base < mine; base += kWave) {
@@ -502,7 +419,12 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
         const int j = min(j0 + u, jn - 1);
         const bool live = j0 + u < jn;  // false for the clamped repeats of the last piece
         const int v = rl(nd.x, j), fl = rl(nd.w, j);
-        const bool pfirst = (fl >> 30) & 1, fin = live && ((fl >> 29) & 1);
+        // A result starts at the node's first piece (or at the wavefront's
+        // first, when split in runs) and ends at its last (or the
+        // wavefront's last).
+        const int pos = base + j0 + u;
+        const bool pfirst = ((fl >> 30) & 1) || (runs && pos == 0);
+        const bool fin = live && (((fl >> 29) & 1) || (runs && pos == mine - 1));
         int b = min(best[u], upd[u]);
         uint64_t c = upd[u] == b ? upc[u] : 0;
 #pragma unroll
@@ -521,8 +443,18 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
           cb = b;
           cc = c;
         }
-        put_dist_count_all(L, fin ? roff(v) : L.sink, b, c);
-        ovf |= fin && L.valid && c == ~0ull;
+        if constexpr (runs) {
+          if (fin) {  // uniform
+            sg.x[seg][L.lane] = c;
+            sg.y[seg][L.lane] = (uint32_t)b;
+            if (L.lane == 0) sg.node[seg] = v;
+            ++seg;
+          }
+          put_dist_count_all(L, L.sink, b, c);
+        } else {
+          put_dist_count_all(L, fin ? roff(v) : L.sink, b, c);
+          ovf |= fin && L.valid && c == ~0ull;
+        }
       }
       wait_vm_left<kNodesF * kStage + 2 * kNodesF>();
     }
@@ -539,13 +471,15 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 // piece by piece; its last piece writes Q and the node flow.  The flows of a
 // piece's entries and the node flows of the step are each one four-way
 // transposed lane sum.
-template <int kNodesB>
+template <int kNodesB, bool runs>  // runs: split in runs, results go to LDS segments
 __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWork &sw,
-                                               const Lane &L, int wave, int depth) {
+                                               const Lane &L, int wave, int depth,
+                                               const Segs &sg) {
   static_assert(kNodesB <= 4 && kStage == 4, "four-way lane sums");
   const int n = tr.n;
   int first, mine;
   level_block(tr.bst_off, tr.bst_cnt, depth, wave, first, mine);
+  int seg = runs ? cld(tr.bst_seg, depth * kSweepWaves + wave) : 0;
   double carry = 0.0;  // the node's flow sum over its earlier pieces
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
@@ -627,7 +561,9 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         const bool live = j0 + u < jn;  // false for the clamped repeats of the last piece
         const int v = rl(nd.x, j), k0 = rl(nd.y, j), k1 = rl(nd.z, j), fl = rl(nd.w, j);
         const int nb = k1 - k0;  // at most kStage
-        const bool pfirst = (fl >> 4) & 1, fin = live && ((fl >> 5) & 1);
+        const int pos = base + j0 + u;  // result boundaries as in the forward pass
+        const bool pfirst = ((fl >> 4) & 1) || (runs && pos == 0);
+        const bool fin = live && (((fl >> 5) & 1) || (runs && pos == mine - 1));
         const double svv = svu[u];
         double f[kStage], acc = 0.0;
 #pragma unroll
@@ -643,9 +579,22 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         // The node's dependency part, summed piece by piece in entry order.
         const double tot = pfirst ? acc : carry + acc;
         if (live) carry = tot;
-        setQ(L, fin ? roff(v) : L.sink, (1.0 + tot) / svv);
-        ns_in[u] = (fin && L.valid && L.s != v) ? tot : 0.0;
-        fin_mask |= (int)fin << u;
+        if constexpr (runs) {
+          if (fin) {  // uniform
+            sg.x[seg][L.lane] = __builtin_bit_cast(uint64_t, tot);
+            sg.y[seg][L.lane] = __builtin_bit_cast(uint64_t, svv);
+            if (L.lane == 0) sg.node[seg] = v;
+            ++seg;
+          }
+          setQ(L, L.sink, 0.0);
+          ns_in[u] = 0.0;
+        } else {
+          double q = 0.0;
+          if (fin) q = (1.0 + tot) / svv;  // uniform branch: only the last piece divides
+          setQ(L, fin ? roff(v) : L.sink, q);
+          ns_in[u] = (fin && L.valid && L.s != v) ? tot : 0.0;
+          fin_mask |= (int)fin << u;
+        }
       }
       const double ns = wave_sum4(ns_in[0], kNodesB > 1 ? ns_in[1 % kNodesB] : 0.0,
                                   kNodesB > 2 ? ns_in[2 % kNodesB] : 0.0,
@@ -662,17 +611,20 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
 
 // One workgroup per tile of 64 sources runs the whole sweep for its tile:
 // forward over depth levels, backward over depth levels, then the ascending
-// states.  Levels narrower than the workgroup's 16 wavefronts are processed
-// node by node with the node's entries split over all wavefronts and the
-// partial results merged in wavefront order, so results are deterministic.
+// states.  Every level runs the pipelined steps over its staged pieces; a
+// level split in runs (few nodes, many pieces) then merges each node's
+// per-wavefront partial results from LDS in a fixed order, so results are
+// deterministic.
 template <int kNodesF, int kNodesB>
 __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, DevTree tr,
                                                                    SweepWork sw, int nsrc,
                                                                    int *ovf_flag,
                                                                    uint64_t *stamps) {
   __shared__ double sh_acc[kSweepWaves][kWave];
-  __shared__ int sh_best[kSweepWaves][kWave];
-  __shared__ uint64_t sh_cnt[kSweepWaves][kWave];
+  // Segments of levels split in runs: per-node partial results.
+  __shared__ uint64_t seg_x[kSweepMaxSegments][kWave], seg_y[kSweepMaxSegments][kWave];
+  __shared__ int seg_node[kSweepMaxSegments];
+  const Segs sg{seg_x, seg_y, seg_node};
   const Lane L = lane_setup(ix, sw, blockIdx.x, nsrc, tr.n);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int n = tr.n, H = tr.height;
@@ -682,73 +634,57 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
 
   // ---- forward: distance and path count ----
   for (int k = 0; k < H; ++k) {
-    const int lo = cld(tr.lvl_off, k), cnt = cld(tr.lvl_off, k + 1) - lo;
-    if (cnt >= kSweepWaves) {
-      fwd_level_wave<kNodesF>(ix, tr, sw, L, wave, k, ovf);
-    } else {
-      for (int q = 0; q < cnt; ++q) {
-        const int v = cld(tr.lvl_v, lo + q);
-        int best = 0x7fffffff;
-        uint64_t c = 0;
-        if (wave == 0 && v <= L.s && L.s < v + cld(tr.subtree, v)) {
-          const int64_t o = L.rs + k;
-          best = ix.dis[o];
-          c = ix.cnt[o];
-        }
-        fwd_bag(tr, sw, L, cld(tr.bag_off, v) + wave, cld(tr.bag_off, v + 1), kSweepWaves, best, c);
-        sh_best[wave][L.lane] = best;
-        sh_cnt[wave][L.lane] = c;
-        __syncthreads();
-        if (wave == 0) {
-          int b = sh_best[0][L.lane];
-          uint64_t cc = sh_cnt[0][L.lane];
-          for (int w = 1; w < kSweepWaves; ++w) {
-            const int bw = sh_best[w][L.lane];
-            if (bw < b) {
-              b = bw;
-              cc = sh_cnt[w][L.lane];
-            } else if (bw == b) {
-              cc = add_sat(cc, sh_cnt[w][L.lane]);
-            }
-          }
-          put_dist_count(L, roff(v), b, cc);
-          ovf |= L.valid && cc == ~0ull;
-        }
-        __syncthreads();
-      }
-    }
+    const int nseg = cld(tr.lvl_nseg, k);
+    if (nseg)
+      fwd_level_wave<kNodesF, true>(ix, tr, sw, L, wave, k, ovf, sg);
+    else
+      fwd_level_wave<kNodesF, false>(ix, tr, sw, L, wave, k, ovf, sg);
     __syncthreads();
+    if (nseg) {
+      // Each node's segments, merged in order (min, then saturating count).
+      for (int s = wave; s < nseg; s += kSweepWaves) {
+        const int v = __builtin_amdgcn_readfirstlane(seg_node[s]);
+        if (s > 0 && __builtin_amdgcn_readfirstlane(seg_node[s - 1]) == v) continue;
+        int b = (int)seg_y[s][L.lane];
+        uint64_t c = seg_x[s][L.lane];
+        for (int t = s + 1; t < nseg && __builtin_amdgcn_readfirstlane(seg_node[t]) == v; ++t) {
+          const int bt = (int)seg_y[t][L.lane];
+          const uint64_t ct = seg_x[t][L.lane];
+          c = bt < b ? ct : bt == b ? add_sat(c, ct) : c;
+          b = min(b, bt);
+        }
+        put_dist_count(L, roff(v), b, c);
+        ovf |= L.valid && c == ~0ull;
+      }
+      __syncthreads();
+    }
   }
 
   if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
 
   // ---- backward: dependencies, shortcut flows, node flows ----
   for (int k = H - 1; k >= 0; --k) {
-    const int lo = cld(tr.blvl_off, k), cnt = cld(tr.blvl_off, k + 1) - lo;
-    if (cnt >= kSweepWaves) {
-      bwd_level_wave<kNodesB>(tr, sw, L, wave, k);
-    } else {
-      for (int q = 0; q < cnt; ++q) {
-        const int v = cld(tr.blvl_v, lo + q);
-        const uint32_t av = roff(v);
-        const int wv = SD(L, av);
-        uint64_t sv = 1;
-        if (!one_of(wv)) sv = SS(L, av);
-        const double svv = (double)sv;
-        sh_acc[wave][L.lane] = bwd_entries(tr, sw, L, cld(tr.ibag_off, v) + wave, cld(tr.ibag_off, v + 1),
-                                           kSweepWaves, dist_of(wv), svv);
-        __syncthreads();
-        if (wave == 0) {
-          double acc = 0.0;
-          for (int w = 0; w < kSweepWaves; ++w) acc += sh_acc[w][L.lane];
-          setQ(L, av, (1.0 + acc) / svv);
-          const double ns = wave_sum_dpp((L.valid && L.s != v) ? acc : 0.0);
-          if (L.lane == 0) sw.node_t[(int64_t)L.t * tr.n + v] = ns;
-        }
-        __syncthreads();
-      }
-    }
+    const int nseg = cld(tr.blvl_nseg, k);
+    if (nseg)
+      bwd_level_wave<kNodesB, true>(tr, sw, L, wave, k, sg);
+    else
+      bwd_level_wave<kNodesB, false>(tr, sw, L, wave, k, sg);
     __syncthreads();
+    if (nseg) {
+      // Each node's segments, summed in order; then its quotient and flow.
+      for (int s = wave; s < nseg; s += kSweepWaves) {
+        const int v = __builtin_amdgcn_readfirstlane(seg_node[s]);
+        if (s > 0 && __builtin_amdgcn_readfirstlane(seg_node[s - 1]) == v) continue;
+        double tot = __builtin_bit_cast(double, seg_x[s][L.lane]);
+        const double svv = __builtin_bit_cast(double, seg_y[s][L.lane]);
+        for (int t = s + 1; t < nseg && __builtin_amdgcn_readfirstlane(seg_node[t]) == v; ++t)
+          tot += __builtin_bit_cast(double, seg_x[t][L.lane]);
+        setQ(L, roff(v), (1.0 + tot) / svv);
+        const double ns = wave_sum_dpp((L.valid && L.s != v) ? tot : 0.0);
+        if (L.lane == 0) sw.node_t[(int64_t)L.t * n + v] = ns;
+      }
+      __syncthreads();
+    }
   }
 
   if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x + 2] = __builtin_amdgcn_s_memrealtime();
