@@ -97,7 +97,8 @@ struct DevTree {
   // wavefronts of a workgroup by balanced work, a node's pieces consecutive;
   // wavefront w's i-th slot is st_off[k] + w * chunk + i, chunk =
   // (st_off[k+1] - st_off[k]) / 16, and the wavefront has st_cnt[16 k + w]
-  // slots.  Unused entry places name node 0 with length 0 and count 0.
+  // slots.  Unused entry places name node 0 with length kSweepNoLen (never
+  // minimal or tight) and count 0.
   // A level whose pieces are split over the wavefronts in contiguous runs
   // (lvl_nseg[k] > 0) leaves per-node partial results (segments) in LDS,
   // wavefront w's first one numbered st_seg[16 k + w], merged after the level.
@@ -109,14 +110,16 @@ struct DevTree {
   const int *bst_cnt;         // 16 * height
   const int *bst_seg;         // 16 * height
   const int *blvl_nseg;       // height
-  const int4 *fs_node;        // {v, first entry, end, subtree[v] | first piece
-                              // << 30 | last piece << 29}
+  const int4 *fs_node;        // {v, entries in the piece, 0, subtree[v] | count-1
+                              // mask << 24 | last piece << 29 | first << 30}:
+                              // mask bit r set when entry r's count is 1
   const int4 *fs_u;           // ancestors of the piece's entries 0..3
   const int4 *fs_len;         // their shortcut lengths
   const ulonglong2 *fs_cnt;   // 2 per slot: counts of entries 0..1, 2..3
-  const int4 *bs_node;        // {v, first entry, end, leaf mask | first piece
-                              // << 4 | last piece << 5}: leaf-mask bit r set
-                              // when entry r's descendant has none
+  const int4 *bs_node;        // {v, entries in the piece, 0, leaf mask | first
+                              // piece << 4 | last piece << 5 | count-1 mask
+                              // << 8}: leaf-mask bit r set when entry r's
+                              // descendant has none
   const int4 *bs_x;           // descendants of the piece's entries 0..3
   const int4 *bs_e;           // their entry indices
   const int4 *bs_len;         // their shortcut lengths
@@ -124,6 +127,10 @@ struct DevTree {
 };
 
 constexpr int kStageEntries = 4;
+// Length of unused staged places: a distance plus it exceeds every true
+// distance (below 999999999) without overflowing, so such an entry is never
+// minimal or tight.
+constexpr int kSweepNoLen = (1 << 30) - 1;
 // Per-node partial results a level split in runs may leave (LDS space).
 constexpr int kSweepMaxSegments = 48;
 

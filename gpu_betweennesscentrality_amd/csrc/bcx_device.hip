@@ -760,15 +760,15 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       const size_t Sb = slots(blvl_off, blvl_v,
                               [&](int32_t v) { return (int64_t)(ioff[(size_t)v + 1] - ioff[(size_t)v]); },
                               bst);
-      std::vector<int32_t> fnode(S * 4, 0), fu(S * 4, 0), flen(S * 4, 0);
+      std::vector<int32_t> fnode(S * 4, 0), fu(S * 4, 0), flen(S * 4, kSweepNoLen);
       std::vector<uint64_t> fcnt(S * 4, 0);
       std::vector<int32_t> bnode(std::max<size_t>(Sb, 1) * 4, 0), bx(std::max<size_t>(Sb, 1) * 4, 0),
-          be(std::max<size_t>(Sb, 1) * 4, 0), blen(std::max<size_t>(Sb, 1) * 4, 0);
+          be(std::max<size_t>(Sb, 1) * 4, 0), blen(std::max<size_t>(Sb, 1) * 4, kSweepNoLen);
       std::vector<double> bcnt(std::max<size_t>(Sb, 1) * 4, 0.0);
-      // Forward piece: {v, first entry, end, subtree size | first piece << 30
-      // | last piece << 29}; backward piece: {v, first entry, end, leaf mask
-      // | first piece << 4 | last piece << 5}.  (The bag sweep refuses graphs
-      // of 2^29 vertices and more long before.)
+      // Forward piece: {v, entries, 0, subtree size | count-1 mask << 24 | last
+      // piece << 29 | first piece << 30}; backward piece: {v, entries, 0, leaf
+      // mask | first piece << 4 | last piece << 5 | count-1 mask << 8}.  (The
+      // bag sweep refuses graphs of 2^24 vertices and more long before.)
       for (int32_t q = 0; q < n; ++q) {
         const int32_t v = lvl_v[(size_t)q];
         const int32_t np = pieces(boff[(size_t)v + 1] - boff[(size_t)v]);
@@ -776,16 +776,18 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
           const size_t o = (size_t)fst.slot[(size_t)(fst.pp[(size_t)q] + pc)] * 4;
           const int32_t e0 = boff[(size_t)v] + pc * kStageEntries;
           const int32_t e1 = std::min(e0 + kStageEntries, boff[(size_t)v + 1]);
-          fnode[o] = v;
-          fnode[o + 1] = e0;
-          fnode[o + 2] = e1;
-          fnode[o + 3] = ix.subtree[(size_t)v] | (pc == 0 ? 1 << 30 : 0) | (pc == np - 1 ? 1 << 29 : 0);
+          int32_t ones = 0;
           for (int32_t r = 0; e0 + r < e1; ++r) {
             const size_t e = (size_t)(e0 + r);
             fu[o + (size_t)r] = ix.bag[e];
             flen[o + (size_t)r] = ix.bag_len[e];
             fcnt[o + (size_t)r] = ix.bag_cnt[e];
+            if (ix.bag_cnt[e] == 1) ones |= 1 << r;
           }
+          fnode[o] = v;
+          fnode[o + 1] = e1 - e0;
+          fnode[o + 3] = ix.subtree[(size_t)v] | ones << 24 | (pc == np - 1 ? 1 << 29 : 0) |
+                         (pc == 0 ? 1 << 30 : 0);
         }
       }
       for (size_t q = 0; q < blvl_v.size(); ++q) {
@@ -795,7 +797,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
           const size_t o = (size_t)bst.slot[(size_t)(bst.pp[q] + pc)] * 4;
           const int32_t k0 = ioff[(size_t)v] + pc * kStageEntries;
           const int32_t k1 = std::min(k0 + kStageEntries, ioff[(size_t)v + 1]);
-          int32_t leafm = 0;
+          int32_t leafm = 0, ones = 0;
           for (int32_t r = 0; k0 + r < k1; ++r) {
             const size_t k = (size_t)(k0 + r);
             bx[o + (size_t)r] = ix_x[k];
@@ -803,11 +805,11 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
             blen[o + (size_t)r] = ix.bag_len[(size_t)ix_e[k]];
             bcnt[o + (size_t)r] = (double)ix.bag_cnt[(size_t)ix_e[k]];
             if (leaf(ix_x[k])) leafm |= 1 << r;
+            if (ix.bag_cnt[(size_t)ix_e[k]] == 1) ones |= 1 << r;
           }
           bnode[o] = v;
-          bnode[o + 1] = k0;
-          bnode[o + 2] = k1;
-          bnode[o + 3] = leafm | (pc == 0 ? 1 << 4 : 0) | (pc == np - 1 ? 1 << 5 : 0);
+          bnode[o + 1] = k1 - k0;
+          bnode[o + 3] = leafm | (pc == 0 ? 1 << 4 : 0) | (pc == np - 1 ? 1 << 5 : 0) | ones << 8;
         }
       }
       // Inverse-bag descendants with the sign bit set for those without

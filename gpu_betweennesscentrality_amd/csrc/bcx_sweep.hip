@@ -98,7 +98,8 @@ constexpr int kOneFlag = 1 << 30;
 // an entry is never minimal or tight.  Selecting it on the scalar length keeps
 // the vector arithmetic on loaded rows free of branches (a uniform select of a
 // loaded value becomes a branch that waits for the load).
-constexpr int kNoLen = kOneFlag - 1;
+constexpr int kNoLen = kSweepNoLen;
+static_assert(kNoLen == kOneFlag - 1, "lengths of unused places stay below the flag bit");
 __device__ __forceinline__ int dist_of(int w) { return w & (kOneFlag - 1); }
 __device__ __forceinline__ bool one_of(int w) { return (w & kOneFlag) != 0; }
 __device__ __forceinline__ void put_dist_count(const Lane &L, uint32_t r, int d, uint64_t c) {
@@ -362,12 +363,11 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
-        const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
         onem[u] = 0;
         best[u] = 0x7fffffff;
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          const int len = r < nb ? rl(slr[r], j) : kNoLen;
+          const int len = rl(slr[r], j);  // kNoLen in unused places
           cand[u][r] = dist_of(w[u][r]) + len;
           onem[u] |= (int)one_of(w[u][r]) << r;
           best[u] = min(best[u], cand[u][r]);
@@ -390,7 +390,7 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
-        const int v = rl(nd.x, j), fl = rl(nd.w, j), size = fl & ((1 << 29) - 1);
+        const int v = rl(nd.x, j), fl = rl(nd.w, j), size = fl & ((1 << 24) - 1);
         const bool anc = (v <= L.s) & (L.s < v + size);
         upd[u] = 0x7fffffff;
         upc[u] = 0;
@@ -429,10 +429,8 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
         uint64_t c = upd[u] == b ? upc[u] : 0;
 #pragma unroll
         for (int r = 0; r < kStage; ++r)
-          if (cand[u][r] == b) {
-            const uint64_t cr = rl64(scr[r], j);  // uniform; 1 for most shortcuts
-            c = add_sat(c, cr == 1 ? m[u][r] : mul_sat(m[u][r], cr));
-          }
+          if (cand[u][r] == b)  // shortcut counts are 1 for most entries (a uniform bit)
+            c = add_sat(c, ((fl >> (24 + r)) & 1) ? m[u][r] : mul_sat(m[u][r], rl64(scr[r], j)));
         // Merge with the node's earlier pieces (saturating sums do not
         // depend on how their terms are grouped).
         if (!pfirst) {
@@ -524,7 +522,6 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
-        const int nb = min(kStage, rl(nd.z, j) - rl(nd.y, j));
         {
           const uint64_t raw = slot_all(!one_of(wv[u]), roff(rl(nd.x, j)), L);
           svr[u] = one_of(wv[u]) ? 1 : raw;
@@ -534,7 +531,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         tm[u] = 0;
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          const int len = r < nb ? rl(slr[r], j) : kNoLen;
+          const int len = rl(slr[r], j);  // kNoLen in unused places
           const bool t = dv + len == dist_of(wx[u][r]);
           tm[u] |= (int)t << r;
           qr[u][r] = slot_if(t && slot_needed((leafm >> r) & 1, wx[u][r]), roff(rl(sxr[r], j)), L);
@@ -550,8 +547,13 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         const int leafm = rl(nd.w, j);
         svu[u] = (double)svr[u];
 #pragma unroll
-        for (int r = 0; r < kStage; ++r)
-          qx[u][r] = ((tm[u] >> r) & 1) ? q_from_slot((leafm >> r) & 1, wx[u][r], qr[u][r]) : 0.0;
+        for (int r = 0; r < kStage; ++r) {
+          // Q(x) where the entry is tight: a quotient from x's backward step,
+          // or, where x has no descendants (a uniform bit), 1 / sigma(x).
+          double q = __builtin_bit_cast(double, qr[u][r]);
+          if ((leafm >> r) & 1) q = 1.0 / (double)(one_of(wx[u][r]) ? 1 : qr[u][r]);
+          qx[u][r] = ((tm[u] >> r) & 1) ? q : 0.0;
+        }
       }
       double ns_in[kNodesB];
       int fin_mask = 0;  // the step's pieces that end their node
@@ -559,8 +561,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
         const bool live = j0 + u < jn;  // false for the clamped repeats of the last piece
-        const int v = rl(nd.x, j), k0 = rl(nd.y, j), k1 = rl(nd.z, j), fl = rl(nd.w, j);
-        const int nb = k1 - k0;  // at most kStage
+        const int v = rl(nd.x, j), nb = rl(nd.y, j), fl = rl(nd.w, j);  // nb <= kStage
         const int pos = base + j0 + u;  // result boundaries as in the forward pass
         const bool pfirst = ((fl >> 4) & 1) || (runs && pos == 0);
         const bool fin = live && (((fl >> 5) & 1) || (runs && pos == mine - 1));
@@ -568,7 +569,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         double f[kStage], acc = 0.0;
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          f[r] = rlf(scr[r], j) * svv * qx[u][r];
+          f[r] = ((fl >> (8 + r)) & 1) ? svv * qx[u][r] : rlf(scr[r], j) * svv * qx[u][r];
           acc += f[r];
         }
         const double fs = wave_sum4(L.valid ? f[0] : 0.0, L.valid ? f[1] : 0.0,
