@@ -532,7 +532,9 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
           const int len = rl(slr[r], j);  // kNoLen in unused places
-          const bool t = dv + len == dist_of(wx[u][r]);
+          // Padding lanes (sources past the batch) count as never tight, so
+          // their flows are 0 and the lane sums below need no mask.
+          const bool t = L.valid && dv + len == dist_of(wx[u][r]);
           tm[u] |= (int)t << r;
           qr[u][r] = slot_if(t && slot_needed((leafm >> r) & 1, wx[u][r]), roff(rl(sxr[r], j)), L);
         }
@@ -572,8 +574,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
           f[r] = ((fl >> (8 + r)) & 1) ? svv * qx[u][r] : rlf(scr[r], j) * svv * qx[u][r];
           acc += f[r];
         }
-        const double fs = wave_sum4(L.valid ? f[0] : 0.0, L.valid ? f[1] : 0.0,
-                                    L.valid ? f[2] : 0.0, L.valid ? f[3] : 0.0);
+        const double fs = wave_sum4(f[0], f[1], f[2], f[3]);
         const int e = row_pick(L.lane, rl(se.x, j), rl(se.y, j), rl(se.z, j), rl(se.w, j));
         store_all(L, live && (L.lane & 15) == 0 && (L.lane >> 4) < nb,
                   sw.phi_t + (int64_t)L.t * tr.bags + e, fs);
