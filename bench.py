@@ -12,6 +12,7 @@ baseline are reported beside it.
   python bench.py --profile-only            # one batch of each form + a 1 GiB copy
   python bench.py --summarize-trace CSV     # rocprofv3 kernel_stats.csv -> markdown
   python bench.py --summarize-pmc FETCH_CSV WRITE_CSV   # HBM traffic per launch
+  python bench.py --summarize-counters CSV...  # any counters of the sweep kernel
 """
 from __future__ import annotations
 
@@ -31,6 +32,7 @@ HBM_PEAK_GBS = 8000.0
 SWEEP_KERNEL = "sweep_tile_kernel"
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 COPY_BYTES = 1 << 30  # the --profile-only device copy
+XCDS, CUS, SIMDS = 8, 256, 1024  # MI355X: accelerator dies, compute units, SIMDs
 PROFILES = ROOT / "profiles"
 
 
@@ -85,6 +87,47 @@ def summarize_pmc(fetch_csv: str, write_csv: str) -> str:
             f"| FETCH_SIZE | {fk:.0f} | {fc:.0f} | {ff:.3f} | {fetch:.4e} |\n"
             f"| WRITE_SIZE | {wk:.0f} | {wc:.0f} | {wf:.3f} | {write:.4e} |\n\n"
             f"traffic_bytes_per_launch: {fetch + write:.6e}\n")
+
+
+def summarize_counters(csv_paths) -> str:
+    """Every counter of the sweep kernel's dispatch in rocprofv3
+    counter_collection.csv files (summed over the rows of that dispatch), and
+    the shares they imply: per-SIMD issue busy fractions from the SQ counters
+    (counted in 4-cycle units) and per-CU busy fractions of the texture
+    address (TA) and data (TD) units, all over the kernel's cycles
+    (GRBM_GUI_ACTIVE, one count per XCD)."""
+    import csv
+    tot = {}
+    for path in csv_paths:
+        disp = {}
+        for r in csv.DictReader(open(path)):
+            if SWEEP_KERNEL not in r["Kernel_Name"]:
+                continue
+            key = (r["Dispatch_Id"], r["Counter_Name"])
+            disp[key] = disp.get(key, 0.0) + float(r["Counter_Value"])
+        last = max(d for d, _ in disp) if disp else None
+        for (d, name), v in disp.items():
+            if d == last:
+                tot[name] = v
+    out = ["| counter | value |", "|---|---|"]
+    out += [f"| {k} | {v:.4e} |" for k, v in sorted(tot.items())]
+    cyc = tot.get("GRBM_GUI_ACTIVE", 0.0) / XCDS
+    if cyc:
+        out += ["", "| derived | value |", "|---|---|", f"| kernel cycles | {cyc:.4e} |"]
+        for k, label in (("SQ_ACTIVE_INST_VALU", "VALU busy per SIMD"),
+                         ("SQ_ACTIVE_INST_SCA", "scalar busy per SIMD")):
+            if k in tot:
+                out.append(f"| {label} | {4 * tot[k] / SIMDS / cyc:.1%} |")
+        for k, label in (("TA_TA_BUSY_sum", "TA busy per CU"), ("TD_TD_BUSY_sum", "TD busy per CU"),
+                         ("TD_TC_STALL_sum", "TD stalled on the L1 per CU")):
+            if k in tot:
+                out.append(f"| {label} | {tot[k] / CUS / cyc:.1%} |")
+        if "SQ_WAVE_CYCLES" in tot:
+            for k, label in (("SQ_WAIT_ANY", "wave cycles waiting on memory counters"),
+                             ("SQ_WAIT_INST_ANY", "wave cycles waiting to issue")):
+                if k in tot:
+                    out.append(f"| {label} | {tot[k] / tot['SQ_WAVE_CYCLES']:.1%} |")
+    return "\n".join(out) + "\n"
 
 
 def summarize_trace(csv_path: str) -> str:
@@ -142,7 +185,12 @@ def main() -> int:
                     help="print a markdown table of a rocprofv3 kernel_stats.csv and exit")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the sweep kernel's HBM traffic from two counter_collection.csv")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="CSV",
+                    help="print every counter of the sweep kernel from counter_collection.csv files")
     args = ap.parse_args()
+    if args.summarize_counters:
+        print(summarize_counters(args.summarize_counters), end="")
+        return 0
     if args.summarize_trace:
         print(summarize_trace(args.summarize_trace), end="")
         return 0
