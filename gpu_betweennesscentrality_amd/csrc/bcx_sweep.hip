@@ -160,7 +160,7 @@ __device__ __forceinline__ void wait_all_vm() {
 // holds sigma (not read when flagged 1); other nodes hold Q itself.  (Storing
 // 1 / sigma from the forward pass instead moves the divisions there, and
 // measured no faster.)
-__device__ __forceinline__ bool slot_needed(bool leaf, int w) { return !(leaf && one_of(w)); }
+__device__ __forceinline__ bool slot_needed(bool leaf, int w) { return !(leaf & one_of(w)); }
 __device__ __forceinline__ double q_from_slot(bool leaf, int w, uint64_t raw) {
   const double inv = 1.0 / (double)(one_of(w) ? 1 : raw);
   return leaf ? inv : __builtin_bit_cast(double, raw);
@@ -407,7 +407,7 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
         const int j = min(j0 + u, jn - 1);
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          const bool need = cand[u][r] == best[u] && !((onem[u] >> r) & 1);
+          const bool need = (cand[u][r] == best[u]) & !((onem[u] >> r) & 1);
           const uint64_t raw = slot_if(need, roff(rl(sur[r], j)), L);
           m[u][r] = need ? raw : 1;
         }
@@ -451,7 +451,7 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
           put_dist_count_all(L, L.sink, b, c);
         } else {
           put_dist_count_all(L, fin ? roff(v) : L.sink, b, c);
-          ovf |= fin && L.valid && c == ~0ull;
+          ovf |= fin & L.valid & (c == ~0ull);
         }
       }
       wait_vm_left<kNodesF * kStage + 2 * kNodesF>();
@@ -534,9 +534,9 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
           const int len = rl(slr[r], j);  // kNoLen in unused places
           // Padding lanes (sources past the batch) count as never tight, so
           // their flows are 0 and the lane sums below need no mask.
-          const bool t = L.valid && dv + len == dist_of(wx[u][r]);
+          const bool t = L.valid & (dv + len == dist_of(wx[u][r]));
           tm[u] |= (int)t << r;
-          qr[u][r] = slot_if(t && slot_needed((leafm >> r) & 1, wx[u][r]), roff(rl(sxr[r], j)), L);
+          qr[u][r] = slot_if(t & slot_needed((leafm >> r) & 1, wx[u][r]), roff(rl(sxr[r], j)), L);
         }
       }
       round1(j0 + kNodesB, wvn, wxn);
@@ -576,7 +576,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         }
         const double fs = wave_sum4(f[0], f[1], f[2], f[3]);
         const int e = row_pick(L.lane, rl(se.x, j), rl(se.y, j), rl(se.z, j), rl(se.w, j));
-        store_all(L, live && (L.lane & 15) == 0 && (L.lane >> 4) < nb,
+        store_all(L, live & ((L.lane & 15) == 0) & ((L.lane >> 4) < nb),
                   sw.phi_t + (int64_t)L.t * tr.bags + e, fs);
         // The node's dependency part, summed piece by piece in entry order.
         const double tot = pfirst ? acc : carry + acc;
@@ -594,7 +594,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
           double q = 0.0;
           if (fin) q = (1.0 + tot) / svv;  // uniform branch: only the last piece divides
           setQ(L, fin ? roff(v) : L.sink, q);
-          ns_in[u] = (fin && L.valid && L.s != v) ? tot : 0.0;
+          ns_in[u] = (fin & L.valid & (L.s != v)) ? tot : 0.0;
           fin_mask |= (int)fin << u;
         }
       }
@@ -604,7 +604,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
       const int u = L.lane >> 4;
       const int v = row_pick(L.lane, rl(nd.x, min(j0, jn - 1)), rl(nd.x, min(j0 + 1, jn - 1)),
                              rl(nd.x, min(j0 + 2, jn - 1)), rl(nd.x, min(j0 + 3, jn - 1)));
-      store_all(L, (L.lane & 15) == 0 && u < kNodesB && ((fin_mask >> u) & 1),
+      store_all(L, ((L.lane & 15) == 0) & (u < kNodesB) & ((fin_mask >> u) & 1),
                 sw.node_t + (int64_t)L.t * n + v, ns);
       wait_vm_left<kNodesB * (1 + kStage) + 2 * kNodesB + 1>();
     }
