@@ -76,6 +76,8 @@ _SIGS = {
     "bcx_index_query": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
     "bcx_index_mean_scan": (C.c_int, [_P, C.c_int64, C.c_uint64, C.POINTER(C.c_double)]),
+    "bcx_index_build_h2h": (C.c_int, [_P]),
+    "bcx_index_h2h_query": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
     "bcx_index_tree_order": (C.c_int, [_P, _I32P]),
     "bcx_index_tree": (C.c_int, [_P, _I32P, _I32P]),
     "bcx_index_save": (C.c_int, [_P, C.c_char_p]),
@@ -96,6 +98,7 @@ _SIGS = {
     "bcx_bc_sources": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, _F64P, C.POINTER(C.c_uint32)]),
     "bcx_query_dist_count": (C.c_int, [_P, C.c_int32, _I32P, _U64P, _U32P]),
     "bcx_subset_bc": (C.c_int, [_P, _I32P, C.c_int32, _F64P]),
+    "bcx_h2h_distances": (C.c_int, [_P, _I32P, C.c_int32, _I32P]),
     "bcx_rq_create": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "bcx_rq_create_mode": (C.c_int, [_P, C.c_int, C.c_size_t, C.POINTER(_P)]),
     "bcx_rq_free": (None, [_P]),
@@ -235,6 +238,16 @@ class Index:
         _check(lib().bcx_index_tree(self._h, par, hgt))
         return par, hgt
 
+    def build_h2h(self) -> None:
+        """QBC's H2H distance labels and separators (buildIndex.cpp:222-278,
+        313-386); needed before creating a Context for h2h_distances."""
+        _check(lib().bcx_index_build_h2h(self._h))
+
+    def h2h_query(self, s: int, t: int) -> int:
+        d = C.c_int32()
+        _check(lib().bcx_index_h2h_query(self._h, s, t, C.byref(d)))
+        return d.value
+
     def mean_scan(self, pairs: int = 1 << 18, seed: int = 1) -> float:
         m = C.c_double()
         _check(lib().bcx_index_mean_scan(self._h, pairs, seed, C.byref(m)))
@@ -325,6 +338,13 @@ class Context:
         h = np.zeros(self.n, np.uint32)
         rc = _check(lib().bcx_query_dist_count(self._h, source, d, s, h))
         return d, s, h, rc == 1
+
+    def h2h_distances(self, sources) -> np.ndarray:
+        """QBC distance-only scan (QBC.cu:243-331): [len(sources), n] int32."""
+        src = np.ascontiguousarray(sources, np.int32)
+        out = np.zeros((len(src), self.n), np.int32)
+        _check(lib().bcx_h2h_distances(self._h, src, len(src), out))
+        return out
 
     def subset_bc(self, points) -> np.ndarray:
         p = np.ascontiguousarray(points, dtype=np.int32)

@@ -119,6 +119,25 @@ int bcx_index_build_with_order(const bcx_graph *g, const int32_t *order, bcx_ind
   BCX_GUARD_END
 }
 
+int bcx_index_build_h2h(bcx_index *ix) {
+  BCX_GUARD_BEGIN
+  if (!ix) return bcx::fail(BCX_EINVAL, "null index");
+  bcx::build_h2h(ix->ix);
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_index_h2h_query(const bcx_index *ix, int32_t s, int32_t t, int32_t *d) {
+  BCX_GUARD_BEGIN
+  if (!ix || !d) return bcx::fail(BCX_EINVAL, "null argument");
+  const auto &x = ix->ix;
+  if (s < 0 || s >= x.n || t < 0 || t >= x.n) return bcx::fail(BCX_EINVAL, "vertex out of range");
+  if (x.sep_off.empty()) return bcx::fail(BCX_EINVAL, "H2H labels not built (bcx_index_build_h2h)");
+  *d = bcx::h2h_query(x, s, t);
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
 int bcx_index_stats_get(const bcx_index *ix, bcx_index_stats *st) {
   if (!ix || !st) return bcx::fail(BCX_EINVAL, "null argument");
   const auto &x = ix->ix;

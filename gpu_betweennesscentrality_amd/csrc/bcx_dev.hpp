@@ -218,6 +218,11 @@ struct bcx_ctx {
   std::vector<int32_t> h_parent, h_depth, h_subtree, h_lvl_off, h_lvl_pair_off;
   int sweep_tiles = 0;
   bcx_dev::SweepWork sw{};
+  // QBC H2H distance labels (rows as the count labels) and separators, when
+  // the index had them at creation.
+  const int *d_gdis = nullptr;
+  const int64_t *d_sep_off = nullptr;
+  const int *d_sep = nullptr;
 
   ~bcx_ctx() {
     if (hipSetDevice(device) != hipSuccess) return;

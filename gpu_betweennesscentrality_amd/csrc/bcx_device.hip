@@ -48,6 +48,60 @@ __device__ __forceinline__ int lca_depth_dev(const DevIndex &ix, int u, int v) {
   return min(x, y) - 1;
 }
 
+// QBC distance-only scan (QBC.cu:243-331 without the P2H branch tables): one
+// workgroup per source s.  The workgroup first puts s's ancestors by depth
+// (the last node of each depth level at or before s in preorder) and s's H2H
+// label row in LDS; then each thread takes targets t: the LCA depth from the
+// sparse table; if s or t is the LCA, one label lookup; otherwise the minimum
+// over the LCA's separator of label(s) + label(t).  Output in preorder ids.
+__global__ __launch_bounds__(256) void h2h_dist_kernel(DevIndex ix, const int *lvl_off,
+                                                       const int *lvl_v, const int *gdis,
+                                                       const int64_t *sep_off, const int *sep,
+                                                       const int *srcs, int *out) {
+  extern __shared__ int h2h_lds[];
+  const int n = ix.n;
+  const int s = srcs[blockIdx.x];
+  const int ds = ix.depth[s];
+  int *path = h2h_lds, *srow = h2h_lds + ds + 1;
+  const int64_t rs = ix.row_off[s];
+  for (int j = threadIdx.x; j <= ds; j += blockDim.x) {
+    int lo = lvl_off[j], hi = lvl_off[j + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (lvl_v[mid] <= s)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    path[j] = lvl_v[lo - 1];
+    srow[j] = gdis[rs + j];
+  }
+  __syncthreads();
+  int *o = out + (int64_t)blockIdx.x * n;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    int d = 0;
+    if (t != s) {
+      const int L = lca_depth_dev(ix, s, t);
+      const int dt = ix.depth[t];
+      const int64_t rt = ix.row_off[t];
+      if (L == ds) {
+        d = gdis[rt + ds];  // s is an ancestor of t
+      } else if (L == dt) {
+        d = srow[dt];  // t is an ancestor of s
+      } else {
+        const int a = path[L];
+        int best = 0x7fffffff;  // label values stay <= 999999999, sums fit
+        for (int64_t i = sep_off[a], e = sep_off[a + 1]; i < e; ++i) {
+          const int k = sep[i];
+          best = min(best, srow[k] + gdis[rt + k]);
+        }
+        d = min(best, kInfDev);
+      }
+    }
+    o[t] = d;
+  }
+}
+
 // min / count-sum / hop-max over entries 0..L of two rows (one thread).
 __device__ __forceinline__ void scan_rows(const DevIndex &ix, int64_t rs, int64_t rt, int L,
                                           int &m, uint64_t &c, uint32_t &h) {
@@ -540,6 +594,11 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
   d.gcol = (const int *)keep(dev_upload(gcol));
   d.gw = (const int *)keep(dev_upload(gw));
   c->ix = d;
+  if (!ix.sep_off.empty()) {
+    c->d_gdis = (const int *)keep(dev_upload(ix.gdis));
+    c->d_sep_off = (const int64_t *)keep(dev_upload(ix.sep_off));
+    c->d_sep = (const int *)keep(dev_upload(ix.sep));
+  }
   // Tree view for the bag sweep: bags, inverse bags and depth levels.
   {
     const int64_t B = ix.bag_off[(size_t)n];
@@ -968,6 +1027,51 @@ int bcx_query_dist_count(bcx_ctx *c, int32_t source, int32_t *d, uint64_t *sigma
     if (hop) hop[v] = hh[p];
   }
   return ovf ? 1 : BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_h2h_distances(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t *dist_out) {
+  BCX_GUARD_BEGIN
+  if (!c || count < 0 || (count > 0 && (!sources || !dist_out))) return bcx::fail(BCX_EINVAL, "bad argument");
+  if (!c->d_sep_off)
+    return bcx::fail(BCX_EINVAL, "context has no H2H labels (call bcx_index_build_h2h before bcx_ctx_create)");
+  const int n = c->ix.n;
+  for (int32_t i = 0; i < count; ++i)
+    if (sources[i] < 0 || sources[i] >= n) return bcx::fail(BCX_EINVAL, "source out of range");
+  if (count == 0) return BCX_OK;
+  const size_t lds = (size_t)2 * (size_t)c->tree.height * 4;
+  if (lds > 65536) return bcx::fail(BCX_EGRAPH, "tree too deep for the H2H scan (height over 8192)");
+  HIP_OK(hipSetDevice(c->device));
+  // Chunks of sources per launch, the output chunk bounded to about 1 GiB.
+  const int32_t chunk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(count, (1ll << 30) / (4ll * n)));
+  int *d_out = nullptr, *d_src = nullptr;
+  HIP_OK(hipMalloc(&d_out, (size_t)chunk * n * 4));
+  HIP_OK(hipMalloc(&d_src, (size_t)chunk * 4));
+  std::vector<int32_t> pre((size_t)chunk), host((size_t)chunk * n);
+  try {
+    for (int32_t b0 = 0; b0 < count; b0 += chunk) {
+      const int32_t nb = std::min(chunk, count - b0);
+      for (int32_t i = 0; i < nb; ++i) pre[(size_t)i] = c->iperm[(size_t)sources[b0 + i]];
+      HIP_OK(hipMemcpyAsync(d_src, pre.data(), (size_t)nb * 4, hipMemcpyHostToDevice, c->stream));
+      h2h_dist_kernel<<<nb, 256, lds, c->stream>>>(c->ix, c->tree.lvl_off, c->tree.lvl_v, c->d_gdis,
+                                                   c->d_sep_off, c->d_sep, d_src, d_out);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipMemcpyAsync(host.data(), d_out, (size_t)nb * n * 4, hipMemcpyDeviceToHost, c->stream));
+      HIP_OK(hipStreamSynchronize(c->stream));
+      for (int32_t i = 0; i < nb; ++i) {
+        int32_t *dst = dist_out + (int64_t)(b0 + i) * n;
+        const int32_t *src = host.data() + (size_t)i * n;
+        for (int32_t v = 0; v < n; ++v) dst[v] = src[c->iperm[(size_t)v]];
+      }
+    }
+  } catch (...) {
+    (void)hipFree(d_out);
+    (void)hipFree(d_src);
+    throw;
+  }
+  HIP_OK(hipFree(d_out));
+  HIP_OK(hipFree(d_src));
+  return BCX_OK;
   BCX_GUARD_END
 }
 

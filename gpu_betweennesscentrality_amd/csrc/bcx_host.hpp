@@ -47,6 +47,13 @@ struct Index {
   std::vector<int32_t> bag_len;
   std::vector<uint64_t> bag_cnt;
   std::vector<uint32_t> bag_hop;
+  // H2H distance labels of the QBC path, built on request (build_h2h): the
+  // global shortest distance from each node to each ancestor and to itself
+  // (rows as dis), and per node the separator depths scanned when the node is
+  // the LCA of a pair whose ends lie in two different child subtrees.
+  std::vector<int32_t> gdis;
+  std::vector<int64_t> sep_off;  // n + 1 (empty until build_h2h)
+  std::vector<int32_t> sep;      // ancestor depths, the node's own depth last
   int32_t tree_height = 0;  // max depth + 1
   int32_t tree_width = 0;   // max bag size
   int32_t count_overflow = 0;
@@ -74,6 +81,11 @@ void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix);
 void index_query(const Index &ix, int32_t s, int32_t t, int32_t &d,
                  uint64_t &sigma, uint32_t &hop, bool &ovf);
 int32_t lca_depth(const Index &ix, int32_t p, int32_t q);  // preorder ids
+// H2H labels and separators (buildIndex.cpp:222-278, 313-386).
+void build_h2h(Index &ix);
+// Distance by the H2H separator scan (QBC.cu:271-331 without the P2H branch
+// table); vertex ids; requires build_h2h.
+int32_t h2h_query(const Index &ix, int32_t s, int32_t t);
 void save_index(const Index &ix, const char *path);
 void load_index(const char *path, Index &ix);
 

@@ -382,6 +382,137 @@ void index_query(const Index &ix, int32_t s, int32_t t, int32_t &d, uint64_t &si
   ovf = c == UINT64_MAX;
 }
 
+// ---- H2H distance labels (QBC path) ---------------------------------------
+// Global distance labels, the top-down recurrence of makeIndexDFS
+// (buildIndex.cpp:245-268) over the bags: for node p and its ancestor y at
+// depth j, d(p, y) = min over bag entries (x at depth k, shortcut length w)
+// of w + d(x, y), where d(x, y) is x's label at depth j when x is deeper
+// (k > j), y's label at depth k when y is deeper (k < j), and 0 when x = y.
+// Parents' rows are complete before their descendants', so each level runs in
+// parallel.  Separators follow optimizeVS (buildIndex.cpp:313-386): a node
+// with fewer than two children needs only its own depth (it is the LCA of a
+// pair only when one end is the node itself); with two, the bag depths of the
+// child with the smaller bag; with more, the bag depths of every child but
+// the one with the most depths unique to it (any two children include one of
+// the others, and a path between two child subtrees crosses both children's
+// bags); in every case the node's own depth last.
+void build_h2h(Index &ix) {
+  const int32_t n = ix.n, H = ix.tree_height;
+  ix.gdis.assign(ix.dis.size(), kInf);
+  std::vector<int32_t> lvl_off((size_t)H + 1, 0), lvl((size_t)n);
+  for (int32_t p = 0; p < n; ++p) lvl_off[(size_t)ix.depth[(size_t)p] + 1]++;
+  for (int32_t k = 0; k < H; ++k) lvl_off[(size_t)k + 1] += lvl_off[(size_t)k];
+  {
+    std::vector<int32_t> f(lvl_off.begin(), lvl_off.end() - 1);
+    for (int32_t p = 0; p < n; ++p) lvl[(size_t)f[(size_t)ix.depth[(size_t)p]]++] = p;
+  }
+  for (int32_t k = 0; k < H; ++k) {
+#pragma omp parallel
+    {
+      std::vector<int32_t> anc;
+#pragma omp for schedule(dynamic, 64)
+      for (int32_t q = lvl_off[(size_t)k]; q < lvl_off[(size_t)k + 1]; ++q) {
+        const int32_t p = lvl[(size_t)q], D = ix.depth[(size_t)p];
+        anc.assign((size_t)D, 0);
+        for (int32_t a = ix.parent[(size_t)p], j = D - 1; j >= 0; a = ix.parent[(size_t)a], --j)
+          anc[(size_t)j] = a;
+        const int64_t row = ix.row_off[(size_t)p];
+        ix.gdis[(size_t)(row + D)] = 0;
+        for (int32_t j = 0; j < D; ++j) {
+          int64_t g = kInf;
+          for (int64_t e = ix.bag_off[(size_t)p]; e < ix.bag_off[(size_t)p + 1]; ++e) {
+            const int32_t x = ix.bag[(size_t)e], kx = ix.depth[(size_t)x];
+            int64_t z;
+            if (kx == j)
+              z = 0;
+            else if (kx < j)
+              z = ix.gdis[(size_t)(ix.row_off[(size_t)anc[(size_t)j]] + kx)];
+            else
+              z = ix.gdis[(size_t)(ix.row_off[(size_t)x] + j)];
+            g = std::min<int64_t>(g, z + ix.bag_len[(size_t)e]);
+          }
+          ix.gdis[(size_t)(row + j)] = (int32_t)std::min<int64_t>(g, kInf);
+        }
+      }
+    }
+  }
+  // Children in creation (preorder) order.
+  std::vector<int32_t> ch_off((size_t)n + 1, 0), ch((size_t)std::max(0, n - 1));
+  for (int32_t p = 1; p < n; ++p) ch_off[(size_t)ix.parent[(size_t)p] + 1]++;
+  for (int32_t p = 0; p < n; ++p) ch_off[(size_t)p + 1] += ch_off[(size_t)p];
+  {
+    std::vector<int32_t> f(ch_off.begin(), ch_off.end() - 1);
+    for (int32_t p = 1; p < n; ++p) ch[(size_t)f[(size_t)ix.parent[(size_t)p]]++] = p;
+  }
+  auto bag_size = [&](int32_t c) { return ix.bag_off[(size_t)c + 1] - ix.bag_off[(size_t)c]; };
+  ix.sep_off.assign((size_t)n + 1, 0);
+  ix.sep.clear();
+  std::vector<int32_t> count((size_t)H, 0), touched;
+  for (int32_t p = 0; p < n; ++p) {
+    const int32_t own = ix.depth[(size_t)p];
+    const int32_t c0 = ch_off[(size_t)p], nch = ch_off[(size_t)p + 1] - c0;
+    if (nch == 2) {
+      const int32_t c1 = ch[(size_t)c0], c2 = ch[(size_t)c0 + 1];
+      const int32_t tmp = bag_size(c1) < bag_size(c2) ? c1 : c2;
+      for (int64_t e = ix.bag_off[(size_t)tmp + 1] - 1; e >= ix.bag_off[(size_t)tmp]; --e) {
+        const int32_t d = ix.depth[(size_t)ix.bag[(size_t)e]];
+        if (d != own) ix.sep.push_back(d);
+      }
+    } else if (nch > 2) {
+      touched.clear();
+      for (int32_t i = 0; i < nch; ++i) {
+        const int32_t c = ch[(size_t)(c0 + i)];
+        for (int64_t e = ix.bag_off[(size_t)c]; e < ix.bag_off[(size_t)c + 1]; ++e) {
+          const int32_t d = ix.depth[(size_t)ix.bag[(size_t)e]];
+          if (count[(size_t)d]++ == 0) touched.push_back(d);
+        }
+      }
+      int32_t idx = 0, best = -1;
+      for (int32_t i = 0; i < nch; ++i) {
+        const int32_t c = ch[(size_t)(c0 + i)];
+        int32_t a = 0;
+        for (int64_t e = ix.bag_off[(size_t)c]; e < ix.bag_off[(size_t)c + 1]; ++e)
+          a += count[(size_t)ix.depth[(size_t)ix.bag[(size_t)e]]] == 1;
+        if (a > best) {
+          best = a;
+          idx = i;
+        }
+      }
+      {
+        const int32_t c = ch[(size_t)(c0 + idx)];
+        for (int64_t e = ix.bag_off[(size_t)c]; e < ix.bag_off[(size_t)c + 1]; ++e)
+          count[(size_t)ix.depth[(size_t)ix.bag[(size_t)e]]]--;
+      }
+      std::sort(touched.begin(), touched.end());
+      for (int32_t d : touched) {
+        if (count[(size_t)d] > 0 && d != own) ix.sep.push_back(d);
+        count[(size_t)d] = 0;
+      }
+    }
+    ix.sep.push_back(own);
+    ix.sep_off[(size_t)p + 1] = (int64_t)ix.sep.size();
+  }
+}
+
+int32_t h2h_query(const Index &ix, int32_t s, int32_t t) {
+  if (ix.sep_off.empty()) throw Error{BCX_EINVAL, "H2H labels not built"};
+  const int32_t p = ix.iperm[(size_t)s], q = ix.iperm[(size_t)t];
+  if (p == q) return 0;
+  const int32_t L = lca_depth(ix, std::min(p, q), std::max(p, q));
+  const int32_t dp = ix.depth[(size_t)p], dq = ix.depth[(size_t)q];
+  const int64_t op = ix.row_off[(size_t)p], oq = ix.row_off[(size_t)q];
+  if (L == dp) return ix.gdis[(size_t)(oq + dp)];  // p is an ancestor of q
+  if (L == dq) return ix.gdis[(size_t)(op + dq)];
+  int32_t a = p;
+  while (ix.depth[(size_t)a] > L) a = ix.parent[(size_t)a];
+  int64_t best = 2 * (int64_t)kInf;
+  for (int64_t i = ix.sep_off[(size_t)a]; i < ix.sep_off[(size_t)a + 1]; ++i) {
+    const int32_t k = ix.sep[(size_t)i];
+    best = std::min<int64_t>(best, (int64_t)ix.gdis[(size_t)(op + k)] + ix.gdis[(size_t)(oq + k)]);
+  }
+  return (int32_t)std::min<int64_t>(best, kInf);
+}
+
 // ---- binary image ---------------------------------------------------------
 namespace {
 const char kMagic[8] = {'B', 'C', 'X', 'I', 'D', 'X', '0', '2'};

@@ -124,6 +124,19 @@ int bcx_index_stats_get(const bcx_index *ix, bcx_index_stats *st);
 /* Host-side (d, sigma, hop) query on the labels; used by tools and tests. */
 int bcx_index_query(const bcx_index *ix, int32_t s, int32_t t, int32_t *d,
                     uint64_t *sigma, uint32_t *hop);
+/* QBC's H2H distance labels on this index's tree (replaces the global-distance
+ * labelling of Graph::makeIndexDFS, src/buildIndex.cpp:222-278, and the
+ * separator reduction Graph::optimizeVS, src/buildIndex.cpp:313-386): the
+ * shortest distance from every node to each ancestor, and per node the
+ * separator scanned when it is the LCA of a pair.  The P2H branch tables of
+ * Graph::SVSP_P2H / P2H_3 (src/buildIndex.cpp:565-719) are not built: their
+ * branch choice is seeded from the clock (src/buildIndex.cpp:608) and they
+ * change which separator is scanned, never the distance.  Needed before
+ * bcx_ctx_create for bcx_h2h_distances.  Host memory: 4 bytes per label
+ * entry plus the separators. */
+int bcx_index_build_h2h(bcx_index *ix);
+/* Host-side H2H distance query (vertex ids); requires bcx_index_build_h2h. */
+int bcx_index_h2h_query(const bcx_index *ix, int32_t s, int32_t t, int32_t *d);
 /* order[p] = vertex id of the p-th tree node in DFS preorder (n entries).
  * Sources processed in this order share label prefixes best. */
 int bcx_index_tree_order(const bcx_index *ix, int32_t *order);
@@ -175,6 +188,13 @@ int bcx_bc_sources(bcx_ctx *ctx, int mode, const int32_t *sources,
 /* Distance, path count and max-hop layer from `source` to every vertex. */
 int bcx_query_dist_count(bcx_ctx *ctx, int32_t source, int32_t *d,
                          uint64_t *sigma, uint32_t *hop);
+/* QBC distance-only scan (replaces the distance part of BC_Kernel,
+ * QBC.cu:243-331: LCA by RMQ, then the ancestor lookup or the minimum over
+ * the LCA's separator of the two H2H labels): dist_out[i * n + v] = d(sources[i],
+ * v), exact (999999999 when unreachable).  The index must have had
+ * bcx_index_build_h2h called before the context was created. */
+int bcx_h2h_distances(bcx_ctx *ctx, const int32_t *sources, int32_t count,
+                      int32_t *dist_out);
 /* Subset BC (RQ): bc_out[q] = BC of points[i] (RQ.cu:451-459, summed). */
 int bcx_subset_bc(bcx_ctx *ctx, const int32_t *points, int32_t q,
                   double *bc_out);

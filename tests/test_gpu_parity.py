@@ -206,6 +206,34 @@ def test_cli_rq_table(tmp_path, grid):
     assert np.all(np.abs(got - want) <= 1e-9 * np.maximum(np.abs(want), 1.0) + 5e-9)
 
 
+# ---- QBC distance-only scan (H2H labels) ------------------------------------
+
+@pytest.mark.parametrize("maker,order", [
+    (lambda: gg.make("grid1k"), False),
+    (lambda: gg.make("grid1k"), True),                      # order-file (QBC) tree
+    (lambda: gg.grid_graph(k=20, seed=3, wmax=2), False),   # many ties
+    (lambda: gg.road_graph(20_000, 25_000, seed=11), False),
+])
+def test_h2h_distances_match_dijkstra(maker, order):
+    g = maker()
+    G = bcx.Graph.from_arcs(g.n, g.u, g.v, g.w)
+    ix = (bcx.Index.build_order(G, gg.permutation(5, 2, g.n).argsort().astype(np.int32))
+          if order else bcx.Index.build_mhl(G))
+    ix.build_h2h()
+    ctx = bcx.Context(G, ix, 0)
+    csr = oracle.CSR.from_arcgraph(g)
+    src = gg.permutation(6, 1, g.n)[: min(g.n, 96)].astype(np.int32)
+    got = ctx.h2h_distances(src)
+    for i, s in enumerate(src.tolist()):
+        assert np.array_equal(got[i].astype(np.int64), oracle.sssp(csr, s)[0]), f"source {s}"
+
+
+def test_h2h_distances_need_labels(grid):
+    g, csr, ctx = grid
+    with pytest.raises(bcx.BcxError):
+        ctx.h2h_distances(np.array([0], np.int32))
+
+
 # ---- bag-sweep form ---------------------------------------------------------
 
 @pytest.mark.parametrize("maker", [

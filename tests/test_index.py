@@ -95,6 +95,47 @@ def test_all_pairs_match_dijkstra(maker):
             assert np.array_equal(got[:, 2].astype(np.uint32), hp)
 
 
+# ---- QBC's H2H distance labels (bcx_index_build_h2h) ------------------------
+
+def test_la_h2h_queries_match_reference(golden_dir, la):
+    # The reference's own H2H index (order file, global-distance labels,
+    # separator scan) answered these pairs; ours must agree on every one.
+    ref = np.load(golden_dir / "la_h2h.npz")
+    G = bcx.Graph.from_arcs(int(la["n"]), la["u"], la["v"], la["w"])
+    ix = bcx.Index.build_order(G, la["order"])
+    ix.build_h2h()
+    for s, t, d in ref["queries"].tolist():
+        assert ix.h2h_query(s - 1, t - 1) == d, (s, t)
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: gg.grid_graph(k=10, seed=4, wmax=2),
+    lambda: gg.unit_grid(9),
+    lambda: gg.road_graph(400, 520, seed=8),
+    lambda: gg.road_graph(300, 700, seed=9),
+])
+def test_h2h_all_pairs_match_dijkstra(maker):
+    # All pairs: the ancestor lookups pin the labels (each is one label
+    # entry), the other pairs pin the separators (a missing separator depth
+    # shows as a distance above Dijkstra's on some pair).
+    g = maker()
+    csr = oracle.CSR.from_arcgraph(g)
+    G = graph_of(g)
+    for ix in (bcx.Index.build_mhl(G),
+               bcx.Index.build_order(G, gg.permutation(3, 3, g.n).argsort().astype(np.int32))):
+        ix.build_h2h()
+        for s in range(g.n):
+            d = oracle.sssp(csr, s)[0]
+            got = np.array([ix.h2h_query(s, t) for t in range(g.n)], dtype=np.int64)
+            assert np.array_equal(got, d), f"source {s}"
+
+
+def test_h2h_query_needs_labels():
+    ix = bcx.Index.build_mhl(graph_of(gg.grid_graph(k=5, seed=1)))
+    with pytest.raises(bcx.BcxError):
+        ix.h2h_query(0, 1)
+
+
 def test_unit_grid_overflow_flag_only_on_final_count():
     g = gg.unit_grid(34)
     ix = bcx.Index.build_mhl(graph_of(g))
