@@ -177,6 +177,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rq", action="store_true")
+    ap.add_argument("--no-h2h", action="store_true", help="skip the QBC distance-only (H2H) scan")
     ap.add_argument("--rq-requests", type=int, default=6)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
@@ -305,6 +306,11 @@ def main() -> int:
     if not args.no_cpu:
         cpu = _cpu_baseline(g, n, args.cpu_seconds)
 
+    h2h = None
+    if not args.no_h2h:
+        ctx.close()  # its sweep buffers hold most of the device memory
+        h2h = _h2h_bench(bcx, G, ix, tree_perm, n)
+
     out = {
         "metric": "sources/sec for full BC (batched vs plain) + RQ p50/p99 latency, 1 MI355X",
         "value": round(value, 2),
@@ -334,6 +340,7 @@ def main() -> int:
             "scan_ms_per_batch": round(scan_ms, 3), "dep_ms_per_batch": round(dep_ms, 3),
             "reduce_ms_per_batch": round(red_ms, 3),
             "count_overflow": bool(ovf),
+            "h2h_scan": h2h,
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -348,6 +355,32 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _h2h_bench(bcx, G, ix, tree_perm, n: int, sources: int = 512) -> dict:
+    """QBC's distance-only path on the same index: H2H labels built on the
+    host, then the device separator scan for `sources` sources (one block of
+    the tree order), timed by HIP events around its kernels."""
+    t0 = time.perf_counter()
+    ix.build_h2h()
+    build_s = time.perf_counter() - t0
+    ctx = bcx.Context(G, ix, 0)
+    src = tree_perm[(n - sources) // 2:(n - sources) // 2 + sources]
+    ctx.h2h_distances(src[:8])  # warm-up
+    ctx.set_profiling(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    ctx.h2h_distances(src)
+    wall = time.perf_counter() - t0
+    tm = ctx.timing()
+    ctx.set_profiling(False)
+    ctx.close()
+    out = {"sources": int(len(src)), "label_build_s": round(build_s, 2),
+           "kernel_ms": round(tm["scan_ms"], 3),
+           "sources_per_s": round(len(src) / (tm["scan_ms"] * 1e-3), 2),
+           "with_copy_back_sources_per_s": round(len(src) / wall, 2)}
+    log(f"[bench] h2h distance scan: {out}")
+    return out
 
 
 def _rq_bench(bcx, gg, ctx, g, nreq: int, batch: int) -> dict:

@@ -1053,9 +1053,19 @@ int bcx_h2h_distances(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t
       const int32_t nb = std::min(chunk, count - b0);
       for (int32_t i = 0; i < nb; ++i) pre[(size_t)i] = c->iperm[(size_t)sources[b0 + i]];
       HIP_OK(hipMemcpyAsync(d_src, pre.data(), (size_t)nb * 4, hipMemcpyHostToDevice, c->stream));
+      if (c->profiling) HIP_OK(hipEventRecord(c->ev[0], c->stream));
       h2h_dist_kernel<<<nb, 256, lds, c->stream>>>(c->ix, c->tree.lvl_off, c->tree.lvl_v, c->d_gdis,
                                                    c->d_sep_off, c->d_sep, d_src, d_out);
       HIP_OK(hipGetLastError());
+      if (c->profiling) {
+        HIP_OK(hipEventRecord(c->ev[1], c->stream));
+        HIP_OK(hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        c->timing.scan_ms += ms;
+        c->timing.scan_launches += 1;
+        c->timing.sources += nb;
+      }
       HIP_OK(hipMemcpyAsync(host.data(), d_out, (size_t)nb * n * 4, hipMemcpyDeviceToHost, c->stream));
       HIP_OK(hipStreamSynchronize(c->stream));
       for (int32_t i = 0; i < nb; ++i) {

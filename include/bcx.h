@@ -81,7 +81,8 @@ typedef struct bcx_tuning {
 } bcx_tuning;
 
 typedef struct bcx_timing {
-  double scan_ms;     /* label scan (distance, count, hop) kernels         */
+  double scan_ms;     /* label scan (distance, count, hop), bag-sweep tile
+                         and H2H distance-scan kernels                     */
   double dep_ms;      /* DAG + layered dependency kernels                  */
   double reduce_ms;   /* BC accumulation kernels                           */
   int64_t scan_launches;
