@@ -77,6 +77,7 @@ _SIGS = {
                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
     "bcx_index_mean_scan": (C.c_int, [_P, C.c_int64, C.c_uint64, C.POINTER(C.c_double)]),
     "bcx_index_build_h2h": (C.c_int, [_P]),
+    "bcx_min_degree_order": (C.c_int, [_P, _I32P, C.POINTER(C.c_int64), _I32P]),
     "bcx_index_h2h_query": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
     "bcx_index_tree_order": (C.c_int, [_P, _I32P]),
     "bcx_index_tree": (C.c_int, [_P, _I32P, _I32P]),
@@ -168,6 +169,15 @@ class Graph:
 
     def connected(self) -> bool:
         return bool(_check(lib().bcx_graph_connected(self._h)))
+
+    def min_degree_order(self) -> np.ndarray:
+        """order[v] = contraction rank (Graph::GraphConstract,
+        buildIndex.cpp:759-925); the order file QBC reads."""
+        order = np.zeros(self.n, np.int32)
+        widths = np.zeros(self.n // 100000 + 1, np.int32)
+        arcs = C.c_int64()
+        _check(lib().bcx_min_degree_order(self._h, order, C.byref(arcs), widths))
+        return order
 
     def close(self) -> None:
         if self._h:

@@ -35,7 +35,8 @@ CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-Wall", "-Wno-unused-fun
              f"-I{INCLUDE}", f"-I{CSRC}"]
 HOST_SOURCES = ["graph_io.cpp", "index_build.cpp", "bcx_api.cpp"]
 HIP_SOURCES = ["bcx_device.hip", "bcx_sweep.hip"]
-CLIS = {"MHLQBC": "cli_mhlqbc.cpp", "QBC": "cli_qbc.cpp", "RQ": "cli_rq.cpp"}
+CLIS = {"MHLQBC": "cli_mhlqbc.cpp", "QBC": "cli_qbc.cpp", "RQ": "cli_rq.cpp",
+        "GraphConstract": "cli_graphconstract.cpp"}
 
 
 def _run(cmd: list[str]) -> None:

@@ -119,6 +119,18 @@ int bcx_index_build_with_order(const bcx_graph *g, const int32_t *order, bcx_ind
   BCX_GUARD_END
 }
 
+int bcx_min_degree_order(const bcx_graph *g, int32_t *order, int64_t *shortcut_arcs,
+                         int32_t *width_log) {
+  BCX_GUARD_BEGIN
+  if (!g || !order) return bcx::fail(BCX_EINVAL, "null argument");
+  std::vector<int32_t> o, wl;
+  bcx::min_degree_order(g->g, o, shortcut_arcs, width_log ? &wl : nullptr);
+  std::memcpy(order, o.data(), o.size() * sizeof(int32_t));
+  if (width_log) std::memcpy(width_log, wl.data(), wl.size() * sizeof(int32_t));
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
 int bcx_index_build_h2h(bcx_index *ix) {
   BCX_GUARD_BEGIN
   if (!ix) return bcx::fail(BCX_EINVAL, "null index");

@@ -81,6 +81,12 @@ void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix);
 void index_query(const Index &ix, int32_t s, int32_t t, int32_t &d,
                  uint64_t &sigma, uint32_t &hop, bool &ovf);
 int32_t lca_depth(const Index &ix, int32_t p, int32_t q);  // preorder ids
+// Min-degree contraction order of Graph::GraphConstract (buildIndex.cpp:
+// 759-925): order[v] = rank; optionally the number of contracted-neighbour
+// arcs (the reference's "super edge number") and the largest contracted
+// neighbourhood seen before every 100,000th contraction (its progress lines).
+void min_degree_order(const Graph &g, std::vector<int32_t> &order,
+                      int64_t *shortcut_arcs = nullptr, std::vector<int32_t> *width_log = nullptr);
 // H2H labels and separators (buildIndex.cpp:222-278, 313-386).
 void build_h2h(Index &ix);
 // Distance by the H2H separator scan (QBC.cu:271-331 without the P2H branch

@@ -382,6 +382,78 @@ void index_query(const Index &ix, int32_t s, int32_t t, int32_t &d, uint64_t &si
   ovf = c == UINT64_MAX;
 }
 
+// ---- min-degree order file (QBC input) --------------------------------------
+// The contraction order of Graph::GraphConstract (buildIndex.cpp:759-925):
+// repeatedly contract a vertex of least current degree in the filled graph
+// (ties to the smaller id: its secondary key compares _DD2[x] with itself,
+// buildIndex.cpp:732, so it never decides), joining the contracted vertex's
+// remaining neighbours pairwise.  Every neighbour whose degree changed is
+// re-keyed at once (buildIndex.cpp:895-904), so the key is always the current
+// degree.  Like the reference built without its parallel branch
+// (buildIndex.cpp:847), a vertex with more than 1000 remaining neighbours
+// adds no fill.  order[v] = contraction rank of v.
+void min_degree_order(const Graph &g, std::vector<int32_t> &order, int64_t *shortcut_arcs,
+                      std::vector<int32_t> *width_log) {
+  const int32_t n = g.n;
+  std::vector<std::vector<int32_t>> adj((size_t)n);
+  for (int32_t v = 0; v < n; ++v) {
+    auto &a = adj[(size_t)v];
+    a.assign(g.col.begin() + g.row[(size_t)v], g.col.begin() + g.row[(size_t)v + 1]);
+    std::sort(a.begin(), a.end());
+    a.erase(std::unique(a.begin(), a.end()), a.end());
+  }
+  auto has = [&](int32_t u, int32_t v) {
+    const auto &a = adj[(size_t)u];
+    return std::binary_search(a.begin(), a.end(), v);
+  };
+  auto add = [&](int32_t u, int32_t v) {
+    auto &a = adj[(size_t)u];
+    a.insert(std::lower_bound(a.begin(), a.end(), v), v);
+  };
+  auto drop = [&](int32_t u, int32_t v) {
+    auto &a = adj[(size_t)u];
+    auto it = std::lower_bound(a.begin(), a.end(), v);
+    if (it != a.end() && *it == v) a.erase(it);
+  };
+  std::vector<int32_t> key((size_t)n);
+  std::set<std::pair<int32_t, int32_t>> q;
+  for (int32_t v = 0; v < n; ++v) {
+    key[(size_t)v] = (int32_t)adj[(size_t)v].size();
+    q.insert({key[(size_t)v], v});
+  }
+  order.assign((size_t)n, 0);
+  int64_t arcs_out = 0;
+  int32_t width = 0;
+  if (width_log) width_log->clear();
+  for (int32_t k = 0; k < n; ++k) {
+    if (width_log && k % 100000 == 0) width_log->push_back(width);
+    const int32_t x = q.begin()->second;
+    q.erase(q.begin());
+    order[(size_t)x] = k;
+    const std::vector<int32_t> nb = std::move(adj[(size_t)x]);
+    adj[(size_t)x].clear();
+    arcs_out += (int64_t)nb.size();
+    width = std::max(width, (int32_t)nb.size());
+    for (int32_t y : nb) drop(y, x);
+    if (nb.size() <= 1000)
+      for (size_t i = 0; i < nb.size(); ++i)
+        for (size_t j = i + 1; j < nb.size(); ++j)
+          if (!has(nb[i], nb[j])) {
+            add(nb[i], nb[j]);
+            add(nb[j], nb[i]);
+          }
+    for (int32_t y : nb) {
+      const int32_t d = (int32_t)adj[(size_t)y].size();
+      if (d != key[(size_t)y]) {
+        q.erase({key[(size_t)y], y});
+        key[(size_t)y] = d;
+        q.insert({d, y});
+      }
+    }
+  }
+  if (shortcut_arcs) *shortcut_arcs = arcs_out;
+}
+
 // ---- H2H distance labels (QBC path) ---------------------------------------
 // Global distance labels, the top-down recurrence of makeIndexDFS
 // (buildIndex.cpp:245-268) over the bags: for node p and its ancestor y at

@@ -125,6 +125,14 @@ int bcx_index_stats_get(const bcx_index *ix, bcx_index_stats *st);
 /* Host-side (d, sigma, hop) query on the labels; used by tools and tests. */
 int bcx_index_query(const bcx_index *ix, int32_t s, int32_t t, int32_t *d,
                     uint64_t *sigma, uint32_t *hop);
+/* Min-degree contraction order (replaces Graph::GraphConstract,
+ * src/buildIndex.cpp:759-925, which writes QBC's order files): order[v] =
+ * contraction rank of v (n entries).  shortcut_arcs (may be NULL) receives
+ * the number of contracted-neighbour arcs (the reference's "super edge
+ * number"); width_log (may be NULL, n / 100000 + 1 entries) the largest
+ * contracted neighbourhood before every 100,000th contraction. */
+int bcx_min_degree_order(const bcx_graph *g, int32_t *order, int64_t *shortcut_arcs,
+                         int32_t *width_log);
 /* QBC's H2H distance labels on this index's tree (replaces the global-distance
  * labelling of Graph::makeIndexDFS, src/buildIndex.cpp:222-278, and the
  * separator reduction Graph::optimizeVS, src/buildIndex.cpp:313-386): the

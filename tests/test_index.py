@@ -95,6 +95,67 @@ def test_all_pairs_match_dijkstra(maker):
             assert np.array_equal(got[:, 2].astype(np.uint32), hp)
 
 
+# ---- QBC's order file (bcx_min_degree_order) ---------------------------------
+
+def test_min_degree_order_reproduces_shipped_la_order(la):
+    # The reference ships LosAngelesOrder.txt, written by its GraphConstract;
+    # the restatement reproduces every rank.
+    G = bcx.Graph.from_arcs(int(la["n"]), la["u"], la["v"], la["w"])
+    assert np.array_equal(G.min_degree_order(), la["order"])
+
+
+def _min_degree_brute(n, u, v):
+    # Textbook greedy elimination: least current degree first, ties to the
+    # smaller id, neighbours joined pairwise.
+    adj = [set() for _ in range(n)]
+    for a, b in zip(u.tolist(), v.tolist()):
+        adj[a].add(b)
+        adj[b].add(a)
+    alive, order = set(range(n)), [0] * n
+    for k in range(n):
+        x = min(alive, key=lambda y: (len(adj[y]), y))
+        alive.remove(x)
+        order[x] = k
+        nb = sorted(adj[x])
+        for y in nb:
+            adj[y].discard(x)
+        for i, a in enumerate(nb):
+            for b in nb[i + 1:]:
+                adj[a].add(b)
+                adj[b].add(a)
+        adj[x].clear()
+    return np.array(order)
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: gg.grid_graph(k=9, seed=4, wmax=5),
+    lambda: gg.road_graph(300, 520, seed=8),
+])
+def test_min_degree_order_matches_greedy_elimination(maker):
+    g = maker()
+    assert np.array_equal(graph_of(g).min_degree_order(), _min_degree_brute(g.n, g.u, g.v))
+
+
+def test_graphconstract_cli_writes_reference_format(tmp_path, la):
+    import subprocess
+    from pathlib import Path
+    n = int(la["n"])
+    gpath = tmp_path / "la.txt"
+    with open(gpath, "w") as f:
+        f.write(f"{n} {len(la['u'])}\n")
+        for a, b, w in zip(la["u"].tolist(), la["v"].tolist(), la["w"].tolist()):
+            f.write(f"{a + 1} {b + 1} {w}\n")
+    exe = Path(bcx.__file__).parent / "bin" / "GraphConstract"
+    out = tmp_path / "order.txt"
+    r = subprocess.run([str(exe), str(gpath), str(out), "min-degree"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    lines = out.read_text().splitlines()
+    assert lines[0] == str(n) and lines[-1] == "min-degree" and len(lines) == n + 2
+    got = np.array([int(x.split()[1]) for x in lines[1:-1]])
+    assert np.array_equal(got, la["order"])
+    assert "super edge number" in r.stdout
+
+
 # ---- QBC's H2H distance labels (bcx_index_build_h2h) ------------------------
 
 def test_la_h2h_queries_match_reference(golden_dir, la):
