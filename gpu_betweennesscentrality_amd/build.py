@@ -36,7 +36,8 @@ CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-Wall", "-Wno-unused-fun
 HOST_SOURCES = ["graph_io.cpp", "index_build.cpp", "bcx_api.cpp"]
 HIP_SOURCES = ["bcx_device.hip", "bcx_sweep.hip"]
 CLIS = {"MHLQBC": "cli_mhlqbc.cpp", "QBC": "cli_qbc.cpp", "RQ": "cli_rq.cpp",
-        "GraphConstract": "cli_graphconstract.cpp"}
+        "GraphConstract": "cli_graphconstract.cpp", "TC_index": "cli_tc_index.cpp",
+        "TC_query": "cli_tc_query.cpp"}
 
 
 def _run(cmd: list[str]) -> None:

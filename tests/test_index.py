@@ -156,6 +156,29 @@ def test_graphconstract_cli_writes_reference_format(tmp_path, la):
     assert "super edge number" in r.stdout
 
 
+def test_tc_tools_answer_reference_counts(tmp_path, golden_dir):
+    # TC_index / TC_query against the reference TreeCount tool's own answers
+    # on the seeded grid (tests/golden/grid1k_tc.npz: 1-based pairs, counts).
+    import subprocess
+    from pathlib import Path
+    ref = np.load(golden_dir / "grid1k_tc.npz")
+    g = gg.make("grid1k")
+    gp = tmp_path / "g.txt"
+    gg.write_graph(g, gp)
+    exe = Path(bcx.__file__).parent / "bin"
+    r = subprocess.run([str(exe / "TC_index"), "-g", str(gp), "-o", str(tmp_path / "ix.bin")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0 and "The index has been written to" in r.stdout, r.stderr
+    q = ref["queries"]
+    (tmp_path / "q.txt").write_text(f"{len(q)}\n" + "".join(f"{s} {t}\n" for s, t, *_ in q.tolist()))
+    r = subprocess.run([str(exe / "TC_query"), "-i", str(tmp_path / "ix.bin"), "-q", str(tmp_path / "q.txt"),
+                        "-o", str(tmp_path / "a.txt")], capture_output=True, text=True)
+    assert r.returncode == 0 and "Query costs" in r.stdout, r.stderr
+    lines = (tmp_path / "a.txt").read_text().split()
+    assert int(lines[0]) == len(q)
+    assert [int(x) for x in lines[1:]] == [int(c) for c in q[:, 3]]
+
+
 # ---- QBC's H2H distance labels (bcx_index_build_h2h) ------------------------
 
 def test_la_h2h_queries_match_reference(golden_dir, la):
