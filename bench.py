@@ -2,7 +2,8 @@
 
 A step is one batched launch sequence over one batch of source vertices on
 the seeded road-like 320,000-vertex graph with its MHL index resident in
-HBM: the bag-sweep form (16,384 sources: per-tile sweep kernel, tile
+HBM: the bag-sweep form (3 tiles of 64 sources per compute unit, 49,152 on
+MI355X: per-tile sweep kernel, tile
 reduction; the shortcut unpacking runs once per full BC and is amortized).
 The value is whole-job full-BC throughput in sources/s; the label-scan form,
 the plain one-source-per-launch form, the RQ serving latencies and the CPU
@@ -168,11 +169,12 @@ def log(*a):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="road320k")
     ap.add_argument("--batch", type=int, default=4096, help="label-scan sources per batch")
-    ap.add_argument("--sweep-batch", type=int, default=16384, help="bag-sweep sources per batch")
+    ap.add_argument("--sweep-batch", type=int, default=0,
+                    help="bag-sweep sources per launch (0: the library default, 3 tiles of 64 per CU)")
     ap.add_argument("--plain-sources", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -223,6 +225,7 @@ def main() -> int:
     # Batches are blocks of the tree order (how a full sweep visits sources),
     # spread evenly over it so the sample covers shallow and deep subtrees.
     tree_perm = ix.tree_order()
+    sweep_default = ctx.tuning()["sweep_batch"]
 
     def tree_batch(k: int, nb: int, size: int = B) -> np.ndarray:
         size = min(size, n)
@@ -234,7 +237,7 @@ def main() -> int:
     if args.profile_only:
         ctx.set_tuning(batch=B)
         ctx.bc_sources(tree_batch(1, 3, B), bcx.MODE_BATCHED)
-        Bs = min(args.sweep_batch, n)
+        Bs = min(args.sweep_batch or sweep_default, n)
         ctx.set_tuning(sweep_batch=Bs)
         ctx.bc_sources(tree_batch(1, 3, Bs), bcx.MODE_SWEEP)
         ms = ctx.copy_probe(1 << 30)
@@ -272,7 +275,7 @@ def main() -> int:
         tuning = ctx.autotune(tree_batch(0, total))
         log(f"[bench] autotune: {tuning} ({time.time() - ta:.1f}s)")
     scan_rate, scan_el, scan_tm, ovf1 = time_mode(bcx.MODE_BATCHED, "label-scan batched", B)
-    Bs = min(args.sweep_batch, n)
+    Bs = min(args.sweep_batch or sweep_default, n)
     value, el, tm, ovf = time_mode(bcx.MODE_SWEEP, "bag-sweep batched", Bs)
     ctx.set_tuning(batch=B)
     ovf = ovf or ovf1

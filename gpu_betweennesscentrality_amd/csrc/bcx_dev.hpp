@@ -194,6 +194,7 @@ T *dev_upload(const std::vector<T> &h) {
 
 struct bcx_ctx {
   int device = 0;
+  int cus = 0;  // compute units of the device
   hipStream_t stream = nullptr;
   bcx_dev::DevIndex ix{};
   std::vector<void *> owned;          // index + graph buffers

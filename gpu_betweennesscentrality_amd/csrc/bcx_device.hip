@@ -531,10 +531,12 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
   HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto &e : c->ev) HIP_OK(hipEventCreate(&e));
   {
-    // One sweep tile per compute unit per launch.
+    // Three sweep tiles per compute unit per launch (two run at a time on a
+    // unit; memory may cut the launch to fewer).
     int cus = 0;
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    c->tn.sweep_batch = std::max(1, cus) * kWave;
+    c->cus = std::max(1, cus);
+    c->tn.sweep_batch = 3 * c->cus * kWave;
   }
   const int n = g.n;
   c->perm = ix.perm;

@@ -617,8 +617,8 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
 // level split in runs (few nodes, many pieces) then merges each node's
 // per-wavefront partial results from LDS in a fixed order, so results are
 // deterministic.
-template <int kNodesF, int kNodesB>
-__global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, DevTree tr,
+template <int kNodesF, int kNodesB, int kWavesPerSimd>
+__global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kernel(DevIndex ix, DevTree tr,
                                                                    SweepWork sw, int nsrc,
                                                                    int *ovf_flag,
                                                                    uint64_t *stamps) {
@@ -779,9 +779,17 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_tile_kernel(DevIndex ix, 
 
 using TileKernel = void (*)(DevIndex, DevTree, SweepWork, int, int *, uint64_t *);
 
-// Nodes per wavefront step (forward, backward): 2 and 2, the fastest on road
-// graphs (with the steps pipelined, more nodes per step only spill registers).
-TileKernel tile_kernel() { return (TileKernel)sweep_tile_kernel<2, 2>; }
+// Two forms of the tile kernel.  Up to one tile per compute unit: 2 pieces per
+// step in each pass, registers for 5 waves per SIMD (one 16-wave workgroup per
+// unit).  More tiles than units: 1 piece per step, registers capped at 64 for
+// 8 waves per SIMD, so two workgroups share each unit and one's waits are
+// covered by the other's work (on road graphs 11-16% more sources per second,
+// with three tiles per unit the best, while one such workgroup alone is 8%
+// slower than the first form).
+TileKernel tile_kernel(int tiles, int cus) {
+  if (tiles > cus) return (TileKernel)sweep_tile_kernel<1, 1, 8>;
+  return (TileKernel)sweep_tile_kernel<2, 2, 1>;
+}
 
 // node_out[v] (+)= sum over tiles of node_t, phi_out[e] (+)= sum over tiles
 // of phi_t, tiles in increasing order.
@@ -971,6 +979,10 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
     cap = cap_now();
   }
   if (cap < 1) throw bcx::Error{BCX_ENOMEM, "not enough device memory for one sweep tile"};
+  // Cut short by memory, a launch of more tiles than compute units takes a
+  // whole number of tiles per unit, so that no unit runs a lone extra tile
+  // after the others are done.
+  if (cap < (size_t)tiles && c->cus > 0 && cap > (size_t)c->cus) cap -= cap % (size_t)c->cus;
   tiles = (int)std::min<size_t>((size_t)tiles, cap);
   c->ensure_sweep(tiles);
   return tiles;
@@ -1027,7 +1039,8 @@ void sweep_launch(bcx_ctx *c, const int32_t *pre, int32_t nsrc, double *node_out
   // Node sums start at zero: nodes without descendants get only ascending flow.
   zero_f64_kernel<<<blocks_for((int64_t)bt * n, 256), 256, 0, c->stream>>>(sw.node_t,
                                                                           (int64_t)bt * n);
-  tile_kernel()<<<bt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf, stamps);
+  tile_kernel(bt, c->cus)<<<bt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf,
+                                                                stamps);
   if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
   if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
   sweep_tile_reduce_kernel<<<blocks_for(n + B, 256), 256, 0, c->stream>>>(

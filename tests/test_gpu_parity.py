@@ -254,6 +254,24 @@ def test_sweep_full_bc(maker):
             f"{np.max(np.abs(out - ref) / np.maximum(np.abs(ref), 1)):.3e}"
 
 
+def test_sweep_two_kernel_forms_agree_bitwise(road):
+    # More tiles in a launch than compute units selects the second form of the
+    # tile kernel (two workgroups per unit, one piece per step); launches of
+    # fewer tiles use the first.  Same tiles, same summation orders: the
+    # results must be bitwise equal.
+    g, csr, ctx = road
+    default = ctx.tuning()["sweep_batch"]  # three tiles of 64 per compute unit
+    cus = default // (3 * 64)
+    src = np.arange(g.n, dtype=np.int32)
+    assert (g.n + 63) // 64 > cus, "the launch must hold more tiles than compute units"
+    ctx.set_tuning(sweep_batch=default)
+    two, o1 = ctx.bc_sources(src, bcx.MODE_SWEEP)
+    ctx.set_tuning(sweep_batch=64 * max(1, cus // 2))
+    one, o2 = ctx.bc_sources(src, bcx.MODE_SWEEP)
+    assert not o1 and not o2
+    assert np.array_equal(two.view(np.uint64), one.view(np.uint64))
+
+
 def test_sweep_scattered_sources(road):
     g, csr, ctx = road
     src = gg.permutation(6, 2, g.n)[:700].astype(np.int32)  # not tree-consecutive
