@@ -77,7 +77,8 @@ typedef struct bcx_tuning {
   int32_t bp_threads;    /* threads per dependency group: 64..512, pow2   */
   int32_t batch;         /* sources per batched launch sequence             */
   int32_t sweep_batch;   /* sources per bag-sweep launch (64 per tile, one
-                            tile per workgroup); default 64 x compute units */
+                            tile per workgroup); default 3 x 64 x compute
+                            units (two workgroups per unit at a time)       */
 } bcx_tuning;
 
 typedef struct bcx_timing {
