@@ -206,6 +206,60 @@ def test_cli_rq_table(tmp_path, grid):
     assert np.all(np.abs(got - want) <= 1e-9 * np.maximum(np.abs(want), 1.0) + 5e-9)
 
 
+# ---- closed forms and edge cases, every form -----------------------------------
+
+def _small(n, edges):
+    e = np.asarray(edges, np.int32).reshape(-1, 2)
+    g = gg._both_ways(n, e[:, 0], e[:, 1], np.ones(len(e), np.int32))
+    G = bcx.Graph.from_arcs(g.n, g.u, g.v, g.w)
+    ix = bcx.Index.build_mhl(G)
+    ix.build_h2h()
+    return g, bcx.Context(G, ix, 0)
+
+
+FORMS = [bcx.MODE_SWEEP, bcx.MODE_BATCHED, bcx.MODE_PLAIN]
+
+
+@pytest.mark.parametrize("name,n,edges,want", [
+    ("path", 50, [(i, i + 1) for i in range(49)], [i * (49 - i) for i in range(50)]),
+    ("star", 40, [(0, i) for i in range(1, 40)], [38 * 39 / 2] + [0] * 39),
+    ("odd cycle", 31, [(i, (i + 1) % 31) for i in range(31)], [30 * 28 / 8] * 31),
+    ("even cycle", 20, [(i, (i + 1) % 20) for i in range(20)], [18 * 18 / 8] * 20),
+    ("complete", 12, [(i, j) for i in range(12) for j in range(i + 1, 12)], [0] * 12),
+    ("two vertices", 2, [(0, 1)], [0, 0]),
+])
+def test_closed_forms_every_form(name, n, edges, want):
+    g, ctx = _small(n, edges)
+    for mode in FORMS:
+        bc, ovf = ctx.full_bc(mode)
+        assert not ovf
+        assert close_bc(bc, np.asarray(want, np.float64)), f"{name}, mode {mode}"
+    csr = oracle.CSR.from_arcgraph(g)
+    dist = ctx.h2h_distances(np.arange(n, dtype=np.int32))
+    for s in range(n):
+        assert np.array_equal(dist[s].astype(np.int64), oracle.sssp(csr, s)[0]), f"{name}, source {s}"
+
+
+def test_ragged_and_empty_source_lists(grid):
+    g, csr, ctx = grid
+    ctx.set_tuning(sweep_batch=4096, batch=256)
+    for k in (0, 1, 63, 65, 129):
+        src = gg.permutation(17, k + 1, g.n)[:k].astype(np.int32)
+        ref = oracle.brandes(csr, src)[0] if k else np.zeros(g.n)
+        for mode in FORMS:
+            out, ovf = ctx.bc_sources(src, mode)
+            assert not ovf and close_bc(out, ref), f"{k} sources, mode {mode}"
+
+
+def test_repeated_sources_count_twice(grid):
+    g, csr, ctx = grid
+    src = np.array([5, 5, 700], np.int32)
+    ref = oracle.brandes(csr, np.array([5, 700], np.int32))[0] + oracle.brandes(csr, np.array([5], np.int32))[0]
+    for mode in FORMS:
+        out, _ = ctx.bc_sources(src, mode)
+        assert close_bc(out, ref), f"mode {mode}"
+
+
 # ---- QBC distance-only scan (H2H labels) ------------------------------------
 
 @pytest.mark.parametrize("maker,order", [

@@ -41,9 +41,9 @@ def test_odd_cycle_closed_form(n):
 
 
 def test_even_cycle_and_complete():
-    n = 8  # even cycle: antipodal pairs split 1/2 over two paths
-    bc, _ = oracle.brandes(csr_from_edges(n, [(i, (i + 1) % n) for i in range(n)]))
-    assert np.allclose(bc, bc[0])
+    for n in (8, 20):  # even cycle: antipodal pairs split 1/2 over two paths
+        bc, _ = oracle.brandes(csr_from_edges(n, [(i, (i + 1) % n) for i in range(n)]))
+        assert np.allclose(bc, (n - 2) ** 2 / 8)
     k = 7
     bc, _ = oracle.brandes(csr_from_edges(k, [(i, j) for i in range(k) for j in range(i + 1, k)]))
     assert np.all(bc == 0)
