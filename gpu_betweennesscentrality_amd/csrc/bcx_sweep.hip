@@ -176,13 +176,6 @@ __device__ __forceinline__ uint64_t slot_if(bool need, uint32_t r, const Lane &L
   if (need) x = SS(L, r);
   return x;
 }
-// The same with the load issued in every lane (lanes that need nothing read
-// the tile's first record), for a value converted right after: the compiler
-// would move a conversion of slot_if's result into its load's branch, where
-// it waits for everything outstanding.
-__device__ __forceinline__ uint64_t slot_all(bool need, uint32_t r, const Lane &L) {
-  return SS(L, need ? r : 0u);
-}
 
 // Lane moves of a double.  dpp_row: a DPP move inside rows of 16 lanes that
 // writes every lane; dpp_move: one with a row mask (rows left out keep 0).
@@ -267,17 +260,6 @@ constexpr int kSweepThreads = 1024;
 constexpr int kSweepWaves = kSweepThreads / kWave;
 
 __device__ __forceinline__ int rl(int x, int j) { return __builtin_amdgcn_readlane(x, j); }
-__device__ __forceinline__ double rlf(double x, int j) {
-  const uint64_t b = __builtin_bit_cast(uint64_t, x);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, j);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), j);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-__device__ __forceinline__ uint64_t rl64(uint64_t x, int j) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, j);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), j);
-  return ((uint64_t)hi << 32) | lo;
-}
 
 constexpr int kStage = kStageEntries;  // bag entries per node staged in registers
 
@@ -334,11 +316,12 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
     const int jn = min(kWave, mine - base);
     const int p = first + base + min(L.lane, jn - 1);
     const int4 nd = tr.fs_node[p], su = tr.fs_u[p], sl = tr.fs_len[p];
-    const ulonglong2 c01 = tr.fs_cnt[2 * p], c23 = tr.fs_cnt[2 * p + 1];
     wait_all_vm();
     const int sur[kStage] = {su.x, su.y, su.z, su.w};
     const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
-    const uint64_t scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
+    // Shortcut counts are 1 for most entries (a flag bit); the others are
+    // read as scalars when used, which keeps them out of the staged registers.
+    const uint64_t *scnt = (const uint64_t *)tr.fs_cnt + 4 * (int64_t)(first + base);
     // Round 1 of the step at j0: the staged entries' distance rows (nodes past
     // the block repeat its last node).
     // Every staged place is loaded, also past the node's bag (node 0's row,
@@ -430,7 +413,7 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 #pragma unroll
         for (int r = 0; r < kStage; ++r)
           if (cand[u][r] == b)  // shortcut counts are 1 for most entries (a uniform bit)
-            c = add_sat(c, ((fl >> (24 + r)) & 1) ? m[u][r] : mul_sat(m[u][r], rl64(scr[r], j)));
+            c = add_sat(c, ((fl >> (24 + r)) & 1) ? m[u][r] : mul_sat(m[u][r], cld(scnt, 4 * j + r)));
         // Merge with the node's earlier pieces (saturating sums do not
         // depend on how their terms are grouped).
         if (!pfirst) {
@@ -482,12 +465,14 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
     const int p = first + base + min(L.lane, jn - 1);
-    const int4 nd = tr.bs_node[p], sx = tr.bs_x[p], se = tr.bs_e[p], sl = tr.bs_len[p];
-    const double2 c01 = tr.bs_cntf[2 * p], c23 = tr.bs_cntf[2 * p + 1];
+    const int4 nd = tr.bs_node[p], sx = tr.bs_x[p], sl = tr.bs_len[p];
     wait_all_vm();
     const int sxr[kStage] = {sx.x, sx.y, sx.z, sx.w};
     const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
-    const double scr[kStage] = {c01.x, c01.y, c23.x, c23.y};
+    // Shortcut counts (1 for most entries, a flag bit) and the entries' shortcut
+    // indices are read as scalars when used, not staged in registers.
+    const double *scnt = (const double *)tr.bs_cntf + 4 * (int64_t)(first + base);
+    const int *sent = (const int *)tr.bs_e + 4 * (int64_t)(first + base);
     // Round 1 of the step at j0 (nodes past the block repeat its last node).
     auto round1 = [&](int j0, int (&wv)[kNodesB], int (&wx)[kNodesB][kStage]) {
 #pragma unroll
@@ -522,10 +507,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
-        {
-          const uint64_t raw = slot_all(!one_of(wv[u]), roff(rl(nd.x, j)), L);
-          svr[u] = one_of(wv[u]) ? 1 : raw;
-        }
+        svr[u] = slot_if(!one_of(wv[u]), roff(rl(nd.x, j)), L);  // 1 where flagged
         const int dv = dist_of(wv[u]);
         const int leafm = rl(nd.w, j);
         tm[u] = 0;
@@ -543,17 +525,25 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
       __builtin_amdgcn_sched_barrier(0);
       double svu[kNodesB];
       double qx[kNodesB][kStage];  // Q(x) in the lanes where entry r is tight
+      // An opaque zero defined here keeps the conversion of the node's count
+      // below this point: the compiler would otherwise move it into the
+      // count's masked-load branch, where it waits for every outstanding load.
+      int zero = 0;
+      __asm__ volatile("" : "+s"(zero));
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
         const int leafm = rl(nd.w, j);
-        svu[u] = (double)svr[u];
+        svu[u] = (double)(svr[u] | (uint32_t)zero);
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
           // Q(x) where the entry is tight: a quotient from x's backward step,
           // or, where x has no descendants (a uniform bit), 1 / sigma(x).
           double q = __builtin_bit_cast(double, qr[u][r]);
-          if ((leafm >> r) & 1) q = 1.0 / (double)(one_of(wx[u][r]) ? 1 : qr[u][r]);
+          if ((leafm >> r) & 1)  // the division only where some tight lane's count is not 1
+            q = __builtin_amdgcn_ballot_w64(((tm[u] >> r) & 1) & !one_of(wx[u][r]))
+                    ? 1.0 / (double)(one_of(wx[u][r]) ? 1 : qr[u][r])
+                    : 1.0;
           qx[u][r] = ((tm[u] >> r) & 1) ? q : 0.0;
         }
       }
@@ -571,11 +561,12 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         double f[kStage], acc = 0.0;
 #pragma unroll
         for (int r = 0; r < kStage; ++r) {
-          f[r] = ((fl >> (8 + r)) & 1) ? svv * qx[u][r] : rlf(scr[r], j) * svv * qx[u][r];
+          f[r] = ((fl >> (8 + r)) & 1) ? svv * qx[u][r] : cld(scnt, 4 * j + r) * svv * qx[u][r];
           acc += f[r];
         }
         const double fs = wave_sum4(f[0], f[1], f[2], f[3]);
-        const int e = row_pick(L.lane, rl(se.x, j), rl(se.y, j), rl(se.z, j), rl(se.w, j));
+        const int e = row_pick(L.lane, cld(sent, 4 * j), cld(sent, 4 * j + 1), cld(sent, 4 * j + 2),
+                               cld(sent, 4 * j + 3));
         store_all(L, live & ((L.lane & 15) == 0) & ((L.lane >> 4) < nb),
                   sw.phi_t + (int64_t)L.t * tr.bags + e, fs);
         // The node's dependency part, summed piece by piece in entry order.
@@ -592,7 +583,8 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
           ns_in[u] = 0.0;
         } else {
           double q = 0.0;
-          if (fin) q = (1.0 + tot) / svv;  // uniform branch: only the last piece divides
+          if (fin)  // uniform branch: only the last piece divides, where some count is not 1
+            q = __builtin_amdgcn_ballot_w64(!one_of(wv[u])) ? (1.0 + tot) / svv : 1.0 + tot;
           setQ(L, fin ? roff(v) : L.sink, q);
           ns_in[u] = (fin & L.valid & (L.s != v)) ? tot : 0.0;
           fin_mask |= (int)fin << u;
