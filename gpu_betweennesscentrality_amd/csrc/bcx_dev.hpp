@@ -102,9 +102,13 @@ struct DevTree {
   // A level whose pieces are split over the wavefronts in contiguous runs
   // (lvl_nseg[k] > 0) leaves per-node partial results (segments) in LDS,
   // wavefront w's first one numbered st_seg[16 k + w], merged after the level.
+  // In a forward level split by node, the last st_nar[16 k + w] slots of
+  // wavefront w are narrow: nodes of one piece with at most kNarrowEntries
+  // entries, whose steps load only that many rows.
   const int *st_off;          // height + 1 (forward: all nodes)
   const int *st_cnt;          // 16 * height
   const int *st_seg;          // 16 * height
+  const int *st_nar;          // 16 * height
   const int *lvl_nseg;        // height: segments of a level split in runs, or 0
   const int *bst_off;         // height + 1 (backward: nodes with descendants)
   const int *bst_cnt;         // 16 * height
@@ -127,6 +131,7 @@ struct DevTree {
 };
 
 constexpr int kStageEntries = 4;
+constexpr int kNarrowEntries = 2;
 // Length of unused staged places: a distance plus it exceeds every true
 // distance (below 999999999) without overflowing, so such an entry is never
 // minimal or tight.

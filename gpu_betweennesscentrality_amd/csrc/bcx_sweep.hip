@@ -283,6 +283,22 @@ __device__ __forceinline__ void level_block(const int *st_off, const int *st_cnt
   count = cld(st_cnt, k * kSweepWaves + wave);
 }
 
+// A staged piece's entry fields (kWidth of the four places of its slot).
+template <int kWidth>
+__device__ __forceinline__ void stage_entries(const int4 *ids, const int4 *lens, int p,
+                                              int (&id)[kWidth], int (&len)[kWidth]) {
+  if constexpr (kWidth == 4) {
+    const int4 a = ids[p], b = lens[p];
+    id[0] = a.x, id[1] = a.y, id[2] = a.z, id[3] = a.w;
+    len[0] = b.x, len[1] = b.y, len[2] = b.z, len[3] = b.w;
+  } else {
+    static_assert(kWidth == 2, "narrow pieces hold two entries");
+    const int2 a = ((const int2 *)ids)[2 * p], b = ((const int2 *)lens)[2 * p];
+    id[0] = a.x, id[1] = a.y;
+    len[0] = b.x, len[1] = b.y;
+  }
+}
+
 // LDS segments of a level split in runs (see the staging): per segment, two
 // 64-bit words per lane and the node it belongs to.
 struct Segs {
@@ -303,22 +319,23 @@ struct Segs {
 // minimum and count are carried from piece to piece and stored by its last
 // piece.  Saturating sums of saturating products do not depend on the order
 // of their terms, so the count equals that of a running min-and-count.
-template <int kNodesF, bool runs>  // runs: split in runs, results go to LDS segments
+// The wavefront's slots [first, first + mine), of at most kWidth entries each
+// (kStage, or kNarrowEntries for the narrow tail of a level split by node).
+template <int kNodesF, bool runs, int kWidth>  // runs: split in runs, results go to LDS segments
 __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree &tr,
-                                               const SweepWork &sw, const Lane &L, int wave,
-                                               int depth, int &ovf, const Segs &sg) {
-  int first, mine;
-  level_block(tr.st_off, tr.st_cnt, depth, wave, first, mine);
-  int seg = runs ? cld(tr.st_seg, depth * kSweepWaves + wave) : 0;
+                                               const SweepWork &sw, const Lane &L, int depth,
+                                               int first, int mine, int seg, int &ovf,
+                                               const Segs &sg) {
+  static_assert(kWidth == kStage || (!runs && kWidth == kNarrowEntries), "piece width");
   int cb = 0x7fffffff;  // the node's minimum and count over its earlier pieces
   uint64_t cc = 0;
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
     const int p = first + base + min(L.lane, jn - 1);
-    const int4 nd = tr.fs_node[p], su = tr.fs_u[p], sl = tr.fs_len[p];
+    const int4 nd = tr.fs_node[p];
+    int sur[kWidth], slr[kWidth];
+    stage_entries<kWidth>(tr.fs_u, tr.fs_len, p, sur, slr);
     wait_all_vm();
-    const int sur[kStage] = {su.x, su.y, su.z, su.w};
-    const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
     // Shortcut counts are 1 for most entries (a flag bit); the others are
     // read as scalars when used, which keeps them out of the staged registers.
     const uint64_t *scnt = (const uint64_t *)tr.fs_cnt + 4 * (int64_t)(first + base);
@@ -327,29 +344,29 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
     // Every staged place is loaded, also past the node's bag (node 0's row,
     // a hot line): a load that may not issue would shorten every count-based
     // wait after it, and skipping them measured slower.
-    auto round1 = [&](int j0, int (&w)[kNodesF][kStage]) {
+    auto round1 = [&](int j0, int (&w)[kNodesF][kWidth]) {
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) w[u][r] = SD(L, roff(rl(sur[r], j)));
+        for (int r = 0; r < kWidth; ++r) w[u][r] = SD(L, roff(rl(sur[r], j)));
       }
     };
-    int w[kNodesF][kStage];
+    int w[kNodesF][kWidth];
     round1(0, w);
     sink_stores<2 * kNodesF>(L);  // a step stores a distance and a count row per node
     for (int j0 = 0; j0 < jn; j0 += kNodesF) {
       // This step's rows have arrived once only the stores issued after them
       // (the last step's, or the sink stores) are outstanding.
       wait_vm_left<2 * kNodesF>();
-      int cand[kNodesF][kStage], best[kNodesF], onem[kNodesF];
+      int cand[kNodesF][kWidth], best[kNodesF], onem[kNodesF];
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
         onem[u] = 0;
         best[u] = 0x7fffffff;
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) {
+        for (int r = 0; r < kWidth; ++r) {
           const int len = rl(slr[r], j);  // kNoLen in unused places
           cand[u][r] = dist_of(w[u][r]) + len;
           onem[u] |= (int)one_of(w[u][r]) << r;
@@ -384,12 +401,12 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
         }
       }
       // Then the count rows of tight candidates not flagged 1.
-      uint64_t m[kNodesF][kStage];
+      uint64_t m[kNodesF][kWidth];
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) {
+        for (int r = 0; r < kWidth; ++r) {
           const bool need = (cand[u][r] == best[u]) & !((onem[u] >> r) & 1);
           const uint64_t raw = slot_if(need, roff(rl(sur[r], j)), L);
           m[u][r] = need ? raw : 1;
@@ -411,7 +428,7 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
         int b = min(best[u], upd[u]);
         uint64_t c = upd[u] == b ? upc[u] : 0;
 #pragma unroll
-        for (int r = 0; r < kStage; ++r)
+        for (int r = 0; r < kWidth; ++r)
           if (cand[u][r] == b)  // shortcut counts are 1 for most entries (a uniform bit)
             c = add_sat(c, ((fl >> (24 + r)) & 1) ? m[u][r] : mul_sat(m[u][r], cld(scnt, 4 * j + r)));
         // Merge with the node's earlier pieces (saturating sums do not
@@ -432,12 +449,13 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
             ++seg;
           }
           put_dist_count_all(L, L.sink, b, c);
+          __asm__ volatile("" ::: "memory");  // not a dead store: the waits count it
         } else {
           put_dist_count_all(L, fin ? roff(v) : L.sink, b, c);
           ovf |= fin & L.valid & (c == ~0ull);
         }
       }
-      wait_vm_left<kNodesF * kStage + 2 * kNodesF>();
+      wait_vm_left<kNodesF * kWidth + 2 * kNodesF>();
     }
   }
 }
@@ -452,57 +470,55 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 // piece by piece; its last piece writes Q and the node flow.  The flows of a
 // piece's entries and the node flows of the step are each one four-way
 // transposed lane sum.
-template <int kNodesB, bool runs>  // runs: split in runs, results go to LDS segments
+template <int kNodesB, bool runs, int kWidth>  // runs: split in runs, results go to LDS segments
 __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWork &sw,
-                                               const Lane &L, int wave, int depth,
+                                               const Lane &L, int first, int mine, int seg,
                                                const Segs &sg) {
   static_assert(kNodesB <= 4 && kStage == 4, "four-way lane sums");
+  static_assert(kWidth == kStage || (!runs && kWidth == kNarrowEntries), "piece width");
   const int n = tr.n;
-  int first, mine;
-  level_block(tr.bst_off, tr.bst_cnt, depth, wave, first, mine);
-  int seg = runs ? cld(tr.bst_seg, depth * kSweepWaves + wave) : 0;
   double carry = 0.0;  // the node's flow sum over its earlier pieces
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
     const int p = first + base + min(L.lane, jn - 1);
-    const int4 nd = tr.bs_node[p], sx = tr.bs_x[p], sl = tr.bs_len[p];
+    const int4 nd = tr.bs_node[p];
+    int sxr[kWidth], slr[kWidth];
+    stage_entries<kWidth>(tr.bs_x, tr.bs_len, p, sxr, slr);
     wait_all_vm();
-    const int sxr[kStage] = {sx.x, sx.y, sx.z, sx.w};
-    const int slr[kStage] = {sl.x, sl.y, sl.z, sl.w};
     // Shortcut counts (1 for most entries, a flag bit) and the entries' shortcut
     // indices are read as scalars when used, not staged in registers.
     const double *scnt = (const double *)tr.bs_cntf + 4 * (int64_t)(first + base);
     const int *sent = (const int *)tr.bs_e + 4 * (int64_t)(first + base);
     // Round 1 of the step at j0 (nodes past the block repeat its last node).
-    auto round1 = [&](int j0, int (&wv)[kNodesB], int (&wx)[kNodesB][kStage]) {
+    auto round1 = [&](int j0, int (&wv)[kNodesB], int (&wx)[kNodesB][kWidth]) {
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
         wv[u] = SD(L, roff(rl(nd.x, j)));
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) wx[u][r] = SD(L, roff(rl(sxr[r], j)));
+        for (int r = 0; r < kWidth; ++r) wx[u][r] = SD(L, roff(rl(sxr[r], j)));
       }
     };
-    int wvn[kNodesB], wxn[kNodesB][kStage];
+    int wvn[kNodesB], wxn[kNodesB][kWidth];
     round1(0, wvn, wxn);
     sink_stores<2 * kNodesB + 1>(L);  // a step stores flows and a quotient per node, then node flows
     for (int j0 = 0; j0 < jn; j0 += kNodesB) {
       // This step's rows have arrived once only the stores issued after them
       // are outstanding (as in the forward pass).
       wait_vm_left<2 * kNodesB + 1>();
-      int wv[kNodesB], wx[kNodesB][kStage];
+      int wv[kNodesB], wx[kNodesB][kWidth];
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
         wv[u] = wvn[u];
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) wx[u][r] = wxn[u][r];
+        for (int r = 0; r < kWidth; ++r) wx[u][r] = wxn[u][r];
       }
       // Round 2: raw slot words only (counts of nodes not flagged 1, and the
       // tight entries' slots); the arithmetic follows once all are issued,
       // and the phase order is kept as written (see the forward pass).
       __builtin_amdgcn_sched_barrier(0);
       uint64_t svr[kNodesB];
-      uint64_t qr[kNodesB][kStage];
+      uint64_t qr[kNodesB][kWidth];
       int tm[kNodesB];
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
@@ -512,7 +528,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         const int leafm = rl(nd.w, j);
         tm[u] = 0;
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) {
+        for (int r = 0; r < kWidth; ++r) {
           const int len = rl(slr[r], j);  // kNoLen in unused places
           // Padding lanes (sources past the batch) count as never tight, so
           // their flows are 0 and the lane sums below need no mask.
@@ -524,7 +540,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
       round1(j0 + kNodesB, wvn, wxn);
       __builtin_amdgcn_sched_barrier(0);
       double svu[kNodesB];
-      double qx[kNodesB][kStage];  // Q(x) in the lanes where entry r is tight
+      double qx[kNodesB][kWidth];  // Q(x) in the lanes where entry r is tight
       // An opaque zero defined here keeps the conversion of the node's count
       // below this point: the compiler would otherwise move it into the
       // count's masked-load branch, where it waits for every outstanding load.
@@ -536,7 +552,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         const int leafm = rl(nd.w, j);
         svu[u] = (double)(svr[u] | (uint32_t)zero);
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) {
+        for (int r = 0; r < kWidth; ++r) {
           // Q(x) where the entry is tight: a quotient from x's backward step,
           // or, where x has no descendants (a uniform bit), 1 / sigma(x).
           double q = __builtin_bit_cast(double, qr[u][r]);
@@ -553,18 +569,19 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
         const bool live = j0 + u < jn;  // false for the clamped repeats of the last piece
-        const int v = rl(nd.x, j), nb = rl(nd.y, j), fl = rl(nd.w, j);  // nb <= kStage
+        const int v = rl(nd.x, j), nb = rl(nd.y, j), fl = rl(nd.w, j);  // nb <= kWidth
         const int pos = base + j0 + u;  // result boundaries as in the forward pass
         const bool pfirst = ((fl >> 4) & 1) || (runs && pos == 0);
         const bool fin = live && (((fl >> 5) & 1) || (runs && pos == mine - 1));
         const double svv = svu[u];
-        double f[kStage], acc = 0.0;
+        double f[kWidth], acc = 0.0;
 #pragma unroll
-        for (int r = 0; r < kStage; ++r) {
+        for (int r = 0; r < kWidth; ++r) {
           f[r] = ((fl >> (8 + r)) & 1) ? svv * qx[u][r] : cld(scnt, 4 * j + r) * svv * qx[u][r];
           acc += f[r];
         }
-        const double fs = wave_sum4(f[0], f[1], f[2], f[3]);
+        const double fs = wave_sum4(f[0], f[1], kWidth > 2 ? f[2 % kWidth] : 0.0,
+                                    kWidth > 3 ? f[3 % kWidth] : 0.0);
         const int e = row_pick(L.lane, cld(sent, 4 * j), cld(sent, 4 * j + 1), cld(sent, 4 * j + 2),
                                cld(sent, 4 * j + 3));
         store_all(L, live & ((L.lane & 15) == 0) & ((L.lane >> 4) < nb),
@@ -580,6 +597,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
             ++seg;
           }
           setQ(L, L.sink, 0.0);
+          __asm__ volatile("" ::: "memory");  // not a dead store: the waits count it
           ns_in[u] = 0.0;
         } else {
           double q = 0.0;
@@ -598,7 +616,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
                              rl(nd.x, min(j0 + 2, jn - 1)), rl(nd.x, min(j0 + 3, jn - 1)));
       store_all(L, ((L.lane & 15) == 0) & (u < kNodesB) & ((fin_mask >> u) & 1),
                 sw.node_t + (int64_t)L.t * n + v, ns);
-      wait_vm_left<kNodesB * (1 + kStage) + 2 * kNodesB + 1>();
+      wait_vm_left<kNodesB * (1 + kWidth) + 2 * kNodesB + 1>();
     }
   }
 }
@@ -629,10 +647,17 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
   // ---- forward: distance and path count ----
   for (int k = 0; k < H; ++k) {
     const int nseg = cld(tr.lvl_nseg, k);
-    if (nseg)
-      fwd_level_wave<kNodesF, true>(ix, tr, sw, L, wave, k, ovf, sg);
-    else
-      fwd_level_wave<kNodesF, false>(ix, tr, sw, L, wave, k, ovf, sg);
+    int first, mine;
+    level_block(tr.st_off, tr.st_cnt, k, wave, first, mine);
+    if (nseg) {
+      fwd_level_wave<kNodesF, true, kStage>(ix, tr, sw, L, k, first, mine,
+                                            cld(tr.st_seg, k * kSweepWaves + wave), ovf, sg);
+    } else {
+      const int nar = cld(tr.st_nar, k * kSweepWaves + wave);
+      fwd_level_wave<kNodesF, false, kStage>(ix, tr, sw, L, k, first, mine - nar, 0, ovf, sg);
+      fwd_level_wave<kNodesF, false, kNarrowEntries>(ix, tr, sw, L, k, first + mine - nar, nar, 0,
+                                                     ovf, sg);
+    }
     __syncthreads();
     if (nseg) {
       // Each node's segments, merged in order (min, then saturating count).
@@ -659,10 +684,14 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
   // ---- backward: dependencies, shortcut flows, node flows ----
   for (int k = H - 1; k >= 0; --k) {
     const int nseg = cld(tr.blvl_nseg, k);
-    if (nseg)
-      bwd_level_wave<kNodesB, true>(tr, sw, L, wave, k, sg);
-    else
-      bwd_level_wave<kNodesB, false>(tr, sw, L, wave, k, sg);
+    int first, mine;
+    level_block(tr.bst_off, tr.bst_cnt, k, wave, first, mine);
+    if (nseg) {
+      bwd_level_wave<kNodesB, true, kStage>(tr, sw, L, first, mine,
+                                            cld(tr.bst_seg, k * kSweepWaves + wave), sg);
+    } else {
+      bwd_level_wave<kNodesB, false, kStage>(tr, sw, L, first, mine, 0, sg);
+    }
     __syncthreads();
     if (nseg) {
       // Each node's segments, summed in order; then its quotient and flow.
