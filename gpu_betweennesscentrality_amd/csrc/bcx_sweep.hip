@@ -608,9 +608,10 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
           fin_mask |= (int)fin << u;
         }
       }
-      const double ns = wave_sum4(ns_in[0], kNodesB > 1 ? ns_in[1 % kNodesB] : 0.0,
-                                  kNodesB > 2 ? ns_in[2 % kNodesB] : 0.0,
-                                  kNodesB > 3 ? ns_in[3 % kNodesB] : 0.0);
+      double ns = 0.0;  // node flows, summed only in steps that end a node (uniform)
+      if (fin_mask)
+        ns = wave_sum4(ns_in[0], kNodesB > 1 ? ns_in[1 % kNodesB] : 0.0,
+                       kNodesB > 2 ? ns_in[2 % kNodesB] : 0.0, kNodesB > 3 ? ns_in[3 % kNodesB] : 0.0);
       const int u = L.lane >> 4;
       const int v = row_pick(L.lane, rl(nd.x, min(j0, jn - 1)), rl(nd.x, min(j0 + 1, jn - 1)),
                              rl(nd.x, min(j0 + 2, jn - 1)), rl(nd.x, min(j0 + 3, jn - 1)));
