@@ -224,11 +224,16 @@ struct bcx_ctx {
   std::vector<int32_t> h_parent, h_depth, h_subtree, h_lvl_off, h_lvl_pair_off;
   int sweep_tiles = 0;
   bcx_dev::SweepWork sw{};
-  // QBC H2H distance labels (rows as the count labels) and separators, when
-  // the index had them at creation.
+  // QBC H2H distance labels and separators, when the index had them at
+  // creation.  The labels are interleaved by groups of 64 consecutive
+  // preorder ids: entry k of node t at d_goff[t / 64] + 64 k + t % 64 (a
+  // group's rows padded to its longest), so that a wavefront of consecutive
+  // targets reads one entry position as one contiguous 256-byte row.
   const int *d_gdis = nullptr;
+  const int64_t *d_goff = nullptr;
   const int64_t *d_sep_off = nullptr;
   const int *d_sep = nullptr;
+  const int *d_iperm = nullptr;  // vertex id -> preorder id
 
   ~bcx_ctx() {
     if (hipSetDevice(device) != hipSuccess) return;

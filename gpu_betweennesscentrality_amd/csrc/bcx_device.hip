@@ -48,14 +48,17 @@ __device__ __forceinline__ int lca_depth_dev(const DevIndex &ix, int u, int v) {
   return min(x, y) - 1;
 }
 
-// QBC distance-only scan (QBC.cu:243-331 without the P2H branch tables): one
-// workgroup per source s.  The workgroup first puts s's ancestors by depth
-// (the last node of each depth level at or before s in preorder) and s's H2H
-// label row in LDS; then each thread takes targets t: the LCA depth from the
-// sparse table; if s or t is the LCA, one label lookup; otherwise the minimum
-// over the LCA's separator of label(s) + label(t).  Output in preorder ids.
+// QBC distance-only scan (QBC.cu:243-331 without the P2H branch tables): a
+// row of workgroups per source s (blockIdx.x), each taking every gridDim.y-th
+// block of 256 targets.  A workgroup first puts s's ancestors by depth (the
+// last node of each depth level at or before s in preorder) and s's H2H label
+// row in LDS; then each thread takes targets t: the LCA depth from the sparse
+// table; if s or t is the LCA, one label lookup; otherwise the minimum over
+// the LCA's separator of label(s) + label(t).  Labels are read in the
+// interleaved layout (gdis, goff); output in preorder ids.
 __global__ __launch_bounds__(256) void h2h_dist_kernel(DevIndex ix, const int *lvl_off,
                                                        const int *lvl_v, const int *gdis,
+                                                       const int64_t *goff,
                                                        const int64_t *sep_off, const int *sep,
                                                        const int *srcs, int *out) {
   extern __shared__ int h2h_lds[];
@@ -63,7 +66,6 @@ __global__ __launch_bounds__(256) void h2h_dist_kernel(DevIndex ix, const int *l
   const int s = srcs[blockIdx.x];
   const int ds = ix.depth[s];
   int *path = h2h_lds, *srow = h2h_lds + ds + 1;
-  const int64_t rs = ix.row_off[s];
   for (int j = threadIdx.x; j <= ds; j += blockDim.x) {
     int lo = lvl_off[j], hi = lvl_off[j + 1];
     while (lo < hi) {
@@ -74,18 +76,18 @@ __global__ __launch_bounds__(256) void h2h_dist_kernel(DevIndex ix, const int *l
         hi = mid;
     }
     path[j] = lvl_v[lo - 1];
-    srow[j] = gdis[rs + j];
+    srow[j] = gdis[goff[s >> 6] + 64 * j + (s & 63)];
   }
   __syncthreads();
   int *o = out + (int64_t)blockIdx.x * n;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+  for (int t = blockIdx.y * blockDim.x + threadIdx.x; t < n; t += gridDim.y * blockDim.x) {
     int d = 0;
     if (t != s) {
       const int L = lca_depth_dev(ix, s, t);
       const int dt = ix.depth[t];
-      const int64_t rt = ix.row_off[t];
+      const int *rt = gdis + goff[t >> 6] + (t & 63);  // entry k at rt[64 k]
       if (L == ds) {
-        d = gdis[rt + ds];  // s is an ancestor of t
+        d = rt[64 * ds];  // s is an ancestor of t
       } else if (L == dt) {
         d = srow[dt];  // t is an ancestor of s
       } else {
@@ -93,13 +95,23 @@ __global__ __launch_bounds__(256) void h2h_dist_kernel(DevIndex ix, const int *l
         int best = 0x7fffffff;  // label values stay <= 999999999, sums fit
         for (int64_t i = sep_off[a], e = sep_off[a + 1]; i < e; ++i) {
           const int k = sep[i];
-          best = min(best, srow[k] + gdis[rt + k]);
+          best = min(best, srow[k] + rt[64 * k]);
         }
         d = min(best, kInfDev);
       }
     }
     o[t] = d;
   }
+}
+
+// out[r][v] = in[r][iperm[v]] for row r = blockIdx.y: distance rows from
+// preorder ids back to vertex ids.
+__global__ __launch_bounds__(256) void h2h_to_vertex_ids_kernel(const int *in, const int *iperm, int n,
+                                                                int *out) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int64_t r = (int64_t)blockIdx.y * n;
+  out[r + v] = in[r + iperm[v]];
 }
 
 // min / count-sum / hop-max over entries 0..L of two rows (one thread).
@@ -597,7 +609,22 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
   d.gw = (const int *)keep(dev_upload(gw));
   c->ix = d;
   if (!ix.sep_off.empty()) {
-    c->d_gdis = (const int *)keep(dev_upload(ix.gdis));
+    const int64_t G = ((int64_t)n + 63) / 64;
+    std::vector<int64_t> goff((size_t)G + 1, 0);
+    for (int64_t g = 0; g < G; ++g) {
+      int32_t len = 0;
+      for (int64_t t = 64 * g; t < std::min<int64_t>(64 * g + 64, n); ++t)
+        len = std::max(len, ix.depth[(size_t)t] + 1);
+      goff[(size_t)g + 1] = goff[(size_t)g] + 64 * (int64_t)len;
+    }
+    std::vector<int32_t> gt((size_t)goff[(size_t)G], 0);
+    for (int64_t t = 0; t < n; ++t) {
+      const int64_t o = goff[(size_t)(t / 64)] + t % 64, r = ix.row_off[(size_t)t];
+      for (int32_t k = 0; k <= ix.depth[(size_t)t]; ++k) gt[(size_t)(o + 64 * k)] = ix.gdis[(size_t)(r + k)];
+    }
+    c->d_gdis = (const int *)keep(dev_upload(gt));
+    c->d_goff = (const int64_t *)keep(dev_upload(goff));
+    c->d_iperm = (const int *)keep(dev_upload(ix.iperm));
     c->d_sep_off = (const int64_t *)keep(dev_upload(ix.sep_off));
     c->d_sep = (const int *)keep(dev_upload(ix.sep));
   }
@@ -1055,20 +1082,31 @@ int bcx_h2h_distances(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t
   const size_t lds = (size_t)2 * (size_t)c->tree.height * 4;
   if (lds > 65536) return bcx::fail(BCX_EGRAPH, "tree too deep for the H2H scan (height over 8192)");
   HIP_OK(hipSetDevice(c->device));
-  // Chunks of sources per launch, the output chunk bounded to about 1 GiB.
-  const int32_t chunk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(count, (1ll << 30) / (4ll * n)));
-  int *d_out = nullptr, *d_src = nullptr;
-  HIP_OK(hipMalloc(&d_out, (size_t)chunk * n * 4));
-  HIP_OK(hipMalloc(&d_src, (size_t)chunk * 4));
-  std::vector<int32_t> pre((size_t)chunk), host((size_t)chunk * n);
+  // Chunks of sources per launch, the output chunk bounded to about 1 GiB
+  // (and to the grid's second dimension, one row of the reorder per source).
+  const int32_t chunk = (int32_t)std::max<int64_t>(
+      1, std::min<int64_t>({(int64_t)count, (1ll << 30) / (4ll * n), 65535}));
+  int *d_out = nullptr, *d_src = nullptr, *d_ids = nullptr;
+  auto release = [&]() {
+    for (int *p : {d_out, d_src, d_ids})
+      if (p) (void)hipFree(p);
+  };
+  std::vector<int32_t> pre((size_t)chunk);
   try {
+    HIP_OK(hipMalloc(&d_out, (size_t)chunk * n * 4));
+    HIP_OK(hipMalloc(&d_ids, (size_t)chunk * n * 4));
+    HIP_OK(hipMalloc(&d_src, (size_t)chunk * 4));
     for (int32_t b0 = 0; b0 < count; b0 += chunk) {
       const int32_t nb = std::min(chunk, count - b0);
       for (int32_t i = 0; i < nb; ++i) pre[(size_t)i] = c->iperm[(size_t)sources[b0 + i]];
       HIP_OK(hipMemcpyAsync(d_src, pre.data(), (size_t)nb * 4, hipMemcpyHostToDevice, c->stream));
       if (c->profiling) HIP_OK(hipEventRecord(c->ev[0], c->stream));
-      h2h_dist_kernel<<<nb, 256, lds, c->stream>>>(c->ix, c->tree.lvl_off, c->tree.lvl_v, c->d_gdis,
-                                                   c->d_sep_off, c->d_sep, d_src, d_out);
+      // Each source's targets are split over several workgroups (interleaved
+      // by 256), so that a launch has about 16 waves per compute unit.
+      const int slices = (int)std::max<int64_t>(
+          1, std::min<int64_t>((int64_t)(n + 255) / 256, (16ll * c->cus + nb - 1) / nb));
+      h2h_dist_kernel<<<dim3((unsigned)nb, (unsigned)slices), 256, lds, c->stream>>>(
+          c->ix, c->tree.lvl_off, c->tree.lvl_v, c->d_gdis, c->d_goff, c->d_sep_off, c->d_sep, d_src, d_out);
       HIP_OK(hipGetLastError());
       if (c->profiling) {
         HIP_OK(hipEventRecord(c->ev[1], c->stream));
@@ -1079,21 +1117,19 @@ int bcx_h2h_distances(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t
         c->timing.scan_launches += 1;
         c->timing.sources += nb;
       }
-      HIP_OK(hipMemcpyAsync(host.data(), d_out, (size_t)nb * n * 4, hipMemcpyDeviceToHost, c->stream));
+      // Rows back to vertex ids on the device, then one copy to the caller.
+      h2h_to_vertex_ids_kernel<<<dim3((unsigned)((n + 255) / 256), (unsigned)nb), 256, 0, c->stream>>>(
+          d_out, c->d_iperm, n, d_ids);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipMemcpyAsync(dist_out + (int64_t)b0 * n, d_ids, (size_t)nb * n * 4, hipMemcpyDeviceToHost,
+                            c->stream));
       HIP_OK(hipStreamSynchronize(c->stream));
-      for (int32_t i = 0; i < nb; ++i) {
-        int32_t *dst = dist_out + (int64_t)(b0 + i) * n;
-        const int32_t *src = host.data() + (size_t)i * n;
-        for (int32_t v = 0; v < n; ++v) dst[v] = src[c->iperm[(size_t)v]];
-      }
     }
   } catch (...) {
-    (void)hipFree(d_out);
-    (void)hipFree(d_src);
+    release();
     throw;
   }
-  HIP_OK(hipFree(d_out));
-  HIP_OK(hipFree(d_src));
+  release();
   return BCX_OK;
   BCX_GUARD_END
 }
