@@ -1102,9 +1102,10 @@ int bcx_h2h_distances(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t
       HIP_OK(hipMemcpyAsync(d_src, pre.data(), (size_t)nb * 4, hipMemcpyHostToDevice, c->stream));
       if (c->profiling) HIP_OK(hipEventRecord(c->ev[0], c->stream));
       // Each source's targets are split over several workgroups (interleaved
-      // by 256), so that a launch has about 16 waves per compute unit.
+      // by 256), about 64 workgroups per compute unit in a launch (16: 7%
+      // slower on road320k).
       const int slices = (int)std::max<int64_t>(
-          1, std::min<int64_t>((int64_t)(n + 255) / 256, (16ll * c->cus + nb - 1) / nb));
+          1, std::min<int64_t>((int64_t)(n + 255) / 256, (64ll * c->cus + nb - 1) / nb));
       h2h_dist_kernel<<<dim3((unsigned)nb, (unsigned)slices), 256, lds, c->stream>>>(
           c->ix, c->tree.lvl_off, c->tree.lvl_v, c->d_gdis, c->d_goff, c->d_sep_off, c->d_sep, d_src, d_out);
       HIP_OK(hipGetLastError());
