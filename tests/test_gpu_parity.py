@@ -282,6 +282,22 @@ def test_h2h_distances_match_dijkstra(maker, order):
         assert np.array_equal(got[i].astype(np.int64), oracle.sssp(csr, s)[0]), f"source {s}"
 
 
+def test_h2h_distances_in_several_chunks():
+    # More than 65,535 sources: several launches, each with its own row reorder.
+    g = gg.make("grid1k")
+    G = bcx.Graph.from_arcs(g.n, g.u, g.v, g.w)
+    ix = bcx.Index.build_mhl(G)
+    ix.build_h2h()
+    ctx = bcx.Context(G, ix, 0)
+    csr = oracle.CSR.from_arcgraph(g)
+    uniq = gg.permutation(8, 1, g.n)[:40].astype(np.int32)
+    ref = np.stack([oracle.sssp(csr, int(s))[0] for s in uniq]).astype(np.int32)
+    reps = 65_535 // len(uniq) + 2
+    got = ctx.h2h_distances(np.tile(uniq, reps))
+    assert got.shape == (reps * len(uniq), g.n)
+    assert np.array_equal(got.reshape(reps, len(uniq), g.n), np.broadcast_to(ref, (reps, len(uniq), g.n)))
+
+
 def test_h2h_distances_need_labels(grid):
     g, csr, ctx = grid
     with pytest.raises(bcx.BcxError):
