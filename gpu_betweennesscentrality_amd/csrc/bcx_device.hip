@@ -471,7 +471,7 @@ void run_sources(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, do
     if (sources[k] < 0 || sources[k] >= n) throw bcx::Error{BCX_EINVAL, "source out of range"};
     pre[(size_t)k] = c->iperm[(size_t)sources[k]];
   }
-  std::sort(pre.begin(), pre.end());
+  sort_tree_ids(pre, n);
   std::vector<double> bc_pre((size_t)n);
   for (int32_t v = 0; v < n; ++v) bc_pre[(size_t)c->iperm[(size_t)v]] = bc_inout[v];
   const int batch = mode == BCX_MODE_PLAIN ? 1 : std::max(1, std::min(c->tn.batch, count));

@@ -1145,7 +1145,7 @@ void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_ino
     if (sources[k] < 0 || sources[k] >= n) throw bcx::Error{BCX_EINVAL, "source out of range"};
     pre[(size_t)k] = c->iperm[(size_t)sources[k]];
   }
-  std::sort(pre.begin(), pre.end());
+  sort_tree_ids(pre, n);
   const int tiles = sweep_tiles_for(c, count);
   std::vector<double> bc_pre((size_t)n);
   for (int32_t v = 0; v < n; ++v) bc_pre[(size_t)c->iperm[(size_t)v]] = bc_inout[v];
