@@ -858,13 +858,13 @@ __global__ __launch_bounds__(256) void sweep_sum_batches_kernel(int n, int64_t B
 // its entries' flows, each sum over pairs in increasing pair order.
 //
 // Wide levels (at least kUnpackWide nodes) take two grid-wide launches, one
-// thread per pair and one thread per node.  Runs of narrower levels go to one
-// resident workgroup that walks them with a barrier after each step, so the
-// long chain of small top levels costs no launches; there a node gets a
+// thread per pair and one wavefront per node.  Runs of narrower levels go to
+// one resident workgroup that walks them with a barrier after each step, so
+// the long chain of small top levels costs no launches; there a node gets a
 // thread on levels of at least kWave nodes and a whole wavefront otherwise
-// (the large bags near the root carry most pairs; the lanes take the pairs in
-// strides and a fixed lane tree sums them).  Every sum has a fixed order, so
-// results are deterministic.
+// (the large bags near the root carry most pairs).  A wavefront's lanes take
+// the node's pairs and entries in strides and a fixed lane tree sums them.
+// Every sum has a fixed order, so results are deterministic.
 constexpr int kUnpackThreads = 1024;
 constexpr int kUnpackWide = 1024;
 
@@ -887,9 +887,30 @@ __global__ __launch_bounds__(256) void sweep_pair_flow_kernel(DevTree tr, SweepW
   if (q < tr.lvl_pair_off[k + 1]) sw.pf[q] = sw.phi[tr.tp_ab[q]] * tr.tp_frac[q];
 }
 
+// One node by a whole wavefront (all 64 lanes must come here together).
+__device__ __forceinline__ void unpack_node_wave(const DevTree &tr, const SweepWork &sw, int z,
+                                                 int lane) {
+  const int p0 = tr.zp_off[z], p1 = tr.zp_off[z + 1];
+  double acc = 0.0;
+  for (int p = p0 + lane; p < p1; p += kWave) acc += sw.pf[tr.zp_pair[p]];
+  acc = wave_sum_dpp(acc);
+  if (lane == 0) sw.interior[z] = acc;
+  for (int e = tr.bag_off[z] + lane; e < tr.bag_off[z + 1]; e += kWave) {
+    const int a0 = tr.ep_off[e], a1 = tr.ep_off[e + 1];
+    if (a1 > a0) {
+      double f = 0.0;
+      for (int a = a0; a < a1; ++a) f += sw.pf[tr.ep_pair[a]];
+      sw.phi[e] += f;
+    }
+  }
+}
+
+// Four nodes of wide level k per workgroup, one per wavefront.
 __global__ __launch_bounds__(256) void sweep_pair_apply_kernel(DevTree tr, SweepWork sw, int k) {
-  const int i = tr.lvl_off[k] + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < tr.lvl_off[k + 1]) unpack_node_serial(tr, sw, tr.lvl_v[i]);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int i = tr.lvl_off[k] + blockIdx.x * 4 + wave;
+  if (i < tr.lvl_off[k + 1]) unpack_node_wave(tr, sw, __builtin_amdgcn_readfirstlane(tr.lvl_v[i]), lane);
 }
 
 __global__ __launch_bounds__(kUnpackThreads) void sweep_unpack_kernel(DevTree tr, SweepWork sw,
@@ -906,22 +927,8 @@ __global__ __launch_bounds__(kUnpackThreads) void sweep_unpack_kernel(DevTree tr
     if (hi - lo >= kWave) {
       for (int i = lo + tid; i < hi; i += kUnpackThreads) unpack_node_serial(tr, sw, tr.lvl_v[i]);
     } else {
-      for (int i = lo + wave; i < hi; i += kWaves) {
-        const int z = __builtin_amdgcn_readfirstlane(tr.lvl_v[i]);
-        const int p0 = tr.zp_off[z], p1 = tr.zp_off[z + 1];
-        double acc = 0.0;
-        for (int p = p0 + lane; p < p1; p += kWave) acc += sw.pf[tr.zp_pair[p]];
-        acc = wave_sum_dpp(acc);
-        if (lane == 0) sw.interior[z] = acc;
-        for (int e = tr.bag_off[z] + lane; e < tr.bag_off[z + 1]; e += kWave) {
-          const int a0 = tr.ep_off[e], a1 = tr.ep_off[e + 1];
-          if (a1 > a0) {
-            double f = 0.0;
-            for (int a = a0; a < a1; ++a) f += sw.pf[tr.ep_pair[a]];
-            sw.phi[e] += f;
-          }
-        }
-      }
+      for (int i = lo + wave; i < hi; i += kWaves)
+        unpack_node_wave(tr, sw, __builtin_amdgcn_readfirstlane(tr.lvl_v[i]), lane);
     }
     __syncthreads();
   }
@@ -1118,7 +1125,7 @@ void sweep_finish(bcx_ctx *c, double *bc_dev) {
       const int qcnt = lpo[(size_t)k + 1] - lpo[(size_t)k];
       if (qcnt > 0)
         sweep_pair_flow_kernel<<<blocks_for(qcnt, 256), 256, 0, c->stream>>>(c->tree, sw, k);
-      sweep_pair_apply_kernel<<<blocks_for(cnt, 256), 256, 0, c->stream>>>(c->tree, sw, k);
+      sweep_pair_apply_kernel<<<blocks_for(cnt, 4), 256, 0, c->stream>>>(c->tree, sw, k);
       ++k;
       continue;
     }
