@@ -868,17 +868,43 @@ __global__ __launch_bounds__(256) void sweep_sum_batches_kernel(int n, int64_t B
 constexpr int kUnpackThreads = 1024;
 constexpr int kUnpackWide = 1024;
 
-__device__ __forceinline__ void unpack_node_serial(const DevTree &tr, const SweepWork &sw, int z) {
+// Sum of pf[idx[p]] for p = p0, p0 + step, ... below p1, added in that
+// order; four index loads, then four value loads, are in flight at a time
+// (a loop of single loads waits twice per term).
+__device__ __forceinline__ double gather_sum(const double *pf, const int *idx, int p, int p1,
+                                             int step) {
   double acc = 0.0;
-  for (int p = tr.zp_off[z]; p < tr.zp_off[z + 1]; ++p) acc += sw.pf[tr.zp_pair[p]];
-  sw.interior[z] = acc;
+  for (; p + 3 * step < p1; p += 4 * step) {
+    const int i0 = idx[p], i1 = idx[p + step], i2 = idx[p + 2 * step], i3 = idx[p + 3 * step];
+    const double x0 = pf[i0], x1 = pf[i1], x2 = pf[i2], x3 = pf[i3];
+    acc += x0;
+    acc += x1;
+    acc += x2;
+    acc += x3;
+  }
+  for (; p < p1; p += step) acc += pf[idx[p]];
+  return acc;
+}
+
+// Pair shares pf[q] for q, q + step, ... below q1 (two at a time): the
+// shortcut flow times the pair's fraction of its restricted path count.
+__device__ __forceinline__ void pair_flows(const DevTree &tr, const SweepWork &sw, int q, int q1,
+                                           int step) {
+  for (; q + step < q1; q += 2 * step) {
+    const int a0 = tr.tp_ab[q], a1 = tr.tp_ab[q + step];
+    const double f0 = tr.tp_frac[q], f1 = tr.tp_frac[q + step];
+    const double p0 = sw.phi[a0], p1 = sw.phi[a1];
+    sw.pf[q] = p0 * f0;
+    sw.pf[q + step] = p1 * f1;
+  }
+  if (q < q1) sw.pf[q] = sw.phi[tr.tp_ab[q]] * tr.tp_frac[q];
+}
+
+__device__ __forceinline__ void unpack_node_serial(const DevTree &tr, const SweepWork &sw, int z) {
+  sw.interior[z] = gather_sum(sw.pf, tr.zp_pair, tr.zp_off[z], tr.zp_off[z + 1], 1);
   for (int e = tr.bag_off[z]; e < tr.bag_off[z + 1]; ++e) {
     const int a0 = tr.ep_off[e], a1 = tr.ep_off[e + 1];
-    if (a1 > a0) {
-      double f = 0.0;
-      for (int a = a0; a < a1; ++a) f += sw.pf[tr.ep_pair[a]];
-      sw.phi[e] += f;
-    }
+    if (a1 > a0) sw.phi[e] += gather_sum(sw.pf, tr.ep_pair, a0, a1, 1);
   }
 }
 
@@ -890,18 +916,12 @@ __global__ __launch_bounds__(256) void sweep_pair_flow_kernel(DevTree tr, SweepW
 // One node by a whole wavefront (all 64 lanes must come here together).
 __device__ __forceinline__ void unpack_node_wave(const DevTree &tr, const SweepWork &sw, int z,
                                                  int lane) {
-  const int p0 = tr.zp_off[z], p1 = tr.zp_off[z + 1];
-  double acc = 0.0;
-  for (int p = p0 + lane; p < p1; p += kWave) acc += sw.pf[tr.zp_pair[p]];
-  acc = wave_sum_dpp(acc);
+  const double acc =
+      wave_sum_dpp(gather_sum(sw.pf, tr.zp_pair, tr.zp_off[z] + lane, tr.zp_off[z + 1], kWave));
   if (lane == 0) sw.interior[z] = acc;
   for (int e = tr.bag_off[z] + lane; e < tr.bag_off[z + 1]; e += kWave) {
     const int a0 = tr.ep_off[e], a1 = tr.ep_off[e + 1];
-    if (a1 > a0) {
-      double f = 0.0;
-      for (int a = a0; a < a1; ++a) f += sw.pf[tr.ep_pair[a]];
-      sw.phi[e] += f;
-    }
+    if (a1 > a0) sw.phi[e] += gather_sum(sw.pf, tr.ep_pair, a0, a1, 1);
   }
 }
 
@@ -919,9 +939,7 @@ __global__ __launch_bounds__(kUnpackThreads) void sweep_unpack_kernel(DevTree tr
   const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
   constexpr int kWaves = kUnpackThreads / kWave;
   for (int k = k0; k < k1; ++k) {
-    const int q0 = tr.lvl_pair_off[k], q1 = tr.lvl_pair_off[k + 1];
-    for (int q = q0 + tid; q < q1; q += kUnpackThreads)
-      sw.pf[q] = sw.phi[tr.tp_ab[q]] * tr.tp_frac[q];
+    pair_flows(tr, sw, tr.lvl_pair_off[k] + tid, tr.lvl_pair_off[k + 1], kUnpackThreads);
     __syncthreads();
     const int lo = tr.lvl_off[k], hi = tr.lvl_off[k + 1];
     if (hi - lo >= kWave) {
