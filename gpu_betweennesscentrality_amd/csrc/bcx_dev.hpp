@@ -76,6 +76,7 @@ struct DevTree {
   const int *ibag_x;       // B: the descendant owning the entry; sign bit set
                            // when that descendant has no descendants
   const int *ibag_e;       // B: the entry index
+  const int *ibag_pos;     // B: entry index -> its position in the inverse bags
   const int *lvl_off;      // height + 1
   const int *lvl_v;        // n: nodes grouped by depth
   const int *blvl_off;     // height + 1: the same for nodes with descendants
@@ -146,7 +147,8 @@ struct SweepWork {
   char *state;     // per (tile, node) record: distance[64], then 64 8-byte
                    // slots (path count, then dependency quotient)
   double *node_t;  // [tile][n]: node flow summed over the tile's lanes
-  double *phi_t;   // [tile][B]: shortcut flow summed over the tile's lanes
+  double *phi_t;   // [tile][B]: shortcut flow summed over the tile's lanes,
+                   // in inverse-bag order (ibag_e gives the entry)
   double *node;    // [n]: node flow summed over tiles
   double *phi;     // [B]: shortcut flow summed over tiles
   double *interior;// [n]: flow through shortcut interiors (after unpacking)

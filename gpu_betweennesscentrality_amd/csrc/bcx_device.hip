@@ -668,6 +668,11 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     }
     t.ibag_off = (const int *)keep(dev_upload(ioff));
     t.ibag_e = (const int *)keep(dev_upload(ix_e));
+    {
+      std::vector<int32_t> pos((size_t)B);
+      for (int64_t k = 0; k < B; ++k) pos[(size_t)ix_e[(size_t)k]] = (int32_t)k;
+      t.ibag_pos = (const int *)keep(dev_upload(pos));
+    }
     t.lvl_off = (const int *)keep(dev_upload(lvl_off));
     t.lvl_v = (const int *)keep(dev_upload(lvl_v));
     // Tight pairs of each bag, found by binary search of a in bag(b) (bags are
@@ -899,7 +904,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
           for (int32_t r = 0; k0 + r < k1; ++r) {
             const size_t k = (size_t)(k0 + r);
             bx[o + (size_t)r] = ix_x[k];
-            be[o + (size_t)r] = ix_e[k];
+            be[o + (size_t)r] = (int32_t)k;  // inverse-bag position: a piece's are consecutive
             blen[o + (size_t)r] = ix.bag_len[(size_t)ix_e[k]];
             bcnt[o + (size_t)r] = (double)ix.bag_cnt[(size_t)ix_e[k]];
             if (leaf(ix_x[k])) leafm |= 1 << r;

@@ -776,7 +776,8 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
           const double fs = wave_sum4(L.valid ? sax * g[0] : 0.0, L.valid ? sax * g[1] : 0.0,
                                       L.valid ? sax * g[2] : 0.0, L.valid ? sax * g[3] : 0.0);
           const int eq = e + (L.lane >> 4) * kSweepWaves;
-          if ((L.lane & 15) == 0 && eq < e1) sw.phi_t[(int64_t)L.t * tr.bags + eq] += fs;
+          if ((L.lane & 15) == 0 && eq < e1)
+            sw.phi_t[(int64_t)L.t * tr.bags + cld(tr.ibag_pos, eq)] += fs;
         }
       }
       sh_acc[wave][L.lane] = up;
@@ -816,17 +817,21 @@ TileKernel tile_kernel(int tiles, int cus) {
 // node_out[v] (+)= sum over tiles of node_t, phi_out[e] (+)= sum over tiles
 // of phi_t, tiles in increasing order.
 __global__ __launch_bounds__(256) void sweep_tile_reduce_kernel(int n, int64_t B, int tiles,
-                                                                SweepWork sw, double *node_out,
-                                                                double *phi_out, int accumulate) {
+                                                                SweepWork sw, const int *tr_ibag_e,
+                                                                double *node_out, double *phi_out,
+                                                                int accumulate) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     double acc = accumulate ? node_out[i] : 0.0;
     for (int t = 0; t < tiles; ++t) acc += sw.node_t[(int64_t)t * n + i];
     node_out[i] = acc;
   } else if (i < n + B) {
-    const int64_t e = i - n;
+    // phi_t is in inverse-bag order (a backward piece's flows are one
+    // contiguous run); phi_out in entry order.
+    const int64_t k = i - n;
+    const int e = tr_ibag_e[k];
     double acc = accumulate ? phi_out[e] : 0.0;
-    for (int t = 0; t < tiles; ++t) acc += sw.phi_t[(int64_t)t * B + e];
+    for (int t = 0; t < tiles; ++t) acc += sw.phi_t[(int64_t)t * B + k];
     phi_out[e] = acc;
   }
 }
@@ -1091,7 +1096,7 @@ void sweep_launch(bcx_ctx *c, const int32_t *pre, int32_t nsrc, double *node_out
   if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
   if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
   sweep_tile_reduce_kernel<<<blocks_for(n + B, 256), 256, 0, c->stream>>>(
-      n, B, bt, sw, node_out, phi_out, accumulate ? 1 : 0);
+      n, B, bt, sw, c->tree.ibag_e, node_out, phi_out, accumulate ? 1 : 0);
   HIP_OK(hipGetLastError());
   if (timed) {
     HIP_OK(hipEventRecord(c->ev[3], c->stream));
