@@ -66,6 +66,9 @@ struct DevTree {
   int n;
   int height;
   int64_t bags;            // total bag entries B
+  int64_t bslots;          // backward staging slots (with padding)
+  const int *bnode_slot;   // n: slot of a node's last backward piece, -1 without descendants
+  const int *bslot_node;   // bslots: the node whose last piece a slot holds, or -1
   const int *subtree;      // n
   const int *bag_off;      // n + 1, entries ordered by ancestor depth
   const int *bag_v;        // B: ancestor
@@ -146,7 +149,8 @@ constexpr int kSweepMaxSegments = 48;
 struct SweepWork {
   char *state;     // per (tile, node) record: distance[64], then 64 8-byte
                    // slots (path count, then dependency quotient)
-  double *node_t;  // [tile][n]: node flow summed over the tile's lanes
+  double *node_t;  // [tile][bslots]: node flow summed over the tile's lanes, at
+                   // the slot of the node's last backward piece (tree.bnode_slot)
   double *phi_t;   // [tile][B]: shortcut flow summed over the tile's lanes,
                    // in inverse-bag order (ibag_e gives the entry)
   double *node;    // [n]: node flow summed over tiles

@@ -931,6 +931,19 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       t.bst_off = (const int *)keep(dev_upload(bst.off));
       t.bst_cnt = (const int *)keep(dev_upload(bst.cnt));
       t.bst_seg = (const int *)keep(dev_upload(bst.seg));
+      {
+        // Node flows go to the slot of the node's last backward piece: the
+        // steps of a wavefront store them one after another, not scattered.
+        std::vector<int32_t> nslot((size_t)n, -1), snode(std::max<size_t>(Sb, 1), -1);
+        for (size_t q = 0; q < blvl_v.size(); ++q) {
+          const int32_t s = bst.slot[(size_t)(bst.pp[q + 1] - 1)];
+          nslot[(size_t)blvl_v[q]] = s;
+          snode[(size_t)s] = blvl_v[q];
+        }
+        t.bslots = (int64_t)std::max<size_t>(Sb, 1);
+        t.bnode_slot = (const int *)keep(dev_upload(nslot));
+        t.bslot_node = (const int *)keep(dev_upload(snode));
+      }
       t.blvl_nseg = (const int *)keep(dev_upload(bst.nseg));
       t.fs_node = (const int4 *)keep(dev_upload(fnode));
       t.fs_u = (const int4 *)keep(dev_upload(fu));

@@ -612,11 +612,13 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
       if (fin_mask)
         ns = wave_sum4(ns_in[0], kNodesB > 1 ? ns_in[1 % kNodesB] : 0.0,
                        kNodesB > 2 ? ns_in[2 % kNodesB] : 0.0, kNodesB > 3 ? ns_in[3 % kNodesB] : 0.0);
-      const int u = L.lane >> 4;
-      const int v = row_pick(L.lane, rl(nd.x, min(j0, jn - 1)), rl(nd.x, min(j0 + 1, jn - 1)),
-                             rl(nd.x, min(j0 + 2, jn - 1)), rl(nd.x, min(j0 + 3, jn - 1)));
+      // Node flows at the slot of the node's last piece (consecutive steps,
+      // consecutive slots).
+      const int u = L.lane >> 4, p0 = first + base;
+      const int ps = row_pick(L.lane, p0 + min(j0, jn - 1), p0 + min(j0 + 1, jn - 1),
+                              p0 + min(j0 + 2, jn - 1), p0 + min(j0 + 3, jn - 1));
       store_all(L, ((L.lane & 15) == 0) & (u < kNodesB) & ((fin_mask >> u) & 1),
-                sw.node_t + (int64_t)L.t * n + v, ns);
+                sw.node_t + (int64_t)L.t * tr.bslots + ps, ns);
       wait_vm_left<kNodesB * (1 + kWidth) + 2 * kNodesB + 1>();
     }
   }
@@ -705,7 +707,7 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
           tot += __builtin_bit_cast(double, seg_x[t][L.lane]);
         setQ(L, roff(v), (1.0 + tot) / svv);
         const double ns = wave_sum_dpp((L.valid && L.s != v) ? tot : 0.0);
-        if (L.lane == 0) sw.node_t[(int64_t)L.t * n + v] = ns;
+        if (L.lane == 0) sw.node_t[(int64_t)L.t * tr.bslots + cld(tr.bnode_slot, v)] = ns;
       }
       __syncthreads();
     }
@@ -791,7 +793,8 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
         const double qx = q_from_slot(xleaf, wx, raw);
         setQ(L, ax, sa != 0 ? qx + u : 0.0);
         const double ns = wave_sum_dpp((L.valid && x != L.s) ? sax * u : 0.0);
-        if (L.lane == 0) sw.node_t[(int64_t)L.t * n + x] += ns;
+        const int xs = cld(tr.bnode_slot, x);  // -1 without descendants, where ns is 0
+        if (L.lane == 0 && xs >= 0) sw.node_t[(int64_t)L.t * tr.bslots + xs] += ns;
       }
       __syncthreads();
     }
@@ -815,21 +818,29 @@ TileKernel tile_kernel(int tiles, int cus) {
 }
 
 // node_out[v] (+)= sum over tiles of node_t, phi_out[e] (+)= sum over tiles
-// of phi_t, tiles in increasing order.
+// of phi_t, tiles in increasing order.  Both are read in slot order (node
+// flows at the slot of the node's last backward piece, shortcut flows in
+// inverse-bag order) and each sum written to its node or entry.  Nodes
+// without descendants have no node flow: 0, or left as they are when
+// accumulating.
 __global__ __launch_bounds__(256) void sweep_tile_reduce_kernel(int n, int64_t B, int tiles,
-                                                                SweepWork sw, const int *tr_ibag_e,
+                                                                SweepWork sw, DevTree tr,
                                                                 double *node_out, double *phi_out,
                                                                 int accumulate) {
+  const int64_t S = tr.bslots;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    double acc = accumulate ? node_out[i] : 0.0;
-    for (int t = 0; t < tiles; ++t) acc += sw.node_t[(int64_t)t * n + i];
-    node_out[i] = acc;
-  } else if (i < n + B) {
-    // phi_t is in inverse-bag order (a backward piece's flows are one
-    // contiguous run); phi_out in entry order.
-    const int64_t k = i - n;
-    const int e = tr_ibag_e[k];
+  if (i < S) {
+    const int v = tr.bslot_node[i];
+    if (v < 0) return;
+    double acc = accumulate ? node_out[v] : 0.0;
+    for (int t = 0; t < tiles; ++t) acc += sw.node_t[(int64_t)t * S + i];
+    node_out[v] = acc;
+  } else if (i < S + n) {
+    const int v = (int)(i - S);
+    if (!accumulate && tr.bnode_slot[v] < 0) node_out[v] = 0.0;
+  } else if (i < S + n + B) {
+    const int64_t k = i - S - n;
+    const int e = tr.ibag_e[k];
     double acc = accumulate ? phi_out[e] : 0.0;
     for (int t = 0; t < tiles; ++t) acc += sw.phi_t[(int64_t)t * B + k];
     phi_out[e] = acc;
@@ -988,7 +999,7 @@ void bcx_ctx::ensure_sweep(int tiles) {
   free_sweep();
   const size_t n = (size_t)ix.n, B = (size_t)tree.bags, T = (size_t)tiles, W = kWave;
   HIP_OK(hipMalloc(&sw.state, T * (n + 1) * kRec));
-  HIP_OK(hipMalloc(&sw.node_t, T * n * 8));
+  HIP_OK(hipMalloc(&sw.node_t, T * (size_t)tree.bslots * 8));
   HIP_OK(hipMalloc(&sw.phi_t, T * std::max<size_t>(B, 1) * 8));
   HIP_OK(hipMalloc(&sw.node, n * 8));
   HIP_OK(hipMalloc(&sw.phi, std::max<size_t>(B, 1) * 8));
@@ -1017,7 +1028,7 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   // next use) and measure again.  A tile's sweep is a latency chain through
   // the levels, so a launch of few tiles costs nearly as much as a full one:
   // fitting the whole batch in one launch matters.
-  const size_t per_tile = ((size_t)n + 1) * kRec + (size_t)n * 8 + (size_t)B * 8;
+  const size_t per_tile = ((size_t)n + 1) * kRec + (size_t)c->tree.bslots * 8 + (size_t)B * 8;
   auto cap_now = [&]() {
     size_t fr = 0, tot = 0;
     HIP_OK(hipMemGetInfo(&fr, &tot));
@@ -1088,15 +1099,12 @@ void sweep_launch(bcx_ctx *c, const int32_t *pre, int32_t nsrc, double *node_out
   if (timed) HIP_OK(hipEventRecord(c->ev[0], c->stream));
   uint64_t *stamps = nullptr;
   if (timed) HIP_OK(hipMalloc(&stamps, (size_t)bt * 4 * 8));
-  // Node sums start at zero: nodes without descendants get only ascending flow.
-  zero_f64_kernel<<<blocks_for((int64_t)bt * n, 256), 256, 0, c->stream>>>(sw.node_t,
-                                                                          (int64_t)bt * n);
   tile_kernel(bt, c->cus)<<<bt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf,
                                                                 stamps);
   if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
   if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
-  sweep_tile_reduce_kernel<<<blocks_for(n + B, 256), 256, 0, c->stream>>>(
-      n, B, bt, sw, c->tree.ibag_e, node_out, phi_out, accumulate ? 1 : 0);
+  sweep_tile_reduce_kernel<<<blocks_for(c->tree.bslots + n + B, 256), 256, 0, c->stream>>>(
+      n, B, bt, sw, c->tree, node_out, phi_out, accumulate ? 1 : 0);
   HIP_OK(hipGetLastError());
   if (timed) {
     HIP_OK(hipEventRecord(c->ev[3], c->stream));
