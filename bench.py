@@ -37,18 +37,21 @@ XCDS, CUS, SIMDS = 8, 256, 1024  # MI355X: accelerator dies, compute units, SIMD
 PROFILES = ROOT / "profiles"
 
 
-def sweep_algorithmic_bytes(n: int, bags: int, inner: int, sources: int) -> float:
+def sweep_algorithmic_bytes(n: int, bags: int, inner: int, height: int, sources: int) -> float:
     """Bytes one sweep launch must move for `sources` sources, a lower bound.
 
     Per source: both passes read one 4-byte distance word per bag entry
     (8 * bags); the forward pass writes one distance word per node (4 * n);
     the backward pass, which runs only on the `inner` nodes (nodes with
     descendants), reads each one's distance word back and writes its 8-byte
-    quotient (12 * inner).  Count words (written and read only where a path
-    count is not 1) and the quotients of tight entries are data-dependent and
-    not counted.
+    quotient (12 * inner).  Every inner node that is not an ancestor of the
+    source (at most `height` are) reaches its distance through at least one
+    tight bag entry, whose ancestor's backward step reads the node's quotient
+    (8 * (inner - height)).  Count words (written and read only where a path
+    count is not 1) and the quotients of further tight entries are
+    data-dependent and not counted.
     """
-    return float(sources) * (8.0 * bags + 4.0 * n + 12.0 * inner)
+    return float(sources) * (8.0 * bags + 4.0 * n + 12.0 * inner + 8.0 * max(0, inner - height))
 
 
 def _kernel_rows(csv_path: str, counter: str):
@@ -298,7 +301,7 @@ def main() -> int:
     parent, _ = ix.tree()
     inner = int(np.unique(parent[parent >= 0]).size)  # nodes with descendants
     sweep_ms = tm["scan_ms"]
-    algo = sweep_algorithmic_bytes(n, bags, inner, Bs)
+    algo = sweep_algorithmic_bytes(n, bags, inner, int(st["tree_height"]), Bs)
     achieved = algo / (sweep_ms * 1e-3) / 1e9
     traffic = committed_traffic(Bs, args.workload)
 
