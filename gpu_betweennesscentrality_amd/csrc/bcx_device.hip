@@ -396,10 +396,6 @@ __global__ void advance_kernel(DevBatch bt, int step) {
   if (threadIdx.x == 0 && blockIdx.x == 0) bt.cursor[0] += step;
 }
 
-__global__ void fill_f64_kernel(double *p, int64_t n, double v) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
 
 
 }  // namespace

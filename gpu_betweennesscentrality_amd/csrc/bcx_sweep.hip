@@ -323,9 +323,8 @@ struct Segs {
 // (kStage, or kNarrowEntries for the narrow tail of a level split by node).
 template <int kNodesF, bool runs, int kWidth>  // runs: split in runs, results go to LDS segments
 __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree &tr,
-                                               const SweepWork &sw, const Lane &L, int depth,
-                                               int first, int mine, int seg, int &ovf,
-                                               const Segs &sg) {
+                                               const Lane &L, int depth, int first, int mine,
+                                               int seg, int &ovf, const Segs &sg) {
   static_assert(kWidth == kStage || (!runs && kWidth == kNarrowEntries), "piece width");
   int cb = 0x7fffffff;  // the node's minimum and count over its earlier pieces
   uint64_t cc = 0;
@@ -476,7 +475,6 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
                                                const Segs &sg) {
   static_assert(kNodesB <= 4 && kStage == 4, "four-way lane sums");
   static_assert(kWidth == kStage || (!runs && kWidth == kNarrowEntries), "piece width");
-  const int n = tr.n;
   double carry = 0.0;  // the node's flow sum over its earlier pieces
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
@@ -642,7 +640,7 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
   const Segs sg{seg_x, seg_y, seg_node};
   const Lane L = lane_setup(ix, sw, blockIdx.x, nsrc, tr.n);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int n = tr.n, H = tr.height;
+  const int H = tr.height;
   int ovf = 0;
   // Optional phase timestamps (constant 100 MHz clock), one row per tile.
   if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x] = __builtin_amdgcn_s_memrealtime();
@@ -653,12 +651,12 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
     int first, mine;
     level_block(tr.st_off, tr.st_cnt, k, wave, first, mine);
     if (nseg) {
-      fwd_level_wave<kNodesF, true, kStage>(ix, tr, sw, L, k, first, mine,
+      fwd_level_wave<kNodesF, true, kStage>(ix, tr, L, k, first, mine,
                                             cld(tr.st_seg, k * kSweepWaves + wave), ovf, sg);
     } else {
       const int nar = cld(tr.st_nar, k * kSweepWaves + wave);
-      fwd_level_wave<kNodesF, false, kStage>(ix, tr, sw, L, k, first, mine - nar, 0, ovf, sg);
-      fwd_level_wave<kNodesF, false, kNarrowEntries>(ix, tr, sw, L, k, first + mine - nar, nar, 0,
+      fwd_level_wave<kNodesF, false, kStage>(ix, tr, L, k, first, mine - nar, 0, ovf, sg);
+      fwd_level_wave<kNodesF, false, kNarrowEntries>(ix, tr, L, k, first + mine - nar, nar, 0,
                                                      ovf, sg);
     }
     __syncthreads();
