@@ -206,6 +206,35 @@ def test_cli_rq_table(tmp_path, grid):
     assert np.all(np.abs(got - want) <= 1e-9 * np.maximum(np.abs(want), 1.0) + 5e-9)
 
 
+# ---- the reference's LosAngeles data ------------------------------------------
+
+def test_la_queries_and_bc_on_the_gpu():
+    """LosAngeles (49,374 vertices): the GPU's distance/count/max-hop rows
+    against the answers of the reference's own index code (golden pairs),
+    and BC of sampled sources in both forms against the oracle."""
+    golden = ROOT / "tests" / "golden"
+    la = np.load(golden / "la_graph.npz")
+    ref = np.load(golden / "la_tc.npz")
+    n = int(la["n"])
+    G = bcx.Graph.from_arcs(n, la["u"], la["v"], la["w"])
+    ix = bcx.Index.build_mhl(G)
+    ctx = bcx.Context(G, ix, 0)
+    q = ref["queries"].astype(np.int64)
+    for s in np.unique(q[:, 0]).tolist():
+        d, sg, hp, _ = ctx.query(s - 1)
+        rows = q[q[:, 0] == s]
+        t = rows[:, 1] - 1
+        assert np.array_equal(d[t].astype(np.int64), rows[:, 2]), f"distances from {s}"
+        assert np.array_equal(sg[t].astype(np.int64), rows[:, 3]), f"counts from {s}"
+        assert np.array_equal(hp[t].astype(np.int64), rows[:, 4]), f"layers from {s}"
+    csr = oracle.CSR(n, la["u"].astype(np.int64), la["v"].astype(np.int64), la["w"].astype(np.int64))
+    src = gg.permutation(21, 1, n)[:192].astype(np.int32)
+    want, _ = oracle.brandes(csr, src)
+    for mode in (bcx.MODE_SWEEP, bcx.MODE_BATCHED):
+        got, ovf = ctx.bc_sources(src, mode)
+        assert not ovf and close_bc(got, want), f"mode {mode}"
+
+
 # ---- closed forms and edge cases, every form -----------------------------------
 
 def _small(n, edges):
