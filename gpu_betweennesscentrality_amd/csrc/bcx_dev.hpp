@@ -107,9 +107,9 @@ struct DevTree {
   // A level whose pieces are split over the wavefronts in contiguous runs
   // (lvl_nseg[k] > 0) leaves per-node partial results (segments) in LDS,
   // wavefront w's first one numbered st_seg[16 k + w], merged after the level.
-  // In a forward level split by node, the last st_nar[16 k + w] slots of
-  // wavefront w are narrow: nodes of one piece with at most kNarrowEntries
-  // entries, whose steps load only that many rows.
+  // In a level split by node, the last st_nar[16 k + w] (backward:
+  // bst_nar) slots of wavefront w are narrow: nodes of one piece with at most
+  // kNarrowEntries entries, whose steps load only that many rows.
   const int *st_off;          // height + 1 (forward: all nodes)
   const int *st_cnt;          // 16 * height
   const int *st_seg;          // 16 * height
@@ -118,6 +118,7 @@ struct DevTree {
   const int *bst_off;         // height + 1 (backward: nodes with descendants)
   const int *bst_cnt;         // 16 * height
   const int *bst_seg;         // 16 * height
+  const int *bst_nar;         // 16 * height
   const int *blvl_nseg;       // height
   const int4 *fs_node;        // {v, entries in the piece, 0, subtree[v] | count-1
                               // mask << 24 | last piece << 29 | first << 30}:

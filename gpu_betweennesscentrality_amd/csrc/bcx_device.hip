@@ -749,11 +749,10 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       //    Costed at 2 pieces for the merge; at most kSweepMaxSegments
       //    segments, so only for levels of up to kSweepMaxSegments - 16 nodes.
       // Either way each wavefront's pieces keep tree order, so the wavefronts
-      // of a workgroup touch neighbouring rows at about the same time.  In the
-      // forward list split by node, a wavefront takes its narrow nodes (one
-      // piece of at most kNarrowEntries entries: most nodes of road graphs)
-      // after the others, each group in tree order, so that their steps load
-      // fewer rows.  (The same in the backward list measured no faster.)
+      // of a workgroup touch neighbouring rows at about the same time.  In a
+      // level split by node, a wavefront takes its narrow nodes (one piece of
+      // at most kNarrowEntries entries: most nodes of road graphs) after the
+      // others, each group in tree order, so that their steps load fewer rows.
       // Wavefront w's i-th slot of level k is off[k] + w * chunk + i (chunk =
       // the largest wavefront count of the level), its count of slots is
       // cnt[16 k + w] and, when split in runs, its first segment is
@@ -777,7 +776,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
         std::vector<int64_t> pp;                              // first piece of each list node
       };
       auto slots = [&](const std::vector<int32_t> &off, const std::vector<int32_t> &list,
-                       auto entries, bool narrow_tail, Staging &st) {
+                       auto entries, Staging &st) {
         const size_t N = (size_t)off[(size_t)H];
         st.pp.assign(N + 1, 0);
         for (size_t i = 0; i < N; ++i) st.pp[i + 1] = st.pp[i] + pieces(entries(list[i]));
@@ -821,7 +820,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
             for (int pass = 0; pass < 2; ++pass)  // other nodes, then narrow ones
               for (int32_t i = 0; i < cnt; ++i) {  // tree order within each wavefront
                 const int64_t e = entries(list[(size_t)(lo + i)]);
-                const bool nar = narrow_tail && e <= kNarrowEntries;
+                const bool nar = e <= kNarrowEntries;
                 if (nar != (pass == 1)) continue;
                 const int w = wave_of[(size_t)i];
                 if (nar) ++st.nar[(size_t)k * kW + (size_t)w];
@@ -855,10 +854,10 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       Staging fst, bst;
       const size_t S = slots(lvl_off, lvl_v,
                              [&](int32_t v) { return (int64_t)(boff[(size_t)v + 1] - boff[(size_t)v]); },
-                             true, fst);
+                             fst);
       const size_t Sb = slots(blvl_off, blvl_v,
                               [&](int32_t v) { return (int64_t)(ioff[(size_t)v + 1] - ioff[(size_t)v]); },
-                              false, bst);
+                              bst);
       std::vector<int32_t> fnode(S * 4, 0), fu(S * 4, 0), flen(S * 4, kSweepNoLen);
       std::vector<uint64_t> fcnt(S * 4, 0);
       std::vector<int32_t> bnode(std::max<size_t>(Sb, 1) * 4, 0), bx(std::max<size_t>(Sb, 1) * 4, 0),
@@ -927,6 +926,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       t.bst_off = (const int *)keep(dev_upload(bst.off));
       t.bst_cnt = (const int *)keep(dev_upload(bst.cnt));
       t.bst_seg = (const int *)keep(dev_upload(bst.seg));
+      t.bst_nar = (const int *)keep(dev_upload(bst.nar));
       {
         // Node flows go to the slot of the node's last backward piece: the
         // steps of a wavefront store them one after another, not scattered.

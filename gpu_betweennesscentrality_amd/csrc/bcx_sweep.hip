@@ -691,7 +691,9 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
       bwd_level_wave<kNodesB, true, kStage>(tr, sw, L, first, mine,
                                             cld(tr.bst_seg, k * kSweepWaves + wave), sg);
     } else {
-      bwd_level_wave<kNodesB, false, kStage>(tr, sw, L, first, mine, 0, sg);
+      const int nar = cld(tr.bst_nar, k * kSweepWaves + wave);
+      bwd_level_wave<kNodesB, false, kStage>(tr, sw, L, first, mine - nar, 0, sg);
+      bwd_level_wave<kNodesB, false, kNarrowEntries>(tr, sw, L, first + mine - nar, nar, 0, sg);
     }
     __syncthreads();
     if (nseg) {
