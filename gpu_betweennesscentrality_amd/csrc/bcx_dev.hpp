@@ -109,16 +109,19 @@ struct DevTree {
   // wavefront w's first one numbered st_seg[16 k + w], merged after the level.
   // In a level split by node, the last st_nar[16 k + w] (backward:
   // bst_nar) slots of wavefront w are narrow: nodes of one piece with at most
-  // kNarrowEntries entries, whose steps load only that many rows.
+  // kNarrowEntries entries, whose steps load only that many rows; the
+  // st_n3 (bst_n3) slots before them hold nodes of exactly three entries.
   const int *st_off;          // height + 1 (forward: all nodes)
   const int *st_cnt;          // 16 * height
   const int *st_seg;          // 16 * height
   const int *st_nar;          // 16 * height
+  const int *st_n3;           // 16 * height
   const int *lvl_nseg;        // height: segments of a level split in runs, or 0
   const int *bst_off;         // height + 1 (backward: nodes with descendants)
   const int *bst_cnt;         // 16 * height
   const int *bst_seg;         // 16 * height
   const int *bst_nar;         // 16 * height
+  const int *bst_n3;          // 16 * height
   const int *blvl_nseg;       // height
   const int4 *fs_node;        // {v, entries in the piece, 0, subtree[v] | count-1
                               // mask << 24 | last piece << 29 | first << 30}:

@@ -750,9 +750,10 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       //    segments, so only for levels of up to kSweepMaxSegments - 16 nodes.
       // Either way each wavefront's pieces keep tree order, so the wavefronts
       // of a workgroup touch neighbouring rows at about the same time.  In a
-      // level split by node, a wavefront takes its narrow nodes (one piece of
-      // at most kNarrowEntries entries: most nodes of road graphs) after the
-      // others, each group in tree order, so that their steps load fewer rows.
+      // level split by node, a wavefront takes its nodes of three entries and
+      // then its narrow nodes (one piece of at most kNarrowEntries entries:
+      // most nodes of road graphs) after the others, each group in tree order,
+      // so that their steps load fewer rows.
       // Wavefront w's i-th slot of level k is off[k] + w * chunk + i (chunk =
       // the largest wavefront count of the level), its count of slots is
       // cnt[16 k + w] and, when split in runs, its first segment is
@@ -772,7 +773,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
         return (int32_t)std::max<int64_t>(1, (e + kStageEntries - 1) / kStageEntries);
       };
       struct Staging {
-        std::vector<int32_t> off, cnt, seg, nar, nseg, slot;  // slot: per piece, list order
+        std::vector<int32_t> off, cnt, seg, nar, n3, nseg, slot;  // slot: per piece, list order
         std::vector<int64_t> pp;                              // first piece of each list node
       };
       auto slots = [&](const std::vector<int32_t> &off, const std::vector<int32_t> &list,
@@ -785,6 +786,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
         st.cnt.assign((size_t)H * kW, 0);
         st.seg.assign((size_t)H * kW, 0);
         st.nar.assign((size_t)H * kW, 0);
+        st.n3.assign((size_t)H * kW, 0);
         st.nseg.assign((size_t)H, 0);
         std::vector<int32_t> order, wave_of, cost;
         for (int k = 0; k < H; ++k) {
@@ -817,13 +819,14 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
               chunk = std::max(chunk, st.cnt[(size_t)k * kW + (size_t)w]);
             }
             int32_t pos[16] = {0};
-            for (int pass = 0; pass < 2; ++pass)  // other nodes, then narrow ones
+            for (int pass = 0; pass < 3; ++pass)  // other nodes, three-entry ones, narrow ones
               for (int32_t i = 0; i < cnt; ++i) {  // tree order within each wavefront
                 const int64_t e = entries(list[(size_t)(lo + i)]);
-                const bool nar = e <= kNarrowEntries;
-                if (nar != (pass == 1)) continue;
+                const int cls = e <= kNarrowEntries ? 2 : e == 3 ? 1 : 0;
+                if (cls != pass) continue;
                 const int w = wave_of[(size_t)i];
-                if (nar) ++st.nar[(size_t)k * kW + (size_t)w];
+                if (cls == 2) ++st.nar[(size_t)k * kW + (size_t)w];
+                if (cls == 1) ++st.n3[(size_t)k * kW + (size_t)w];
                 for (int32_t pc = 0; pc < cost[(size_t)i]; ++pc)
                   st.slot[(size_t)(st.pp[(size_t)(lo + i)] + pc)] = st.off[(size_t)k] + w * chunk + pos[w]++;
               }
@@ -922,11 +925,13 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
       t.st_cnt = (const int *)keep(dev_upload(fst.cnt));
       t.st_seg = (const int *)keep(dev_upload(fst.seg));
       t.st_nar = (const int *)keep(dev_upload(fst.nar));
+      t.st_n3 = (const int *)keep(dev_upload(fst.n3));
       t.lvl_nseg = (const int *)keep(dev_upload(fst.nseg));
       t.bst_off = (const int *)keep(dev_upload(bst.off));
       t.bst_cnt = (const int *)keep(dev_upload(bst.cnt));
       t.bst_seg = (const int *)keep(dev_upload(bst.seg));
       t.bst_nar = (const int *)keep(dev_upload(bst.nar));
+      t.bst_n3 = (const int *)keep(dev_upload(bst.n3));
       {
         // Node flows go to the slot of the node's last backward piece: the
         // steps of a wavefront store them one after another, not scattered.

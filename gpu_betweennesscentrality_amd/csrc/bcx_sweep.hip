@@ -291,6 +291,10 @@ __device__ __forceinline__ void stage_entries(const int4 *ids, const int4 *lens,
     const int4 a = ids[p], b = lens[p];
     id[0] = a.x, id[1] = a.y, id[2] = a.z, id[3] = a.w;
     len[0] = b.x, len[1] = b.y, len[2] = b.z, len[3] = b.w;
+  } else if constexpr (kWidth == 3) {
+    const int4 a = ids[p], b = lens[p];
+    id[0] = a.x, id[1] = a.y, id[2] = a.z;
+    len[0] = b.x, len[1] = b.y, len[2] = b.z;
   } else {
     static_assert(kWidth == 2, "narrow pieces hold two entries");
     const int2 a = ((const int2 *)ids)[2 * p], b = ((const int2 *)lens)[2 * p];
@@ -325,7 +329,8 @@ template <int kNodesF, bool runs, int kWidth>  // runs: split in runs, results g
 __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree &tr,
                                                const Lane &L, int depth, int first, int mine,
                                                int seg, int &ovf, const Segs &sg) {
-  static_assert(kWidth == kStage || (!runs && kWidth == kNarrowEntries), "piece width");
+  static_assert(kWidth == kStage || (!runs && (kWidth == 3 || kWidth == kNarrowEntries)),
+                "piece width");
   int cb = 0x7fffffff;  // the node's minimum and count over its earlier pieces
   uint64_t cc = 0;
   for (int base = 0; base < mine; base += kWave) {
@@ -474,7 +479,8 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
                                                const Lane &L, int first, int mine, int seg,
                                                const Segs &sg) {
   static_assert(kNodesB <= 4 && kStage == 4, "four-way lane sums");
-  static_assert(kWidth == kStage || (!runs && kWidth == kNarrowEntries), "piece width");
+  static_assert(kWidth == kStage || (!runs && (kWidth == 3 || kWidth == kNarrowEntries)),
+                "piece width");
   double carry = 0.0;  // the node's flow sum over its earlier pieces
   for (int base = 0; base < mine; base += kWave) {
     const int jn = min(kWave, mine - base);
@@ -654,8 +660,9 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
       fwd_level_wave<kNodesF, true, kStage>(ix, tr, L, k, first, mine,
                                             cld(tr.st_seg, k * kSweepWaves + wave), ovf, sg);
     } else {
-      const int nar = cld(tr.st_nar, k * kSweepWaves + wave);
-      fwd_level_wave<kNodesF, false, kStage>(ix, tr, L, k, first, mine - nar, 0, ovf, sg);
+      const int nar = cld(tr.st_nar, k * kSweepWaves + wave), n3 = cld(tr.st_n3, k * kSweepWaves + wave);
+      fwd_level_wave<kNodesF, false, kStage>(ix, tr, L, k, first, mine - nar - n3, 0, ovf, sg);
+      fwd_level_wave<kNodesF, false, 3>(ix, tr, L, k, first + mine - nar - n3, n3, 0, ovf, sg);
       fwd_level_wave<kNodesF, false, kNarrowEntries>(ix, tr, L, k, first + mine - nar, nar, 0,
                                                      ovf, sg);
     }
@@ -691,8 +698,9 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
       bwd_level_wave<kNodesB, true, kStage>(tr, sw, L, first, mine,
                                             cld(tr.bst_seg, k * kSweepWaves + wave), sg);
     } else {
-      const int nar = cld(tr.bst_nar, k * kSweepWaves + wave);
-      bwd_level_wave<kNodesB, false, kStage>(tr, sw, L, first, mine - nar, 0, sg);
+      const int nar = cld(tr.bst_nar, k * kSweepWaves + wave), n3 = cld(tr.bst_n3, k * kSweepWaves + wave);
+      bwd_level_wave<kNodesB, false, kStage>(tr, sw, L, first, mine - nar - n3, 0, sg);
+      bwd_level_wave<kNodesB, false, 3>(tr, sw, L, first + mine - nar - n3, n3, 0, sg);
       bwd_level_wave<kNodesB, false, kNarrowEntries>(tr, sw, L, first + mine - nar, nar, 0, sg);
     }
     __syncthreads();
