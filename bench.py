@@ -205,7 +205,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by source ranges, see DESIGN.md)", file=sys.stderr)
+              "(a full BC would shard by source ranges, one GPU each)", file=sys.stderr)
         return 2
 
     import gpu_betweennesscentrality_amd as bcx

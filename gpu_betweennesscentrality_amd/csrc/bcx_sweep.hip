@@ -120,7 +120,7 @@ __device__ __forceinline__ void store_all(const Lane &L, bool put, double *p, do
   double *sink = (double *)&SS(L, L.sink);
   *(put ? p : sink) = x;
 }
-// kCount stores to the sink after a pipelined loop's first round of row
+// kCount stores to the sink after a pipelined loop's first set of row
 // loads: the loop body's first wait then sees as many stores after the rows
 // on entry as on every later iteration, and leaves them outstanding.
 // The compiler barrier keeps the repeated stores from being merged.
@@ -313,12 +313,12 @@ struct Segs {
 
 // Forward over this wavefront's block of a level (all at tree depth `depth`).
 // Lane j stages piece j (a node's fields and up to kStage of its bag entries)
-// with one round of loads; the wavefront then handles kNodesF pieces per
-// step.  Round 1 of a step loads the distance words of the pieces' entries;
-// round 2 the up-label (first piece, where the node is an ancestor of the
+// with one set of loads; the wavefront then handles kNodesF pieces per
+// step.  Load set 1 of a step loads the distance words of the pieces' entries;
+// load set 2 the up-label (first piece, where the node is an ancestor of the
 // lane's source) and the path-count rows of tight candidates not flagged 1.
-// The steps are pipelined: a step issues the next step's round 1 after its
-// own round 2 and before its stores, so every wait is for loads that are
+// The steps are pipelined: a step issues the next step's load set 1 after its
+// own load set 2 and before its stores, so every wait is for loads that are
 // already old (stores and loads share one completion counter).  A node's
 // minimum and count are carried from piece to piece and stored by its last
 // piece.  Saturating sums of saturating products do not depend on the order
@@ -343,12 +343,12 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
     // Shortcut counts are 1 for most entries (a flag bit); the others are
     // read as scalars when used, which keeps them out of the staged registers.
     const uint64_t *scnt = (const uint64_t *)tr.fs_cnt + 4 * (int64_t)(first + base);
-    // Round 1 of the step at j0: the staged entries' distance rows (nodes past
+    // Load set 1 of the step at j0: the staged entries' distance rows (nodes past
     // the block repeat its last node).
     // Every staged place is loaded, also past the node's bag (node 0's row,
     // a hot line): a load that may not issue would shorten every count-based
     // wait after it, and skipping them measured slower.
-    auto round1 = [&](int j0, int (&w)[kNodesF][kWidth]) {
+    auto rows1 = [&](int j0, int (&w)[kNodesF][kWidth]) {
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
@@ -357,7 +357,7 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
       }
     };
     int w[kNodesF][kWidth];
-    round1(0, w);
+    rows1(0, w);
     sink_stores<2 * kNodesF>(L);  // a step stores a distance and a count row per node
     for (int j0 = 0; j0 < jn; j0 += kNodesF) {
       // This step's rows have arrived once only the stores issued after them
@@ -381,7 +381,7 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
       // the step (a load under a lane mask may not issue, so the one wait
       // that follows covers all of them), then the arithmetic and stores.
       __builtin_amdgcn_sched_barrier(0);
-      // Round 2, first the up-label where v is an ancestor of the lane's
+      // Load set 2, first the up-label where v is an ancestor of the lane's
       // source.  A tile's sources are neighbours in tree order, so for nearly
       // every node no lane needs it and the wavefront skips both loads; other
       // lanes read the entry that matches nothing, so the values need no
@@ -416,7 +416,7 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
           m[u][r] = need ? raw : 1;
         }
       }
-      round1(j0 + kNodesF, w);
+      rows1(j0 + kNodesF, w);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
@@ -467,8 +467,8 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 // Backward over this wavefront's block of a level's nodes that have
 // descendants (a node without descendants has dD = 0 and no flows; its Q is
 // derived from its count by the readers), with the same staging by pieces
-// (of the inverse bag) and kNodesB pieces per step.  Round 1 loads the node's
-// distance word and the piece's entries' distance words; round 2 the node's
+// (of the inverse bag) and kNodesB pieces per step.  Load set 1 loads the node's
+// distance word and the piece's entries' distance words; load set 2 the node's
 // count (lanes not flagged 1) and the tight entries' quotients; steps are
 // pipelined as in the forward pass.  The node's dependency part is summed
 // piece by piece; its last piece writes Q and the node flow.  The flows of a
@@ -493,8 +493,8 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
     // indices are read as scalars when used, not staged in registers.
     const double *scnt = (const double *)tr.bs_cntf + 4 * (int64_t)(first + base);
     const int *sent = (const int *)tr.bs_e + 4 * (int64_t)(first + base);
-    // Round 1 of the step at j0 (nodes past the block repeat its last node).
-    auto round1 = [&](int j0, int (&wv)[kNodesB], int (&wx)[kNodesB][kWidth]) {
+    // Load set 1 of the step at j0 (nodes past the block repeat its last node).
+    auto rows1 = [&](int j0, int (&wv)[kNodesB], int (&wx)[kNodesB][kWidth]) {
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
@@ -504,7 +504,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
       }
     };
     int wvn[kNodesB], wxn[kNodesB][kWidth];
-    round1(0, wvn, wxn);
+    rows1(0, wvn, wxn);
     sink_stores<2 * kNodesB + 1>(L);  // a step stores flows and a quotient per node, then node flows
     for (int j0 = 0; j0 < jn; j0 += kNodesB) {
       // This step's rows have arrived once only the stores issued after them
@@ -517,7 +517,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
 #pragma unroll
         for (int r = 0; r < kWidth; ++r) wx[u][r] = wxn[u][r];
       }
-      // Round 2: raw slot words only (counts of nodes not flagged 1, and the
+      // Load set 2: raw slot words only (counts of nodes not flagged 1, and the
       // tight entries' slots); the arithmetic follows once all are issued,
       // and the phase order is kept as written (see the forward pass).
       __builtin_amdgcn_sched_barrier(0);
@@ -541,7 +541,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
           qr[u][r] = slot_if(t & slot_needed((leafm >> r) & 1, wx[u][r]), roff(rl(sxr[r], j)), L);
         }
       }
-      round1(j0 + kNodesB, wvn, wxn);
+      rows1(j0 + kNodesB, wvn, wxn);
       __builtin_amdgcn_sched_barrier(0);
       double svu[kNodesB];
       double qx[kNodesB][kWidth];  // Q(x) in the lanes where entry r is tight
