@@ -26,6 +26,9 @@ __all__ = ["BcxError", "Graph", "Index", "Context", "RQServer", "lib", "MODE_PLA
 MODE_PLAIN = 0
 MODE_BATCHED = 1
 MODE_SWEEP = 2
+SWEEP_FORM_AUTO = 0
+SWEEP_FORM_ONE_PER_CU = 1
+SWEEP_FORM_TWO_PER_CU = 2
 _PKG = Path(__file__).resolve().parent
 _LIB = None
 
@@ -45,7 +48,8 @@ class IndexStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile", C.c_int32), ("tchunk", C.c_int32), ("bp_threads", C.c_int32),
-                ("batch", C.c_int32), ("sweep_batch", C.c_int32)]
+                ("batch", C.c_int32), ("sweep_batch", C.c_int32), ("sweep_form", C.c_int32),
+                ("unpack_wide", C.c_int32), ("sweep_mem_limit", C.c_int64)]
 
 
 class Timing(C.Structure):
@@ -98,6 +102,7 @@ _SIGS = {
     "bcx_full_bc": (C.c_int, [_P, C.c_int, _F64P, C.POINTER(C.c_uint32)]),
     "bcx_bc_sources": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, _F64P, C.POINTER(C.c_uint32)]),
     "bcx_query_dist_count": (C.c_int, [_P, C.c_int32, _I32P, _U64P, _U32P]),
+    "bcx_query_dist_count_mode": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, _I32P, _U64P, C.c_void_p]),
     "bcx_subset_bc": (C.c_int, [_P, _I32P, C.c_int32, _F64P]),
     "bcx_h2h_distances": (C.c_int, [_P, _I32P, C.c_int32, _I32P]),
     "bcx_rq_create": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
@@ -108,6 +113,7 @@ _SIGS = {
     "bcx_rq_step": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "bcx_rq_poll": (C.c_int, [_P, C.c_int64, _F64P]),
     "bcx_rq_bytes_needed": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_size_t)]),
+    "bcx_rq_memory": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
 }
 
 
@@ -349,6 +355,17 @@ class Context:
         rc = _check(lib().bcx_query_dist_count(self._h, source, d, s, h))
         return d, s, h, rc == 1
 
+    def query_rows(self, sources, mode: int = MODE_SWEEP):
+        """(d, sigma, hop, overflow) rows [len(sources), n] of one form's
+        distance/count stage; hop is None for the bag sweep (no layers)."""
+        src = np.ascontiguousarray(sources, np.int32)
+        d = np.zeros((len(src), self.n), np.int32)
+        s = np.zeros((len(src), self.n), np.uint64)
+        h = None if mode == MODE_SWEEP else np.zeros((len(src), self.n), np.uint32)
+        rc = _check(lib().bcx_query_dist_count_mode(
+            self._h, mode, src, len(src), d, s, None if h is None else h.ctypes.data_as(C.c_void_p)))
+        return d, s, h, rc == 1
+
     def h2h_distances(self, sources) -> np.ndarray:
         """QBC distance-only scan (QBC.cu:243-331): [len(sources), n] int32."""
         src = np.ascontiguousarray(sources, np.int32)
@@ -388,6 +405,12 @@ class RQServer:
         b = C.c_size_t()
         _check(lib().bcx_rq_bytes_needed(self._h, q, C.byref(b)))
         return b.value
+
+    def memory(self) -> tuple[int, int]:
+        """(bytes held now, budget)."""
+        u, b = C.c_size_t(), C.c_size_t()
+        _check(lib().bcx_rq_memory(self._h, C.byref(u), C.byref(b)))
+        return u.value, b.value
 
     def submit(self, points) -> int | None:
         """Request id, or None when it cannot be admitted now (BCX_EBUSY)."""

@@ -81,7 +81,8 @@ def build_library(force: bool = True) -> Path:
 def build_oracle() -> Path:
     odir = ROOT / "oracle"
     lib = odir / "liboracle.so"
-    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall", odir / "oracle.c", "-o", lib, "-lm"])
+    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-fopenmp", "-Wall", odir / "oracle.c", "-o", lib,
+          "-lm"])
     ref = os.environ.get("BCX_REFERENCE_DIR", "/root/reference")
     if Path(ref, "src", "TreeCount", "TreeDecomp.cpp").exists():
         _run(["sh", odir / "build_ref.sh"])

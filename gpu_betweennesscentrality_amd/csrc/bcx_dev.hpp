@@ -161,11 +161,13 @@ struct SweepWork {
   double *phi;     // [B]: shortcut flow summed over tiles
   double *interior;// [n]: flow through shortcut interiors (after unpacking)
   double *pf;      // [P]: flow share of each tight pair
-  int *tile_src;   // [tiles * 64] sources (tree ids)
-  int *cand;       // candidate lists of the ascending pass
-  int *cand_off;   // [tiles + 1]
-  int cand_cap;
+  int *tile_src;   // [job tiles * 64] sources (tree ids) of every tile of a job
+  int *cand;       // candidate lists of the ascending pass, every tile of a job
+  int *cand_off;   // [job tiles + 1]
+  int64_t cand_cap;
+  int64_t src_cap; // tiles tile_src and cand_off hold
   int tstride;     // tiles the state arrays are laid out for
+  int tile0;       // job tile of a launch's first workgroup (per launch)
 };
 
 // Saturating count arithmetic: ~0 marks "overflowed" and is absorbing under
@@ -215,11 +217,12 @@ struct bcx_ctx {
   bcx_dev::DevIndex ix{};
   std::vector<void *> owned;          // index + graph buffers
   std::vector<int32_t> perm, iperm;   // preorder <-> vertex
-  bcx_tuning tn{64, 32, 256, 4096, 16384};
+  bcx_tuning tn{64, 32, 256, 4096, 16384, BCX_SWEEP_FORM_AUTO, 1024, 0};
   // batch workspace
   int ws_slots = 0;
   bcx_dev::DevWork ws{};
-  int *d_srcs = nullptr;
+  int *d_srcs = nullptr;  // label-scan source list (srcs_cap entries)
+  int64_t srcs_cap = 0;
   int *d_cursor = nullptr;
   int *d_total = nullptr;
   int *d_ovf = nullptr;
@@ -235,6 +238,14 @@ struct bcx_ctx {
   std::vector<int32_t> h_parent, h_depth, h_subtree, h_lvl_off, h_lvl_pair_off;
   int sweep_tiles = 0;
   bcx_dev::SweepWork sw{};
+  // A sweep job's plan on the host (tile sources, ascending-pass candidate
+  // lists); it stays alive until the next job starts, which first waits for
+  // the stream, so the uploads from it may run asynchronously.
+  std::vector<int32_t> plan_src, plan_cand, plan_cand_off;
+  // The launch sequence of the last sweep job, captured once and replayed
+  // while the job's shape and buffers stay the same.
+  hipGraphExec_t sweep_exec = nullptr;
+  std::vector<int64_t> sweep_exec_key;
   // QBC H2H distance labels and separators, when the index had them at
   // creation.  The labels are interleaved by groups of 64 consecutive
   // preorder ids: entry k of node t at d_goff[t / 64] + 64 k + t % 64 (a
@@ -250,7 +261,8 @@ struct bcx_ctx {
     if (hipSetDevice(device) != hipSuccess) return;
     if (stream) (void)hipStreamSynchronize(stream);
     free_ws();  // destroys graph_exec once and clears it
-    free_sweep();
+    free_sweep();  // destroys sweep_exec
+    if (d_srcs) (void)hipFree(d_srcs);
     for (void *p : owned) (void)hipFree(p);
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -268,6 +280,22 @@ struct bcx_ctx {
       if (p) (void)hipFree(p);
     ws = bcx_dev::DevWork{};
     ws_slots = 0;
+  }
+  // Room for `count` label-scan sources; the captured batch graph names the
+  // list, so it is dropped when the list moves.
+  void ensure_srcs(int64_t count) {
+    if (count <= srcs_cap && d_srcs) return;
+    if (stream) HIP_OK(hipStreamSynchronize(stream));
+    if (graph_exec) {
+      (void)hipGraphExecDestroy(graph_exec);
+      graph_exec = nullptr;
+    }
+    if (d_srcs) HIP_OK(hipFree(d_srcs));
+    d_srcs = nullptr;
+    srcs_cap = 0;
+    const int64_t cap = std::max<int64_t>({count, (int64_t)ix.n, 1});
+    HIP_OK(hipMalloc(&d_srcs, (size_t)cap * 4));
+    srcs_cap = cap;
   }
   static size_t slot_bytes(int n) {
     return (size_t)n * (4 + 8 + 4 + 4 + 8) + ((size_t)n + 2) * 4;
@@ -302,9 +330,14 @@ inline void sort_tree_ids(std::vector<int32_t> &ids, int32_t n) {
   for (int32_t x : ids) out[(size_t)start[(size_t)x]++] = x;
   ids.swap(out);
 }
-// Bag-sweep full BC over `count` sources (vertex ids), added into bc_inout.
+// Bag-sweep full BC over `count` sources (vertex ids, repeats allowed),
+// added into bc_inout.
 void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_inout,
                uint32_t *overflow);
+// Distance and path-count rows of the bag sweep's forward pass for `count`
+// sources (vertex ids): d[i * n + v], sigma[i * n + v] for sources[i].
+// Returns 1 if some count saturated.
+int sweep_query(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t *d, uint64_t *sigma);
 // Pieces of the bag sweep, for callers that keep flows per batch.
 // Tiles per launch for `count` sources (tuning batch, capped by device
 // memory); allocates the sweep buffers for that many tiles.

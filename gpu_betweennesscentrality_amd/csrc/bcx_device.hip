@@ -472,6 +472,7 @@ void run_sources(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, do
   for (int32_t v = 0; v < n; ++v) bc_pre[(size_t)c->iperm[(size_t)v]] = bc_inout[v];
   const int batch = mode == BCX_MODE_PLAIN ? 1 : std::max(1, std::min(c->tn.batch, count));
   c->ensure_ws(batch);
+  c->ensure_srcs(count);
   if (count > 0)
     HIP_OK(hipMemcpyAsync(c->d_srcs, pre.data(), (size_t)count * 4, hipMemcpyHostToDevice,
                           c->stream));
@@ -962,7 +963,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     c->h_subtree = ix.subtree;
     c->h_lvl_off = lvl_off;
   }
-  c->d_srcs =(int *)keep(dev_upload(std::vector<int32_t>((size_t)n, 0)));
+  c->ensure_srcs(n);
   c->d_cursor = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
   c->d_total = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
   c->d_ovf = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
@@ -984,6 +985,11 @@ int bcx_ctx_set_tuning(bcx_ctx *c, const bcx_tuning *tn) {
   if (tn->batch < 1 || tn->batch > (1 << 20)) return bcx::fail(BCX_EINVAL, "batch out of range");
   if (tn->sweep_batch < 1 || tn->sweep_batch > (1 << 22))
     return bcx::fail(BCX_EINVAL, "sweep_batch out of range");
+  if (tn->sweep_form != BCX_SWEEP_FORM_AUTO && tn->sweep_form != BCX_SWEEP_FORM_ONE_PER_CU &&
+      tn->sweep_form != BCX_SWEEP_FORM_TWO_PER_CU)
+    return bcx::fail(BCX_EINVAL, "sweep_form must be BCX_SWEEP_FORM_AUTO, _ONE_PER_CU or _TWO_PER_CU");
+  if (tn->unpack_wide < 1) return bcx::fail(BCX_EINVAL, "unpack_wide must be at least 1");
+  if (tn->sweep_mem_limit < 0) return bcx::fail(BCX_EINVAL, "sweep_mem_limit must not be negative");
   c->tn = *tn;
   return BCX_OK;
 }
@@ -1035,7 +1041,7 @@ int bcx_bc_sources(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, 
     return bcx::fail(BCX_EINVAL, "bad argument");
   if (mode != BCX_MODE_PLAIN && mode != BCX_MODE_BATCHED && mode != BCX_MODE_SWEEP)
     return bcx::fail(BCX_EINVAL, "bad mode");
-  if (count > c->ix.n) return bcx::fail(BCX_EINVAL, "more sources than vertices");
+  if (count > (1 << 30)) return bcx::fail(BCX_EINVAL, "more than 2^30 sources in one call");
   HIP_OK(hipSetDevice(c->device));
   if (mode == BCX_MODE_SWEEP)
     run_sweep(c, sources, count, bc_inout, overflow);
@@ -1085,6 +1091,77 @@ int bcx_query_dist_count(bcx_ctx *c, int32_t source, int32_t *d, uint64_t *sigma
     if (sigma) sigma[v] = ss[p];
     if (hop) hop[v] = hh[p];
   }
+  return ovf ? 1 : BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_query_dist_count_mode(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, int32_t *d,
+                              uint64_t *sigma, uint32_t *hop) {
+  BCX_GUARD_BEGIN
+  if (!c || count < 0 || (count > 0 && !sources)) return bcx::fail(BCX_EINVAL, "bad argument");
+  const int n = c->ix.n;
+  for (int32_t i = 0; i < count; ++i)
+    if (sources[i] < 0 || sources[i] >= n) return bcx::fail(BCX_EINVAL, "source out of range");
+  if (count == 0) return BCX_OK;
+  HIP_OK(hipSetDevice(c->device));
+  if (mode == BCX_MODE_SWEEP) {
+    if (hop) return bcx::fail(BCX_EINVAL, "the bag sweep keeps no hop layers: pass hop = NULL");
+    return bcx_dev::sweep_query(c, sources, count, d, sigma);
+  }
+  if (mode == BCX_MODE_PLAIN) {
+    int any = 0;
+    for (int32_t i = 0; i < count; ++i) {
+      const int64_t o = (int64_t)i * n;
+      const int rc = bcx_query_dist_count(c, sources[i], d ? d + o : nullptr, sigma ? sigma + o : nullptr,
+                                          hop ? hop + o : nullptr);
+      if (rc < 0) return rc;
+      any |= rc;
+    }
+    return any ? 1 : BCX_OK;
+  }
+  if (mode != BCX_MODE_BATCHED) return bcx::fail(BCX_EINVAL, "bad mode");
+  // The label-scan tile kernel over batches of the given sources, in the
+  // given order (lane j of a tile <-> the batch's j-th source).
+  const int batch = std::max(1, std::min(c->tn.batch, count));
+  c->ensure_ws(batch);
+  c->ensure_srcs(count);
+  std::vector<int32_t> pre((size_t)count);
+  for (int32_t i = 0; i < count; ++i) pre[(size_t)i] = c->iperm[(size_t)sources[i]];
+  const int zero = 0;
+  HIP_OK(hipMemcpyAsync(c->d_srcs, pre.data(), (size_t)count * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->d_total, &count, 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->d_ovf, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  std::vector<int32_t> hd;
+  std::vector<uint64_t> hs;
+  std::vector<uint32_t> hh;
+  for (int32_t b0 = 0; b0 < count; b0 += batch) {
+    const int32_t nb = std::min(batch, count - b0);
+    HIP_OK(hipMemcpyAsync(c->d_cursor, &b0, 4, hipMemcpyHostToDevice, c->stream));
+    const int waves = 16;
+    dim3 grid((batch + kWave - 1) / kWave, (n + waves * c->tn.tchunk - 1) / (waves * c->tn.tchunk));
+    scan_tile_kernel<<<grid, waves * kWave, 0, c->stream>>>(c->ix, c->ix.dis, c->ix.cnt, c->ix.hop, c->ix.row_off,
+                                                            batch_of(c), c->ws.d, c->ws.sigma, c->ws.hop, batch,
+                                                            c->tn.tchunk, c->d_ovf);
+    HIP_OK(hipGetLastError());
+    hd.resize((size_t)nb * n);
+    hs.resize((size_t)nb * n);
+    hh.resize((size_t)nb * n);
+    HIP_OK(hipMemcpyAsync(hd.data(), c->ws.d, hd.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(hs.data(), c->ws.sigma, hs.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(hh.data(), c->ws.hop, hh.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (int32_t r = 0; r < nb; ++r) {
+      const int64_t o = (int64_t)(b0 + r) * n;
+      for (int32_t v = 0; v < n; ++v) {
+        const size_t p = (size_t)r * n + (size_t)c->iperm[(size_t)v];
+        if (d) d[o + v] = hd[p];
+        if (sigma) sigma[o + v] = hs[p];
+        if (hop) hop[o + v] = hh[p];
+      }
+    }
+  }
+  int ovf = 0;
+  HIP_OK(hipMemcpy(&ovf, c->d_ovf, 4, hipMemcpyDeviceToHost));
   return ovf ? 1 : BCX_OK;
   BCX_GUARD_END
 }
@@ -1198,7 +1275,9 @@ struct bcx_rq {
   };
   bcx_ctx *ctx = nullptr;
   int mode = BCX_MODE_SWEEP;
-  size_t budget = 0, used = 0;
+  // The budget covers the engine's fixed memory (per-batch flow slots or the
+  // sweep's source list, and the batch sum) and every admitted request.
+  size_t budget = 0, used = 0, fixed = 0;
   int32_t batch = 0, nb = 0, cur = 0;
   int64_t next_id = 1;
   double *bsum = nullptr;
@@ -1246,6 +1325,11 @@ int bcx_rq_create_mode(bcx_ctx *c, int mode, size_t hbm_budget_bytes, bcx_rq **o
     r->batch = bcx_dev::sweep_tiles_for(c, n) * bcx_dev::kWave;
     r->nb = (n + r->batch - 1) / r->batch;
     const size_t B = (size_t)std::max<int64_t>(c->tree.bags, 1);
+    r->fixed = (size_t)r->nb * ((size_t)n + B) * 8 + (size_t)n * 8;
+    if (r->fixed > r->budget)
+      return bcx::fail(BCX_EINVAL, "budget below the engine's fixed device memory (" +
+                                       std::to_string(r->fixed) + " bytes)");
+    r->used = r->fixed;
     HIP_OK(hipMalloc(&r->nodeb, (size_t)r->nb * n * 8));
     HIP_OK(hipMalloc(&r->phib, (size_t)r->nb * B * 8));
     HIP_OK(hipMalloc(&r->bsum, (size_t)n * 8));
@@ -1256,6 +1340,11 @@ int bcx_rq_create_mode(bcx_ctx *c, int mode, size_t hbm_budget_bytes, bcx_rq **o
   }
   r->batch = std::max(1, std::min(c->tn.batch, n));
   r->nb = (n + r->batch - 1) / r->batch;
+  r->fixed = (size_t)n * (8 + 4) + 32;
+  if (r->fixed > r->budget)
+    return bcx::fail(BCX_EINVAL, "budget below the engine's fixed device memory (" +
+                                     std::to_string(r->fixed) + " bytes)");
+  r->used = r->fixed;
   c->ensure_ws(r->batch);
   HIP_OK(hipMalloc(&r->bsum, (size_t)n * 8));
   // The sweep's source list is the identity over tree order.
@@ -1280,6 +1369,13 @@ void bcx_rq_free(bcx_rq *rq) { delete rq; }
 int bcx_rq_bytes_needed(const bcx_rq *rq, int32_t q, size_t *bytes) {
   if (!rq || !bytes || q < 0) return bcx::fail(BCX_EINVAL, "bad argument");
   *bytes = rq->need(q);
+  return BCX_OK;
+}
+
+int bcx_rq_memory(const bcx_rq *rq, size_t *used, size_t *budget) {
+  if (!rq) return bcx::fail(BCX_EINVAL, "null engine");
+  if (used) *used = rq->used;
+  if (budget) *budget = rq->budget;
   return BCX_OK;
 }
 

@@ -54,12 +54,14 @@ struct Lane {
                  // not land anywhere go there (see store_all)
 };
 
+// Tile t of a launch is tile sw.tile0 + t of the job: its sources and
+// candidate lists are the job's, its state and flow sums the launch's.
 __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &sw, int t,
                                            int nsrc, int n) {
   Lane L;
   L.t = t;
   L.lane = threadIdx.x & (kWave - 1);
-  const int slot = t * kWave + L.lane;
+  const int64_t slot = (int64_t)(sw.tile0 + t) * kWave + L.lane;
   L.valid = slot < nsrc;
   L.s = sw.tile_src[slot];  // padded with a valid source beyond nsrc
   L.rs = ix.row_off[L.s];
@@ -634,11 +636,14 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
 // level split in runs (few nodes, many pieces) then merges each node's
 // per-wavefront partial results from LDS in a fixed order, so results are
 // deterministic.
+// With fwd_only, the kernel stops after the forward pass (distances and path
+// counts left in the tile state, for probes of that pass).
 template <int kNodesF, int kNodesB, int kWavesPerSimd>
 __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kernel(DevIndex ix, DevTree tr,
                                                                    SweepWork sw, int nsrc,
                                                                    int *ovf_flag,
-                                                                   uint64_t *stamps) {
+                                                                   uint64_t *stamps,
+                                                                   int fwd_only) {
   __shared__ double sh_acc[kSweepWaves][kWave];
   // Segments of levels split in runs: per-node partial results.
   __shared__ uint64_t seg_x[kSweepMaxSegments][kWave], seg_y[kSweepMaxSegments][kWave];
@@ -688,6 +693,10 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
   }
 
   if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
+  if (fwd_only) {  // uniform
+    if (ovf) atomicOr(ovf_flag, 1);
+    return;
+  }
 
   // ---- backward: dependencies, shortcut flows, node flows ----
   for (int k = H - 1; k >= 0; --k) {
@@ -737,7 +746,7 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
   // their ancestors' quotients, and a lane whose source is not below x never
   // reads x again.
   {
-    const int c0 = cld(sw.cand_off, L.t), c1 = cld(sw.cand_off, L.t + 1);
+    const int c0 = cld(sw.cand_off, sw.tile0 + L.t), c1 = cld(sw.cand_off, sw.tile0 + L.t + 1);
     for (int kk = c0; kk < c1; ++kk) {
       const int x = cld(sw.cand, kk);
       const bool anc = x <= L.s && L.s < x + cld(tr.subtree, x);
@@ -811,7 +820,7 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
   if (ovf) atomicOr(ovf_flag, 1);
 }
 
-using TileKernel = void (*)(DevIndex, DevTree, SweepWork, int, int *, uint64_t *);
+using TileKernel = void (*)(DevIndex, DevTree, SweepWork, int, int *, uint64_t *, int);
 
 // Two forms of the tile kernel.  Up to one tile per compute unit: 2 pieces per
 // step in each pass, registers for 5 waves per SIMD (one 16-wave workgroup per
@@ -819,10 +828,30 @@ using TileKernel = void (*)(DevIndex, DevTree, SweepWork, int, int *, uint64_t *
 // 8 waves per SIMD, so two workgroups share each unit and one's waits are
 // covered by the other's work (on road graphs 11-16% more sources per second,
 // with three tiles per unit the best, while one such workgroup alone is 8%
-// slower than the first form).
-TileKernel tile_kernel(int tiles, int cus) {
-  if (tiles > cus) return (TileKernel)sweep_tile_kernel<1, 1, 8>;
+// slower than the first form).  The tuning may force either form.
+int tile_form(int tiles, int cus, int form) {
+  if (form == BCX_SWEEP_FORM_ONE_PER_CU || form == BCX_SWEEP_FORM_TWO_PER_CU) return form;
+  return tiles > cus ? BCX_SWEEP_FORM_TWO_PER_CU : BCX_SWEEP_FORM_ONE_PER_CU;
+}
+TileKernel tile_kernel(int form) {
+  if (form == BCX_SWEEP_FORM_TWO_PER_CU) return (TileKernel)sweep_tile_kernel<1, 1, 8>;
   return (TileKernel)sweep_tile_kernel<2, 2, 1>;
+}
+
+// Distance and count rows of a launch's tiles after a forward-only launch:
+// row r (tile r / 64, lane r % 64) of node p goes to d[r * n + p], s[r * n + p]
+// (tree ids), for r below nrows.
+__global__ __launch_bounds__(256) void sweep_extract_kernel(SweepWork sw, int n, int *d,
+                                                            uint64_t *s) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int r = blockIdx.y;
+  const char *rec = sw.state + ((int64_t)(r / kWave) * (n + 1) + p) * kRec;
+  const int lane = r % kWave;
+  const int w = *(const int *)(rec + lane * 4);
+  const uint64_t c = one_of(w) ? 1 : *(const uint64_t *)(rec + kSlotOff + lane * 8);
+  d[(int64_t)r * n + p] = dist_of(w);
+  s[(int64_t)r * n + p] = c;
 }
 
 // node_out[v] (+)= sum over tiles of node_t, phi_out[e] (+)= sum over tiles
@@ -881,7 +910,8 @@ __global__ __launch_bounds__(256) void sweep_sum_batches_kernel(int n, int64_t B
 // level, step 2 gives each node of the level its interior flow and adds to
 // its entries' flows, each sum over pairs in increasing pair order.
 //
-// Wide levels (at least kUnpackWide nodes) take two grid-wide launches, one
+// Wide levels (at least the tuning's unpack_wide nodes, 1024 by default)
+// take two grid-wide launches, one
 // thread per pair and one wavefront per node.  Runs of narrower levels go to
 // one resident workgroup that walks them with a barrier after each step, so
 // the long chain of small top levels costs no launches; there a node gets a
@@ -890,7 +920,6 @@ __global__ __launch_bounds__(256) void sweep_sum_batches_kernel(int n, int64_t B
 // the node's pairs and entries in strides and a fixed lane tree sums them.
 // Every sum has a fixed order, so results are deterministic.
 constexpr int kUnpackThreads = 1024;
-constexpr int kUnpackWide = 1024;
 
 // Sum of pf[idx[p]] for p = p0, p0 + step, ... below p1, added in that
 // order; four index loads, then four value loads, are in flight at a time
@@ -993,6 +1022,9 @@ inline unsigned blocks_for(int64_t items, int per) { return (unsigned)((items + 
 }  // namespace
 
 void bcx_ctx::free_sweep() {
+  if (sweep_exec) (void)hipGraphExecDestroy(sweep_exec);
+  sweep_exec = nullptr;
+  sweep_exec_key.clear();
   void *bufs[] = {sw.state, sw.node_t, sw.phi_t,
                   sw.node, sw.phi,     sw.interior, sw.tile_src, sw.cand, sw.cand_off,
                   sw.pf};
@@ -1005,7 +1037,7 @@ void bcx_ctx::free_sweep() {
 void bcx_ctx::ensure_sweep(int tiles) {
   if (tiles <= sweep_tiles) return;
   free_sweep();
-  const size_t n = (size_t)ix.n, B = (size_t)tree.bags, T = (size_t)tiles, W = kWave;
+  const size_t n = (size_t)ix.n, B = (size_t)tree.bags, T = (size_t)tiles;
   HIP_OK(hipMalloc(&sw.state, T * (n + 1) * kRec));
   HIP_OK(hipMalloc(&sw.node_t, T * (size_t)tree.bslots * 8));
   HIP_OK(hipMalloc(&sw.phi_t, T * std::max<size_t>(B, 1) * 8));
@@ -1013,10 +1045,7 @@ void bcx_ctx::ensure_sweep(int tiles) {
   HIP_OK(hipMalloc(&sw.phi, std::max<size_t>(B, 1) * 8));
   HIP_OK(hipMalloc(&sw.interior, n * 8));
   HIP_OK(hipMalloc(&sw.pf, std::max<size_t>((size_t)tree.pairs, 1) * 8));
-  HIP_OK(hipMalloc(&sw.tile_src, T * W * 4));
-  HIP_OK(hipMalloc(&sw.cand_off, (T + 1) * 4));
-  sw.cand_cap = (int)std::min<size_t>(T * ((size_t)tree.height + W), (size_t)1 << 30);
-  HIP_OK(hipMalloc(&sw.cand, (size_t)sw.cand_cap * 4));
+  // The job arrays (tile sources, candidate lists) are sized by sweep_plan.
   sw.tstride = tiles;
   sweep_tiles = tiles;
 }
@@ -1028,20 +1057,23 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   const int64_t B = c->tree.bags;
   if (((uint64_t)n + 1) * (uint64_t)kRec > 0x100000000ull)
     throw bcx::Error{BCX_EGRAPH, "bag-sweep records of one tile exceed 4 GiB (more than 5592404 vertices)"};
-  const int total_tiles = (count + kWave - 1) / kWave;
+  const int total_tiles = (int)(((int64_t)count + kWave - 1) / kWave);
   int tiles = std::max(1, std::min(c->tn.sweep_batch / kWave, std::max(1, total_tiles)));
   // Keep the per-tile state within 85% of the device memory not already held
-  // by this context's sweep buffers.  When that cuts the launch short and the
-  // label-scan workspace is allocated, release it (it is re-allocated on its
-  // next use) and measure again.  A tile's sweep is a latency chain through
-  // the levels, so a launch of few tiles costs nearly as much as a full one:
-  // fitting the whole batch in one launch matters.
+  // by this context's sweep buffers (and within the tuning's memory limit).
+  // When that cuts the launch short and the label-scan workspace is
+  // allocated, release it (it is re-allocated on its next use) and measure
+  // again.  A tile's sweep is a latency chain through the levels, so a launch
+  // of few tiles costs nearly as much as a full one: fitting the whole batch
+  // in one launch matters.
   const size_t per_tile = ((size_t)n + 1) * kRec + (size_t)c->tree.bslots * 8 + (size_t)B * 8;
   auto cap_now = [&]() {
     size_t fr = 0, tot = 0;
     HIP_OK(hipMemGetInfo(&fr, &tot));
     const size_t held = (size_t)c->sweep_tiles * per_tile;
-    return (size_t)((double)(fr + held) * 0.85) / std::max<size_t>(per_tile, 1);
+    size_t cap = (size_t)((double)(fr + held) * 0.85) / std::max<size_t>(per_tile, 1);
+    if (c->tn.sweep_mem_limit > 0) cap = std::min(cap, (size_t)c->tn.sweep_mem_limit / per_tile);
+    return cap;
   };
   size_t cap = cap_now();
   if (cap < (size_t)tiles && c->ws_slots > 0) {
@@ -1059,22 +1091,29 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   return tiles;
 }
 
-void sweep_launch(bcx_ctx *c, const int32_t *pre, int32_t nsrc, double *node_out,
-                  double *phi_out, bool accumulate) {
-  const int n = c->ix.n;
-  const int64_t B = c->tree.bags;
-  const int bt = (nsrc + kWave - 1) / kWave;
-  if (bt < 1 || bt > c->sweep_tiles) throw bcx::Error{BCX_EINVAL, "sweep launch exceeds its tiles"};
-  SweepWork sw = c->sw;
+// Plans a job over the sorted tree ids pre[0 .. cnt): tile t holds sources
+// pre[64 t .. 64 t + 63] (the last tile padded with its first source), and
+// its candidate list is the union of its sources' root paths with every
+// ancestor first (a source's path stops where it meets the previous
+// source's).  Uploads both to the job arrays, growing them when needed.  The
+// host plan lives in the context until the next plan, which first waits for
+// the stream, so the uploads need no wait of their own.
+void sweep_plan(bcx_ctx *c, const int32_t *pre, int64_t cnt) {
+  HIP_OK(hipStreamSynchronize(c->stream));
+  const int64_t T = (cnt + kWave - 1) / kWave;
   const auto &par = c->h_parent;
   const auto &sub = c->h_subtree;
-  // Tile sources (padded with the tile's first source) and the union of their
-  // root paths in an order where every ancestor comes first.
-  std::vector<int32_t> tsrc((size_t)bt * kWave, 0), cand, cand_off((size_t)bt + 1, 0), path;
-  for (int t = 0; t < bt; ++t) {
-    const int t0 = t * kWave;
-    const int tn = std::min(kWave, nsrc - t0);
-    for (int j = 0; j < kWave; ++j) tsrc[(size_t)t * kWave + j] = pre[(size_t)(t0 + (j < tn ? j : 0))];
+  auto &tsrc = c->plan_src;
+  auto &cand = c->plan_cand;
+  auto &coff = c->plan_cand_off;
+  tsrc.assign((size_t)T * kWave, 0);
+  cand.clear();
+  coff.assign((size_t)T + 1, 0);
+  std::vector<int32_t> path;
+  for (int64_t t = 0; t < T; ++t) {
+    const int64_t t0 = t * kWave;
+    const int tn = (int)std::min<int64_t>(kWave, cnt - t0);
+    for (int j = 0; j < kWave; ++j) tsrc[(size_t)(t * kWave + j)] = pre[(size_t)(t0 + (j < tn ? j : 0))];
     int prev = -1;
     for (int j = 0; j < tn; ++j) {
       const int s = pre[(size_t)(t0 + j)];
@@ -1086,34 +1125,65 @@ void sweep_launch(bcx_ctx *c, const int32_t *pre, int32_t nsrc, double *node_out
       for (size_t q = path.size(); q-- > 0;) cand.push_back(path[q]);
       prev = s;
     }
-    cand_off[(size_t)t + 1] = (int32_t)cand.size();
+    if (cand.size() > 0x7fffffffu) throw bcx::Error{BCX_EINVAL, "sweep job too large"};
+    coff[(size_t)t + 1] = (int32_t)cand.size();
   }
-  if ((int64_t)cand.size() > c->sw.cand_cap) {
-    HIP_OK(hipStreamSynchronize(c->stream));
-    HIP_OK(hipFree(c->sw.cand));
-    c->sw.cand = nullptr;
-    HIP_OK(hipMalloc(&c->sw.cand, cand.size() * 4));
-    c->sw.cand_cap = (int)cand.size();
-    sw.cand = c->sw.cand;
+  SweepWork &sw = c->sw;
+  if (T > sw.src_cap || !sw.tile_src) {
+    if (sw.tile_src) HIP_OK(hipFree(sw.tile_src));
+    if (sw.cand_off) HIP_OK(hipFree(sw.cand_off));
+    sw.tile_src = nullptr;
+    sw.cand_off = nullptr;
+    sw.src_cap = 0;
+    const int64_t cap = std::max<int64_t>(T, 1);
+    HIP_OK(hipMalloc(&sw.tile_src, (size_t)cap * kWave * 4));
+    HIP_OK(hipMalloc(&sw.cand_off, (size_t)(cap + 1) * 4));
+    sw.src_cap = cap;
   }
-  HIP_OK(hipMemcpyAsync(sw.tile_src, tsrc.data(), tsrc.size() * 4, hipMemcpyHostToDevice,
-                        c->stream));
-  HIP_OK(hipMemcpyAsync(sw.cand_off, cand_off.data(), cand_off.size() * 4,
-                        hipMemcpyHostToDevice, c->stream));
+  if ((int64_t)cand.size() > sw.cand_cap || !sw.cand) {
+    if (sw.cand) HIP_OK(hipFree(sw.cand));
+    sw.cand = nullptr;
+    sw.cand_cap = 0;
+    const int64_t cap = std::max<int64_t>((int64_t)cand.size(), 1);
+    HIP_OK(hipMalloc(&sw.cand, (size_t)cap * 4));
+    sw.cand_cap = cap;
+  }
+  if (T > 0)
+    HIP_OK(hipMemcpyAsync(sw.tile_src, tsrc.data(), tsrc.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(sw.cand_off, coff.data(), coff.size() * 4, hipMemcpyHostToDevice, c->stream));
   if (!cand.empty())
-    HIP_OK(hipMemcpyAsync(sw.cand, cand.data(), cand.size() * 4, hipMemcpyHostToDevice,
-                          c->stream));
-  const bool timed = c->profiling;
-  if (timed) HIP_OK(hipEventRecord(c->ev[0], c->stream));
+    HIP_OK(hipMemcpyAsync(sw.cand, cand.data(), cand.size() * 4, hipMemcpyHostToDevice, c->stream));
+}
+
+// One launch over tiles [tile0, tile0 + nt) of the planned job of nsrc
+// sources, in tile-kernel form `form`.  Unless fwd_only, the launch's node
+// and shortcut flows then go to node_out[n] / phi_out[B] (added when
+// `accumulate`, else stored).  With `timed`, HIP events bracket the kernels
+// and per-tile phase stamps are read back (eager launches only).
+void sweep_tiles_launch(bcx_ctx *c, int tile0, int nt, int nsrc, int form, bool fwd_only,
+                        double *node_out, double *phi_out, bool accumulate, bool timed) {
+  const int n = c->ix.n;
+  const int64_t B = c->tree.bags;
+  if (nt < 1 || nt > c->sweep_tiles) throw bcx::Error{BCX_EINVAL, "sweep launch exceeds its tiles"};
+  SweepWork sw = c->sw;
+  sw.tile0 = tile0;
   uint64_t *stamps = nullptr;
-  if (timed) HIP_OK(hipMalloc(&stamps, (size_t)bt * 4 * 8));
-  tile_kernel(bt, c->cus)<<<bt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf,
-                                                                stamps);
-  if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
-  if (timed) HIP_OK(hipEventRecord(c->ev[2], c->stream));
-  sweep_tile_reduce_kernel<<<blocks_for(c->tree.bslots + n + B, 256), 256, 0, c->stream>>>(
-      n, B, bt, sw, c->tree, node_out, phi_out, accumulate ? 1 : 0);
+  if (timed) {
+    HIP_OK(hipMalloc(&stamps, (size_t)nt * 4 * 8));
+    HIP_OK(hipEventRecord(c->ev[0], c->stream));
+  }
+  tile_kernel(form)<<<nt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf, stamps,
+                                                        fwd_only ? 1 : 0);
   HIP_OK(hipGetLastError());
+  if (timed) {
+    HIP_OK(hipEventRecord(c->ev[1], c->stream));
+    HIP_OK(hipEventRecord(c->ev[2], c->stream));
+  }
+  if (!fwd_only) {
+    sweep_tile_reduce_kernel<<<blocks_for(c->tree.bslots + n + B, 256), 256, 0, c->stream>>>(
+        n, B, nt, sw, c->tree, node_out, phi_out, accumulate ? 1 : 0);
+    HIP_OK(hipGetLastError());
+  }
   if (timed) {
     HIP_OK(hipEventRecord(c->ev[3], c->stream));
     HIP_OK(hipEventSynchronize(c->ev[3]));
@@ -1124,23 +1194,30 @@ void sweep_launch(bcx_ctx *c, const int32_t *pre, int32_t nsrc, double *node_out
     c->timing.scan_ms += a;
     c->timing.dep_ms += b;
     c->timing.reduce_ms += r;
-    std::vector<uint64_t> st((size_t)bt * 4);
+    std::vector<uint64_t> st((size_t)nt * 4);
     HIP_OK(hipMemcpy(st.data(), stamps, st.size() * 8, hipMemcpyDeviceToHost));
     HIP_OK(hipFree(stamps));
     double f = 0, b2 = 0, as = 0;
-    for (int t = 0; t < bt; ++t) {
+    for (int t = 0; t < nt; ++t) {
       f += (double)(st[4 * t + 1] - st[4 * t]);
       b2 += (double)(st[4 * t + 2] - st[4 * t + 1]);
       as += (double)(st[4 * t + 3] - st[4 * t + 2]);
     }
-    c->timing.sweep_fwd_ms += f / bt * 1e-5;  // 100 MHz ticks
-    c->timing.sweep_bwd_ms += b2 / bt * 1e-5;
-    c->timing.sweep_asc_ms += as / bt * 1e-5;
+    c->timing.sweep_fwd_ms += f / nt * 1e-5;  // 100 MHz ticks
+    c->timing.sweep_bwd_ms += fwd_only ? 0.0 : b2 / nt * 1e-5;
+    c->timing.sweep_asc_ms += fwd_only ? 0.0 : as / nt * 1e-5;
   }
+}
+
+void sweep_launch(bcx_ctx *c, const int32_t *pre, int32_t nsrc, double *node_out,
+                  double *phi_out, bool accumulate) {
+  const int bt = (nsrc + kWave - 1) / kWave;
+  if (bt < 1 || bt > c->sweep_tiles) throw bcx::Error{BCX_EINVAL, "sweep launch exceeds its tiles"};
+  sweep_plan(c, pre, nsrc);
+  sweep_tiles_launch(c, 0, bt, nsrc, tile_form(bt, c->cus, c->tn.sweep_form), false, node_out,
+                     phi_out, accumulate, c->profiling);
   c->timing.scan_launches += 1;
   c->timing.dep_launches += 1;
-  // The host vectors above go out of scope: wait for the copies.
-  HIP_OK(hipStreamSynchronize(c->stream));
 }
 
 void sweep_sum_batches(bcx_ctx *c, const double *nodeb, const double *phib, int nb) {
@@ -1158,9 +1235,10 @@ void sweep_finish(bcx_ctx *c, double *bc_dev) {
   const auto &lvl = c->h_lvl_off;
   const auto &lpo = c->h_lvl_pair_off;
   const int H = c->tree.height;
+  const int wide = std::max(1, c->tn.unpack_wide);
   for (int k = 0; k < H;) {
     const int cnt = lvl[(size_t)k + 1] - lvl[(size_t)k];
-    if (cnt >= kUnpackWide) {
+    if (cnt >= wide) {
       const int qcnt = lpo[(size_t)k + 1] - lpo[(size_t)k];
       if (qcnt > 0)
         sweep_pair_flow_kernel<<<blocks_for(qcnt, 256), 256, 0, c->stream>>>(c->tree, sw, k);
@@ -1169,7 +1247,7 @@ void sweep_finish(bcx_ctx *c, double *bc_dev) {
       continue;
     }
     int k1 = k + 1;
-    while (k1 < H && lvl[(size_t)k1 + 1] - lvl[(size_t)k1] < kUnpackWide) ++k1;
+    while (k1 < H && lvl[(size_t)k1 + 1] - lvl[(size_t)k1] < wide) ++k1;
     sweep_unpack_kernel<<<1, kUnpackThreads, 0, c->stream>>>(c->tree, sw, k, k1);
     k = k1;
   }
@@ -1193,17 +1271,64 @@ void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_ino
   }
   sort_tree_ids(pre, n);
   const int tiles = sweep_tiles_for(c, count);
+  sweep_plan(c, pre.data(), count);
+  const int T = (int)(((int64_t)count + kWave - 1) / kWave);
+  const int form = tile_form(std::min(tiles, std::max(T, 1)), c->cus, c->tn.sweep_form);
   std::vector<double> bc_pre((size_t)n);
   for (int32_t v = 0; v < n; ++v) bc_pre[(size_t)c->iperm[(size_t)v]] = bc_inout[v];
   const int zero = 0;
   HIP_OK(hipMemcpyAsync(c->d_ovf, &zero, 4, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(c->d_bc, bc_pre.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
-  sweep_zero(c, c->sw.node, n);
-  sweep_zero(c, c->sw.phi, B);
-  for (int32_t first = 0; first < count; first += tiles * kWave)
-    sweep_launch(c, pre.data() + first, std::min(tiles * kWave, count - first), c->sw.node,
-                 c->sw.phi, true);
-  sweep_finish(c, c->d_bc);
+  // The job: flows zeroed, one launch (tile kernel + reduction over its
+  // tiles) per `tiles` tiles, then the unpack and the BC update.
+  auto job = [&](bool timed) {
+    sweep_zero(c, c->sw.node, n);
+    sweep_zero(c, c->sw.phi, B);
+    for (int t0 = 0; t0 < T; t0 += tiles)
+      sweep_tiles_launch(c, t0, std::min(tiles, T - t0), count, form, false, c->sw.node, c->sw.phi,
+                         true, timed);
+    sweep_finish(c, c->d_bc);
+  };
+  if (c->profiling) {
+    job(true);
+  } else {
+    // Captured once per job shape and buffer set, then replayed: the kernel
+    // arguments depend only on these; the sources and candidate lists are
+    // data in the job arrays.
+    const std::vector<int64_t> key = {count,
+                                      tiles,
+                                      form,
+                                      c->tn.unpack_wide,
+                                      (int64_t)(intptr_t)c->sw.state,
+                                      (int64_t)(intptr_t)c->sw.tile_src,
+                                      (int64_t)(intptr_t)c->sw.cand,
+                                      (int64_t)(intptr_t)c->sw.cand_off,
+                                      (int64_t)(intptr_t)c->d_bc};
+    if (!c->sweep_exec || c->sweep_exec_key != key) {
+      if (c->sweep_exec) HIP_OK(hipGraphExecDestroy(c->sweep_exec));
+      c->sweep_exec = nullptr;
+      c->sweep_exec_key.clear();
+      hipGraph_t g;
+      HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      try {
+        job(false);
+      } catch (...) {
+        hipGraph_t dead;
+        (void)hipStreamEndCapture(c->stream, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+      }
+      HIP_OK(hipStreamEndCapture(c->stream, &g));
+      const hipError_t e = hipGraphInstantiate(&c->sweep_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIP_OK(e);
+      c->sweep_exec_key = key;
+    }
+    HIP_OK(hipGraphLaunch(c->sweep_exec, c->stream));
+  }
+  const int launches = (T + tiles - 1) / tiles;
+  c->timing.scan_launches += launches;
+  c->timing.dep_launches += launches;
   c->timing.sources += count;
   int ovf = 0;
   HIP_OK(hipMemcpyAsync(bc_pre.data(), c->d_bc, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1211,6 +1336,69 @@ void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_ino
   HIP_OK(hipStreamSynchronize(c->stream));
   for (int32_t v = 0; v < n; ++v) bc_inout[v] = bc_pre[(size_t)c->iperm[(size_t)v]];
   if (overflow) *overflow = (uint32_t)ovf;
+}
+
+int sweep_query(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t *d, uint64_t *sigma) {
+  const int n = c->ix.n;
+  if (count <= 0) return 0;
+  std::vector<std::pair<int32_t, int32_t>> key((size_t)count);
+  for (int32_t k = 0; k < count; ++k) {
+    if (sources[k] < 0 || sources[k] >= n) throw bcx::Error{BCX_EINVAL, "source out of range"};
+    key[(size_t)k] = {c->iperm[(size_t)sources[k]], k};
+  }
+  std::stable_sort(key.begin(), key.end());
+  std::vector<int32_t> pre((size_t)count);
+  for (int32_t k = 0; k < count; ++k) pre[(size_t)k] = key[(size_t)k].first;
+  const int tiles = sweep_tiles_for(c, count);
+  // Launches of up to `chunk` tiles, their rows extracted to about 1 GiB of
+  // scratch at a time (and at most 65535 rows, the grid's second dimension).
+  const int64_t rows_cap = std::max<int64_t>(kWave, (1ll << 30) / (12ll * n));
+  const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)tiles, rows_cap / kWave, 1023}));
+  const int form = tile_form(chunk, c->cus, c->tn.sweep_form);
+  sweep_plan(c, pre.data(), count);
+  const int zero = 0;
+  HIP_OK(hipMemcpyAsync(c->d_ovf, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  const int T = (count + kWave - 1) / kWave;
+  int *dd = nullptr;
+  uint64_t *ds = nullptr;
+  auto release = [&]() {
+    if (dd) (void)hipFree(dd);
+    if (ds) (void)hipFree(ds);
+  };
+  try {
+    HIP_OK(hipMalloc(&dd, (size_t)chunk * kWave * n * 4));
+    HIP_OK(hipMalloc(&ds, (size_t)chunk * kWave * n * 8));
+    std::vector<int32_t> hd;
+    std::vector<uint64_t> hs;
+    for (int t0 = 0; t0 < T; t0 += chunk) {
+      const int nt = std::min(chunk, T - t0);
+      const int rows = std::min(nt * kWave, count - t0 * kWave);
+      sweep_tiles_launch(c, t0, nt, count, form, true, nullptr, nullptr, false, c->profiling);
+      sweep_extract_kernel<<<dim3((unsigned)((n + 255) / 256), (unsigned)rows), 256, 0, c->stream>>>(
+          c->sw, n, dd, ds);
+      HIP_OK(hipGetLastError());
+      hd.resize((size_t)rows * n);
+      hs.resize((size_t)rows * n);
+      HIP_OK(hipMemcpyAsync(hd.data(), dd, hd.size() * 4, hipMemcpyDeviceToHost, c->stream));
+      HIP_OK(hipMemcpyAsync(hs.data(), ds, hs.size() * 8, hipMemcpyDeviceToHost, c->stream));
+      HIP_OK(hipStreamSynchronize(c->stream));
+      for (int r = 0; r < rows; ++r) {
+        const int64_t i = key[(size_t)(t0 * kWave + r)].second;
+        for (int32_t v = 0; v < n; ++v) {
+          const size_t p = (size_t)r * n + (size_t)c->iperm[(size_t)v];
+          if (d) d[i * n + v] = hd[p];
+          if (sigma) sigma[i * n + v] = hs[p];
+        }
+      }
+    }
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
+  int ovf = 0;
+  HIP_OK(hipMemcpy(&ovf, c->d_ovf, 4, hipMemcpyDeviceToHost));
+  return ovf ? 1 : 0;
 }
 
 }  // namespace bcx_dev

@@ -79,7 +79,25 @@ typedef struct bcx_tuning {
   int32_t sweep_batch;   /* sources per bag-sweep launch (64 per tile, one
                             tile per workgroup); default 3 x 64 x compute
                             units (two workgroups per unit at a time)       */
+  int32_t sweep_form;    /* bag-sweep tile-kernel form: BCX_SWEEP_FORM_AUTO
+                            (by tiles per launch), _ONE_PER_CU (2 pieces per
+                            step, one workgroup per compute unit) or
+                            _TWO_PER_CU (1 piece per step, registers capped
+                            so two workgroups share a unit).  Every form
+                            gives bitwise the same results.                */
+  int32_t unpack_wide;   /* bag-sweep unpack: depth levels of at least this
+                            many nodes run as two grid-wide launches, the
+                            rest in one resident workgroup (default 1024)  */
+  int64_t sweep_mem_limit; /* bytes of device memory a sweep launch's state
+                            may use beyond the 85%-of-free cap (0: no
+                            further limit)                                  */
 } bcx_tuning;
+
+enum {
+  BCX_SWEEP_FORM_AUTO = 0,
+  BCX_SWEEP_FORM_ONE_PER_CU = 1,
+  BCX_SWEEP_FORM_TWO_PER_CU = 2,
+};
 
 typedef struct bcx_timing {
   double scan_ms;     /* label scan (distance, count, hop), bag-sweep tile
@@ -198,6 +216,17 @@ int bcx_bc_sources(bcx_ctx *ctx, int mode, const int32_t *sources,
 /* Distance, path count and max-hop layer from `source` to every vertex. */
 int bcx_query_dist_count(bcx_ctx *ctx, int32_t source, int32_t *d,
                          uint64_t *sigma, uint32_t *hop);
+/* The same rows for `count` sources, as computed by one form of the full-BC
+ * path (stage 2 of BC_Kernel, MHLQBC.cu:334-412): BCX_MODE_PLAIN (thread per
+ * target), BCX_MODE_BATCHED (the 64-source label-scan tile) or BCX_MODE_SWEEP
+ * (the forward pass of the bag-sweep tile kernel, in the kernel form the
+ * tuning selects).  Row i of d / sigma / hop (i * n + v) belongs to
+ * sources[i].  The bag sweep orders its Brandes pass by tree depth and keeps
+ * no hop layers: with BCX_MODE_SWEEP, hop must be NULL.  Returns 1 instead of
+ * BCX_OK when some count saturated at UINT64_MAX. */
+int bcx_query_dist_count_mode(bcx_ctx *ctx, int mode, const int32_t *sources,
+                              int32_t count, int32_t *d, uint64_t *sigma,
+                              uint32_t *hop);
 /* QBC distance-only scan (replaces the distance part of BC_Kernel,
  * QBC.cu:243-331: LCA by RMQ, then the ancestor lookup or the minimum over
  * the LCA's separator of the two H2H labels): dist_out[i * n + v] = d(sources[i],
@@ -223,6 +252,12 @@ int bcx_subset_bc(bcx_ctx *ctx, const int32_t *points, int32_t q,
  * batch node and shortcut flows kept in shared slots; a finished request is
  * unpacked once) or BCX_MODE_BATCHED (label scan; per request a [batch][q]
  * table).  bcx_rq_create is bcx_rq_create_mode with BCX_MODE_SWEEP. */
+/* hbm_budget_bytes bounds the engine's device memory: its fixed part (in the
+ * bag-sweep form one node and one shortcut flow slot per batch, nb (n + B) 8
+ * bytes, and n 8 bytes of batch sums; in the label-scan form the source list
+ * and batch sums, 12 n bytes) is taken at creation (BCX_EINVAL if the budget
+ * is smaller), and each request takes bcx_rq_bytes_needed(q) while in flight
+ * and returns it when it completes or is cancelled. */
 int bcx_rq_create(bcx_ctx *ctx, size_t hbm_budget_bytes, bcx_rq **out);
 int bcx_rq_create_mode(bcx_ctx *ctx, int mode, size_t hbm_budget_bytes,
                        bcx_rq **out);
@@ -236,6 +271,9 @@ int bcx_rq_step(bcx_rq *rq, int32_t *active);
 /* 1 and result copied (bc_out[q]) if done, 0 if still running. */
 int bcx_rq_poll(bcx_rq *rq, int64_t id, double *bc_out);
 int bcx_rq_bytes_needed(const bcx_rq *rq, int32_t q, size_t *bytes);
+/* Device memory the engine holds now (fixed part plus admitted requests) and
+ * its budget. */
+int bcx_rq_memory(const bcx_rq *rq, size_t *used, size_t *budget);
 
 #ifdef __cplusplus
 }

@@ -185,6 +185,39 @@ out:
   return rc;
 }
 
+/* orc_brandes over `threads` contiguous slices of the source list, each into
+ * its own array, then added into bc[n] in slice order (OpenMP; the result is
+ * the same for a given thread count, and within rounding of orc_brandes). */
+int orc_brandes_mt(int32_t n, const int64_t *row, const int32_t *col, const int32_t *w,
+                   const int32_t *sources, int32_t ns, double *bc, int32_t *overflow,
+                   int32_t threads) {
+  if (threads < 1) threads = 1;
+  if (threads > ns) threads = ns > 0 ? ns : 1;
+  double *part = (double *)calloc((size_t)threads * (size_t)n, sizeof(double));
+  int32_t *ovf = (int32_t *)calloc((size_t)threads, sizeof(int32_t));
+  int *rcs = (int *)calloc((size_t)threads, sizeof(int));
+  int rc = 0;
+  if (!part || !ovf || !rcs) {
+    rc = -1;
+    goto out;
+  }
+#pragma omp parallel for num_threads(threads) schedule(static, 1)
+  for (int32_t t = 0; t < threads; ++t) {
+    const int32_t a = (int32_t)((int64_t)ns * t / threads), b = (int32_t)((int64_t)ns * (t + 1) / threads);
+    rcs[t] = orc_brandes(n, row, col, w, sources + a, b - a, part + (size_t)t * n, &ovf[t]);
+  }
+  for (int32_t t = 0; t < threads; ++t) {
+    if (rcs[t]) rc = -1;
+    if (ovf[t] && overflow) *overflow = 1;
+    for (int32_t i = 0; i < n; ++i) bc[i] += part[(size_t)t * n + i];
+  }
+out:
+  free(part);
+  free(ovf);
+  free(rcs);
+  return rc;
+}
+
 /* RQ pair formula over all unordered pairs; O(n^2 q) after n Dijkstras. */
 int orc_rq_pairs(int32_t n, const int64_t *row, const int32_t *col, const int32_t *w,
                  const int32_t *points, int32_t q, double *out) {

@@ -8,10 +8,15 @@
 //   tc_ref tc  <graph> [pairs]          TreeCount index (TreeDecomp.cpp:12-511)
 //   tc_ref h2h <graph> <order> [pairs]  H2H index from an order file
 //                                       (buildIndex.cpp:4-278)
+//   tc_ref tcfull <graph>               TreeCount index, every node's label row
+//                                       and bag (the data moveCountIndexToDevice
+//                                       reads, MHLQBC.cu:6-128, plus the bags)
 // Output (text): a stats line, the elimination order, one "V" line per vertex
 // (id, parent vertex id, tree height, label length; 1-based ids), label rows
 // of the vertices listed in the pairs file, and one "Q s t d count hop" line
-// per pair.  Pair queries evaluate the reference's scan (MHLQBC.cu:334-412 for
+// per pair.  tcfull prints, per vertex, "R v k d:c:h ..." (its label row over
+// its k ancestors, root first) and "B v k u:len:c:h ..." (its bag: neighbour
+// at elimination, shortcut length, count and hop count).  Pair queries evaluate the reference's scan (MHLQBC.cu:334-412 for
 // TreeCount labels; min over the LCA's pos set for H2H distance labels) on the
 // reference's own in-memory labels, with 64-bit count products.
 #include <cstdint>
@@ -128,6 +133,29 @@ static int run_tc(const char *graph, const char *pairs_path) {
   return 0;
 }
 
+static int run_tcfull(const char *graph) {
+  TreeDecomp td(graph, "/dev/null");
+  td.Reduce();
+  td.MakeTree();
+  td.MakeIndex();
+  const int n = td.G_.n();
+  std::printf("n %d height %d width %d\n", n, td.tree_height_, td.tree_width_);
+  for (int v = 1; v <= n; ++v) {
+    const auto &nd = td.tree_[td.rank_[v]];
+    int pv = nd.parent < 0 ? 0 : td.tree_[nd.parent].vertex;
+    std::printf("V %d %d %d %zu\n", v, pv, nd.height, nd.dis.size());
+    std::printf("R %d %zu", v, nd.dis.size());
+    for (size_t j = 0; j < nd.dis.size(); ++j)
+      std::printf(" %d:%u:%u", nd.dis[j], (unsigned)nd.cnt[j], (unsigned)nd.my_cnt[j]);
+    std::printf("\nB %d %zu", v, nd.vertices.size());
+    for (size_t i = 0; i < nd.vertices.size(); ++i)
+      std::printf(" %d:%d:%u:%u", nd.vertices[i], nd.vertices_length[i],
+                  (unsigned)td.edge_count_[nd.vertex][i], (unsigned)td.my_edge_count_[nd.vertex][i]);
+    std::printf("\n");
+  }
+  return 0;
+}
+
 static int run_h2h(const char *graph, const char *order, const char *pairs_path) {
   Graph g;
   g.ReadGraph(graph);
@@ -176,8 +204,10 @@ static int run_h2h(const char *graph, const char *order, const char *pairs_path)
 
 int main(int argc, char **argv) {
   if (argc >= 3 && std::string(argv[1]) == "tc") return run_tc(argv[2], argc > 3 ? argv[3] : nullptr);
+  if (argc >= 3 && std::string(argv[1]) == "tcfull") return run_tcfull(argv[2]);
   if (argc >= 4 && std::string(argv[1]) == "h2h")
     return run_h2h(argv[2], argv[3], argc > 4 ? argv[4] : nullptr);
-  std::fprintf(stderr, "usage: tc_ref tc <graph> [pairs] | tc_ref h2h <graph> <order> [pairs]\n");
+  std::fprintf(stderr, "usage: tc_ref tc <graph> [pairs] | tc_ref tcfull <graph> | "
+                       "tc_ref h2h <graph> <order> [pairs]\n");
   return 2;
 }

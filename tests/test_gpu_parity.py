@@ -386,3 +386,228 @@ def test_sweep_overflow_flag():
     G, ix, ctx = build(g)
     _, ovf = ctx.bc_sources(np.array([0], np.int32), bcx.MODE_SWEEP)
     assert ovf
+
+
+# ---- distance / count rows of every form, bit-exact -------------------------
+
+def _rows_match(g, csr, got, sources, hop=True):
+    d, s, h, _ = got
+    for i, src in enumerate(np.asarray(sources).tolist()):
+        od, osg, ohp, _ = oracle.sssp(csr, src)
+        assert np.array_equal(d[i].astype(np.int64), od), f"distances from {src}"
+        assert np.array_equal(s[i], osg), f"counts from {src}"
+        if hop and h is not None:
+            assert np.array_equal(h[i], ohp), f"layers from {src}"
+
+
+SWEEP_FORMS = [bcx.SWEEP_FORM_ONE_PER_CU, bcx.SWEEP_FORM_TWO_PER_CU]
+
+
+def test_rows_every_form_grid_all_sources(grid):
+    """The label-scan tile kernel and the bag sweep's forward pass (both
+    tile-kernel forms) give the oracle's distances and counts from every
+    source of the grid, bit for bit (and the label scan its layers)."""
+    g, csr, ctx = grid
+    src = np.arange(g.n, dtype=np.int32)
+    ctx.set_tuning(batch=256)
+    _rows_match(g, csr, ctx.query_rows(src, bcx.MODE_BATCHED), src)
+    for form in SWEEP_FORMS:
+        ctx.set_tuning(sweep_form=form)
+        _rows_match(g, csr, ctx.query_rows(src[::-1].copy(), bcx.MODE_SWEEP), src[::-1])
+    ctx.set_tuning(sweep_form=bcx.SWEEP_FORM_AUTO, batch=4096)
+
+
+def test_rows_sweep_needs_no_hop(grid):
+    g, csr, ctx = grid
+    d = np.zeros(g.n, np.int32)
+    s = np.zeros(g.n, np.uint64)
+    h = np.zeros(g.n, np.uint32)
+    rc = bcx.lib().bcx_query_dist_count_mode(ctx._h, bcx.MODE_SWEEP, np.array([0], np.int32), 1, d, s,
+                                             h.ctypes.data)
+    assert rc == -1
+
+
+@pytest.mark.parametrize("k", [34, 35])
+def test_rows_unit_grid_saturation_every_form(k):
+    """C(2k-2, k-1) paths corner to corner: fits uint64 at k = 34, saturates
+    (flagged) at k = 35, identically in every form and in the oracle."""
+    from math import comb
+
+    g = gg.unit_grid(k)
+    G, ix, ctx = build(g)
+    csr = oracle.CSR.from_arcgraph(g)
+    src = np.array([0, k - 1, k * k - 1, k * (k // 2) + k // 2], np.int32)
+    od, osg, _, oovf = oracle.sssp(csr, 0)
+    assert oovf == (k == 35)
+    want = comb(2 * k - 2, k - 1)
+    runs = [(bcx.MODE_BATCHED, None)] + [(bcx.MODE_SWEEP, f) for f in SWEEP_FORMS]
+    for mode, form in runs:
+        if form is not None:
+            ctx.set_tuning(sweep_form=form)
+        d, s, h, ovf = ctx.query_rows(src, mode)
+        assert ovf == (k == 35), f"mode {mode} form {form}"
+        assert int(s[0, k * k - 1]) == (want if k == 34 else 2**64 - 1)
+        _rows_match(g, csr, (d, s, h, ovf), src)
+
+
+def test_rows_road_every_form(road):
+    g, csr, ctx = road
+    src = np.concatenate([np.arange(200, 264), gg.permutation(31, 1, g.n)[:16]]).astype(np.int32)
+    ctx.set_tuning(batch=64)
+    _rows_match(g, csr, ctx.query_rows(src, bcx.MODE_BATCHED), src)
+    for form in SWEEP_FORMS:
+        ctx.set_tuning(sweep_form=form)
+        _rows_match(g, csr, ctx.query_rows(src, bcx.MODE_SWEEP), src)
+    ctx.set_tuning(sweep_form=bcx.SWEEP_FORM_AUTO, batch=4096)
+
+
+def test_rows_la_golden_pairs_every_form():
+    """On the reference's LosAngeles graph, the rows of the label-scan tile
+    kernel and of the bag sweep's forward pass equal the answers of the
+    reference's own index code on its golden pairs."""
+    golden = ROOT / "tests" / "golden"
+    la = np.load(golden / "la_graph.npz")
+    ref = np.load(golden / "la_tc.npz")
+    n = int(la["n"])
+    G = bcx.Graph.from_arcs(n, la["u"], la["v"], la["w"])
+    ix = bcx.Index.build_mhl(G)
+    ctx = bcx.Context(G, ix, 0)
+    q = ref["queries"].astype(np.int64)
+    srcs = np.unique(q[:, 0])
+    src0 = (srcs - 1).astype(np.int32)
+    runs = [(bcx.MODE_BATCHED, None)] + [(bcx.MODE_SWEEP, f) for f in SWEEP_FORMS]
+    for mode, form in runs:
+        if form is not None:
+            ctx.set_tuning(sweep_form=form)
+        d, s, h, _ = ctx.query_rows(src0, mode)
+        for i, sv in enumerate(srcs.tolist()):
+            rows = q[q[:, 0] == sv]
+            t = rows[:, 1] - 1
+            assert np.array_equal(d[i, t].astype(np.int64), rows[:, 2]), f"distances from {sv}, mode {mode}"
+            assert np.array_equal(s[i, t].astype(np.int64), rows[:, 3]), f"counts from {sv}, mode {mode}"
+            if h is not None:
+                assert np.array_equal(h[i, t].astype(np.int64), rows[:, 4]), f"layers from {sv}"
+    ctx.close()
+
+
+# ---- unpack, memory-capped launches, captured jobs ------------------------------
+
+@pytest.mark.parametrize("maker", [
+    lambda: gg.make("grid1k"),
+    lambda: gg.road_graph(3000, 3750, seed=21),
+])
+def test_sweep_wide_unpack_path(maker):
+    """Every depth level through the grid-wide unpack launches (threshold 1)
+    against the oracle, and against the default split within rounding."""
+    g = maker()
+    G, ix, ctx = build(g)
+    ref, _ = oracle.brandes(oracle.CSR.from_arcgraph(g), threads=oracle.default_threads())
+    base, _ = ctx.full_bc(bcx.MODE_SWEEP)
+    ctx.set_tuning(unpack_wide=1)
+    wide, ovf = ctx.full_bc(bcx.MODE_SWEEP)
+    assert not ovf
+    assert close_bc(wide, ref)
+    assert np.all(np.abs(wide - base) <= 1e-12 * np.maximum(np.abs(base), 1.0))
+
+
+def test_sweep_memory_capped_launches(road):
+    """A launch cut short by the memory cap: the label-scan workspace is
+    released first, the launch rounds down to whole tiles per compute unit,
+    and the job runs as several launches; BC still matches the oracle."""
+    g, csr, _ = road
+    G, ix, ctx = build(g)  # library default tuning (the shared context's is changed by other tests)
+    default = ctx.tuning()["sweep_batch"]
+    cus = default // (3 * 64)
+    tiles = (g.n + 63) // 64
+    if tiles <= cus + 8:
+        pytest.skip("needs more tiles than compute units")
+    ctx.set_tuning(batch=256)
+    ctx.bc_sources(np.arange(256, dtype=np.int32), bcx.MODE_BATCHED)  # holds the workspace
+    want_cap = (cus + tiles) // 2  # between the compute units and the job's tiles
+    ctx.set_tuning(sweep_batch=default, sweep_mem_limit=want_cap * (g.n + 1) * 768)
+    ctx.reset_timing()
+    capped, ovf = ctx.full_bc(bcx.MODE_SWEEP)
+    launches = ctx.timing()["scan_launches"]
+    ctx.set_tuning(sweep_mem_limit=0)
+    ctx.reset_timing()
+    whole, _ = ctx.full_bc(bcx.MODE_SWEEP)
+    assert ctx.timing()["scan_launches"] == 1
+    assert launches == -(-tiles // cus), f"{launches} launches for {tiles} tiles"
+    ref, _ = oracle.brandes(csr, threads=oracle.default_threads())
+    assert not ovf
+    assert close_bc(capped, ref)
+    assert close_bc(whole, ref)
+    ctx.close()
+
+
+def test_sweep_captured_job_replays(road):
+    """The captured launch sequence of a job, replayed several times with
+    eager (profiled, uncaptured) jobs in between, gives the eager result bit
+    for bit every time."""
+    g, csr, ctx = road
+    src = np.arange(0, g.n, 3, dtype=np.int32)
+    ctx.set_profiling(True)
+    eager, _ = ctx.bc_sources(src, bcx.MODE_SWEEP)
+    ctx.set_profiling(False)
+    for rep in range(4):
+        got, _ = ctx.bc_sources(src, bcx.MODE_SWEEP)
+        assert np.array_equal(got.view(np.uint64), eager.view(np.uint64)), f"replay {rep}"
+        ctx.set_profiling(True)
+        other, _ = ctx.bc_sources(src[:777], bcx.MODE_SWEEP)
+        ctx.set_profiling(False)
+    ref, _ = oracle.brandes(csr, src, threads=oracle.default_threads())
+    assert close_bc(eager, ref)
+
+
+# ---- RQ admission ----------------------------------------------------------------
+
+@pytest.mark.parametrize("mode", [bcx.MODE_SWEEP, bcx.MODE_BATCHED])
+def test_rq_admission_busy_readmit_cancel(grid, mode):
+    """A budget that holds one request at a time: the second is refused with
+    BCX_EBUSY, admitted once the first completes; a request cancelled in the
+    middle of its cycle returns its bytes; results match the oracle and the
+    request alone, bit for bit."""
+    g, csr, ctx = grid
+    ref, _ = oracle.brandes(csr)
+    saved = ctx.tuning()
+    ctx.set_tuning(batch=128, sweep_batch=128)
+    a = np.array([3, 9, 400, 1000], np.int32)
+    b = np.arange(0, 1024, 5, dtype=np.int32)
+    probe = bcx.RQServer(ctx, 1 << 30, mode)
+    fixed, _ = probe.memory()
+    need_b = probe.bytes_needed(len(b))
+    probe.close()
+    srv = bcx.RQServer(ctx, fixed + need_b, mode)  # room for b alone, or a alone
+    with pytest.raises(bcx.BcxError):
+        bcx.RQServer(ctx, fixed - 1, mode)
+    ra = srv.submit(a)
+    assert ra is not None
+    assert srv.submit(b) is None, "b must not fit while a is in flight"
+    res_a = None
+    steps = 0
+    while res_a is None:
+        srv.step()
+        steps += 1
+        res_a = srv.poll(ra)
+    assert srv.memory()[0] == fixed
+    rb = srv.submit(b)
+    assert rb is not None
+    srv.step()
+    srv.step()
+    srv.cancel(rb)
+    assert srv.memory()[0] == fixed
+    rb = srv.submit(b)
+    res_b = None
+    while res_b is None:
+        srv.step()
+        res_b = srv.poll(rb)
+    assert close_bc(res_a, ref[a])
+    assert close_bc(res_b, ref[b])
+    alone = bcx.RQServer(ctx, 1 << 30, mode)
+    rid = alone.submit(a)
+    got = None
+    while got is None:
+        alone.step()
+        got = alone.poll(rid)
+    assert np.array_equal(got.view(np.uint64), res_a.view(np.uint64))
+    ctx.set_tuning(**saved)
