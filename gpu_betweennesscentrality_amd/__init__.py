@@ -102,6 +102,7 @@ _SIGS = {
     "bcx_full_bc": (C.c_int, [_P, C.c_int, _F64P, C.POINTER(C.c_uint32)]),
     "bcx_bc_sources": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, _F64P, C.POINTER(C.c_uint32)]),
     "bcx_query_dist_count": (C.c_int, [_P, C.c_int32, _I32P, _U64P, _U32P]),
+    "bcx_query_dist_count_noalloc": (C.c_int, [_P, C.c_int32, _I32P, _U64P, _U32P]),
     "bcx_query_dist_count_mode": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, _I32P, _U64P, C.c_void_p]),
     "bcx_subset_bc": (C.c_int, [_P, _I32P, C.c_int32, _F64P]),
     "bcx_h2h_distances": (C.c_int, [_P, _I32P, C.c_int32, _I32P]),
@@ -353,6 +354,15 @@ class Context:
         s = np.zeros(self.n, np.uint64)
         h = np.zeros(self.n, np.uint32)
         rc = _check(lib().bcx_query_dist_count(self._h, source, d, s, h))
+        return d, s, h, rc == 1
+
+    def query_noalloc(self, source: int) -> tuple[np.ndarray, np.ndarray, np.ndarray, bool]:
+        """query() through the allocation-free entry point (library-held
+        staging, graphs up to 1,572,864 vertices)."""
+        d = np.zeros(self.n, np.int32)
+        s = np.zeros(self.n, np.uint64)
+        h = np.zeros(self.n, np.uint32)
+        rc = _check(lib().bcx_query_dist_count_noalloc(self._h, source, d, s, h))
         return d, s, h, rc == 1
 
     def query_rows(self, sources, mode: int = MODE_SWEEP):

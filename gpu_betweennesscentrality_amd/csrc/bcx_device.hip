@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -112,6 +113,19 @@ __global__ __launch_bounds__(256) void h2h_to_vertex_ids_kernel(const int *in, c
   if (v >= n) return;
   const int64_t r = (int64_t)blockIdx.y * n;
   out[r + v] = in[r + iperm[v]];
+}
+
+// Rows from preorder ids to vertex ids: d_out[v] = d_in[iperm[v]], and the
+// same for counts and hops.
+__global__ __launch_bounds__(256) void rows_to_vertex_ids_kernel(const int *iperm, int n, const int *d_in,
+                                                                 const uint64_t *s_in, const uint32_t *h_in,
+                                                                 int *d_out, uint64_t *s_out, uint32_t *h_out) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int p = iperm[v];
+  d_out[v] = d_in[p];
+  s_out[v] = s_in[p];
+  h_out[v] = h_in[p];
 }
 
 // min / count-sum / hop-max over entries 0..L of two rows (one thread).
@@ -621,7 +635,6 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     }
     c->d_gdis = (const int *)keep(dev_upload(gt));
     c->d_goff = (const int64_t *)keep(dev_upload(goff));
-    c->d_iperm = (const int *)keep(dev_upload(ix.iperm));
     c->d_sep_off = (const int64_t *)keep(dev_upload(ix.sep_off));
     c->d_sep = (const int *)keep(dev_upload(ix.sep));
   }
@@ -967,6 +980,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
   c->d_cursor = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
   c->d_total = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
   c->d_ovf = (int *)keep(dev_upload(std::vector<int32_t>(4, 0)));
+  c->d_iperm = (const int *)keep(dev_upload(ix.iperm));
   c->d_bc = (double *)keep(dev_upload(std::vector<double>((size_t)n, 0.0)));
   *out = c.release();
   return BCX_OK;
@@ -1246,6 +1260,81 @@ int bcx_subset_bc(bcx_ctx *c, const int32_t *points, int32_t q, double *bc_out) 
 }
 
 }  // extern "C"
+
+// ---- allocation-free probe ------------------------------------------------
+// bcx_query_dist_count_noalloc keeps every device buffer it needs beyond the
+// context's index in `scratch`, a file-scope device array (one copy per GPU,
+// held until the process ends):
+//   [0, 256)                     source list (1 entry), cursor, total, overflow flag
+//   then, for kNoAllocMaxN vertices each: distance, count and hop rows in
+//   preorder ids (the plain scan's output), and the same rows in vertex ids.
+// Host code never passes `scratch` itself to a kernel or copy (there it names
+// a host-side stand-in): it uses the device address hipGetSymbolAddress
+// returns, looked up once per device with that device current.
+constexpr int64_t kNoAllocMaxN = 1572864;
+constexpr size_t kNoAllocHead = 256;
+constexpr size_t kNoAllocBytes = kNoAllocHead + 2 * (size_t)kNoAllocMaxN * (4 + 8 + 4);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of this device's scratch array (the device must be current).
+char *scratch_of(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices)
+    throw bcx::Error{BCX_EINVAL, "device id beyond the allocation-free probe's table"};
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw bcx::Error{BCX_EDEVICE, "hipGetSymbolAddress returned no address for the scratch array"};
+    g_scratch_ptr[device] = (char *)p;
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
+extern "C" int bcx_query_dist_count_noalloc(bcx_ctx *c, int32_t source, int32_t *d, uint64_t *sigma,
+                                            uint32_t *hop) {
+  BCX_GUARD_BEGIN
+  if (!c || !d || !sigma || !hop) return bcx::fail(BCX_EINVAL, "null argument");
+  const int n = c->ix.n;
+  if (n > kNoAllocMaxN)
+    return bcx::fail(BCX_EINVAL, "graph has more than 1572864 vertices, the allocation-free probe's maximum");
+  if (source < 0 || source >= n) return bcx::fail(BCX_EINVAL, "source out of range");
+  HIP_OK(hipSetDevice(c->device));
+  char *base = scratch_of(c->device);
+  int *head = (int *)base;  // source, cursor, total, overflow flag
+  const size_t N = (size_t)kNoAllocMaxN;
+  char *rows = base + kNoAllocHead;
+  DevWork ws{};
+  ws.d = (int *)rows;
+  ws.sigma = (uint64_t *)(rows + 4 * N);
+  ws.hop = (uint32_t *)(rows + 12 * N);
+  char *out = rows + 16 * N;
+  int *d_v = (int *)out;
+  uint64_t *s_v = (uint64_t *)(out + 4 * N);
+  uint32_t *h_v = (uint32_t *)(out + 12 * N);
+  // head[0] = source (tree id), head[1] = cursor 0, head[2] = total 1, head[3] = overflow 0;
+  // written from the host, so the flag starts at zero on every call.
+  const int32_t init[4] = {c->iperm[(size_t)source], 0, 1, 0};
+  HIP_OK(hipMemcpyAsync(head, init, sizeof init, hipMemcpyHostToDevice, c->stream));
+  const DevBatch bt{head, head + 1, head + 2};
+  scan_plain_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(c->ix, bt, ws, head + 3);
+  rows_to_vertex_ids_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(c->d_iperm, n, ws.d, ws.sigma, ws.hop,
+                                                                    d_v, s_v, h_v);
+  HIP_OK(hipGetLastError());
+  int32_t ovf = 0;
+  HIP_OK(hipMemcpyAsync(d, d_v, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(sigma, s_v, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(hop, h_v, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(&ovf, head + 3, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return ovf ? 1 : BCX_OK;
+  BCX_GUARD_END
+}
 
 // ---- RQ serving engine ------------------------------------------------------
 // One cyclic pass over all sources (tree order, in batches) serves every

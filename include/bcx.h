@@ -227,6 +227,16 @@ int bcx_query_dist_count(bcx_ctx *ctx, int32_t source, int32_t *d,
 int bcx_query_dist_count_mode(bcx_ctx *ctx, int mode, const int32_t *sources,
                               int32_t count, int32_t *d, uint64_t *sigma,
                               uint32_t *hop);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same rows as bcx_query_dist_count (all three outputs
+ * required), computed by the same kernel, with no allocation of any kind.
+ * Its device staging lives in a fixed array of the library (48 MiB per GPU,
+ * held until the process ends) shared by every call on that GPU: graphs of
+ * more than 1,572,864 vertices are refused with BCX_EINVAL, and calls that
+ * overlap on one GPU (other threads, other contexts) race. */
+int bcx_query_dist_count_noalloc(bcx_ctx *ctx, int32_t source, int32_t *d,
+                                 uint64_t *sigma, uint32_t *hop);
 /* QBC distance-only scan (replaces the distance part of BC_Kernel,
  * QBC.cu:243-331: LCA by RMQ, then the ancestor lookup or the minimum over
  * the LCA's separator of the two H2H labels): dist_out[i * n + v] = d(sources[i],
