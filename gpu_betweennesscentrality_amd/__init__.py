@@ -61,6 +61,7 @@ class Timing(C.Structure):
 
 _P = C.c_void_p
 _I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_I64P = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _U32P = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _U64P = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 _F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -88,6 +89,9 @@ _SIGS = {
     "bcx_index_save": (C.c_int, [_P, C.c_char_p]),
     "bcx_index_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "bcx_index_free": (None, [_P]),
+    "bcx_index_from_arrays": (C.c_int, [C.c_int32, _I32P, _I64P, _I32P, _U64P, _U32P, _I64P, _I32P, _I32P,
+                                        _U64P, _U32P, _P, C.POINTER(_P)]),
+    "bcx_index_export": (C.c_int, [_P, _I32P, _I64P, _I32P, _U64P, _U32P, _I64P, _I32P, _I32P, _U64P, _U32P]),
     "bcx_ctx_create": (C.c_int, [C.c_int, _P, _P, C.POINTER(_P)]),
     "bcx_ctx_free": (None, [_P]),
     "bcx_ctx_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
@@ -226,6 +230,31 @@ class Index:
         h = _P()
         _check(lib().bcx_index_load(str(path).encode(), C.byref(h)))
         return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, a: dict, graph: "Graph | None" = None) -> "Index":
+        """Index from the reference's host structure (keys as export())."""
+        n = len(a["parent"])
+        args = [np.ascontiguousarray(a[k], dt) for k, dt in (
+            ("parent", np.int32), ("label_off", np.int64), ("dis", np.int32), ("cnt", np.uint64),
+            ("hop", np.uint32), ("bag_off", np.int64), ("bag_v", np.int32), ("bag_len", np.int32),
+            ("bag_cnt", np.uint64), ("bag_hop", np.uint32))]
+        h = _P()
+        _check(lib().bcx_index_from_arrays(n, *args, graph._h if graph is not None else None, C.byref(h)))
+        return cls(h.value)
+
+    def export(self) -> dict:
+        """Tree, label rows over ancestors and bags, indexed by vertex id."""
+        st = self.stats()
+        n, E, B = st["n"], st["label_entries"] - st["n"], st["bag_entries"]
+        a = {"parent": np.zeros(n, np.int32), "label_off": np.zeros(n + 1, np.int64),
+             "dis": np.zeros(E, np.int32), "cnt": np.zeros(E, np.uint64), "hop": np.zeros(E, np.uint32),
+             "bag_off": np.zeros(n + 1, np.int64), "bag_v": np.zeros(B, np.int32),
+             "bag_len": np.zeros(B, np.int32), "bag_cnt": np.zeros(B, np.uint64),
+             "bag_hop": np.zeros(B, np.uint32)}
+        _check(lib().bcx_index_export(self._h, a["parent"], a["label_off"], a["dis"], a["cnt"], a["hop"],
+                                      a["bag_off"], a["bag_v"], a["bag_len"], a["bag_cnt"], a["bag_hop"]))
+        return a
 
     def save(self, path) -> None:
         _check(lib().bcx_index_save(self._h, str(path).encode()))

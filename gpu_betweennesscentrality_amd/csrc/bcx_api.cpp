@@ -233,6 +233,41 @@ int bcx_index_load(const char *path, bcx_index **out) {
   BCX_GUARD_END
 }
 
+int bcx_index_from_arrays(int32_t n, const int32_t *parent, const int64_t *label_off, const int32_t *dis,
+                          const uint64_t *cnt, const uint32_t *hop, const int64_t *bag_off, const int32_t *bag_v,
+                          const int32_t *bag_len, const uint64_t *bag_cnt, const uint32_t *bag_hop,
+                          const bcx_graph *g, bcx_index **out) {
+  BCX_GUARD_BEGIN
+  if (!parent || !label_off || !dis || !cnt || !hop || !bag_off || !bag_v || !bag_len || !bag_cnt || !bag_hop ||
+      !out)
+    return bcx::fail(BCX_EINVAL, "null argument");
+  if (g && g->g.n != n) return bcx::fail(BCX_EINVAL, "graph and arrays have different vertex counts");
+  auto *h = new bcx_index();
+  try {
+    bcx::index_from_arrays(n, parent, label_off, dis, cnt, hop, bag_off, bag_v, bag_len, bag_cnt, bag_hop, h->ix);
+    if (g) {
+      h->ix.fp_arcs = g->g.arcs();
+      h->ix.fp_hash = bcx::graph_fingerprint(g->g);
+    }
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_index_export(const bcx_index *ix, int32_t *parent, int64_t *label_off, int32_t *dis, uint64_t *cnt,
+                     uint32_t *hop, int64_t *bag_off, int32_t *bag_v, int32_t *bag_len, uint64_t *bag_cnt,
+                     uint32_t *bag_hop) {
+  BCX_GUARD_BEGIN
+  if (!ix) return bcx::fail(BCX_EINVAL, "null index");
+  bcx::index_export(ix->ix, parent, label_off, dis, cnt, hop, bag_off, bag_v, bag_len, bag_cnt, bag_hop);
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
 void bcx_index_free(bcx_index *ix) { delete ix; }
 
 }  // extern "C"

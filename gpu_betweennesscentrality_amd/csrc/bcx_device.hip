@@ -544,6 +544,8 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
   const bcx::Graph &g = bcx::graph_of(gh);
   const bcx::Index &ix = bcx::index_of(ih);
   if (g.n != ix.n) return bcx::fail(BCX_EINVAL, "graph and index sizes differ");
+  if (ix.fp_hash != 0 && (ix.fp_arcs != g.arcs() || ix.fp_hash != bcx::graph_fingerprint(g)))
+    return bcx::fail(BCX_EINVAL, "the index was built for another graph (fingerprints differ)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return bcx::fail(BCX_EDEVICE, "no HIP device available");
@@ -576,7 +578,9 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     int32_t o = grow[(size_t)p];
     for (int64_t a = g.row[(size_t)v]; a < g.row[(size_t)v + 1]; ++a, ++o) {
       gcol[(size_t)o] = ix.iperm[(size_t)g.col[(size_t)a]];
-      gw[(size_t)o] = g.w[(size_t)a];
+      // An arc of kInf or more is on no shortest path (every distance is
+      // below kInf); the clamp keeps distance + weight in int32.
+      gw[(size_t)o] = std::min(g.w[(size_t)a], bcx::kInf);
     }
   }
   int levels = 1;

@@ -54,6 +54,10 @@ struct Index {
   std::vector<int32_t> gdis;
   std::vector<int64_t> sep_off;  // n + 1 (empty until build_h2h)
   std::vector<int32_t> sep;      // ancestor depths, the node's own depth last
+  // Fingerprint of the graph the index was built for (arc count and an
+  // order-independent hash of its arcs; 0 when unknown).
+  int64_t fp_arcs = 0;
+  uint64_t fp_hash = 0;
   int32_t tree_height = 0;  // max depth + 1
   int32_t tree_width = 0;   // max bag size
   int32_t count_overflow = 0;
@@ -72,6 +76,8 @@ void read_graph_file(const char *path, Graph &g);
 void graph_from_arcs(int32_t n, int64_t arcs, const int32_t *u, const int32_t *v,
                      const int32_t *w, Graph &g);
 bool graph_connected(const Graph &g);
+// Order-independent hash of the arcs (u, v, w); never 0.
+uint64_t graph_fingerprint(const Graph &g);
 void read_order_file(const char *path, int32_t n, std::vector<int32_t> &order);
 
 // index_build.cpp
@@ -94,5 +100,17 @@ void build_h2h(Index &ix);
 int32_t h2h_query(const Index &ix, int32_t s, int32_t t);
 void save_index(const Index &ix, const char *path);
 void load_index(const char *path, Index &ix);
+// Throws BCX_EGRAPH when shortest distances may reach kInf.
+void check_distance_bound(const Index &ix);
+// Validates an index from outside; derives iperm and subtree.
+void validate_index(Index &ix, const std::string &what);
+// The reference's host index structure, indexed by vertex id (see
+// index_build.cpp), in and out.
+void index_from_arrays(int32_t n, const int32_t *parent, const int64_t *label_off, const int32_t *dis,
+                       const uint64_t *cnt, const uint32_t *hop, const int64_t *bag_off, const int32_t *bag_v,
+                       const int32_t *bag_len, const uint64_t *bag_cnt, const uint32_t *bag_hop, Index &ix);
+void index_export(const Index &ix, int32_t *parent, int64_t *label_off, int32_t *dis, uint64_t *cnt,
+                  uint32_t *hop, int64_t *bag_off, int32_t *bag_v, int32_t *bag_len, uint64_t *bag_cnt,
+                  uint32_t *bag_hop);
 
 }  // namespace bcx

@@ -53,6 +53,21 @@ inline int apply_env_tuning(bcx_ctx *ctx) {
   return BCX_MODE_SWEEP;
 }
 
+// The MHL index: loaded from an image when BCX_INDEX_LOAD names one (a
+// restart without rebuilding; the context refuses an image of another
+// graph), else built; saved when BCX_INDEX_SAVE names a path.
+inline bcx_index *mhl_index(const bcx_graph *g) {
+  bcx_index *ix = nullptr;
+  const char *load = std::getenv("BCX_INDEX_LOAD");
+  if (load && *load)
+    check(bcx_index_load(load, &ix), "loading index");
+  else
+    check(bcx_index_build_mhl(g, &ix), "building index");
+  const char *save = std::getenv("BCX_INDEX_SAVE");
+  if (save && *save) check(bcx_index_save(ix, save), "saving index");
+  return ix;
+}
+
 // "vertexID  BCValue" then "id value" with 2 decimals (QBC.cu:614-626).
 inline void write_bc_file(const std::string &path, const std::vector<double> &bc) {
   std::ofstream ofs;

@@ -20,10 +20,7 @@ int main(int argc, char *argv[]) {
 
   auto t0 = std::chrono::high_resolution_clock::now();
   std::cout << "Read Weighted Graph_ OK!\nn = " << n << ", m = " << m << std::endl;
-  bcx_index *ix = nullptr;
-  cli::check(bcx_index_build_mhl(g, &ix), "building index");
-  if (const char *dump = std::getenv("BCX_INDEX_SAVE"))
-    cli::check(bcx_index_save(ix, dump), "saving index");
+  bcx_index *ix = cli::mhl_index(g);
   bcx_ctx *ctx = nullptr;
   cli::check(bcx_ctx_create(device, g, ix, &ctx), "creating device context");
   const int mode = cli::apply_env_tuning(ctx);

@@ -111,13 +111,8 @@ void graph_from_arcs(int32_t n, int64_t m, const int32_t *u, const int32_t *v,
       throw Error{BCX_EGRAPH, "edge " + std::to_string(u[a] + 1) + " " + std::to_string(v[a] + 1) +
                                   " has different weights in its two directions"};
   }
-  // Every path must stay below the label INF (999999999) so that INF sums in
-  // the int32 label arithmetic never meet a real distance.
-  int64_t total = 0;
-  for (int64_t a : uniq)
-    if (u[a] < v[a]) total += w[a];
-  if (total >= kInf)
-    throw Error{BCX_EGRAPH, "sum of edge weights reaches the label INF 999999999"};
+  // Shortest distances must stay below the label INF (999999999); the index
+  // build checks that bound on the distances themselves.
   g.n = n;
   g.row.assign((size_t)n + 1, 0);
   for (int64_t i = 0; i < m; ++i)
@@ -155,6 +150,19 @@ void read_graph_file(const char *path, Graph &g) {
     w[(size_t)i] = (int32_t)c;
   }
   graph_from_arcs((int32_t)n, m, u.data(), v.data(), w.data(), g);
+}
+
+uint64_t graph_fingerprint(const Graph &g) {
+  uint64_t h = 0;
+  for (int32_t x = 0; x < g.n; ++x)
+    for (int64_t k = g.row[(size_t)x]; k < g.row[(size_t)x + 1]; ++k) {
+      uint64_t z = ((uint64_t)(uint32_t)x << 32 | (uint32_t)g.col[(size_t)k]) ^
+                   ((uint64_t)(uint32_t)g.w[(size_t)k] * 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;  // splitmix64 finaliser
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      h += z ^ (z >> 31);
+    }
+  return h ? h : 1;
 }
 
 bool graph_connected(const Graph &g) {

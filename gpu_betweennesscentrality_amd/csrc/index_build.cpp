@@ -80,7 +80,7 @@ void eliminate(const Graph &g, const std::vector<int32_t> &order,
   for (int32_t x = 0; x < n; ++x) {
     auto &a = adj[(size_t)x];
     for (int64_t k = g.row[(size_t)x]; k < g.row[(size_t)x + 1]; ++k)
-      a.push_back(Arc{g.col[(size_t)k], g.w[(size_t)k], 1, 1});
+      a.push_back(Arc{g.col[(size_t)k], std::min(g.w[(size_t)k], kInf), 1, 1});
     std::sort(a.begin(), a.end(), [](const Arc &p, const Arc &q) { return p.nb < q.nb; });
   }
   elim.clear();
@@ -142,7 +142,10 @@ void eliminate(const Graph &g, const std::vector<int32_t> &order,
     for (size_t i = 0; i < nb.size(); ++i) {
       for (size_t j = i + 1; j < nb.size(); ++j) {
         const Arc &a = nb[i], &b = nb[j];
-        const int32_t L = a.len + b.len;  // < kInf: total weight is checked at load
+        // Lengths are kept at most kInf: a shortcut that long is on no
+        // shortest path (every distance stays below kInf, checked after the
+        // labels), and the clamp keeps every sum of two lengths in int32.
+        const int32_t L = (int32_t)std::min<int64_t>((int64_t)a.len + b.len, kInf);
         const uint64_t c = mul_sat(a.cnt, b.cnt, ovf);
         const uint32_t h = a.hop + b.hop;
         auto &au = adj[(size_t)a.nb];
@@ -183,6 +186,20 @@ void eliminate(const Graph &g, const std::vector<int32_t> &order,
 }
 
 }  // namespace
+
+// Every shortest distance must stay below the label INF (999999999), which
+// also keeps it below 2^30 for the bag sweep's flag bit.  Entry 0 of a row is
+// the label towards the root, whose restriction (vertices eliminated before
+// the root) is no restriction: the true distance d(p, root).  Since
+// d(s, t) <= d(s, root) + d(root, t), twice the largest such entry bounds every
+// distance; graphs where it reaches the INF are refused.
+void check_distance_bound(const Index &ix) {
+  int64_t far = 0;
+  for (int32_t p = 1; p < ix.n; ++p) far = std::max<int64_t>(far, ix.dis[(size_t)ix.row_off[(size_t)p]]);
+  if (2 * far >= kInf)
+    throw Error{BCX_EGRAPH, "shortest distances may reach the label INF 999999999 (farthest vertex from the "
+                            "tree root at " + std::to_string(far) + ")"};
+}
 
 void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix) {
   auto t0 = std::chrono::steady_clock::now();
@@ -336,6 +353,9 @@ void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix) {
     }
   }
   ix.count_overflow = ovf_all;
+  check_distance_bound(ix);
+  ix.fp_arcs = g.arcs();
+  ix.fp_hash = graph_fingerprint(g);
   ix.build_seconds =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -587,7 +607,7 @@ int32_t h2h_query(const Index &ix, int32_t s, int32_t t) {
 
 // ---- binary image ---------------------------------------------------------
 namespace {
-const char kMagic[8] = {'B', 'C', 'X', 'I', 'D', 'X', '0', '2'};
+const char kMagic[8] = {'B', 'C', 'X', 'I', 'D', 'X', '0', '3'};
 template <class T>
 void wvec(FILE *f, const std::vector<T> &v) {
   int64_t k = (int64_t)v.size();
@@ -596,9 +616,9 @@ void wvec(FILE *f, const std::vector<T> &v) {
     throw Error{BCX_EIO, "index write failed"};
 }
 template <class T>
-void rvec(FILE *f, std::vector<T> &v) {
+void rvec(FILE *f, std::vector<T> &v, int64_t max_len) {
   int64_t k;
-  if (std::fread(&k, sizeof k, 1, f) != 1 || k < 0 || k > ((int64_t)1 << 40))
+  if (std::fread(&k, sizeof k, 1, f) != 1 || k < 0 || k > max_len)
     throw Error{BCX_EIO, "index file corrupt"};
   v.resize((size_t)k);
   if (k && std::fread(v.data(), sizeof(T), (size_t)k, f) != (size_t)k)
@@ -606,12 +626,104 @@ void rvec(FILE *f, std::vector<T> &v) {
 }
 }  // namespace
 
+// Checks an index assembled from outside (a loaded image or caller arrays)
+// against every property the kernels and the host planning rely on, and
+// derives iperm and subtree.  Throws BCX_EIO naming `what` on the first
+// violation.
+void validate_index(Index &ix, const std::string &what) {
+  auto bad = [&](const std::string &m) { throw Error{BCX_EIO, what + ": " + m}; };
+  const int32_t n = ix.n;
+  if (n <= 0) bad("no vertices");
+  if ((int64_t)ix.perm.size() != n || (int64_t)ix.depth.size() != n || (int64_t)ix.parent.size() != n)
+    bad("tree arrays of the wrong length");
+  ix.iperm.assign((size_t)n, -1);
+  for (int32_t p = 0; p < n; ++p) {
+    const int32_t v = ix.perm[(size_t)p];
+    if (v < 0 || v >= n || ix.iperm[(size_t)v] != -1) bad("the preorder is not a permutation");
+    ix.iperm[(size_t)v] = p;
+  }
+  if (ix.parent[0] != -1 || ix.depth[0] != 0) bad("node 0 is not the root");
+  int32_t maxd = 0;
+  for (int32_t p = 1; p < n; ++p) {
+    const int32_t a = ix.parent[(size_t)p];
+    if (a < 0 || a >= p) bad("a parent does not precede its child in preorder");
+    if (ix.depth[(size_t)p] != ix.depth[(size_t)a] + 1) bad("a depth is not its parent's plus one");
+    maxd = std::max(maxd, ix.depth[(size_t)p]);
+  }
+  if (ix.tree_height != maxd + 1) bad("tree height is not the largest depth plus one");
+  ix.subtree.assign((size_t)n, 1);
+  for (int32_t p = n - 1; p > 0; --p) ix.subtree[(size_t)ix.parent[(size_t)p]] += ix.subtree[(size_t)p];
+  for (int32_t p = 1; p < n; ++p) {  // preorder: a subtree is a contiguous id range
+    const int32_t a = ix.parent[(size_t)p];
+    if (!(a < p && p + ix.subtree[(size_t)p] <= a + ix.subtree[(size_t)a])) bad("ids are not a preorder");
+  }
+  if ((int64_t)ix.row_off.size() != (int64_t)n + 1 || ix.row_off[0] != 0) bad("label offsets");
+  for (int32_t p = 0; p < n; ++p)
+    if (ix.row_off[(size_t)p + 1] - ix.row_off[(size_t)p] != (int64_t)ix.depth[(size_t)p] + 1)
+      bad("a label row's length is not its depth plus one");
+  const int64_t E = ix.row_off[(size_t)n];
+  if ((int64_t)ix.dis.size() != E || (int64_t)ix.cnt.size() != E || (int64_t)ix.hop.size() != E)
+    bad("label arrays of the wrong length");
+  for (int32_t p = 0; p < n; ++p) {
+    const int64_t o = ix.row_off[(size_t)p];
+    const int32_t D = ix.depth[(size_t)p];
+    if (ix.dis[(size_t)(o + D)] != 0 || ix.cnt[(size_t)(o + D)] != 1 || ix.hop[(size_t)(o + D)] != 0)
+      bad("a label row's own entry is not (0, 1, 0)");
+    for (int32_t j = 0; j < D; ++j) {
+      const int32_t d = ix.dis[(size_t)(o + j)];
+      if (d < 0 || d > kInf) bad("a label distance outside [0, 999999999]");
+      if (d == kInf && ix.cnt[(size_t)(o + j)] != 0) bad("an unreachable label entry with a count");
+    }
+  }
+  if ((int64_t)ix.bag_off.size() != (int64_t)n + 1 || ix.bag_off[0] != 0) bad("bag offsets");
+  for (int32_t p = 0; p < n; ++p)
+    if (ix.bag_off[(size_t)p + 1] < ix.bag_off[(size_t)p]) bad("bag offsets are not monotone");
+  const int64_t B = ix.bag_off[(size_t)n];
+  if ((int64_t)ix.bag.size() != B || (int64_t)ix.bag_len.size() != B || (int64_t)ix.bag_cnt.size() != B ||
+      (int64_t)ix.bag_hop.size() != B)
+    bad("bag arrays of the wrong length");
+  int32_t width = 0;
+  for (int32_t p = 0; p < n; ++p) {
+    const int64_t e0 = ix.bag_off[(size_t)p], e1 = ix.bag_off[(size_t)p + 1];
+    width = std::max(width, (int32_t)(e1 - e0));
+    if (p == 0 ? e1 != e0 : e1 == e0) bad(p == 0 ? "the root has a bag" : "a non-root node has an empty bag");
+    int32_t prev = -1;
+    for (int64_t e = e0; e < e1; ++e) {
+      const int32_t a = ix.bag[(size_t)e];
+      if (a < 0 || a >= p || p >= a + ix.subtree[(size_t)a]) bad("a bag entry is not an ancestor");
+      if (ix.depth[(size_t)a] <= prev) bad("bag entries are not in increasing depth");
+      prev = ix.depth[(size_t)a];
+      if (ix.bag_len[(size_t)e] < 1 || ix.bag_len[(size_t)e] > kInf) bad("a shortcut length outside [1, INF]");
+      if (ix.bag_cnt[(size_t)e] == 0) bad("a shortcut with no path");
+    }
+    if (p > 0 && ix.bag[(size_t)(e1 - 1)] != ix.parent[(size_t)p]) bad("a node's deepest bag entry is not its parent");
+  }
+  if (ix.tree_width != width) bad("tree width is not the largest bag");
+  if (!ix.sep_off.empty() || !ix.gdis.empty() || !ix.sep.empty()) {
+    if ((int64_t)ix.gdis.size() != E) bad("H2H labels of the wrong length");
+    if ((int64_t)ix.sep_off.size() != (int64_t)n + 1 || ix.sep_off[0] != 0) bad("separator offsets");
+    for (int32_t p = 0; p < n; ++p) {
+      if (ix.sep_off[(size_t)p + 1] <= ix.sep_off[(size_t)p]) bad("a node without its own separator entry");
+      for (int64_t i = ix.sep_off[(size_t)p]; i < ix.sep_off[(size_t)p + 1]; ++i)
+        if ((int64_t)ix.sep.size() <= i || ix.sep[(size_t)i] < 0 || ix.sep[(size_t)i] > ix.depth[(size_t)p])
+          bad("a separator depth out of range");
+    }
+    if ((int64_t)ix.sep.size() != ix.sep_off[(size_t)n]) bad("separator array of the wrong length");
+    for (int64_t i = 0; i < E; ++i)
+      if (ix.gdis[(size_t)i] < 0 || ix.gdis[(size_t)i] > kInf) bad("an H2H distance outside [0, INF]");
+  }
+  check_distance_bound(ix);
+}
+
 void save_index(const Index &ix, const char *path) {
   FILE *f = std::fopen(path, "wb");
   if (!f) throw Error{BCX_EIO, std::string("cannot write ") + path};
   try {
-    int32_t hdr[4] = {ix.n, ix.tree_height, ix.tree_width, ix.count_overflow};
-    if (std::fwrite(kMagic, 1, 8, f) != 8 || std::fwrite(hdr, sizeof hdr, 1, f) != 1)
+    const int32_t hdr[4] = {ix.n, ix.tree_height, ix.tree_width, ix.count_overflow};
+    const int64_t fp[2] = {ix.fp_arcs, (int64_t)ix.fp_hash};
+    const int32_t h2h[2] = {ix.sep_off.empty() ? 0 : 1, 0};
+    if (std::fwrite(kMagic, 1, 8, f) != 8 || std::fwrite(hdr, sizeof hdr, 1, f) != 1 ||
+        std::fwrite(fp, sizeof fp, 1, f) != 1 || std::fwrite(h2h, sizeof h2h, 1, f) != 1)
       throw Error{BCX_EIO, "index write failed"};
     wvec(f, ix.perm);
     wvec(f, ix.depth);
@@ -625,6 +737,11 @@ void save_index(const Index &ix, const char *path) {
     wvec(f, ix.bag_len);
     wvec(f, ix.bag_cnt);
     wvec(f, ix.bag_hop);
+    if (h2h[0]) {
+      wvec(f, ix.gdis);
+      wvec(f, ix.sep_off);
+      wvec(f, ix.sep);
+    }
   } catch (...) {
     std::fclose(f);
     throw;
@@ -638,50 +755,186 @@ void load_index(const char *path, Index &ix) {
   ix = Index();
   try {
     char m[8];
-    int32_t hdr[4];
+    int32_t hdr[4], h2h[2];
+    int64_t fp[2];
     if (std::fread(m, 1, 8, f) != 8 || std::memcmp(m, kMagic, 8) != 0 ||
-        std::fread(hdr, sizeof hdr, 1, f) != 1)
+        std::fread(hdr, sizeof hdr, 1, f) != 1 || std::fread(fp, sizeof fp, 1, f) != 1 ||
+        std::fread(h2h, sizeof h2h, 1, f) != 1)
       throw Error{BCX_EIO, std::string(path) + " is not a bcx index image"};
     ix.n = hdr[0];
     ix.tree_height = hdr[1];
     ix.tree_width = hdr[2];
     ix.count_overflow = hdr[3];
-    rvec(f, ix.perm);
-    rvec(f, ix.depth);
-    rvec(f, ix.parent);
-    rvec(f, ix.row_off);
-    rvec(f, ix.dis);
-    rvec(f, ix.cnt);
-    rvec(f, ix.hop);
-    rvec(f, ix.bag_off);
-    rvec(f, ix.bag);
-    rvec(f, ix.bag_len);
-    rvec(f, ix.bag_cnt);
-    rvec(f, ix.bag_hop);
+    ix.fp_arcs = fp[0];
+    ix.fp_hash = (uint64_t)fp[1];
+    if (ix.n <= 0 || ix.tree_height <= 0 || ix.tree_height > ix.n)
+      throw Error{BCX_EIO, std::string(path) + ": bad header"};
+    const int64_t n = ix.n, cap = (int64_t)1 << 40;
+    rvec(f, ix.perm, n);
+    rvec(f, ix.depth, n);
+    rvec(f, ix.parent, n);
+    rvec(f, ix.row_off, n + 1);
+    rvec(f, ix.dis, cap);
+    rvec(f, ix.cnt, cap);
+    rvec(f, ix.hop, cap);
+    rvec(f, ix.bag_off, n + 1);
+    rvec(f, ix.bag, cap);
+    rvec(f, ix.bag_len, cap);
+    rvec(f, ix.bag_cnt, cap);
+    rvec(f, ix.bag_hop, cap);
+    if (h2h[0]) {
+      rvec(f, ix.gdis, cap);
+      rvec(f, ix.sep_off, n + 1);
+      rvec(f, ix.sep, cap);
+    }
+    char extra;
+    if (std::fread(&extra, 1, 1, f) != 0) throw Error{BCX_EIO, std::string(path) + ": trailing data"};
   } catch (...) {
     std::fclose(f);
     throw;
   }
   std::fclose(f);
-  const int32_t n = ix.n;
-  if (n <= 0 || (int32_t)ix.perm.size() != n || (int32_t)ix.depth.size() != n ||
-      (int32_t)ix.parent.size() != n || (int64_t)ix.row_off.size() != (int64_t)n + 1 ||
-      (int64_t)ix.dis.size() != ix.row_off[(size_t)n] || ix.cnt.size() != ix.dis.size() ||
-      ix.hop.size() != ix.dis.size())
-    throw Error{BCX_EIO, std::string(path) + ": inconsistent index image"};
+  validate_index(ix, path);
+}
+
+// ---- host arrays (the reference's TreeDecomp handed over) -----------------
+// The tree, label rows and bags indexed by vertex id: parent[v] (-1 at the
+// root); v's label row over its ancestors, root first (depth(v) entries, the
+// reference's Node_::dis / cnt / my_cnt, TreeDecomp.h:95-118); v's bag, its
+// neighbours at elimination with the shortcut's length, count and hop count
+// (Node_::vertices, vertices_length, and TreeDecomp::edge_count_ /
+// my_edge_count_, TreeDecomp.cpp:157-200).  Nodes are renumbered in DFS
+// preorder with children in increasing vertex id, rows get their own entry
+// (0, 1, 0) appended, bags are ordered by ancestor depth, and the result is
+// validated like a loaded image.
+void index_from_arrays(int32_t n, const int32_t *parent, const int64_t *label_off, const int32_t *dis,
+                       const uint64_t *cnt, const uint32_t *hop, const int64_t *bag_off, const int32_t *bag_v,
+                       const int32_t *bag_len, const uint64_t *bag_cnt, const uint32_t *bag_hop, Index &ix) {
+  if (n <= 0) throw Error{BCX_EINVAL, "no vertices"};
+  ix = Index();
+  ix.n = n;
+  int32_t root = -1;
+  std::vector<int32_t> ch_off((size_t)n + 1, 0), ch((size_t)n);
+  for (int32_t v = 0; v < n; ++v) {
+    const int32_t a = parent[v];
+    if (a == -1) {
+      if (root >= 0) throw Error{BCX_EINVAL, "more than one root"};
+      root = v;
+    } else if (a < 0 || a >= n || a == v) {
+      throw Error{BCX_EINVAL, "parent of vertex " + std::to_string(v) + " out of range"};
+    } else {
+      ch_off[(size_t)a + 1]++;
+    }
+  }
+  if (root < 0) throw Error{BCX_EINVAL, "no root"};
+  for (int32_t v = 0; v < n; ++v) ch_off[(size_t)v + 1] += ch_off[(size_t)v];
+  {
+    std::vector<int32_t> f(ch_off.begin(), ch_off.end() - 1);
+    for (int32_t v = 0; v < n; ++v)  // increasing vertex id within each parent
+      if (parent[v] >= 0) ch[(size_t)f[(size_t)parent[v]]++] = v;
+  }
+  ix.perm.assign((size_t)n, -1);
   ix.iperm.assign((size_t)n, -1);
+  ix.depth.assign((size_t)n, 0);
+  ix.parent.assign((size_t)n, -1);
+  {
+    std::vector<std::pair<int32_t, int32_t>> st{{root, -1}};  // (vertex, parent preorder id)
+    int32_t next = 0;
+    while (!st.empty()) {
+      auto [v, pp] = st.back();
+      st.pop_back();
+      if (next >= n || ix.iperm[(size_t)v] != -1) throw Error{BCX_EINVAL, "parent array has a cycle"};
+      const int32_t p = next++;
+      ix.perm[(size_t)p] = v;
+      ix.iperm[(size_t)v] = p;
+      ix.parent[(size_t)p] = pp;
+      ix.depth[(size_t)p] = pp < 0 ? 0 : ix.depth[(size_t)pp] + 1;
+      for (int32_t k = ch_off[(size_t)v + 1]; k-- > ch_off[(size_t)v];) st.emplace_back(ch[(size_t)k], p);
+    }
+    if (next != n) throw Error{BCX_EINVAL, "parent array does not form one tree"};
+  }
+  ix.tree_height = 1 + *std::max_element(ix.depth.begin(), ix.depth.end());
+  ix.row_off.assign((size_t)n + 1, 0);
+  ix.bag_off.assign((size_t)n + 1, 0);
   for (int32_t p = 0; p < n; ++p) {
-    int32_t v = ix.perm[(size_t)p];
-    if (v < 0 || v >= n || ix.iperm[(size_t)v] != -1)
-      throw Error{BCX_EIO, std::string(path) + ": bad permutation"};
-    ix.iperm[(size_t)v] = p;
+    const int32_t v = ix.perm[(size_t)p];
+    if (label_off[v + 1] - label_off[v] != ix.depth[(size_t)p])
+      throw Error{BCX_EINVAL, "label row of vertex " + std::to_string(v) + " does not have depth(v) entries"};
+    if (bag_off[v + 1] < bag_off[v]) throw Error{BCX_EINVAL, "bag offsets are not monotone"};
+    ix.row_off[(size_t)p + 1] = ix.row_off[(size_t)p] + ix.depth[(size_t)p] + 1;
+    ix.bag_off[(size_t)p + 1] = ix.bag_off[(size_t)p] + (bag_off[v + 1] - bag_off[v]);
   }
-  ix.subtree.assign((size_t)n, 1);
-  for (int32_t p = n - 1; p > 0; --p) {
-    int32_t a = ix.parent[(size_t)p];
-    if (a < 0 || a >= p) throw Error{BCX_EIO, std::string(path) + ": bad tree"};
-    ix.subtree[(size_t)a] += ix.subtree[(size_t)p];
+  const size_t E = (size_t)ix.row_off[(size_t)n], B = (size_t)ix.bag_off[(size_t)n];
+  ix.dis.resize(E);
+  ix.cnt.resize(E);
+  ix.hop.resize(E);
+  ix.bag.resize(B);
+  ix.bag_len.resize(B);
+  ix.bag_cnt.resize(B);
+  ix.bag_hop.resize(B);
+  std::vector<int64_t> ord;
+  for (int32_t p = 0; p < n; ++p) {
+    const int32_t v = ix.perm[(size_t)p];
+    const int64_t src = label_off[v], dst = ix.row_off[(size_t)p];
+    const int32_t D = ix.depth[(size_t)p];
+    for (int32_t j = 0; j < D; ++j) {
+      ix.dis[(size_t)(dst + j)] = dis[src + j];
+      ix.cnt[(size_t)(dst + j)] = cnt[src + j];
+      ix.hop[(size_t)(dst + j)] = hop[src + j];
+    }
+    ix.dis[(size_t)(dst + D)] = 0;
+    ix.cnt[(size_t)(dst + D)] = 1;
+    ix.hop[(size_t)(dst + D)] = 0;
+    ord.clear();
+    for (int64_t e = bag_off[v]; e < bag_off[v + 1]; ++e) {
+      if (bag_v[e] < 0 || bag_v[e] >= n) throw Error{BCX_EINVAL, "bag vertex out of range"};
+      ord.push_back(e);
+    }
+    std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+      return ix.depth[(size_t)ix.iperm[(size_t)bag_v[a]]] < ix.depth[(size_t)ix.iperm[(size_t)bag_v[b]]];
+    });
+    int64_t k = ix.bag_off[(size_t)p];
+    for (int64_t e : ord) {
+      ix.bag[(size_t)k] = ix.iperm[(size_t)bag_v[e]];
+      ix.bag_len[(size_t)k] = bag_len[e];
+      ix.bag_cnt[(size_t)k] = bag_cnt[e];
+      ix.bag_hop[(size_t)k] = bag_hop[e];
+      ++k;
+    }
+    ix.tree_width = std::max(ix.tree_width, (int32_t)ord.size());
   }
+  for (size_t e = 0; e < E; ++e)
+    if (ix.cnt[e] == UINT64_MAX) ix.count_overflow = 1;
+  validate_index(ix, "index arrays");
+}
+
+void index_export(const Index &ix, int32_t *parent, int64_t *label_off, int32_t *dis, uint64_t *cnt,
+                  uint32_t *hop, int64_t *bag_off, int32_t *bag_v, int32_t *bag_len, uint64_t *bag_cnt,
+                  uint32_t *bag_hop) {
+  const int32_t n = ix.n;
+  int64_t lo = 0, bo = 0;
+  for (int32_t v = 0; v < n; ++v) {
+    const int32_t p = ix.iperm[(size_t)v];
+    const int32_t D = ix.depth[(size_t)p];
+    if (parent) parent[v] = ix.parent[(size_t)p] < 0 ? -1 : ix.perm[(size_t)ix.parent[(size_t)p]];
+    if (label_off) label_off[v] = lo;
+    const int64_t r = ix.row_off[(size_t)p];
+    for (int32_t j = 0; j < D; ++j) {
+      if (dis) dis[lo + j] = ix.dis[(size_t)(r + j)];
+      if (cnt) cnt[lo + j] = ix.cnt[(size_t)(r + j)];
+      if (hop) hop[lo + j] = ix.hop[(size_t)(r + j)];
+    }
+    lo += D;
+    if (bag_off) bag_off[v] = bo;
+    for (int64_t e = ix.bag_off[(size_t)p]; e < ix.bag_off[(size_t)p + 1]; ++e, ++bo) {
+      if (bag_v) bag_v[bo] = ix.perm[(size_t)ix.bag[(size_t)e]];
+      if (bag_len) bag_len[bo] = ix.bag_len[(size_t)e];
+      if (bag_cnt) bag_cnt[bo] = ix.bag_cnt[(size_t)e];
+      if (bag_hop) bag_hop[bo] = ix.bag_hop[(size_t)e];
+    }
+  }
+  if (label_off) label_off[n] = lo;
+  if (bag_off) bag_off[n] = bo;
 }
 
 }  // namespace bcx

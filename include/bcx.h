@@ -174,9 +174,45 @@ int bcx_index_tree(const bcx_index *ix, int32_t *parent, int32_t *height);
 /* Mean scan length pos(lca)+1 over `pairs` seeded random pairs. */
 int bcx_index_mean_scan(const bcx_index *ix, int64_t pairs, uint64_t seed,
                         double *mean);
-/* Write / read a device-ready binary image of the index (own format). */
+/* Write / read a device-ready binary image of the index (own format; replaces
+ * the headerless tmpRoadCountIndex dump, TreeDecomp.cpp:460-467,496-506).  The
+ * image holds the tree, labels, bags, H2H labels and separators when built,
+ * and a fingerprint of the graph (arc count and a hash of the arcs) that
+ * bcx_ctx_create checks.  Loading validates every array (BCX_EIO with the
+ * first violation): tree shape and preorder, row lengths, own entries, bag
+ * entries being ancestors in depth order ending at the parent, value ranges,
+ * and the distance bound. */
 int bcx_index_save(const bcx_index *ix, const char *path);
 int bcx_index_load(const char *path, bcx_index **out);
+/* The host-array seam of moveCountIndexToDevice (MHLQBC.cu:6-128): an index
+ * from the reference's TreeDecomp data, indexed by vertex id (0-based).
+ *   parent[v]            tree parent vertex, -1 for the root (Node_::parent)
+ *   label_off[n + 1]     v's label row is [label_off[v], label_off[v + 1]),
+ *                        depth(v) entries over its ancestors, root first
+ *   dis / cnt / hop      the rows' restricted distance, shortest-path count
+ *                        and max hop count (Node_::dis, cnt, my_cnt,
+ *                        TreeDecomp.h:95-118)
+ *   bag_off[n + 1]       v's bag is [bag_off[v], bag_off[v + 1])
+ *   bag_v / bag_len /    its neighbours at elimination with the shortcut's
+ *   bag_cnt / bag_hop    length, count and hop count (Node_::vertices,
+ *                        vertices_length, TreeDecomp::edge_count_,
+ *                        my_edge_count_, TreeDecomp.cpp:157-200)
+ * g (may be NULL) records the graph's fingerprint for bcx_ctx_create.  The
+ * arrays are copied and validated as a loaded image is (BCX_EIO or
+ * BCX_EINVAL on the first violation). */
+int bcx_index_from_arrays(int32_t n, const int32_t *parent,
+                          const int64_t *label_off, const int32_t *dis,
+                          const uint64_t *cnt, const uint32_t *hop,
+                          const int64_t *bag_off, const int32_t *bag_v,
+                          const int32_t *bag_len, const uint64_t *bag_cnt,
+                          const uint32_t *bag_hop, const bcx_graph *g,
+                          bcx_index **out);
+/* The same arrays out of an index (each pointer may be NULL): label rows
+ * hold label_entries - n entries and the bags bag_entries (bcx_index_stats). */
+int bcx_index_export(const bcx_index *ix, int32_t *parent, int64_t *label_off,
+                     int32_t *dis, uint64_t *cnt, uint32_t *hop,
+                     int64_t *bag_off, int32_t *bag_v, int32_t *bag_len,
+                     uint64_t *bag_cnt, uint32_t *bag_hop);
 void bcx_index_free(bcx_index *ix);
 
 /* ---- device context --------------------------------------------------- */

@@ -6,7 +6,9 @@ oracle/_ref/tc_ref, the reference's own host index code (TreeDecomp.cpp,
 buildIndex.cpp) compiled by oracle/build_ref.sh with a dumping harness:
 tree shape (parent, height, label length per vertex), the elimination order
 and (distance, count, max-hop) for seeded vertex pairs evaluated with the
-reference's scan on the reference's labels.  Fixtures are data only.
+reference's scan on the reference's labels; for the grid also the reference's
+whole host index structure (label rows and bags), the input of the host-array
+seam.  Fixtures are data only.
 
   python tests/golden/make_golden.py [--reference /root/reference]
 """
@@ -64,6 +66,38 @@ def run_ref(args, pairs: np.ndarray, n: int):
     return parse_dump(out, n)
 
 
+def run_ref_full(graph_path, n: int) -> dict:
+    """The reference TreeDecomp's whole host structure (tc_ref tcfull):
+    parent, label rows over ancestors and bags, indexed by 0-based vertex id."""
+    exe = ROOT / "oracle" / "_ref" / "tc_ref"
+    out = subprocess.run([str(exe), "tcfull", str(graph_path)], capture_output=True, text=True,
+                         check=True).stdout
+    parent = np.full(n, -1, np.int32)
+    rows = [None] * n
+    bags = [None] * n
+    for line in out.splitlines():
+        f = line.split()
+        if not f:
+            continue
+        if f[0] == "V":
+            parent[int(f[1]) - 1] = int(f[2]) - 1
+        elif f[0] == "R":
+            rows[int(f[1]) - 1] = [tuple(int(x) for x in e.split(":")) for e in f[3:]]
+        elif f[0] == "B":
+            bags[int(f[1]) - 1] = [tuple(int(x) for x in e.split(":")) for e in f[3:]]
+    lo = np.zeros(n + 1, np.int64)
+    bo = np.zeros(n + 1, np.int64)
+    for v in range(n):
+        lo[v + 1] = lo[v] + len(rows[v])
+        bo[v + 1] = bo[v] + len(bags[v])
+    R = np.array([e for r in rows for e in r], np.int64).reshape(-1, 3)
+    Bg = np.array([e for b in bags for e in b], np.int64).reshape(-1, 4)
+    return dict(parent=parent, label_off=lo, dis=R[:, 0].astype(np.int32), cnt=R[:, 1].astype(np.uint64),
+                hop=R[:, 2].astype(np.uint32), bag_off=bo, bag_v=(Bg[:, 0] - 1).astype(np.int32),
+                bag_len=Bg[:, 1].astype(np.int32), bag_cnt=Bg[:, 2].astype(np.uint64),
+                bag_hop=Bg[:, 3].astype(np.uint32))
+
+
 def seeded_pairs(n: int, k: int, seed: int) -> np.ndarray:
     a = gg.randint(seed, 1, k, 1, n)
     b = gg.randint(seed, 2, k, 1, n)
@@ -89,6 +123,12 @@ def main() -> None:
                         height=height, rowlen=rowlen, order=order, queries=q,
                         stats=np.array([st["height"], st["width"], st["entries"]]))
     print("grid1k:", st, len(q), "queries")
+    with tempfile.TemporaryDirectory() as d:
+        gp = Path(d) / "grid.txt"
+        gg.write_graph(g, gp)
+        full = run_ref_full(gp, g.n)
+    np.savez_compressed(HERE / "grid1k_tcfull.npz", **full)
+    print("grid1k full TreeDecomp:", len(full["dis"]), "label entries,", len(full["bag_v"]), "bag entries")
 
     # LosAngeles fixture shipped with the reference.
     gpath = ref / "data" / "roadData" / "LosAngeles.txt"

@@ -50,7 +50,6 @@ def write(tmp_path, text, name="g.txt"):
     ("2 2\n1 1 5\n1 1 5\n", "self loop"),
     ("x y\n", "bad header"),
     ("2 4\n1 2 5\n2 1 5\n1 2 7\n2 1 7\n", "listed twice"),
-    ("2 2\n1 2 999999999\n2 1 999999999\n", "INF"),
 ])
 def test_graph_reader_rejects_bad_input(tmp_path, text, match):
     with pytest.raises(bcx.BcxError, match=match):

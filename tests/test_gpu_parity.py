@@ -611,3 +611,73 @@ def test_rq_admission_busy_readmit_cancel(grid, mode):
         got = alone.poll(rid)
     assert np.array_equal(got.view(np.uint64), res_a.view(np.uint64))
     ctx.set_tuning(**saved)
+
+
+# ---- persisted index and the host-array seam ----------------------------------
+
+def test_bc_from_loaded_image_and_cli_restart(tmp_path, grid):
+    """A saved image (with H2H labels) loaded in a new context gives the
+    oracle's BC in both forms and the same H2H distances; MHLQBC restarted
+    from the image (BCX_INDEX_LOAD) writes the same BC."""
+    g, csr, _ = grid
+    ref, _ = oracle.brandes(csr)
+    G = bcx.Graph.from_arcs(g.n, g.u, g.v, g.w)
+    ix = bcx.Index.build_mhl(G)
+    ix.build_h2h()
+    p = tmp_path / "grid.idx"
+    ix.save(p)
+    ctx = bcx.Context(G, bcx.Index.load(p), 0)
+    for mode in (bcx.MODE_SWEEP, bcx.MODE_BATCHED):
+        bc, ovf = ctx.full_bc(mode)
+        assert not ovf and close_bc(bc, ref), f"mode {mode}"
+    d = ctx.h2h_distances(np.array([0, 500], np.int32))
+    assert np.array_equal(d[1].astype(np.int64), oracle.sssp(csr, 500)[0])
+    ctx.close()
+    gp = tmp_path / "g.txt"
+    gg.write_graph(g, gp)
+    exe = ROOT / "gpu_betweennesscentrality_amd" / "bin" / "MHLQBC"
+    env = dict(os.environ, BCX_INDEX_LOAD=str(p), BCX_BC_DUMP=str(tmp_path / "bc.bin"))
+    out = subprocess.run([exe, gp, "0"], env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert close_bc(np.fromfile(tmp_path / "bc.bin", np.float64), ref)
+    other = gg.grid_graph(k=32, seed=2)
+    op = tmp_path / "other.txt"
+    gg.write_graph(other, op)
+    out = subprocess.run([exe, op, "0"], env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 1 and "another graph" in out.stderr
+
+
+def test_bc_from_reference_treedecomp_through_the_seam():
+    """The reference's own host index for the grid (its label rows and bags,
+    dumped by the reference's TreeDecomp code) handed over through
+    bcx_index_from_arrays: BC in both forms equals the oracle's."""
+    golden = ROOT / "tests" / "golden"
+    full = dict(np.load(golden / "grid1k_tcfull.npz"))
+    tc = np.load(golden / "grid1k_tc.npz")
+    n = int(tc["n"])
+    G = bcx.Graph.from_arcs(n, tc["u"], tc["v"], tc["w"])
+    ctx = bcx.Context(G, bcx.Index.from_arrays(full, G), 0)
+    ref, _ = oracle.brandes(oracle.CSR(n, tc["u"], tc["v"], tc["w"]))
+    for mode in (bcx.MODE_SWEEP, bcx.MODE_BATCHED):
+        bc, ovf = ctx.full_bc(mode)
+        assert not ovf and close_bc(bc, ref), f"mode {mode}"
+    ctx.close()
+
+
+def test_weights_summing_past_inf():
+    """Graphs the old total-weight cap refused: a star whose weights sum past
+    the label INF, and a triangle with an arc longer than any distance."""
+    n = 3001
+    e = np.array([(0, i) for i in range(1, n)], np.int32)
+    star = gg._both_ways(n, e[:, 0], e[:, 1], np.full(n - 1, 400_000, np.int64))
+    G, ix, ctx = build(star)
+    want = np.zeros(n)
+    want[0] = (n - 1) * (n - 2) / 2
+    for mode in FORMS:
+        bc, ovf = ctx.full_bc(mode)
+        assert not ovf and close_bc(bc, want), f"star, mode {mode}"
+    tri = gg._both_ways(3, np.array([0, 1, 0]), np.array([1, 2, 2]), np.array([5, 5, 2_000_000_000]))
+    G, ix, ctx = build(tri)
+    for mode in FORMS:
+        bc, ovf = ctx.full_bc(mode)
+        assert not ovf and close_bc(bc, np.array([0.0, 1.0, 0.0])), f"triangle, mode {mode}"
