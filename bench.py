@@ -1,13 +1,16 @@
 """Benchmark of the betweenness hot path on one MI355X.
 
-A step is one batched launch sequence over one batch of source vertices on
-the seeded road-like 320,000-vertex graph with its MHL index resident in
-HBM: the bag-sweep form (3 tiles of 64 sources per compute unit, 49,152 on
-MI355X: per-tile sweep kernel, tile
-reduction; the shortcut unpacking runs once per full BC and is amortized).
-The value is whole-job full-BC throughput in sources/s; the label-scan form,
-the plain one-source-per-launch form, the RQ serving latencies and the CPU
-baseline are reported beside it.
+A step is one bag-sweep launch over one batch of source vertices (49,152 on
+MI355X: three tiles of 64 sources per compute unit, fewer where device
+memory caps the launch) on a seeded road-like graph with its MHL index
+resident in HBM.  W untimed steps, then exactly K timed steps as one full-BC
+call: K launches (tile kernel + reduction over tiles), then the shortcut
+unpacking once; the sources walk the tree order cyclically.  `value` is
+sources/s over that call.  Beside it: a full BC of every source, the
+label-scan form (the reference's method) and the plain one-source-per-launch
+form, the roofline of the tile kernel, self-checks of the timed result, RQ
+serving at saturation and the CPU baseline (the reference's method on every
+host core).
 
   python bench.py [--steps K] [--warmup W] [--workload road320k] [--gpus 1]
   python bench.py --profile-only            # one batch of each form + a 1 GiB copy
@@ -169,24 +172,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def max_rel(a, b) -> float:
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0))) if len(a) else 0.0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6, help="timed bag-sweep launches")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed bag-sweep launches before them")
     ap.add_argument("--workload", default="road320k")
     ap.add_argument("--batch", type=int, default=4096, help="label-scan sources per batch")
     ap.add_argument("--sweep-batch", type=int, default=0,
                     help="bag-sweep sources per launch (0: the library default, 3 tiles of 64 per CU)")
+    ap.add_argument("--labelscan-full-max-n", type=int, default=400_000,
+                    help="run the label scan over every source (the cross-check of the full BC) up to this n")
     ap.add_argument("--plain-sources", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rq", action="store_true")
     ap.add_argument("--no-h2h", action="store_true", help="skip the QBC distance-only (H2H) scan")
-    ap.add_argument("--rq-requests", type=int, default=6)
+    ap.add_argument("--rq-requests", type=int, default=200)
+    ap.add_argument("--rq-slots", type=int, default=16,
+                    help="RQ budget: the engine's fixed memory plus this many mean-size requests")
+    ap.add_argument("--rq-load", type=float, default=1.25,
+                    help="offered RQ load as a multiple of the engine's capacity")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
-                    help="one label-scan batch, one sweep batch and a 1 GiB copy (for rocprofv3)")
+                    help="one label-scan batch, one sweep launch and a 1 GiB copy (for rocprofv3)")
     ap.add_argument("--summarize-trace", metavar="CSV",
                     help="print a markdown table of a rocprofv3 kernel_stats.csv and exit")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -207,6 +220,9 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(a full BC would shard by source ranges, one GPU each)", file=sys.stderr)
         return 2
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be at least 1 and --warmup at least 0", file=sys.stderr)
+        return 2
 
     import gpu_betweennesscentrality_amd as bcx
     from gpu_betweennesscentrality_amd import graphgen as gg
@@ -219,156 +235,192 @@ def main() -> int:
     st = ix.stats()
     lbar = ix.mean_scan(1 << 18, 1)
     log(f"[bench] index: height={st['tree_height']} width={st['tree_width']} "
-        f"entries={st['label_entries']} Lbar={lbar:.1f} build={st['build_seconds']:.1f}s")
+        f"entries={st['label_entries']} bags={st['bag_entries']} Lbar={lbar:.1f} build={st['build_seconds']:.1f}s")
     ctx = bcx.Context(G, ix, 0)
     n = g.n
     B = min(args.batch, n)
     ctx.set_tuning(batch=B)
-
-    # Batches are blocks of the tree order (how a full sweep visits sources),
-    # spread evenly over it so the sample covers shallow and deep subtrees.
+    if args.sweep_batch:
+        ctx.set_tuning(sweep_batch=args.sweep_batch)
+    Bs = ctx.sweep_capacity()  # sources per launch after the memory cap
     tree_perm = ix.tree_order()
-    sweep_default = ctx.tuning()["sweep_batch"]
 
-    def tree_batch(k: int, nb: int, size: int = B) -> np.ndarray:
-        size = min(size, n)
-        start = (k * (n - size)) // max(1, nb - 1) if nb > 1 else 0
-        start -= start % 64
-        return tree_perm[start:start + size]
+    def cyc(start: int, count: int) -> np.ndarray:
+        """`count` sources from position `start` of the tree order, cyclically."""
+        return tree_perm[(start + np.arange(count, dtype=np.int64)) % n].astype(np.int32)
 
-    total = args.warmup + args.steps
+    W, K = args.warmup, args.steps
     if args.profile_only:
-        ctx.set_tuning(batch=B)
-        ctx.bc_sources(tree_batch(1, 3, B), bcx.MODE_BATCHED)
-        Bs = min(args.sweep_batch or sweep_default, n)
-        ctx.set_tuning(sweep_batch=Bs)
-        ctx.bc_sources(tree_batch(1, 3, Bs), bcx.MODE_SWEEP)
-        ms = ctx.copy_probe(1 << 30)
-        log(f"[bench] profile-only done; 1 GiB device copy {ms:.3f} ms")
+        # The same bag-sweep launches as the timed call (W + K, one job),
+        # one label-scan batch and a 1 GiB copy, for a kernel trace.
+        ctx.bc_sources(cyc(n // 3, B), bcx.MODE_BATCHED)
+        ctx.bc_sources(cyc(0, (W + K) * Bs), bcx.MODE_SWEEP)
+        ms = ctx.copy_probe(COPY_BYTES)
+        log(f"[bench] profile-only done ({W + K} launches of {Bs} sources); 1 GiB device copy {ms:.3f} ms")
         return 0
 
-    def time_mode(mode: int, label: str, size: int):
-        # One call per phase, as a full BC runs: the call launches one batch
-        # per step and finishes the job once (BC accumulation, and for the
-        # bag-sweep form the shortcut unpacking).
-        ctx.set_tuning(**({"sweep_batch": size} if mode == bcx.MODE_SWEEP else {"batch": size}))
-
-        def job(k0: int, k1: int) -> np.ndarray:
-            # Batches k0..k1-1; where they would cover the graph, every
-            # source once (a whole full BC).
-            if (k1 - k0) * size >= n:
-                return tree_perm
-            return np.concatenate([tree_batch(k, total, size) for k in range(k0, k1)])
-
-        if args.warmup:
-            ctx.bc_sources(job(0, args.warmup), mode)
-        src = job(args.warmup, total)
-        t = time.perf_counter()
-        bc, ovf = ctx.bc_sources(src, mode)
-        el = time.perf_counter() - t
-        done = len(src)
-        rate = done / el
-        ctx.set_profiling(True)
-        ctx.reset_timing()
-        ctx.bc_sources(tree_batch(total // 2, total, size), mode)
-        tm = ctx.timing()
-        ctx.set_profiling(False)
-        if tm.get("sweep_fwd_ms"):
-            log(f"[bench]   sweep phases per tile: fwd {tm['sweep_fwd_ms']:.1f} bwd {tm['sweep_bwd_ms']:.1f} "
-                f"asc {tm['sweep_asc_ms']:.1f} ms")
-        log(f"[bench] {label}: {done} sources in {el:.3f}s -> {rate:.1f} sources/s; "
-            f"phases {tm['scan_ms']:.1f} / {tm['dep_ms']:.1f} / {tm['reduce_ms']:.1f} ms")
-        return rate, el, tm, ovf, done
-
+    checks = {}
     tuning = ctx.tuning()
     if not args.no_autotune:
         ta = time.time()
-        tuning = ctx.autotune(tree_batch(0, total))
+        tuning = ctx.autotune(cyc(0, B))
         log(f"[bench] autotune: {tuning} ({time.time() - ta:.1f}s)")
-    scan_rate, scan_el, scan_tm, ovf1, _ = time_mode(bcx.MODE_BATCHED, "label-scan batched", B)
-    Bs = min(args.sweep_batch or sweep_default, n)
-    value, el, tm, ovf, done = time_mode(bcx.MODE_SWEEP, "bag-sweep batched", Bs)
-    launches = -(-done // Bs)  # the steps, or fewer where they would pass a full BC
-    ctx.set_tuning(batch=B)
-    ovf = ovf or ovf1
-    scan_ms = scan_tm["scan_ms"]
-    dep_ms = scan_tm["dep_ms"]
-    red_ms = scan_tm["reduce_ms"]
-    S = 64
-    # Roofline of the dominant kernel, the sweep tile kernel: algorithmic bytes
-    # of one launch over its duration (HIP events on the context's stream).
+
+    # ---- label scan (the reference's method): rate, and the cross-check ------
+    ls_full = n <= args.labelscan_full_max_n
+    ls_src = tree_perm.astype(np.int32) if ls_full else cyc(n // 5, 4 * B)
+    ctx.bc_sources(cyc(0, B), bcx.MODE_BATCHED)  # warm-up
+    t = time.perf_counter()
+    ls_bc, ls_ovf = ctx.bc_sources(ls_src, bcx.MODE_BATCHED)
+    ls_el = time.perf_counter() - t
+    ls_rate = len(ls_src) / ls_el
+    log(f"[bench] label scan: {len(ls_src)} sources in {ls_el:.2f}s -> {ls_rate:.1f} sources/s")
+
+    # ---- headline: W untimed, then K timed bag-sweep launches ---------------
+    if W:
+        ctx.bc_sources(cyc(0, W * Bs), bcx.MODE_SWEEP)
+    src = cyc(W * Bs, K * Bs)
+    ctx.reset_timing()
+    t = time.perf_counter()
+    timed_bc, ovf = ctx.bc_sources(src, bcx.MODE_SWEEP)
+    el = time.perf_counter() - t
+    launches = ctx.timing()["scan_launches"]
+    value = K * Bs / el
+    log(f"[bench] bag sweep: {K} launches of {Bs} sources in {el:.3f}s -> {value:.1f} sources/s "
+        f"({1e3 * el / K:.2f} ms per launch)")
+    checks["timed_call_launches"] = {"want": K, "got": int(launches), "ok": int(launches) == K}
+
+    # Tile-kernel time per launch: the same K launches again, each bracketed
+    # by HIP events on the context's stream (the time depends on where in
+    # the tree order a launch's sources lie, so the mean is over these).
+    ctx.set_profiling(True)
+    ctx.reset_timing()
+    prof_bc, _ = ctx.bc_sources(src, bcx.MODE_SWEEP)
+    tm = ctx.timing()
+    ctx.set_profiling(False)
+    kernel_ms = tm["scan_ms"] / K
+    log(f"[bench] sweep tile kernel {kernel_ms:.2f} ms per launch of {Bs} (mean of {K}); per tile fwd "
+        f"{tm['sweep_fwd_ms'] / K:.1f} bwd {tm['sweep_bwd_ms'] / K:.1f} asc {tm['sweep_asc_ms'] / K:.1f} ms; "
+        f"reduce {tm['reduce_ms'] / K:.2f} ms")
+    checks["timed_equals_eager_bitwise"] = {"ok": bool(np.array_equal(timed_bc.view(np.uint64),
+                                                                      prof_bc.view(np.uint64)))}
+
+    # ---- a full BC of every source -----------------------------------------
+    t = time.perf_counter()
+    full_bc, full_ovf = ctx.full_bc(bcx.MODE_SWEEP)
+    full_first = time.perf_counter() - t
+    t = time.perf_counter()
+    full_bc2, _ = ctx.full_bc(bcx.MODE_SWEEP)
+    full_el = time.perf_counter() - t
+    log(f"[bench] full BC (bag sweep): {full_el:.3f}s ({n / full_el:.1f} sources/s; first call {full_first:.3f}s)")
+    checks["full_bc_repeat_bitwise"] = {"ok": bool(np.array_equal(full_bc.view(np.uint64),
+                                                                  full_bc2.view(np.uint64)))}
+
+    # ---- self-checks of the timed result -------------------------------------
+    # The timed sources are q whole passes over the tree order plus r more:
+    # BC(timed) = q * BC(all) + BC(those r), within rounding.
+    q, r = divmod(K * Bs, n)
+    rest = ctx.bc_sources(cyc(W * Bs + q * n, r), bcx.MODE_SWEEP)[0] if r else np.zeros(n)
+    e = max_rel(timed_bc, q * full_bc + rest)
+    checks["timed_equals_passes_plus_rest"] = {"max_rel": e, "tol": 1e-9, "ok": e <= 1e-9}
+    # The sweep against the reference's method (label scan) on the same sources.
+    sw_ls = full_bc if ls_full else ctx.bc_sources(ls_src, bcx.MODE_SWEEP)[0]
+    e = max_rel(sw_ls, ls_bc)
+    checks["sweep_equals_label_scan"] = {"sources": int(len(ls_src)), "max_rel": e, "tol": 1e-9, "ok": e <= 1e-9}
+    checks["no_count_overflow"] = {"ok": not (ovf or full_ovf or ls_ovf)}
+    for k, v in checks.items():
+        log(f"[bench] check {k}: {v}")
+
+    # ---- roofline ------------------------------------------------------------
     bags = int(st["bag_entries"])
     parent, _ = ix.tree()
     inner = int(np.unique(parent[parent >= 0]).size)  # nodes with descendants
-    sweep_ms = tm["scan_ms"]
     algo = sweep_algorithmic_bytes(n, bags, inner, int(st["tree_height"]), Bs)
-    achieved = algo / (sweep_ms * 1e-3) / 1e9
+    achieved = algo / (kernel_ms * 1e-3) / 1e9
     traffic = committed_traffic(Bs, args.workload)
+    # Label-scan form against its own byte model, B_src(64) with the measured
+    # mean DAG arcs per source.
+    d_rows = ctx.query_rows(cyc(n // 7, 4), bcx.MODE_BATCHED)[0].astype(np.int64)
+    dbar = float(np.mean([np.count_nonzero(d[g.v] == d[g.u] + g.w) for d in d_rows]))
+    bsrc = 16.0 * n * lbar / 64 + 48.0 * n + 4.0 * g.arcs + 8.0 * dbar
 
-    # Plain form: one source per launch sequence.
-    ps = tree_perm[: args.plain_sources]
+    # ---- plain form: one source per launch sequence ---------------------------
+    ps = cyc(n // 3, args.plain_sources)
     ctx.bc_sources(ps[:2], bcx.MODE_PLAIN)
     t = time.perf_counter()
     ctx.bc_sources(ps, bcx.MODE_PLAIN)
     plain = len(ps) / (time.perf_counter() - t)
-    log(f"[bench] plain: {plain:.1f} sources/s ({value / plain:.2f}x batched/plain)")
+    log(f"[bench] plain: {plain:.1f} sources/s")
 
     rq = None
     if not args.no_rq:
-        rq = _rq_bench(bcx, gg, ctx, g, args.rq_requests, Bs)
+        rq = _rq_bench(bcx, gg, ctx, g, args.rq_requests, args.rq_slots, args.rq_load)
 
     cpu = None
     if not args.no_cpu:
-        cpu = _cpu_baseline(g, n, args.cpu_seconds)
+        cpu = _cpu_baseline(g, ix, n, args.cpu_seconds, st)
 
     h2h = None
     if not args.no_h2h:
         ctx.close()  # its sweep buffers hold most of the device memory
         h2h = _h2h_bench(bcx, G, ix, tree_perm, n)
 
+    ok = all(v["ok"] for v in checks.values())
     out = {
         "metric": "sources/sec for full BC (batched vs plain) + RQ p50/p99 latency, 1 MI355X",
         "value": round(value, 2),
         "unit": "sources/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * el / launches, 3),
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(1e3 * el / K, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32+u64 (labels, counts) / f64 (dependencies, BC)",
         "data": "synthetic (seeded road-like Delaunay graph, generated in-process)",
         "config": {
-            "workload": f"{args.workload}: MHL full BC (bag-sweep form), n={n}, arcs={g.arcs}, "
-                        f"{Bs} sources per batched launch",
-            "n": n, "arcs": int(g.arcs), "sweep_batch": Bs, "labelscan_batch": B, "tile": S,
-            "tchunk": tuning["tchunk"], "dep_threads": tuning["bp_threads"],
+            "workload": f"{args.workload}: MHL full BC, bag-sweep form, n={n}, arcs={g.arcs}; a step is one launch "
+                        f"of {Bs} sources, the timed call {K} launches plus one unpack",
+            "n": n, "arcs": int(g.arcs), "sources_per_launch": Bs, "timed_launches": int(launches),
+            "timed_sources": K * Bs, "timed_seconds": round(el, 4),
+            "full_bc_sources_per_s": round(n / full_el, 2), "full_bc_seconds": round(full_el, 4),
+            "full_bc_first_call_seconds": round(full_first, 4),
+            "labelscan_batch": B, "tchunk": tuning["tchunk"], "dep_threads": tuning["bp_threads"],
             "tree_height": st["tree_height"], "tree_width": st["tree_width"],
             "label_entries": st["label_entries"], "bag_entries": bags, "inner_nodes": inner,
-            "Lbar": round(lbar, 2),
-            "labelscan_sources_per_s": round(scan_rate, 2),
-            "sweep_phase_ms": [round(tm["scan_ms"], 3), round(tm["dep_ms"], 3), round(tm["reduce_ms"], 3)],
+            "Lbar": round(lbar, 2), "Dbar": round(dbar, 1),
+            "labelscan_sources_per_s": round(ls_rate, 2), "labelscan_sources": int(len(ls_src)),
             "plain_sources_per_s": round(plain, 2),
-            "batched_over_plain": round(value / plain, 2),
-            "full_bc_seconds_est": round(n / value, 2),
-            "scan_ms_per_batch": round(scan_ms, 3), "dep_ms_per_batch": round(dep_ms, 3),
-            "reduce_ms_per_batch": round(red_ms, 3),
+            "sweep_over_plain": round(value / plain, 2),
+            "labelscan_over_plain": round(ls_rate / plain, 2),
+            "sweep_over_labelscan": round(value / ls_rate, 2),
             "count_overflow": bool(ovf),
             "h2h_scan": h2h,
         },
+        "checks": checks,
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None if traffic is None else round(traffic, 0),
-            "kernel": SWEEP_KERNEL, "kernel_ms": round(sweep_ms, 3),
+            "model": "bag-sweep lower bound per source: 8*bag_entries + 4*n + 12*inner_nodes "
+                     "+ 8*(inner_nodes - tree_height) bytes",
+            "kernel": SWEEP_KERNEL, "kernel_ms": round(kernel_ms, 3), "sources_per_launch": Bs,
             "algorithmic_bytes": round(algo, 0),
-            "traffic_GBs": None if traffic is None else round(traffic / (sweep_ms * 1e-3) / 1e9, 2),
+            "traffic_GBs": None if traffic is None else round(traffic / (kernel_ms * 1e-3) / 1e9, 2),
+            "labelscan": {
+                "model": "B_src(S) = 16*n*Lbar/S + 48*n + 4*arcs + 8*Dbar, S = 64",
+                "bytes_per_source": round(bsrc, 0),
+                "achieved_GBs": round(ls_rate * bsrc / 1e9, 2),
+                "frac": round(ls_rate * bsrc / 1e9 / HBM_PEAK_GBS, 4)},
         },
         "cpu_baseline": cpu,
         "rq": rq,
     }
     print(json.dumps(out), flush=True)
+    if not ok:
+        log("[bench] a self-check failed: " + ", ".join(k for k, v in checks.items() if not v["ok"]))
+        return 1
     return 0
 
 
@@ -398,79 +450,134 @@ def _h2h_bench(bcx, G, ix, tree_perm, n: int, sources: int = 512) -> dict:
     return out
 
 
-def _rq_bench(bcx, gg, ctx, g, nreq: int, batch: int) -> dict:
-    """Continuous-batching RQ serving on the bag-sweep form: Poisson arrivals,
-    admission by budget; every request rides the shared cyclic sweep for one
-    full cycle of batches."""
+def _rq_bench(bcx, gg, ctx, g, nreq: int, slots: int, load: float) -> dict:
+    """Continuous-batching RQ serving at saturation on the bag-sweep form.
+
+    Requests (Q log-uniform in [1e3, 5e4], uniform or BFS-ball sets) arrive
+    as a seeded Poisson stream at `load` times the engine's capacity.  The
+    engine's budget is its fixed memory plus `slots` mean-size requests, so
+    admission binds: a request that does not fit gets BCX_EBUSY and waits in
+    arrival order.  Every admitted request rides the shared cyclic sweep for
+    one full cycle.  Latency = completion - arrival, split into queue wait
+    (arrival to admission) and service (admission to completion)."""
     n = g.n
-    ctx.set_tuning(sweep_batch=batch)
     row = np.zeros(n + 1, np.int64)
     np.add.at(row, g.u.astype(np.int64) + 1, 1)
     row = np.cumsum(row)
     sets = gg.query_sets(n, nreq, seed=4, adjacency=(row, g.v))
-    free, _ = ctx.mem_info()
-    srv = bcx.RQServer(ctx, int(free * 0.25))
-    # Saturating arrival rate: mean gap = one batch time / 2.
-    rng_gap = gg.uniform01(4, 30, nreq)
-    t_batch = None
-    arrivals = []
-    submitted = {}
-    done = {}
-    t0 = time.perf_counter()
-    pending = list(range(nreq))
-    next_arrival = 0.0
-    while len(done) < nreq:
-        now = time.perf_counter() - t0
-        while pending and now >= next_arrival:
-            i = pending[0]
-            rid = srv.submit(sets[i])
-            if rid is None:
-                break
-            pending.pop(0)
-            submitted[rid] = (i, time.perf_counter())
-            gap = -np.log(1.0 - rng_gap[i]) * (t_batch or 0.01) * 0.5
-            next_arrival = now + gap
-        ts = time.perf_counter()
+    probe = bcx.RQServer(ctx, 1 << 40)
+    fixed, _ = probe.memory()
+    need = np.array([probe.bytes_needed(len(s)) for s in sets], np.float64)
+    probe.close()
+    budget = int(fixed + max(slots * need.mean(), need.max()))  # the largest request always fits alone
+    srv = bcx.RQServer(ctx, budget)
+    # One step is one sweep launch; a request is served after nb steps.
+    ts = time.perf_counter()
+    rid = srv.submit(sets[0][:1])
+    steps_cycle = 0
+    while srv.poll(rid) is None:
         srv.step()
-        t_step = time.perf_counter() - ts
-        t_batch = t_step if t_batch is None else 0.9 * t_batch + 0.1 * t_step
-        for rid in list(submitted):
-            r = srv.poll(rid)
-            if r is not None:
-                i, ta = submitted.pop(rid)
-                done[i] = time.perf_counter() - ta
-        if not submitted and pending:
-            next_arrival = min(next_arrival, time.perf_counter() - t0)
+        steps_cycle += 1
+    cycle_s = time.perf_counter() - ts
+    capacity = slots / cycle_s  # requests per second with `slots` in flight
+    rate = load * capacity
+    gaps = -np.log(1.0 - gg.uniform01(4, 30, nreq)) / rate
+    arrive = np.cumsum(gaps)
+    admit = np.full(nreq, np.nan)
+    done = np.full(nreq, np.nan)
+    inflight = {}
+    nxt = 0
+    ebusy = 0
+    steps = 0
+    max_inflight = 0
+    t0 = time.perf_counter()
+    while np.isnan(done).any():
+        now = time.perf_counter() - t0
+        while nxt < nreq and arrive[nxt] <= now:
+            r = srv.submit(sets[nxt])
+            if r is None:
+                ebusy += 1
+                break
+            admit[nxt] = time.perf_counter() - t0
+            inflight[r] = nxt
+            nxt += 1
+        max_inflight = max(max_inflight, len(inflight))
+        if not inflight:
+            time.sleep(max(0.0, arrive[nxt] - (time.perf_counter() - t0)))
+            continue
+        srv.step()
+        steps += 1
+        for r in list(inflight):
+            if srv.poll(r) is not None:
+                done[inflight.pop(r)] = time.perf_counter() - t0
     wall = time.perf_counter() - t0
-    lat = np.array([done[i] for i in range(nreq)])
+    srv.close()
+    lat, wait, serv = done - arrive, admit - arrive, done - admit
     qs = np.array([len(s) for s in sets])
-    res = {"requests": nreq, "p50_s": round(float(np.percentile(lat, 50)), 4),
-           "p99_s": round(float(np.percentile(lat, 99)), 4),
-           "p99_over_p50": round(float(np.percentile(lat, 99) / np.percentile(lat, 50)), 3),
-           "requests_per_s": round(nreq / wall, 4), "queries_per_s": round(float(qs.sum()) / wall, 1),
-           "q_sizes": qs.tolist(), "batch_seconds": round(t_batch or 0.0, 4),
-           "form": "bag-sweep", "sources_per_batch": batch}
+
+    def pct(x, p):
+        return round(float(np.percentile(x, p)), 4)
+
+    res = {"requests": nreq, "form": "bag-sweep", "steps": steps, "cycles": round(steps / steps_cycle, 2),
+           "steps_per_cycle": steps_cycle, "cycle_s": round(cycle_s, 4),
+           "budget_bytes": budget, "fixed_bytes": int(fixed), "slots": slots, "max_in_flight": max_inflight,
+           "ebusy": ebusy, "offered_per_s": round(rate, 3), "capacity_per_s": round(capacity, 3),
+           "p50_s": pct(lat, 50), "p99_s": pct(lat, 99), "p99_over_p50": round(pct(lat, 99) / pct(lat, 50), 3),
+           "wait_p50_s": pct(wait, 50), "wait_p99_s": pct(wait, 99),
+           "service_p50_s": pct(serv, 50), "service_p99_s": pct(serv, 99),
+           "service_p99_over_p50": round(pct(serv, 99) / pct(serv, 50), 3),
+           "requests_per_s": round(nreq / wall, 3), "queries_per_s": round(float(qs.sum()) / wall, 1),
+           "q_min": int(qs.min()), "q_median": int(np.median(qs)), "q_max": int(qs.max())}
     log(f"[bench] rq: {res}")
     return res
 
 
-def _cpu_baseline(g, n: int, budget_s: float) -> dict:
-    """Oracle Dijkstra-Brandes (one thread) on a bounded sample of sources."""
+def _cpu_baseline(g, ix, n: int, budget_s: float, st: dict) -> dict:
+    """The reference's method on the host cores: the index's label rows
+    (built by the host index builder, timed separately), then per source the
+    label query of every target and the layered Brandes pass
+    (oracle.label_bc, OpenMP over sources), on a bounded sample of the same
+    workload; Dijkstra-Brandes (oracle.brandes) timed beside it."""
+    import os
+
     import oracle
     from gpu_betweennesscentrality_amd import graphgen as gg
 
+    threads = oracle.default_threads()
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     csr = oracle.CSR.from_arcgraph(g)
+    tree = ix.export()
     src = gg.permutation(9, 1, n).astype(np.int32)
-    t = time.perf_counter()
-    oracle.brandes(csr, src[:2])
-    per = max(1e-4, (time.perf_counter() - t) / 2)
-    k = int(max(4, min(n, budget_s / per)))
-    t = time.perf_counter()
-    oracle.brandes(csr, src[:k])
-    el = time.perf_counter() - t
-    res = {"value": round(k / el, 3), "unit": "sources/s", "cores": 1, "kind": "port",
-           "sample": f"{k} seeded random sources of the same graph, Dijkstra + Brandes "
-                     f"(oracle/oracle.c), {el:.1f}s"}
+
+    def rate(fn, seconds):
+        t = time.perf_counter()
+        fn(src[:threads])
+        per = max(1e-4, (time.perf_counter() - t) / threads)
+        k = int(max(threads, min(n, seconds / per)))
+        k -= k % threads
+        t = time.perf_counter()
+        fn(src[:k])
+        el = time.perf_counter() - t
+        return k / el, k, el
+
+    lr, lk, lel = rate(lambda s: oracle.label_bc(csr, tree, s, threads), budget_s)
+    dr, dk, dlel = rate(lambda s: oracle.brandes(csr, s, threads), budget_s / 3)
+    res = {"value": round(lr, 3), "unit": "sources/s", "cores": threads, "kind": "port",
+           "sample": f"{lk} seeded random sources of the same graph: label queries over the host index's rows "
+                     f"to every target, then layered Brandes (oracle/oracle.c orc_label_bc), {lel:.1f}s "
+                     f"on {threads} threads",
+           "cpu_model": model, "nproc": os.cpu_count(), "threads": threads,
+           "index_build_s": round(float(st["build_seconds"]), 3),
+           "full_bc_seconds_est": round(n / lr, 1),
+           "dijkstra_brandes_sources_per_s": round(dr, 3),
+           "dijkstra_brandes_sample": f"{dk} sources, {dlel:.1f}s on {threads} threads"}
     log(f"[bench] cpu baseline: {res}")
     return res
 

@@ -97,6 +97,7 @@ _SIGS = {
     "bcx_ctx_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "bcx_ctx_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "bcx_ctx_autotune": (C.c_int, [_P, _I32P, C.c_int32, C.POINTER(Tuning)]),
+    "bcx_ctx_sweep_capacity": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "bcx_ctx_workspace_bytes": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_size_t)]),
     "bcx_ctx_mem_info": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "bcx_ctx_timing": (C.c_int, [_P, C.POINTER(Timing)]),
@@ -354,6 +355,12 @@ class Context:
 
     def reset_timing(self) -> None:
         lib().bcx_ctx_timing_reset(self._h)
+
+    def sweep_capacity(self) -> int:
+        """Sources one bag-sweep launch takes after the device-memory cap."""
+        k = C.c_int32()
+        _check(lib().bcx_ctx_sweep_capacity(self._h, C.byref(k)))
+        return k.value
 
     def workspace_bytes(self, batch: int) -> int:
         b = C.c_size_t()

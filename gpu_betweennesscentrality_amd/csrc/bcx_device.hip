@@ -1018,6 +1018,15 @@ int bcx_ctx_get_tuning(const bcx_ctx *c, bcx_tuning *tn) {
   return BCX_OK;
 }
 
+int bcx_ctx_sweep_capacity(bcx_ctx *c, int32_t *sources_per_launch) {
+  BCX_GUARD_BEGIN
+  if (!c || !sources_per_launch) return bcx::fail(BCX_EINVAL, "null argument");
+  HIP_OK(hipSetDevice(c->device));
+  *sources_per_launch = sweep_tiles_for(c, c->tn.sweep_batch) * kWave;
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
 int bcx_ctx_workspace_bytes(const bcx_ctx *c, int32_t batch, size_t *bytes) {
   if (!c || !bytes || batch < 1) return bcx::fail(BCX_EINVAL, "bad argument");
   *bytes = bcx_ctx::slot_bytes(c->ix.n) * (size_t)batch;

@@ -227,6 +227,9 @@ int bcx_ctx_get_tuning(const bcx_ctx *ctx, bcx_tuning *tn);
  * in the context and copies it to *best (may be NULL). */
 int bcx_ctx_autotune(bcx_ctx *ctx, const int32_t *sources, int32_t count,
                      bcx_tuning *best);
+/* Sources one bag-sweep launch takes (the tuning's sweep_batch, cut to whole
+ * tiles of 64 by device memory); allocates the launch state. */
+int bcx_ctx_sweep_capacity(bcx_ctx *ctx, int32_t *sources_per_launch);
 /* Bytes of device memory one batched sequence of `batch` sources needs. */
 int bcx_ctx_workspace_bytes(const bcx_ctx *ctx, int32_t batch, size_t *bytes);
 int bcx_ctx_mem_info(const bcx_ctx *ctx, size_t *free_bytes, size_t *total_bytes);

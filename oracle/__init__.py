@@ -46,6 +46,9 @@ def _lib():
         L.orc_brandes_mt.restype = C.c_int
         L.orc_brandes_mt.argtypes = [C.c_int32, _I64P, _I32P, _I32P, _I32P, C.c_int32, _F64P,
                                      C.POINTER(C.c_int32), C.c_int32]
+        L.orc_label_bc.restype = C.c_int
+        L.orc_label_bc.argtypes = [C.c_int32, _I32P, _I64P, _I32P, _U64P, _U32P, _I64P, _I32P, _I32P, _I32P,
+                                   C.c_int32, _F64P, C.c_int32]
         L.orc_rq_pairs.restype = C.c_int
         L.orc_rq_pairs.argtypes = [C.c_int32, _I64P, _I32P, _I32P, _I32P, C.c_int32, _F64P]
         _LIB = L
@@ -106,6 +109,22 @@ def brandes(g: CSR, sources=None, threads: int = 1):
     if rc != 0:
         raise MemoryError("oracle brandes")
     return bc, bool(ovf.value)
+
+
+def label_bc(g: CSR, tree: dict, sources=None, threads: int = 1):
+    """Reference-method BC (label queries + layered Brandes) over `sources`
+    from an index's host structure `tree` (parent, label_off, dis, cnt, hop
+    by vertex id, as bcx's Index.export() or the reference's TreeDecomp
+    give it)."""
+    src = np.arange(g.n, dtype=np.int32) if sources is None else np.ascontiguousarray(sources, np.int32)
+    bc = np.zeros(g.n, np.float64)
+    t = {k: np.ascontiguousarray(tree[k], dt) for k, dt in (
+        ("parent", np.int32), ("label_off", np.int64), ("dis", np.int32), ("cnt", np.uint64),
+        ("hop", np.uint32))}
+    if _lib().orc_label_bc(g.n, t["parent"], t["label_off"], t["dis"], t["cnt"], t["hop"], g.row, g.col, g.w,
+                           src, len(src), bc, int(threads)) != 0:
+        raise MemoryError("oracle label_bc (allocation failed or not one tree)")
+    return bc
 
 
 def rq_pairs(g: CSR, points):

@@ -125,3 +125,37 @@ def test_planted_mistakes_are_caught():
     _, s, _, _ = oracle.sssp(oracle.CSR.from_arcgraph(g), 0)
     assert int(s[k * k - 1]) != 1 and int(s[k * k - 1]) == comb(2 * k - 2, k - 1)
     del d
+
+
+def test_label_bc_on_reference_rows_equals_brandes():
+    """The reference-method CPU path (label queries over the reference's own
+    TreeDecomp rows, then layered Brandes) equals Dijkstra-Brandes; a row with
+    one entry changed does not."""
+    from pathlib import Path
+
+    golden = Path(__file__).resolve().parent / "golden"
+    full = dict(np.load(golden / "grid1k_tcfull.npz"))
+    tc = np.load(golden / "grid1k_tc.npz")
+    csr = oracle.CSR(int(tc["n"]), tc["u"], tc["v"], tc["w"])
+    ref, _ = oracle.brandes(csr)
+    got = oracle.label_bc(csr, full, threads=4)
+    assert np.all(np.abs(got - ref) <= 1e-9 * np.maximum(np.abs(ref), 1.0))
+    bad = dict(full, dis=full["dis"].copy())
+    row = np.arange(full["label_off"][500], full["label_off"][501])
+    k = row[np.flatnonzero(full["dis"][row] < 999_999_999)[0]]
+    bad["dis"][k] //= 2  # a much shorter restricted path that does not exist
+    wrong = oracle.label_bc(csr, bad, threads=4)
+    assert not np.all(np.abs(wrong - ref) <= 1e-9 * np.maximum(np.abs(ref), 1.0))
+
+
+def test_label_bc_on_built_rows_equals_brandes():
+    import gpu_betweennesscentrality_amd as bcx
+
+    g = gg.road_graph(3000, 3750, seed=21)
+    G = bcx.Graph.from_arcs(g.n, g.u, g.v, g.w)
+    tree = bcx.Index.build_mhl(G).export()
+    csr = oracle.CSR.from_arcgraph(g)
+    src = gg.permutation(3, 1, g.n)[:300].astype(np.int32)
+    ref, _ = oracle.brandes(csr, src)
+    got = oracle.label_bc(csr, tree, src, threads=3)
+    assert np.all(np.abs(got - ref) <= 1e-9 * np.maximum(np.abs(ref), 1.0))
