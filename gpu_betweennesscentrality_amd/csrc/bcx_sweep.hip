@@ -44,20 +44,31 @@ namespace {
 constexpr int64_t kSlotOff = kWave * 4;
 constexpr int64_t kRec = kWave * (4 + 8);
 
+// Offset beyond every buffer resource below: a raw buffer store given it in a
+// lane writes nothing there (out-of-range buffer stores are dropped), while
+// the instruction still issues and is counted like any other.  The pipelined
+// steps issue every store of a step (see wait_vm_left); lanes and steps that
+// must not write use it, which costs no memory traffic.
+constexpr uint32_t kDrop = 0xF0000000u;
+// Word 3 of a gfx9-family buffer resource for untyped accesses (32-bit data
+// format; no swizzle, no index stride).
+constexpr int kBufferWord3 = 0x00020000;
+
 struct Lane {
   int t, lane, s;
   bool valid;
   int64_t rs;
   char *tb;      // the tile's records (uniform); a record is addressed by its
-                 // 32-bit offset (node * kRec), the tile spanning under 4 GB
-  uint32_t sink; // offset of record n, which no node owns: stores that must
-                 // not land anywhere go there (see store_all)
+                 // 32-bit offset (node * kRec), the tile spanning under kDrop bytes
+  __amdgpu_buffer_rsrc_t rec;   // the tile's records as a buffer resource
+  __amdgpu_buffer_rsrc_t phi;   // the tile's shortcut flows (sw.phi_t row)
+  __amdgpu_buffer_rsrc_t nodef; // the tile's node flows (sw.node_t row)
 };
 
 // Tile t of a launch is tile sw.tile0 + t of the job: its sources and
 // candidate lists are the job's, its state and flow sums the launch's.
-__device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &sw, int t,
-                                           int nsrc, int n) {
+__device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const DevTree &tr, const SweepWork &sw,
+                                           int t, int nsrc, int n) {
   Lane L;
   L.t = t;
   L.lane = threadIdx.x & (kWave - 1);
@@ -66,7 +77,11 @@ __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &
   L.s = sw.tile_src[slot];  // padded with a valid source beyond nsrc
   L.rs = ix.row_off[L.s];
   L.tb = sw.state + (int64_t)t * (n + 1) * kRec;
-  L.sink = (uint32_t)n * (uint32_t)kRec;
+  L.rec = __builtin_amdgcn_make_buffer_rsrc(L.tb, 0, (int)((uint32_t)(n + 1) * (uint32_t)kRec), kBufferWord3);
+  L.phi = __builtin_amdgcn_make_buffer_rsrc(sw.phi_t + (int64_t)t * tr.bags, 0, (int)(uint32_t)(tr.bags * 8),
+                                            kBufferWord3);
+  L.nodef = __builtin_amdgcn_make_buffer_rsrc(sw.node_t + (int64_t)t * tr.bslots, 0,
+                                              (int)(uint32_t)(tr.bslots * 8), kBufferWord3);
   return L;
 }
 
@@ -75,6 +90,16 @@ __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const SweepWork &
 template <class T>
 __device__ __forceinline__ T cld(const T *p, int64_t i) {
   return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
+// Raw buffer stores (32- and 64-bit) at a byte offset of a resource.
+__device__ __forceinline__ void st32(__amdgpu_buffer_rsrc_t r, uint32_t off, int x) {
+  __builtin_amdgcn_raw_buffer_store_b32(x, r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void st64(__amdgpu_buffer_rsrc_t r, uint32_t off, uint64_t x) {
+  typedef int v2i __attribute__((ext_vector_type(2)));
+  const v2i v = {(int)(uint32_t)x, (int)(uint32_t)(x >> 32)};
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, (int)off, 0, 0);
 }
 
 // Record offset of node v (uniform).  A row access is the uniform tile base
@@ -110,27 +135,31 @@ __device__ __forceinline__ void put_dist_count(const Lane &L, uint32_t r, int d,
 }
 
 // Stores inside the pipelined steps are issued unconditionally, with lanes
-// (or whole steps) that must not write sent to the tile's sink record.  Loads
-// and stores share one completion counter, so the wait for the next step's
-// rows (issued before these stores) can then leave exactly these stores
+// (or whole steps) that must not write given the kDrop offset.  Loads and
+// stores share one completion counter, so the wait for the next step's rows
+// (issued before these stores) can then leave exactly these stores
 // outstanding; a store under a condition would make that wait drain them.
-__device__ __forceinline__ void put_dist_count_all(const Lane &L, uint32_t r, int d, uint64_t c) {
-  SD(L, r) = d | (c == 1 ? kOneFlag : 0);
-  SS(L, c == 1 ? L.sink : r) = c;
+// put: the lane's node result (distance word, and the count where it is not
+// 1) goes to record r.
+__device__ __forceinline__ void put_dist_count_all(const Lane &L, bool put, uint32_t r, int d, uint64_t c) {
+  st32(L.rec, put ? r + (uint32_t)L.lane * 4u : kDrop, d | (c == 1 ? kOneFlag : 0));
+  st64(L.rec, (put & (c != 1)) ? r + (uint32_t)kSlotOff + (uint32_t)L.lane * 8u : kDrop, c);
 }
-__device__ __forceinline__ void store_all(const Lane &L, bool put, double *p, double x) {
-  double *sink = (double *)&SS(L, L.sink);
-  *(put ? p : sink) = x;
+__device__ __forceinline__ void put_q_all(const Lane &L, bool put, uint32_t r, double q) {
+  st64(L.rec, put ? r + (uint32_t)kSlotOff + (uint32_t)L.lane * 8u : kDrop, __builtin_bit_cast(uint64_t, q));
 }
-// kCount stores to the sink after a pipelined loop's first set of row
-// loads: the loop body's first wait then sees as many stores after the rows
-// on entry as on every later iteration, and leaves them outstanding.
-// The compiler barrier keeps the repeated stores from being merged.
+__device__ __forceinline__ void put_flow_all(__amdgpu_buffer_rsrc_t res, bool put, int64_t idx, double x) {
+  st64(res, put ? (uint32_t)idx * 8u : kDrop, __builtin_bit_cast(uint64_t, x));
+}
+// kCount dropped stores after a pipelined loop's first set of row loads: the
+// loop body's first wait then sees as many stores after the rows on entry as
+// on every later iteration, and leaves them outstanding.  The compiler
+// barrier keeps the repeated stores from being merged.
 template <int kCount>
 __device__ __forceinline__ void sink_stores(const Lane &L) {
 #pragma unroll
   for (int i = 0; i < kCount; ++i) {
-    SD(L, L.sink) = 0;
+    st32(L.rec, kDrop, 0);
     __asm__ volatile("" ::: "memory");
   }
 }
@@ -454,10 +483,10 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
             if (L.lane == 0) sg.node[seg] = v;
             ++seg;
           }
-          put_dist_count_all(L, L.sink, b, c);
+          put_dist_count_all(L, false, 0, b, c);
           __asm__ volatile("" ::: "memory");  // not a dead store: the waits count it
         } else {
-          put_dist_count_all(L, fin ? roff(v) : L.sink, b, c);
+          put_dist_count_all(L, fin, roff(v), b, c);
           ovf |= fin & L.valid & (c == ~0ull);
         }
       }
@@ -590,8 +619,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
                                     kWidth > 3 ? f[3 % kWidth] : 0.0);
         const int e = row_pick(L.lane, cld(sent, 4 * j), cld(sent, 4 * j + 1), cld(sent, 4 * j + 2),
                                cld(sent, 4 * j + 3));
-        store_all(L, live & ((L.lane & 15) == 0) & ((L.lane >> 4) < nb),
-                  sw.phi_t + (int64_t)L.t * tr.bags + e, fs);
+        put_flow_all(L.phi, live & ((L.lane & 15) == 0) & ((L.lane >> 4) < nb), e, fs);
         // The node's dependency part, summed piece by piece in entry order.
         const double tot = pfirst ? acc : carry + acc;
         if (live) carry = tot;
@@ -602,14 +630,14 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
             if (L.lane == 0) sg.node[seg] = v;
             ++seg;
           }
-          setQ(L, L.sink, 0.0);
+          put_q_all(L, false, 0, 0.0);
           __asm__ volatile("" ::: "memory");  // not a dead store: the waits count it
           ns_in[u] = 0.0;
         } else {
           double q = 0.0;
           if (fin)  // uniform branch: only the last piece divides, where some count is not 1
             q = __builtin_amdgcn_ballot_w64(!one_of(wv[u])) ? (1.0 + tot) / svv : 1.0 + tot;
-          setQ(L, fin ? roff(v) : L.sink, q);
+          put_q_all(L, fin, roff(v), q);
           ns_in[u] = (fin & L.valid & (L.s != v)) ? tot : 0.0;
           fin_mask |= (int)fin << u;
         }
@@ -623,8 +651,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
       const int u = L.lane >> 4, p0 = first + base;
       const int ps = row_pick(L.lane, p0 + min(j0, jn - 1), p0 + min(j0 + 1, jn - 1),
                               p0 + min(j0 + 2, jn - 1), p0 + min(j0 + 3, jn - 1));
-      store_all(L, ((L.lane & 15) == 0) & (u < kNodesB) & ((fin_mask >> u) & 1),
-                sw.node_t + (int64_t)L.t * tr.bslots + ps, ns);
+      put_flow_all(L.nodef, ((L.lane & 15) == 0) & (u < kNodesB) & ((fin_mask >> u) & 1), ps, ns);
       wait_vm_left<kNodesB * (1 + kWidth) + 2 * kNodesB + 1>();
     }
   }
@@ -649,7 +676,7 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
   __shared__ uint64_t seg_x[kSweepMaxSegments][kWave], seg_y[kSweepMaxSegments][kWave];
   __shared__ int seg_node[kSweepMaxSegments];
   const Segs sg{seg_x, seg_y, seg_node};
-  const Lane L = lane_setup(ix, sw, blockIdx.x, nsrc, tr.n);
+  const Lane L = lane_setup(ix, tr, sw, blockIdx.x, nsrc, tr.n);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int H = tr.height;
   int ovf = 0;
@@ -1055,8 +1082,9 @@ namespace bcx_dev {
 int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   const int n = c->ix.n;
   const int64_t B = c->tree.bags;
-  if (((uint64_t)n + 1) * (uint64_t)kRec > 0x100000000ull)
-    throw bcx::Error{BCX_EGRAPH, "bag-sweep records of one tile exceed 4 GiB (more than 5592404 vertices)"};
+  if (((uint64_t)n + 1) * (uint64_t)kRec > (uint64_t)kDrop || (uint64_t)B * 8 > (uint64_t)kDrop ||
+      (uint64_t)c->tree.bslots * 8 > (uint64_t)kDrop)
+    throw bcx::Error{BCX_EGRAPH, "bag-sweep tile state exceeds its 32-bit offsets (more than 5242879 vertices)"};
   const int total_tiles = (int)(((int64_t)count + kWave - 1) / kWave);
   int tiles = std::max(1, std::min(c->tn.sweep_batch / kWave, std::max(1, total_tiles)));
   // Keep the per-tile state within 85% of the device memory not already held
