@@ -36,6 +36,8 @@ HBM_PEAK_GBS = 8000.0
 SWEEP_KERNEL = "sweep_tile_kernel"
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 COPY_BYTES = 1 << 30  # the --profile-only device copy
+READ_KERNEL = "stream_read_kernel"
+READ_BYTES = 4 << 30  # the --profile-only streaming read, larger than the 256 MB MALL
 XCDS, CUS, SIMDS = 8, 256, 1024  # MI355X: accelerator dies, compute units, SIMDs
 PROFILES = ROOT / "profiles"
 
@@ -89,11 +91,17 @@ def summarize_pmc(fetch_csv: str, write_csv: str) -> str:
     wf = (COPY_BYTES / 1024.0) / wc
     fetch = fk * 1024.0 * ff
     write = wk * 1024.0 * wf
-    return (f"| counter | sweep kernel (KiB) | 1 GiB copy (KiB) | scale | bytes per launch |\n"
-            f"|---|---|---|---|---|\n"
-            f"| FETCH_SIZE | {fk:.0f} | {fc:.0f} | {ff:.3f} | {fetch:.4e} |\n"
-            f"| WRITE_SIZE | {wk:.0f} | {wc:.0f} | {wf:.3f} | {write:.4e} |\n\n"
-            f"traffic_bytes_per_launch: {fetch + write:.6e}\n")
+    out = (f"| counter | sweep kernel (KiB) | 1 GiB copy (KiB) | scale | bytes per launch |\n"
+           f"|---|---|---|---|---|\n"
+           f"| FETCH_SIZE | {fk:.0f} | {fc:.0f} | {ff:.3f} | {fetch:.4e} |\n"
+           f"| WRITE_SIZE | {wk:.0f} | {wc:.0f} | {wf:.3f} | {write:.4e} |\n")
+    try:
+        fr = pick(f, READ_KERNEL)
+        out += (f"\nCross-check of the FETCH_SIZE scale: the 4 GiB streaming read (`{READ_KERNEL}`) reads "
+                f"{fr:.0f} KiB, scale {(READ_BYTES / 1024.0) / fr:.3f}.\n")
+    except SystemExit:
+        pass
+    return out + f"\ntraffic_bytes_per_launch: {fetch + write:.6e}\n"
 
 
 def summarize_counters(csv_paths) -> str:
@@ -256,7 +264,9 @@ def main() -> int:
         ctx.bc_sources(cyc(n // 3, B), bcx.MODE_BATCHED)
         ctx.bc_sources(cyc(0, (W + K) * Bs), bcx.MODE_SWEEP)
         ms = ctx.copy_probe(COPY_BYTES)
-        log(f"[bench] profile-only done ({W + K} launches of {Bs} sources); 1 GiB device copy {ms:.3f} ms")
+        rms = ctx.read_probe(READ_BYTES)
+        log(f"[bench] profile-only done ({W + K} launches of {Bs} sources); 1 GiB device copy {ms:.3f} ms; "
+            f"4 GiB streaming read {rms:.3f} ms ({READ_BYTES / rms / 1e6:.0f} GB/s)")
         return 0
 
     checks = {}

@@ -104,6 +104,7 @@ _SIGS = {
     "bcx_ctx_timing_reset": (None, [_P]),
     "bcx_ctx_set_profiling": (C.c_int, [_P, C.c_int]),
     "bcx_ctx_copy_probe": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_double)]),
+    "bcx_ctx_read_probe": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_double)]),
     "bcx_full_bc": (C.c_int, [_P, C.c_int, _F64P, C.POINTER(C.c_uint32)]),
     "bcx_bc_sources": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, _F64P, C.POINTER(C.c_uint32)]),
     "bcx_query_dist_count": (C.c_int, [_P, C.c_int32, _I32P, _U64P, _U32P]),
@@ -346,6 +347,12 @@ class Context:
         """Milliseconds for one device-to-device copy of nbytes."""
         ms = C.c_double()
         _check(lib().bcx_ctx_copy_probe(self._h, nbytes, C.byref(ms)))
+        return ms.value
+
+    def read_probe(self, nbytes: int) -> float:
+        """Milliseconds for one streaming read of nbytes (multiple of 16)."""
+        ms = C.c_double()
+        _check(lib().bcx_ctx_read_probe(self._h, nbytes, C.byref(ms)))
         return ms.value
 
     def timing(self) -> dict:

@@ -14,11 +14,12 @@
 #include "bcx_host.hpp"
 #include "bcx_internal.hpp"
 
-#define HIP_OK(x)                                                                  \
-  do {                                                                             \
-    hipError_t e_ = (x);                                                           \
-    if (e_ != hipSuccess)                                                          \
-      throw bcx::Error{BCX_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)}; \
+#define HIP_OK(x)                                                                            \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess)                                                                    \
+      throw bcx::Error{e_ == hipErrorOutOfMemory ? BCX_ENOMEM : BCX_EDEVICE,                 \
+                       std::string(#x) + ": " + hipGetErrorString(e_)};                      \
   } while (0)
 
 namespace bcx_dev {
@@ -300,8 +301,23 @@ struct bcx_ctx {
   static size_t slot_bytes(int n) {
     return (size_t)n * (4 + 8 + 4 + 4 + 8) + ((size_t)n + 2) * 4;
   }
+  // The label-scan workspace for `slots` sources.  When device memory runs
+  // out and the bag sweep's buffers are held, they are released (they are
+  // re-allocated on the sweep's next use) and the allocation is tried again,
+  // as the sweep releases this workspace in the other direction.
   void ensure_ws(int slots) {
     if (slots <= ws_slots) return;
+    try {
+      alloc_ws(slots);
+    } catch (const bcx::Error &e) {
+      if (e.code != BCX_ENOMEM || sweep_tiles == 0) throw;
+      (void)hipGetLastError();  // clear the failed allocation's error
+      if (stream) HIP_OK(hipStreamSynchronize(stream));
+      free_sweep();
+      alloc_ws(slots);
+    }
+  }
+  void alloc_ws(int slots) {
     free_ws();
     const size_t n = (size_t)ix.n;
     HIP_OK(hipMalloc(&ws.d, n * slots * 4));

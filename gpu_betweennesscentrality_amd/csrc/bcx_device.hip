@@ -1620,11 +1620,15 @@ int bcx_rq_poll(bcx_rq *rq, int64_t id, double *bc_out) {
 }  // extern "C"
 
 // ---- autotuner --------------------------------------------------------------
-// The two phases are tuned separately on one sample batch: the target chunk
-// of the label scan (targets per wavefront work item) by the scan time alone,
-// the width of the dependency group (threads sharing one source's layers) by
-// the dependency time alone.  Each candidate runs once to warm up and once
-// timed with HIP events on the context stream.
+// Label scan: the two phases are tuned separately on one sample batch: the
+// target chunk (targets per wavefront work item) by the scan time alone, the
+// width of the dependency group (threads sharing one source's layers) by the
+// dependency time alone.  Bag sweep: tiles per compute unit (1 to 4) and the
+// tile-kernel form (one workgroup per unit with 2 pieces per step, or two
+// with 1 piece per step), each timed on one launch of the sample's sources
+// (repeated to fill it) by its tile kernel and reduction per source;
+// launches that device memory would cut short are skipped.  Each candidate
+// runs once to warm up and once timed with HIP events on the context stream.
 extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t count,
                                 bcx_tuning *best_out) {
   BCX_GUARD_BEGIN
@@ -1632,14 +1636,14 @@ extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t coun
   HIP_OK(hipSetDevice(c->device));
   const bcx_tuning start = c->tn;
   const int32_t cnt = std::min(count, start.batch);
-  std::vector<double> scratch((size_t)c->ix.n, 0.0);
+  std::vector<double> bc_tmp((size_t)c->ix.n, 0.0);
   const bool was_profiling = c->profiling;
   c->profiling = true;
   auto trial = [&](const bcx_tuning &t, double &scan, double &dep) {
     c->tn = t;
-    run_sources(c, BCX_MODE_BATCHED, sources, cnt, scratch.data(), nullptr);  // warm-up
+    run_sources(c, BCX_MODE_BATCHED, sources, cnt, bc_tmp.data(), nullptr);  // warm-up
     c->timing = bcx_timing{};
-    run_sources(c, BCX_MODE_BATCHED, sources, cnt, scratch.data(), nullptr);
+    run_sources(c, BCX_MODE_BATCHED, sources, cnt, bc_tmp.data(), nullptr);
     scan = c->timing.scan_ms;
     dep = c->timing.dep_ms;
   };
@@ -1666,6 +1670,32 @@ extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t coun
         best.bp_threads = bp;
       }
     }
+    const int32_t n = c->ix.n;
+    const int32_t most = std::min<int64_t>((int64_t)4 * c->cus * kWave, std::max<int64_t>(n, kWave));
+    std::vector<int32_t> sample((size_t)most);
+    for (int32_t i = 0; i < most; ++i) sample[(size_t)i] = sources[i % count];
+    double best_per = 1e300;
+    for (int tpc = 1; tpc <= 4; ++tpc) {
+      for (int form : {BCX_SWEEP_FORM_ONE_PER_CU, BCX_SWEEP_FORM_TWO_PER_CU}) {
+        bcx_tuning t = best;
+        t.sweep_batch = tpc * c->cus * kWave;
+        t.sweep_form = form;
+        c->tn = t;
+        if (t.sweep_batch > most) continue;  // more sources than the graph has
+        if (sweep_tiles_for(c, t.sweep_batch) * kWave < t.sweep_batch) continue;  // cut by memory
+        c->profiling = false;
+        run_sweep(c, sample.data(), t.sweep_batch, bc_tmp.data(), nullptr);  // warm-up (and capture)
+        c->profiling = true;
+        c->timing = bcx_timing{};
+        run_sweep(c, sample.data(), t.sweep_batch, bc_tmp.data(), nullptr);
+        const double per = (c->timing.scan_ms + c->timing.reduce_ms) / t.sweep_batch;
+        if (per < best_per) {
+          best_per = per;
+          best.sweep_batch = t.sweep_batch;
+          best.sweep_form = form;
+        }
+      }
+    }
   } catch (...) {
     c->tn = start;
     c->profiling = was_profiling;
@@ -1675,6 +1705,43 @@ extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t coun
   c->profiling = was_profiling;
   c->timing = bcx_timing{};
   if (best_out) *best_out = best;
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+// Streaming read of `words` 16-byte words (grid-stride, one 16-byte load
+// per lane per step), for calibrating the HBM read counters; the sum keeps
+// the loads alive and goes to *out.
+__global__ __launch_bounds__(256) void stream_read_kernel(const int4 *p, int64_t words, int *out) {
+  int acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x) {
+    const int4 v = p[i];
+    acc ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  if (acc == 0x7fffffff) out[0] = acc;  // never true for the probe's zero-filled data
+}
+
+// Measurement hook: one streaming read of `bytes` (a multiple of 16) on the
+// context stream, timed with HIP events; no reference counterpart.
+extern "C" int bcx_ctx_read_probe(bcx_ctx *c, size_t bytes, double *ms) {
+  BCX_GUARD_BEGIN
+  if (!c || bytes < 16 || bytes % 16) return bcx::fail(BCX_EINVAL, "bad argument");
+  HIP_OK(hipSetDevice(c->device));
+  void *a = nullptr;
+  HIP_OK(hipMalloc(&a, bytes));
+  float t = 0;
+  hipError_t e = hipMemsetAsync(a, 0, bytes, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(c->ev[0], c->stream);
+  if (e == hipSuccess) {
+    stream_read_kernel<<<c->cus * 16, 256, 0, c->stream>>>((const int4 *)a, (int64_t)(bytes / 16), c->d_ovf + 1);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
+  (void)hipFree(a);
+  HIP_OK(e);
+  if (ms) *ms = t;
   return BCX_OK;
   BCX_GUARD_END
 }

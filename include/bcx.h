@@ -222,9 +222,12 @@ void bcx_ctx_free(bcx_ctx *ctx);
 int bcx_ctx_set_tuning(bcx_ctx *ctx, const bcx_tuning *tn);
 int bcx_ctx_get_tuning(const bcx_ctx *ctx, bcx_tuning *tn);
 /* Autotuner (no reference counterpart: the reference fixes 256x256 launches
- * at compile time, MHLQBC.cu:3).  Times candidate target chunks and
- * dependency-group widths on `sources` (one batch at most), keeps the fastest
- * in the context and copies it to *best (may be NULL). */
+ * at compile time, MHLQBC.cu:3).  Times candidate label-scan target chunks
+ * and dependency-group widths on `sources` (one batch at most), and bag-sweep
+ * launch sizes (1 to 4 tiles of 64 sources per compute unit, those that fit
+ * device memory) in both tile-kernel forms on `sources` repeated to fill a
+ * launch; keeps the fastest of each in the context and copies the tuning to
+ * *best (may be NULL). */
 int bcx_ctx_autotune(bcx_ctx *ctx, const int32_t *sources, int32_t count,
                      bcx_tuning *best);
 /* Sources one bag-sweep launch takes (the tuning's sweep_batch, cut to whole
@@ -241,6 +244,10 @@ int bcx_ctx_set_profiling(bcx_ctx *ctx, int on);
 /* Measurement hook (no reference counterpart): one device-to-device copy of
  * `bytes`, timed; used to calibrate HBM traffic counters. */
 int bcx_ctx_copy_probe(bcx_ctx *ctx, size_t bytes, double *ms);
+/* Measurement hook (no reference counterpart): one streaming read of `bytes`
+ * (a multiple of 16, 16-byte loads), timed; calibrates the HBM read
+ * counters beside the copy. */
+int bcx_ctx_read_probe(bcx_ctx *ctx, size_t bytes, double *ms);
 
 /* ---- hot path --------------------------------------------------------- */
 /* Full BC over all sources: bc_out[n] (undirected, unnormalised, each

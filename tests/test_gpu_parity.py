@@ -681,3 +681,21 @@ def test_weights_summing_past_inf():
     for mode in FORMS:
         bc, ovf = ctx.full_bc(mode)
         assert not ovf and close_bc(bc, np.array([0.0, 1.0, 0.0])), f"triangle, mode {mode}"
+
+
+def test_autotune_picks_a_sweep_launch_and_stays_exact(road):
+    """The autotuner times bag-sweep launch sizes and tile-kernel forms; the
+    tuning it keeps gives the oracle's BC."""
+    g, csr, _ = road
+    G, ix, ctx = build(g)
+    default = ctx.tuning()["sweep_batch"]
+    cus = default // (3 * 64)
+    best = ctx.autotune(gg.permutation(12, 1, g.n)[:4096].astype(np.int32))
+    assert best["sweep_form"] in (bcx.SWEEP_FORM_ONE_PER_CU, bcx.SWEEP_FORM_TWO_PER_CU)
+    assert best["sweep_batch"] % (64 * cus) == 0 and 1 <= best["sweep_batch"] // (64 * cus) <= 4
+    assert ctx.tuning() == best
+    src = gg.permutation(13, 1, g.n)[:3000].astype(np.int32)
+    got, ovf = ctx.bc_sources(src, bcx.MODE_SWEEP)
+    want, _ = oracle.brandes(csr, src, threads=oracle.default_threads())
+    assert not ovf and close_bc(got, want)
+    ctx.close()
