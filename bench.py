@@ -435,12 +435,15 @@ def main() -> int:
 
 
 def _h2h_bench(bcx, G, ix, tree_perm, n: int, sources: int = 512) -> dict:
-    """QBC's distance-only path on the same index: H2H labels built on the
-    host, then the device separator scan for `sources` sources (one block of
-    the tree order), timed by HIP events around its kernels."""
+    """QBC's distance-only path on the same index: H2H labels, optimizeVS
+    separators and P2H branch tables built on the host, then the device
+    separator scan for `sources` sources (one block of the tree order), timed
+    by HIP events around its kernels; the mean scan length with and without
+    the P2H tables beside it."""
     t0 = time.perf_counter()
     ix.build_h2h()
     build_s = time.perf_counter() - t0
+    scan_sep, scan_p2h = ix.h2h_scan_stats(1 << 18, 1)
     ctx = bcx.Context(G, ix, 0)
     src = tree_perm[(n - sources) // 2:(n - sources) // 2 + sources]
     ctx.h2h_distances(src[:8])  # warm-up
@@ -453,6 +456,7 @@ def _h2h_bench(bcx, G, ix, tree_perm, n: int, sources: int = 512) -> dict:
     ctx.set_profiling(False)
     ctx.close()
     out = {"sources": int(len(src)), "label_build_s": round(build_s, 2),
+           "mean_scan_optimizeVS": round(scan_sep, 2), "mean_scan_p2h": round(scan_p2h, 2),
            "kernel_ms": round(tm["scan_ms"], 3),
            "sources_per_s": round(len(src) / (tm["scan_ms"] * 1e-3), 2),
            "with_copy_back_sources_per_s": round(len(src) / wall, 2)}

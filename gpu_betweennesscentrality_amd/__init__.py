@@ -83,6 +83,8 @@ _SIGS = {
     "bcx_index_mean_scan": (C.c_int, [_P, C.c_int64, C.c_uint64, C.POINTER(C.c_double)]),
     "bcx_index_build_h2h": (C.c_int, [_P]),
     "bcx_min_degree_order": (C.c_int, [_P, _I32P, C.POINTER(C.c_int64), _I32P]),
+    "bcx_index_h2h_scan_stats": (C.c_int, [_P, C.c_int64, C.c_uint64, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double)]),
     "bcx_index_h2h_query": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
     "bcx_index_tree_order": (C.c_int, [_P, _I32P]),
     "bcx_index_tree": (C.c_int, [_P, _I32P, _I32P]),
@@ -295,6 +297,12 @@ class Index:
         d = C.c_int32()
         _check(lib().bcx_index_h2h_query(self._h, s, t, C.byref(d)))
         return d.value
+
+    def h2h_scan_stats(self, pairs: int = 1 << 18, seed: int = 1) -> tuple[float, float]:
+        """Mean H2H scan length: optimizeVS separators only, and with P2H."""
+        a, b = C.c_double(), C.c_double()
+        _check(lib().bcx_index_h2h_scan_stats(self._h, pairs, seed, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def mean_scan(self, pairs: int = 1 << 18, seed: int = 1) -> float:
         m = C.c_double()

@@ -150,6 +150,18 @@ int bcx_index_h2h_query(const bcx_index *ix, int32_t s, int32_t t, int32_t *d) {
   BCX_GUARD_END
 }
 
+int bcx_index_h2h_scan_stats(const bcx_index *ix, int64_t pairs, uint64_t seed, double *mean_separator,
+                             double *mean_p2h) {
+  BCX_GUARD_BEGIN
+  if (!ix || pairs < 0) return bcx::fail(BCX_EINVAL, "bad argument");
+  double a = 0, b = 0;
+  bcx::h2h_scan_stats(ix->ix, pairs, seed, a, b);
+  if (mean_separator) *mean_separator = a;
+  if (mean_p2h) *mean_p2h = b;
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
 int bcx_index_stats_get(const bcx_index *ix, bcx_index_stats *st) {
   if (!ix || !st) return bcx::fail(BCX_EINVAL, "null argument");
   const auto &x = ix->ix;

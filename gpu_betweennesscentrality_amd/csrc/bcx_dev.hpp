@@ -257,6 +257,11 @@ struct bcx_ctx {
   const int64_t *d_sep_off = nullptr;
   const int *d_sep = nullptr;
   const int *d_iperm = nullptr;  // vertex id -> preorder id
+  // P2H branch tables (see bcx::Index): per-node slot, per-slot base, offsets, lists.
+  const int *d_p2h_slot = nullptr;
+  const int64_t *d_p2h_base = nullptr;
+  const int64_t *d_p2h_off = nullptr;
+  const int *d_p2h = nullptr;
 
   ~bcx_ctx() {
     if (hipSetDevice(device) != hipSuccess) return;

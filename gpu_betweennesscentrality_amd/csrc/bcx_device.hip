@@ -57,11 +57,21 @@ __device__ __forceinline__ int lca_depth_dev(const DevIndex &ix, int u, int v) {
 // table; if s or t is the LCA, one label lookup; otherwise the minimum over
 // the LCA's separator of label(s) + label(t).  Labels are read in the
 // interleaved layout (gdis, goff); output in preorder ids.
+// P2H tables on the device: slot[node] = b for the kept branch nodes (-1
+// elsewhere); the list of node s below branch node a with slot b is
+// list[off[base[b] + s - a] .. off[base[b] + s - a + 1]).
+struct DevP2H {
+  const int *slot;
+  const int64_t *base;
+  const int64_t *off;
+  const int *list;
+};
+
 __global__ __launch_bounds__(256) void h2h_dist_kernel(DevIndex ix, const int *lvl_off,
                                                        const int *lvl_v, const int *gdis,
                                                        const int64_t *goff,
                                                        const int64_t *sep_off, const int *sep,
-                                                       const int *srcs, int *out) {
+                                                       DevP2H p2h, const int *srcs, int *out) {
   extern __shared__ int h2h_lds[];
   const int n = ix.n;
   const int s = srcs[blockIdx.x];
@@ -94,8 +104,20 @@ __global__ __launch_bounds__(256) void h2h_dist_kernel(DevIndex ix, const int *l
       } else {
         const int a = path[L];
         int best = 0x7fffffff;  // label values stay <= 999999999, sums fit
-        for (int64_t i = sep_off[a], e = sep_off[a + 1]; i < e; ++i) {
-          const int k = sep[i];
+        // The LCA's separator (QBC.cu:313-329), or, at a P2H branch node,
+        // the shorter of the two ends' projected lists (QBC.cu:290-312).
+        const int *list = sep + sep_off[a];
+        int64_t len = sep_off[a + 1] - sep_off[a];
+        const int b = p2h.slot[a];
+        if (b >= 0) {
+          const int64_t o = p2h.base[b] - a;
+          const int64_t os = p2h.off[o + s], ls = p2h.off[o + s + 1] - os;
+          const int64_t ot = p2h.off[o + t], lt = p2h.off[o + t + 1] - ot;
+          list = p2h.list + (ls <= lt ? os : ot);
+          len = ls <= lt ? ls : lt;
+        }
+        for (int64_t i = 0; i < len; ++i) {
+          const int k = list[i];
           best = min(best, srow[k] + rt[64 * k]);
         }
         d = min(best, kInfDev);
@@ -641,6 +663,12 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     c->d_goff = (const int64_t *)keep(dev_upload(goff));
     c->d_sep_off = (const int64_t *)keep(dev_upload(ix.sep_off));
     c->d_sep = (const int *)keep(dev_upload(ix.sep));
+    std::vector<int32_t> slot((size_t)n, -1);
+    for (size_t b = 0; b < ix.p2h_node.size(); ++b) slot[(size_t)ix.p2h_node[b]] = (int32_t)b;
+    c->d_p2h_slot = (const int *)keep(dev_upload(slot));
+    c->d_p2h_base = (const int64_t *)keep(dev_upload(ix.p2h_base));
+    c->d_p2h_off = (const int64_t *)keep(dev_upload(ix.p2h_off));
+    c->d_p2h = (const int *)keep(dev_upload(ix.p2h));
   }
   // Tree view for the bag sweep: bags, inverse bags and depth levels.
   {
@@ -1230,7 +1258,8 @@ int bcx_h2h_distances(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t
       const int slices = (int)std::max<int64_t>(
           1, std::min<int64_t>((int64_t)(n + 255) / 256, (64ll * c->cus + nb - 1) / nb));
       h2h_dist_kernel<<<dim3((unsigned)nb, (unsigned)slices), 256, lds, c->stream>>>(
-          c->ix, c->tree.lvl_off, c->tree.lvl_v, c->d_gdis, c->d_goff, c->d_sep_off, c->d_sep, d_src, d_out);
+          c->ix, c->tree.lvl_off, c->tree.lvl_v, c->d_gdis, c->d_goff, c->d_sep_off, c->d_sep,
+          DevP2H{c->d_p2h_slot, c->d_p2h_base, c->d_p2h_off, c->d_p2h}, d_src, d_out);
       HIP_OK(hipGetLastError());
       if (c->profiling) {
         HIP_OK(hipEventRecord(c->ev[1], c->stream));

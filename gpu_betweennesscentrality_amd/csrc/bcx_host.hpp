@@ -54,6 +54,14 @@ struct Index {
   std::vector<int32_t> gdis;
   std::vector<int64_t> sep_off;  // n + 1 (empty until build_h2h)
   std::vector<int32_t> sep;      // ancestor depths, the node's own depth last
+  // P2H branch tables (built with the H2H labels): for each kept branch node
+  // b (preorder id p2h_node[b]) and each node s of its subtree, the depths of
+  // b's separator that no other separator vertex dominates from s; list of s
+  // = p2h[p2h_off[p2h_base[b] + s - p2h_node[b]] .. next).
+  std::vector<int32_t> p2h_node;
+  std::vector<int64_t> p2h_base;
+  std::vector<int64_t> p2h_off;
+  std::vector<int32_t> p2h;
   // Fingerprint of the graph the index was built for (arc count and an
   // order-independent hash of its arcs; 0 when unknown).
   int64_t fp_arcs = 0;
@@ -98,6 +106,9 @@ void build_h2h(Index &ix);
 // Distance by the H2H separator scan (QBC.cu:271-331 without the P2H branch
 // table); vertex ids; requires build_h2h.
 int32_t h2h_query(const Index &ix, int32_t s, int32_t t);
+// Mean scan length (label entries read) of the H2H distance query over
+// `pairs` seeded random pairs, with and without the P2H tables.
+void h2h_scan_stats(const Index &ix, int64_t pairs, uint64_t seed, double &mean_sep, double &mean_p2h);
 void save_index(const Index &ix, const char *path);
 void load_index(const char *path, Index &ix);
 // Throws BCX_EGRAPH when shortest distances may reach kInf.

@@ -488,6 +488,157 @@ void min_degree_order(const Graph &g, std::vector<int32_t> &order, int64_t *shor
 // the one with the most depths unique to it (any two children include one of
 // the others, and a path between two child subtrees crosses both children's
 // bags); in every case the node's own depth last.
+namespace {
+
+// Depth of LCA and the LCA's preorder id for preorder ids p, q.
+int32_t lca_node(const Index &ix, int32_t p, int32_t q) {
+  if (p > q) std::swap(p, q);
+  while (!(p <= q && q < p + ix.subtree[(size_t)p])) p = ix.parent[(size_t)p];
+  return p;
+}
+
+// P2H list of node s below branch node v (preorder ids), or null when v has
+// no table.
+const int32_t *p2h_list(const Index &ix, int32_t v, int32_t s, int64_t &len) {
+  for (size_t b = 0; b < ix.p2h_node.size(); ++b)
+    if (ix.p2h_node[b] == v) {
+      const int64_t i = ix.p2h_base[b] + (s - v);
+      len = ix.p2h_off[(size_t)i + 1] - ix.p2h_off[(size_t)i];
+      return ix.p2h.data() + ix.p2h_off[(size_t)i];
+    }
+  return nullptr;
+}
+
+// The P2H branch tables of QBC (SVSP_P2H and P2H_3, buildIndex.cpp:565-719),
+// with the reference's parameters k = 10, d = 5, R = 1e6 (QBC.cu:431) and a
+// fixed seed instead of the clock (buildIndex.cpp:608):
+//  1. branch nodes (two or more children) scored by the share of vertex
+//     pairs whose LCA they are, 2 sum_{i<j} |T_i| |T_j| / n^2 over their
+//     child subtrees; the k best (ties to the smaller vertex id);
+//  2. for each, and each node s of its subtree, the separator depths c such
+//     that no other separator vertex u has d(s,u) <= d(s,c) and
+//     d(s,c) = d(s,u) + d(u,c): a path from s through c can go through u
+//     instead, so the minimum over s's list alone is the distance;
+//  3. R seeded random pairs whose LCA is one of them (neither end the LCA)
+//     credit it |separator| - |shorter of the two lists|; the d best keep
+//     their tables.
+void build_p2h(Index &ix, int k, int d, int64_t R, uint64_t seed) {
+  const int32_t n = ix.n;
+  ix.p2h_node.clear();
+  ix.p2h_base.clear();
+  ix.p2h_off.clear();
+  ix.p2h.clear();
+  std::vector<int32_t> nch((size_t)n, 0);
+  for (int32_t p = 1; p < n; ++p) nch[(size_t)ix.parent[(size_t)p]]++;
+  std::vector<std::pair<double, int32_t>> score;  // (p, vertex id)
+  {
+    std::vector<double> acc((size_t)n, 0.0), sq((size_t)n, 0.0);
+    for (int32_t p = 1; p < n; ++p) {
+      const double t = ix.subtree[(size_t)p];
+      acc[(size_t)ix.parent[(size_t)p]] += t;
+      sq[(size_t)ix.parent[(size_t)p]] += t * t;
+    }
+    for (int32_t p = 0; p < n; ++p)
+      if (nch[(size_t)p] >= 2)  // sum_{i<j} a_i a_j = ((sum a)^2 - sum a^2) / 2
+        score.emplace_back((acc[(size_t)p] * acc[(size_t)p] - sq[(size_t)p]) / ((double)n * n),
+                           ix.perm[(size_t)p]);
+  }
+  std::sort(score.begin(), score.end(), [](const auto &a, const auto &b) {
+    return a.first != b.first ? a.first > b.first : a.second < b.second;
+  });
+  if ((int)score.size() > k) score.resize((size_t)k);
+  if (score.empty()) return;
+  // Tables of the candidates.
+  struct Table {
+    int32_t v;
+    std::vector<int64_t> off;  // subtree size + 1
+    std::vector<int32_t> list;
+  };
+  std::vector<Table> tabs(score.size());
+  for (size_t b = 0; b < score.size(); ++b) {
+    Table &T = tabs[b];
+    const int32_t v = ix.iperm[(size_t)score[b].second];
+    T.v = v;
+    const int32_t size = ix.subtree[(size_t)v];
+    const int64_t s0 = ix.sep_off[(size_t)v], s1 = ix.sep_off[(size_t)v + 1];
+    const int32_t m = (int32_t)(s1 - s0);
+    const int32_t *sepv = ix.sep.data() + s0;
+    // Ancestors of v by depth, then d(u, c) between separator vertices: the
+    // deeper one's label at the shallower one's depth.
+    const int32_t Dv = ix.depth[(size_t)v];
+    std::vector<int32_t> anc((size_t)Dv + 1);
+    for (int32_t a = v, j = Dv; j >= 0; a = ix.parent[(size_t)a], --j) anc[(size_t)j] = a;
+    std::vector<int64_t> duc((size_t)m * m, 0);
+    for (int32_t i = 0; i < m; ++i)
+      for (int32_t j = 0; j < m; ++j) {
+        const int32_t lo = std::max(sepv[i], sepv[j]), hi = std::min(sepv[i], sepv[j]);
+        duc[(size_t)i * m + j] = ix.gdis[(size_t)(ix.row_off[(size_t)anc[(size_t)lo]] + hi)];
+      }
+    std::vector<std::vector<int32_t>> lists((size_t)size);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int32_t r = 1; r < size; ++r) {
+      const int32_t s = v + r;
+      const int64_t row = ix.row_off[(size_t)s];
+      auto &out = lists[(size_t)r];
+      for (int32_t c = 0; c < m; ++c) {
+        const int64_t dsc = ix.gdis[(size_t)(row + sepv[c])];
+        bool keep = true;
+        for (int32_t u = 0; u < m && keep; ++u) {
+          if (u == c) continue;
+          const int64_t dsu = ix.gdis[(size_t)(row + sepv[u])];
+          if (dsu <= dsc && dsc == dsu + duc[(size_t)u * m + c]) keep = false;
+        }
+        if (keep) out.push_back(sepv[c]);
+      }
+    }
+    T.off.assign((size_t)size + 1, 0);
+    for (int32_t r = 0; r < size; ++r) T.off[(size_t)r + 1] = T.off[(size_t)r] + (int64_t)lists[(size_t)r].size();
+    T.list.reserve((size_t)T.off[(size_t)size]);
+    for (auto &l : lists) T.list.insert(T.list.end(), l.begin(), l.end());
+  }
+  // Credit each candidate over R seeded random pairs (splitmix64 stream).
+  std::vector<double> gain(tabs.size(), 0.0);
+  uint64_t z = seed;
+  auto next = [&]() {
+    z += 0x9E3779B97F4A7C15ull;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+  };
+  for (int64_t i = 0; i < R; ++i) {
+    const int32_t p = (int32_t)(next() % (uint64_t)n), q = (int32_t)(next() % (uint64_t)n);
+    if (p == q) {
+      --i;
+      continue;
+    }
+    const int32_t a = lca_node(ix, p, q);
+    if (a == p || a == q) continue;
+    for (size_t b = 0; b < tabs.size(); ++b)
+      if (tabs[b].v == a) {
+        const auto &T = tabs[b];
+        const int64_t lp = T.off[(size_t)(p - a) + 1] - T.off[(size_t)(p - a)];
+        const int64_t lq = T.off[(size_t)(q - a) + 1] - T.off[(size_t)(q - a)];
+        gain[b] += (double)(ix.sep_off[(size_t)a + 1] - ix.sep_off[(size_t)a]) - (double)std::min(lp, lq);
+      }
+  }
+  std::vector<size_t> ord(tabs.size());
+  for (size_t b = 0; b < ord.size(); ++b) ord[b] = b;
+  std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return gain[a] > gain[b]; });
+  if ((int)ord.size() > d) ord.resize((size_t)d);
+  std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return tabs[a].v < tabs[b].v; });
+  for (size_t b : ord) {
+    const Table &T = tabs[b];
+    ix.p2h_node.push_back(T.v);
+    ix.p2h_base.push_back((int64_t)ix.p2h_off.size());
+    const int64_t shift = (int64_t)ix.p2h.size();
+    for (int64_t o : T.off) ix.p2h_off.push_back(o + shift);
+    ix.p2h.insert(ix.p2h.end(), T.list.begin(), T.list.end());
+  }
+}
+
+}  // namespace
+
 void build_h2h(Index &ix) {
   const int32_t n = ix.n, H = ix.tree_height;
   ix.gdis.assign(ix.dis.size(), kInf);
@@ -584,6 +735,7 @@ void build_h2h(Index &ix) {
     ix.sep.push_back(own);
     ix.sep_off[(size_t)p + 1] = (int64_t)ix.sep.size();
   }
+  build_p2h(ix, 10, 5, 1000000, 0x5eed);
 }
 
 int32_t h2h_query(const Index &ix, int32_t s, int32_t t) {
@@ -598,11 +750,51 @@ int32_t h2h_query(const Index &ix, int32_t s, int32_t t) {
   int32_t a = p;
   while (ix.depth[(size_t)a] > L) a = ix.parent[(size_t)a];
   int64_t best = 2 * (int64_t)kInf;
-  for (int64_t i = ix.sep_off[(size_t)a]; i < ix.sep_off[(size_t)a + 1]; ++i) {
-    const int32_t k = ix.sep[(size_t)i];
+  // A P2H branch node: the shorter of the two ends' projected lists
+  // (QBC.cu:290-312); otherwise the optimizeVS separator (QBC.cu:313-329).
+  int64_t lp = 0, lq = 0;
+  const int32_t *Lp = p2h_list(ix, a, p, lp), *Lq = Lp ? p2h_list(ix, a, q, lq) : nullptr;
+  const int32_t *list = ix.sep.data() + ix.sep_off[(size_t)a];
+  int64_t len = ix.sep_off[(size_t)a + 1] - ix.sep_off[(size_t)a];
+  if (Lp) {
+    list = lp <= lq ? Lp : Lq;
+    len = std::min(lp, lq);
+  }
+  for (int64_t i = 0; i < len; ++i) {
+    const int32_t k = list[i];
     best = std::min<int64_t>(best, (int64_t)ix.gdis[(size_t)(op + k)] + ix.gdis[(size_t)(oq + k)]);
   }
   return (int32_t)std::min<int64_t>(best, kInf);
+}
+
+void h2h_scan_stats(const Index &ix, int64_t pairs, uint64_t seed, double &mean_sep, double &mean_p2h) {
+  if (ix.sep_off.empty()) throw Error{BCX_EINVAL, "H2H labels not built"};
+  uint64_t z = seed;
+  auto next = [&]() {
+    z += 0x9E3779B97F4A7C15ull;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+  };
+  double a_sep = 0, a_p2h = 0;
+  for (int64_t i = 0; i < pairs; ++i) {
+    const int32_t p = (int32_t)(next() % (uint64_t)ix.n), q = (int32_t)(next() % (uint64_t)ix.n);
+    const int32_t a = lca_node(ix, p, q);
+    if (a == p || a == q) {  // one lookup (QBC.cu:282-288)
+      a_sep += 1;
+      a_p2h += 1;
+      continue;
+    }
+    const double m = (double)(ix.sep_off[(size_t)a + 1] - ix.sep_off[(size_t)a]);
+    int64_t lp = 0, lq = 0;
+    const int32_t *Lp = p2h_list(ix, a, p, lp);
+    if (Lp) p2h_list(ix, a, q, lq);
+    a_sep += m;
+    a_p2h += Lp ? (double)std::min(lp, lq) : m;
+  }
+  mean_sep = pairs ? a_sep / (double)pairs : 0.0;
+  mean_p2h = pairs ? a_p2h / (double)pairs : 0.0;
 }
 
 // ---- binary image ---------------------------------------------------------
@@ -711,6 +903,23 @@ void validate_index(Index &ix, const std::string &what) {
     if ((int64_t)ix.sep.size() != ix.sep_off[(size_t)n]) bad("separator array of the wrong length");
     for (int64_t i = 0; i < E; ++i)
       if (ix.gdis[(size_t)i] < 0 || ix.gdis[(size_t)i] > kInf) bad("an H2H distance outside [0, INF]");
+    if (ix.p2h_base.size() != ix.p2h_node.size()) bad("P2H tables of the wrong length");
+    int64_t next = 0;
+    for (size_t b = 0; b < ix.p2h_node.size(); ++b) {
+      const int32_t v = ix.p2h_node[b];
+      if (v < 0 || v >= n || (b > 0 && v <= ix.p2h_node[b - 1])) bad("P2H branch nodes out of order");
+      if (ix.p2h_base[b] != next) bad("P2H table offsets");
+      const int64_t size = ix.subtree[(size_t)v];
+      next += size + 1;
+      if ((int64_t)ix.p2h_off.size() < next) bad("P2H offsets of the wrong length");
+      for (int64_t r = 0; r < size; ++r) {
+        const int64_t o0 = ix.p2h_off[(size_t)(ix.p2h_base[b] + r)], o1 = ix.p2h_off[(size_t)(ix.p2h_base[b] + r + 1)];
+        if (o0 < 0 || o1 < o0 || o1 > (int64_t)ix.p2h.size()) bad("P2H offsets are not monotone");
+        for (int64_t i = o0; i < o1; ++i)
+          if (ix.p2h[(size_t)i] < 0 || ix.p2h[(size_t)i] > ix.depth[(size_t)v]) bad("a P2H depth out of range");
+      }
+    }
+    if ((int64_t)ix.p2h_off.size() != next) bad("P2H offsets of the wrong length");
   }
   check_distance_bound(ix);
 }
@@ -741,6 +950,10 @@ void save_index(const Index &ix, const char *path) {
       wvec(f, ix.gdis);
       wvec(f, ix.sep_off);
       wvec(f, ix.sep);
+      wvec(f, ix.p2h_node);
+      wvec(f, ix.p2h_base);
+      wvec(f, ix.p2h_off);
+      wvec(f, ix.p2h);
     }
   } catch (...) {
     std::fclose(f);
@@ -786,6 +999,10 @@ void load_index(const char *path, Index &ix) {
       rvec(f, ix.gdis, cap);
       rvec(f, ix.sep_off, n + 1);
       rvec(f, ix.sep, cap);
+      rvec(f, ix.p2h_node, n);
+      rvec(f, ix.p2h_base, n);
+      rvec(f, ix.p2h_off, cap);
+      rvec(f, ix.p2h, cap);
     }
     char extra;
     if (std::fread(&extra, 1, 1, f) != 0) throw Error{BCX_EIO, std::string(path) + ": trailing data"};

@@ -156,13 +156,19 @@ int bcx_min_degree_order(const bcx_graph *g, int32_t *order, int64_t *shortcut_a
  * labelling of Graph::makeIndexDFS, src/buildIndex.cpp:222-278, and the
  * separator reduction Graph::optimizeVS, src/buildIndex.cpp:313-386): the
  * shortest distance from every node to each ancestor, and per node the
- * separator scanned when it is the LCA of a pair.  The P2H branch tables of
- * Graph::SVSP_P2H / P2H_3 (src/buildIndex.cpp:565-719) are not built: their
- * branch choice is seeded from the clock (src/buildIndex.cpp:608) and they
- * change which separator is scanned, never the distance.  Needed before
- * bcx_ctx_create for bcx_h2h_distances.  Host memory: 4 bytes per label
- * entry plus the separators. */
+ * separator scanned when it is the LCA of a pair; and the P2H branch tables
+ * of Graph::SVSP_P2H / P2H_3 (src/buildIndex.cpp:565-719) with the
+ * reference's k = 10, d = 5, R = 1e6 (QBC.cu:431) and a fixed seed where the
+ * reference seeds from the clock (src/buildIndex.cpp:608): for the kept
+ * branch nodes, per descendant the separator entries not dominated from it.
+ * Needed before bcx_ctx_create for bcx_h2h_distances.  Host memory: 4 bytes
+ * per label entry plus the separators and tables. */
 int bcx_index_build_h2h(bcx_index *ix);
+/* Mean number of label entries the H2H distance query reads over `pairs`
+ * seeded random pairs: with the optimizeVS separators only, and with the P2H
+ * branch tables as the device scan uses them. */
+int bcx_index_h2h_scan_stats(const bcx_index *ix, int64_t pairs, uint64_t seed,
+                             double *mean_separator, double *mean_p2h);
 /* Host-side H2H distance query (vertex ids); requires bcx_index_build_h2h. */
 int bcx_index_h2h_query(const bcx_index *ix, int32_t s, int32_t t, int32_t *d);
 /* order[p] = vertex id of the p-th tree node in DFS preorder (n entries).

@@ -6,6 +6,7 @@ LosAngeles graph and the seeded grid) and (b) the oracle's Dijkstra on all
 pairs of small graphs, including tie-heavy and overflowing unit grids.
 """
 from math import comb
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -254,3 +255,21 @@ def test_disconnected_graph_is_rejected():
     assert not G.connected()
     with pytest.raises(bcx.BcxError, match="disconnected"):
         bcx.Index.build_mhl(G)
+
+
+def test_p2h_tables_on_the_shipped_order_file():
+    """QBC's index from the shipped LosAngeles order file with the P2H branch
+    tables (SVSP_P2H / P2H_3, buildIndex.cpp:565-719; k = 10, d = 5, R = 1e6,
+    fixed seed): the reference's own H2H answers on its golden pairs hold, and
+    the mean scan falls as on the reference (optimizeVS about 61 entries, P2H
+    about 26, measured on the reference's index)."""
+    golden = Path(__file__).resolve().parent / "golden"
+    la = np.load(golden / "la_graph.npz")
+    n = int(la["n"])
+    G = bcx.Graph.from_arcs(n, la["u"], la["v"], la["w"])
+    ix = bcx.Index.build_order(G, la["order"])
+    ix.build_h2h()
+    for s, t, d in np.load(golden / "la_h2h.npz")["queries"].tolist():
+        assert ix.h2h_query(s - 1, t - 1) == d, (s, t)
+    sep, p2h = ix.h2h_scan_stats(200_000, 7)
+    assert 55 <= sep <= 67 and 22 <= p2h <= 30, (sep, p2h)

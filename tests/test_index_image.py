@@ -15,7 +15,8 @@ GOLDEN = Path(__file__).resolve().parent / "golden"
 # follow when the header's H2H flag is set).
 VECS = [("perm", "i4"), ("depth", "i4"), ("parent", "i4"), ("row_off", "i8"), ("dis", "i4"), ("cnt", "u8"),
         ("hop", "u4"), ("bag_off", "i8"), ("bag", "i4"), ("bag_len", "i4"), ("bag_cnt", "u8"),
-        ("bag_hop", "u4"), ("gdis", "i4"), ("sep_off", "i8"), ("sep", "i4")]
+        ("bag_hop", "u4"), ("gdis", "i4"), ("sep_off", "i8"), ("sep", "i4"), ("p2h_node", "i4"),
+        ("p2h_base", "i8"), ("p2h_off", "i8"), ("p2h", "i4")]
 HEAD = 8 + 16 + 16 + 8
 
 
@@ -23,7 +24,7 @@ def vector_offsets(raw: bytes) -> dict:
     """name -> (offset of the first element, element count, dtype)."""
     has_h2h = struct.unpack_from("<i", raw, 8 + 16 + 16)[0]
     out, o = {}, HEAD
-    for name, dt in VECS[: 15 if has_h2h else 12]:
+    for name, dt in VECS[: 19 if has_h2h else 12]:
         k = struct.unpack_from("<q", raw, o)[0]
         out[name] = (o + 8, k, np.dtype(dt))
         o += 8 + k * np.dtype(dt).itemsize
@@ -103,6 +104,21 @@ def test_mutated_images_are_refused(tmp_path, grid, name, fn, match):
     ix.save(p)
     q = _mutated(tmp_path, p.read_bytes(), fn, name)
     with pytest.raises(bcx.BcxError, match=match):
+        bcx.Index.load(q)
+
+
+def test_p2h_tables_in_the_image(tmp_path, grid):
+    g, G, ix = grid
+    ix.build_h2h()
+    p = tmp_path / "grid.idx"
+    ix.save(p)
+    raw = p.read_bytes()
+    off = vector_offsets(raw)
+    assert off["p2h_node"][1] > 0 and off["p2h"][1] > 0
+    sep, p2h = ix.h2h_scan_stats(20000, 3)
+    assert p2h < sep
+    q = _mutated(tmp_path, raw, lambda b, o: _put(b, o["p2h"][0], o["p2h"][2], 0, 10 ** 6), "p2h")
+    with pytest.raises(bcx.BcxError, match="P2H"):
         bcx.Index.load(q)
 
 
