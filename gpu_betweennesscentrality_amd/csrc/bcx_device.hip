@@ -717,38 +717,10 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     }
     t.lvl_off = (const int *)keep(dev_upload(lvl_off));
     t.lvl_v = (const int *)keep(dev_upload(lvl_v));
-    // Tight pairs of each bag, found by binary search of a in bag(b) (bags are
-    // ordered by depth and every pair of a bag is joined by a fill shortcut).
+    // Tight pairs of each bag, level by level (host, OpenMP).
     std::vector<int32_t> tz, tab;
     std::vector<double> tfr;
-    for (int32_t k = 0; k < ix.tree_height; ++k)
-      for (int32_t q = lvl_off[(size_t)k]; q < lvl_off[(size_t)k + 1]; ++q) {
-        const int32_t z = lvl_v[(size_t)q];
-        for (int32_t i = boff[(size_t)z]; i < boff[(size_t)z + 1]; ++i)
-          for (int32_t j = i + 1; j < boff[(size_t)z + 1]; ++j) {
-            const int32_t a = ix.bag[(size_t)i], b = ix.bag[(size_t)j];
-            const int32_t da = ix.depth[(size_t)a];
-            int32_t lo = boff[(size_t)b], hi = boff[(size_t)b + 1];
-            while (lo < hi) {
-              const int32_t mid = (lo + hi) / 2;
-              if (ix.depth[(size_t)ix.bag[(size_t)mid]] < da)
-                lo = mid + 1;
-              else
-                hi = mid;
-            }
-            if (lo == boff[(size_t)b + 1] || ix.bag[(size_t)lo] != a)
-              return bcx::fail(BCX_EGRAPH, "index bag is not closed under its shortcuts");
-            if ((int64_t)ix.bag_len[(size_t)i] + ix.bag_len[(size_t)j] != ix.bag_len[(size_t)lo])
-              continue;
-            tz.push_back(z);
-            tab.push_back(lo);
-            tfr.push_back((double)ix.bag_cnt[(size_t)i] * (double)ix.bag_cnt[(size_t)j] /
-                          (double)ix.bag_cnt[(size_t)lo]);
-            // remember the two entries through the entry -> pair lists below
-            tab.push_back(i);
-            tab.push_back(j);
-          }
-      }
+    bcx::tight_pairs(ix, lvl_v, tz, tab, tfr);
     const int64_t P = (int64_t)tz.size();
     std::vector<int32_t> lpo((size_t)ix.tree_height + 1, 0), tp_ab((size_t)P), zpo((size_t)n + 1, 0),
         zpp((size_t)P), epo((size_t)B + 1, 0), epp((size_t)(2 * P));

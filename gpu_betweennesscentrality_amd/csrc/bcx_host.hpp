@@ -109,6 +109,9 @@ int32_t h2h_query(const Index &ix, int32_t s, int32_t t);
 // Mean scan length (label entries read) of the H2H distance query over
 // `pairs` seeded random pairs, with and without the P2H tables.
 void h2h_scan_stats(const Index &ix, int64_t pairs, uint64_t seed, double &mean_sep, double &mean_p2h);
+// Tight pairs of every bag in `nodes` order (see index_build.cpp).
+void tight_pairs(const Index &ix, const std::vector<int32_t> &nodes, std::vector<int32_t> &tz,
+                 std::vector<int32_t> &tab, std::vector<double> &tfr);
 void save_index(const Index &ix, const char *path);
 void load_index(const char *path, Index &ix);
 // Throws BCX_EGRAPH when shortest distances may reach kInf.

@@ -797,6 +797,73 @@ void h2h_scan_stats(const Index &ix, int64_t pairs, uint64_t seed, double &mean_
   mean_p2h = pairs ? a_p2h / (double)pairs : 0.0;
 }
 
+// ---- tight bag pairs (bag-sweep unpacking) --------------------------------
+// For every node z, in the order `nodes`, and every pair of its bag entries
+// i < j (ancestors a above b), the pair is tight when the shortcut z-a plus
+// z-b is as short as the shortcut a-b, which is entry e of b's bag (found by
+// binary search over b's depth-ordered bag; every pair of a bag is joined by
+// a fill shortcut).  Out: tz[q] = z, tab[3q..3q+2] = (e, i, j), and
+// tfr[q] = c(a,z) c(z,b) / c(a,b), in (nodes order, i, j) order.  Two
+// parallel passes (count, then fill at prefix offsets) keep that order.
+void tight_pairs(const Index &ix, const std::vector<int32_t> &nodes, std::vector<int32_t> &tz,
+                 std::vector<int32_t> &tab, std::vector<double> &tfr) {
+  const int64_t N = (int64_t)nodes.size();
+  std::vector<int64_t> cnt((size_t)N + 1, 0);
+  int bad = 0;
+  auto find = [&](int64_t i, int64_t j) -> int64_t {
+    const int32_t a = ix.bag[(size_t)i], b = ix.bag[(size_t)j];
+    const int32_t da = ix.depth[(size_t)a];
+    int64_t lo = ix.bag_off[(size_t)b], hi = ix.bag_off[(size_t)b + 1];
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (ix.depth[(size_t)ix.bag[(size_t)mid]] < da)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (lo == ix.bag_off[(size_t)b + 1] || ix.bag[(size_t)lo] != a) return -1;
+    return lo;
+  };
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
+  for (int64_t q = 0; q < N; ++q) {
+    const int32_t z = nodes[(size_t)q];
+    int64_t c = 0;
+    for (int64_t i = ix.bag_off[(size_t)z]; i < ix.bag_off[(size_t)z + 1]; ++i)
+      for (int64_t j = i + 1; j < ix.bag_off[(size_t)z + 1]; ++j) {
+        const int64_t e = find(i, j);
+        if (e < 0) {
+          bad = 1;
+          continue;
+        }
+        c += (int64_t)ix.bag_len[(size_t)i] + ix.bag_len[(size_t)j] == ix.bag_len[(size_t)e];
+      }
+    cnt[(size_t)q + 1] = c;
+  }
+  if (bad) throw Error{BCX_EGRAPH, "index bag is not closed under its shortcuts"};
+  for (int64_t q = 0; q < N; ++q) cnt[(size_t)q + 1] += cnt[(size_t)q];
+  const int64_t P = cnt[(size_t)N];
+  if (P > 0x7fffffff) throw Error{BCX_EGRAPH, "more than 2^31 tight bag pairs"};
+  tz.assign((size_t)P, 0);
+  tab.assign((size_t)(3 * P), 0);
+  tfr.assign((size_t)P, 0.0);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t q = 0; q < N; ++q) {
+    const int32_t z = nodes[(size_t)q];
+    int64_t k = cnt[(size_t)q];
+    for (int64_t i = ix.bag_off[(size_t)z]; i < ix.bag_off[(size_t)z + 1]; ++i)
+      for (int64_t j = i + 1; j < ix.bag_off[(size_t)z + 1]; ++j) {
+        const int64_t e = find(i, j);
+        if ((int64_t)ix.bag_len[(size_t)i] + ix.bag_len[(size_t)j] != ix.bag_len[(size_t)e]) continue;
+        tz[(size_t)k] = z;
+        tab[(size_t)(3 * k)] = (int32_t)e;
+        tab[(size_t)(3 * k + 1)] = (int32_t)i;
+        tab[(size_t)(3 * k + 2)] = (int32_t)j;
+        tfr[(size_t)k] = (double)ix.bag_cnt[(size_t)i] * (double)ix.bag_cnt[(size_t)j] / (double)ix.bag_cnt[(size_t)e];
+        ++k;
+      }
+  }
+}
+
 // ---- binary image ---------------------------------------------------------
 namespace {
 const char kMagic[8] = {'B', 'C', 'X', 'I', 'D', 'X', '0', '3'};

@@ -206,6 +206,10 @@ WORKLOADS = {
     "road100k": dict(kind="road", n=100_000, m=125_000, seed=2),
     "road320k": dict(kind="road", n=320_000, m=400_000, seed=3),
     "road1070k": dict(kind="road", n=1_070_000, m=1_350_000, seed=5, arterial_frac=0.01),
+    # The other reading of the configs' edge counts: undirected edges, 2.5 per
+    # vertex (1.6M arc lines at 320k) -- a near-triangulation, far denser
+    # than road networks (tree height 2,368 against 526).
+    "road320k_m800k": dict(kind="road", n=320_000, m=800_000, seed=3),
 }
 
 
