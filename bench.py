@@ -250,7 +250,6 @@ def main() -> int:
     ctx.set_tuning(batch=B)
     if args.sweep_batch:
         ctx.set_tuning(sweep_batch=args.sweep_batch)
-    Bs = ctx.sweep_capacity()  # sources per launch after the memory cap
     tree_perm = ix.tree_order()
 
     def cyc(start: int, count: int) -> np.ndarray:
@@ -258,6 +257,15 @@ def main() -> int:
         return tree_perm[(start + np.arange(count, dtype=np.int64)) % n].astype(np.int32)
 
     W, K = args.warmup, args.steps
+    checks = {}
+    tuning = ctx.tuning()
+    if not args.no_autotune and not args.profile_only:
+        ta = time.time()
+        tuning = ctx.autotune(cyc(0, B))
+        if args.sweep_batch:  # an explicit launch size wins over the tuned one
+            ctx.set_tuning(sweep_batch=args.sweep_batch)
+        log(f"[bench] autotune: {tuning} ({time.time() - ta:.1f}s)")
+    Bs = ctx.sweep_capacity()  # sources per launch after the memory cap
     if args.profile_only:
         # The same bag-sweep launches as the timed call (W + K, one job),
         # one label-scan batch and a 1 GiB copy, for a kernel trace.
@@ -268,13 +276,6 @@ def main() -> int:
         log(f"[bench] profile-only done ({W + K} launches of {Bs} sources); 1 GiB device copy {ms:.3f} ms; "
             f"4 GiB streaming read {rms:.3f} ms ({READ_BYTES / rms / 1e6:.0f} GB/s)")
         return 0
-
-    checks = {}
-    tuning = ctx.tuning()
-    if not args.no_autotune:
-        ta = time.time()
-        tuning = ctx.autotune(cyc(0, B))
-        log(f"[bench] autotune: {tuning} ({time.time() - ta:.1f}s)")
 
     # ---- label scan (the reference's method): rate, and the cross-check ------
     ls_full = n <= args.labelscan_full_max_n
@@ -308,10 +309,11 @@ def main() -> int:
     prof_bc, _ = ctx.bc_sources(src, bcx.MODE_SWEEP)
     tm = ctx.timing()
     ctx.set_profiling(False)
-    kernel_ms = tm["scan_ms"] / K
-    log(f"[bench] sweep tile kernel {kernel_ms:.2f} ms per launch of {Bs} (mean of {K}); per tile fwd "
-        f"{tm['sweep_fwd_ms'] / K:.1f} bwd {tm['sweep_bwd_ms'] / K:.1f} asc {tm['sweep_asc_ms'] / K:.1f} ms; "
-        f"reduce {tm['reduce_ms'] / K:.2f} ms")
+    pl = max(1, int(tm["scan_launches"]))
+    kernel_ms = tm["scan_ms"] / pl
+    log(f"[bench] sweep tile kernel {kernel_ms:.2f} ms per launch of {Bs} (mean of {pl}); per tile fwd "
+        f"{tm['sweep_fwd_ms'] / pl:.1f} bwd {tm['sweep_bwd_ms'] / pl:.1f} asc {tm['sweep_asc_ms'] / pl:.1f} ms; "
+        f"reduce {tm['reduce_ms'] / pl:.2f} ms")
     checks["timed_equals_eager_bitwise"] = {"ok": bool(np.array_equal(timed_bc.view(np.uint64),
                                                                       prof_bc.view(np.uint64)))}
 

@@ -1624,12 +1624,11 @@ int bcx_rq_poll(bcx_rq *rq, int64_t id, double *bc_out) {
 // Label scan: the two phases are tuned separately on one sample batch: the
 // target chunk (targets per wavefront work item) by the scan time alone, the
 // width of the dependency group (threads sharing one source's layers) by the
-// dependency time alone.  Bag sweep: tiles per compute unit (1 to 4) and the
-// tile-kernel form (one workgroup per unit with 2 pieces per step, or two
-// with 1 piece per step), each timed on one launch of the sample's sources
-// (repeated to fill it) by its tile kernel and reduction per source;
-// launches that device memory would cut short are skipped.  Each candidate
-// runs once to warm up and once timed with HIP events on the context stream.
+// dependency time alone, each candidate once to warm up and once timed with
+// HIP events on the context stream.  Bag sweep: tiles per compute unit (1 to
+// 4) and the tile-kernel form (one workgroup per unit with 2 pieces per step,
+// or two with 1 piece per step), each timed on the same job (below);
+// launches that device memory would cut short are skipped.
 extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t count,
                                 bcx_tuning *best_out) {
   BCX_GUARD_BEGIN
@@ -1671,27 +1670,43 @@ extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t coun
         best.bp_threads = bp;
       }
     }
+    // Bag sweep: every candidate runs the same job, 12 x compute units x 64
+    // consecutive tree-order sources (a whole number of launches for 1 to 4
+    // tiles per unit), timed by wall clock after a warm-up run, so launch
+    // counts, per-launch work and the unpack all count.  The context's own
+    // setting is timed first and is kept unless a candidate is at least 2%
+    // faster (close candidates would otherwise be picked by timing noise).
     const int32_t n = c->ix.n;
-    const int32_t most = std::min<int64_t>((int64_t)4 * c->cus * kWave, std::max<int64_t>(n, kWave));
-    std::vector<int32_t> sample((size_t)most);
-    for (int32_t i = 0; i < most; ++i) sample[(size_t)i] = sources[i % count];
-    double best_per = 1e300;
+    const int32_t total = (int32_t)std::min<int64_t>((int64_t)12 * c->cus * kWave, n);
+    std::vector<int32_t> job((size_t)total);
+    for (int32_t i = 0; i < total; ++i) job[(size_t)i] = c->perm[(size_t)((n / 4 + (int64_t)i) % n)];
+    double best_s = 1e300;
+    c->profiling = false;
+    auto time_sweep = [&](const bcx_tuning &t) {
+      c->tn = t;
+      if (t.sweep_batch > total) return -1.0;  // more sources per launch than the job
+      if (sweep_tiles_for(c, t.sweep_batch) * kWave < t.sweep_batch) return -1.0;  // cut by memory
+      run_sweep(c, job.data(), total, bc_tmp.data(), nullptr);  // warm-up (and capture)
+      const auto t0 = std::chrono::steady_clock::now();
+      run_sweep(c, job.data(), total, bc_tmp.data(), nullptr);
+      return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    {
+      const double sec = time_sweep(best);
+      if (sec > 0) best_s = sec;
+    }
     for (int tpc = 1; tpc <= 4; ++tpc) {
       for (int form : {BCX_SWEEP_FORM_ONE_PER_CU, BCX_SWEEP_FORM_TWO_PER_CU}) {
         bcx_tuning t = best;
         t.sweep_batch = tpc * c->cus * kWave;
         t.sweep_form = form;
-        c->tn = t;
-        if (t.sweep_batch > most) continue;  // more sources than the graph has
-        if (sweep_tiles_for(c, t.sweep_batch) * kWave < t.sweep_batch) continue;  // cut by memory
-        c->profiling = false;
-        run_sweep(c, sample.data(), t.sweep_batch, bc_tmp.data(), nullptr);  // warm-up (and capture)
-        c->profiling = true;
-        c->timing = bcx_timing{};
-        run_sweep(c, sample.data(), t.sweep_batch, bc_tmp.data(), nullptr);
-        const double per = (c->timing.scan_ms + c->timing.reduce_ms) / t.sweep_batch;
-        if (per < best_per) {
-          best_per = per;
+        const int start_form = start.sweep_form != BCX_SWEEP_FORM_AUTO ? start.sweep_form
+                               : start.sweep_batch / kWave > c->cus  ? BCX_SWEEP_FORM_TWO_PER_CU
+                                                                      : BCX_SWEEP_FORM_ONE_PER_CU;
+        if (t.sweep_batch == start.sweep_batch && form == start_form) continue;  // already timed
+        const double sec = time_sweep(t);
+        if (sec > 0 && sec < 0.98 * best_s) {
+          best_s = sec;
           best.sweep_batch = t.sweep_batch;
           best.sweep_form = form;
         }

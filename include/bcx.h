@@ -231,9 +231,10 @@ int bcx_ctx_get_tuning(const bcx_ctx *ctx, bcx_tuning *tn);
  * at compile time, MHLQBC.cu:3).  Times candidate label-scan target chunks
  * and dependency-group widths on `sources` (one batch at most), and bag-sweep
  * launch sizes (1 to 4 tiles of 64 sources per compute unit, those that fit
- * device memory) in both tile-kernel forms on `sources` repeated to fill a
- * launch; keeps the fastest of each in the context and copies the tuning to
- * *best (may be NULL). */
+ * device memory) in both tile-kernel forms, each on the same job of 12 x 64
+ * sources per compute unit taken in tree order; keeps the fastest of each in
+ * the context (the current sweep setting unless a candidate is 2% faster) and
+ * copies the tuning to *best (may be NULL). */
 int bcx_ctx_autotune(bcx_ctx *ctx, const int32_t *sources, int32_t count,
                      bcx_tuning *best);
 /* Sources one bag-sweep launch takes (the tuning's sweep_batch, cut to whole

@@ -691,7 +691,7 @@ def test_autotune_picks_a_sweep_launch_and_stays_exact(road):
     default = ctx.tuning()["sweep_batch"]
     cus = default // (3 * 64)
     best = ctx.autotune(gg.permutation(12, 1, g.n)[:4096].astype(np.int32))
-    assert best["sweep_form"] in (bcx.SWEEP_FORM_ONE_PER_CU, bcx.SWEEP_FORM_TWO_PER_CU)
+    assert best["sweep_form"] in (bcx.SWEEP_FORM_AUTO, bcx.SWEEP_FORM_ONE_PER_CU, bcx.SWEEP_FORM_TWO_PER_CU)
     assert best["sweep_batch"] % (64 * cus) == 0 and 1 <= best["sweep_batch"] // (64 * cus) <= 4
     assert ctx.tuning() == best
     src = gg.permutation(13, 1, g.n)[:3000].astype(np.int32)
