@@ -66,6 +66,8 @@ struct DevWork {
 struct DevTree {
   int n;
   int height;
+  int dbits;               // width of the distance field of the sweep's record
+                           // words (every shortest distance is below 2^dbits)
   int64_t bags;            // total bag entries B
   int64_t bslots;          // backward staging slots (with padding)
   const int *bnode_slot;   // n: slot of a node's last backward piece, -1 without descendants

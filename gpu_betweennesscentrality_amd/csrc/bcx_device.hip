@@ -697,6 +697,16 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     DevTree t{};
     t.n = n;
     t.height = ix.tree_height;
+    {
+      // Every shortest distance is at most the path through the tree root,
+      // twice the farthest vertex's root distance (label entry 0 of its row),
+      // which the index's distance bound keeps below 2^30.
+      int64_t far = 0;
+      for (int32_t p = 1; p < n; ++p) far = std::max<int64_t>(far, ix.dis[(size_t)ix.row_off[(size_t)p]]);
+      int b = 1;
+      while (b < 30 && (int64_t{1} << b) <= 2 * far) ++b;
+      t.dbits = b;
+    }
     t.bags = B;
     t.subtree = (const int *)keep(dev_upload(ix.subtree));
     t.bag_off = (const int *)keep(dev_upload(boff));

@@ -63,6 +63,9 @@ struct Lane {
   __amdgpu_buffer_rsrc_t rec;   // the tile's records as a buffer resource
   __amdgpu_buffer_rsrc_t phi;   // the tile's shortcut flows (sw.phi_t row)
   __amdgpu_buffer_rsrc_t nodef; // the tile's node flows (sw.node_t row)
+  uint32_t dbits;  // distance field width of the record words (uniform)
+  uint32_t dmask;  // 2^dbits - 1
+  uint32_t cmax;   // largest count kept in a word, 2^(32 - dbits) - 1
 };
 
 // Tile t of a launch is tile sw.tile0 + t of the job: its sources and
@@ -82,6 +85,9 @@ __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const DevTree &tr
                                             kBufferWord3);
   L.nodef = __builtin_amdgcn_make_buffer_rsrc(sw.node_t + (int64_t)t * tr.bslots, 0,
                                               (int)(uint32_t)(tr.bslots * 8), kBufferWord3);
+  L.dbits = (uint32_t)tr.dbits;
+  L.dmask = (1u << L.dbits) - 1u;
+  L.cmax = 0xffffffffu >> L.dbits;
   return L;
 }
 
@@ -115,23 +121,31 @@ __device__ __forceinline__ void setQ(const Lane &L, uint32_t r, double q) {
   SS(L, r) = __builtin_bit_cast(uint64_t, q);
 }
 
-// A distance word carries a flag in bit 30, set when the lane's path count is
-// 1 (distances stay below 2^30: the reader caps the weight sum at 999999999).
-// A flagged lane's count slot is neither written nor read, which on road
-// graphs, where ties are rare, removes nearly all count-row traffic.
-constexpr int kOneFlag = 1 << 30;
+// A distance word holds the lane's distance in its low dbits bits and the
+// lane's path count in the bits above, when the count is between 1 and
+// 2^(32 - dbits) - 1; a count field of 0 says the count is in the record's
+// 8-byte slot.  dbits covers the largest possible distance (twice the
+// farthest root distance, below 2^30), so on road graphs, with 17 to 20 bits
+// of distance, counts up to several thousand travel with the distance: the
+// count slot is neither written nor read for them, also where ties are
+// common (a third to two thirds of the lanes on weights of about 100).
 // Length given to entries past the end of a bag: a candidate distance plus it
 // exceeds every true distance (below 999999999) without overflowing, so such
 // an entry is never minimal or tight.  Selecting it on the scalar length keeps
 // the vector arithmetic on loaded rows free of branches (a uniform select of a
 // loaded value becomes a branch that waits for the load).
 constexpr int kNoLen = kSweepNoLen;
-static_assert(kNoLen == kOneFlag - 1, "lengths of unused places stay below the flag bit");
-__device__ __forceinline__ int dist_of(int w) { return w & (kOneFlag - 1); }
-__device__ __forceinline__ bool one_of(int w) { return (w & kOneFlag) != 0; }
+static_assert(kNoLen >= (1 << 30) - 1 && kNoLen < (1 << 30), "a distance plus kNoLen stays below 2^31");
+__device__ __forceinline__ int dist_of(const Lane &L, int w) { return w & (int)L.dmask; }
+// The count field of a word: the count, or 0 where the count is in the slot.
+__device__ __forceinline__ uint32_t cnt_of(const Lane &L, int w) { return (uint32_t)w >> L.dbits; }
+__device__ __forceinline__ bool cnt_in_word(const Lane &L, uint64_t c) { return c - 1 < (uint64_t)L.cmax; }
+__device__ __forceinline__ int pack_word(const Lane &L, int d, uint64_t c) {
+  return d | (int)((cnt_in_word(L, c) ? (uint32_t)c : 0u) << L.dbits);
+}
 __device__ __forceinline__ void put_dist_count(const Lane &L, uint32_t r, int d, uint64_t c) {
-  SD(L, r) = d | (c == 1 ? kOneFlag : 0);
-  if (c != 1) SS(L, r) = c;
+  SD(L, r) = pack_word(L, d, c);
+  if (!cnt_in_word(L, c)) SS(L, r) = c;
 }
 
 // Stores inside the pipelined steps are issued unconditionally, with lanes
@@ -142,8 +156,8 @@ __device__ __forceinline__ void put_dist_count(const Lane &L, uint32_t r, int d,
 // put: the lane's node result (distance word, and the count where it is not
 // 1) goes to record r.
 __device__ __forceinline__ void put_dist_count_all(const Lane &L, bool put, uint32_t r, int d, uint64_t c) {
-  st32(L.rec, put ? r + (uint32_t)L.lane * 4u : kDrop, d | (c == 1 ? kOneFlag : 0));
-  st64(L.rec, (put & (c != 1)) ? r + (uint32_t)kSlotOff + (uint32_t)L.lane * 8u : kDrop, c);
+  st32(L.rec, put ? r + (uint32_t)L.lane * 4u : kDrop, pack_word(L, d, c));
+  st64(L.rec, (put & !cnt_in_word(L, c)) ? r + (uint32_t)kSlotOff + (uint32_t)L.lane * 8u : kDrop, c);
 }
 __device__ __forceinline__ void put_q_all(const Lane &L, bool put, uint32_t r, double q) {
   st64(L.rec, put ? r + (uint32_t)kSlotOff + (uint32_t)L.lane * 8u : kDrop, __builtin_bit_cast(uint64_t, q));
@@ -186,24 +200,26 @@ __device__ __forceinline__ void wait_all_vm() {
   __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0), expcnt and lgkmcnt not waited
   __builtin_amdgcn_sched_barrier(0);
 }
-// Quotient Q = (1 + dD) / sigma from a raw slot word.  A node without
-// descendants has dD = 0 and its backward step is skipped, so its slot still
-// holds sigma (not read when flagged 1); other nodes hold Q itself.  (Storing
-// 1 / sigma from the forward pass instead moves the divisions there, and
-// measured no faster.)
-__device__ __forceinline__ bool slot_needed(bool leaf, int w) { return !(leaf & one_of(w)); }
-__device__ __forceinline__ double q_from_slot(bool leaf, int w, uint64_t raw) {
-  const double inv = 1.0 / (double)(one_of(w) ? 1 : raw);
-  return leaf ? inv : __builtin_bit_cast(double, raw);
+// Quotient Q = (1 + dD) / sigma of a node from its word and slot.  A node
+// without descendants has dD = 0 and its backward step is skipped, so its
+// slot still holds sigma (read only where the word does not hold it); other
+// nodes hold Q itself.  (Storing 1 / sigma from the forward pass instead
+// moves the divisions there, and measured no faster.)
+__device__ __forceinline__ bool slot_needed(const Lane &L, bool leaf, int w) {
+  return !(leaf & (cnt_of(L, w) != 0));
+}
+__device__ __forceinline__ double q_from_slot(const Lane &L, bool leaf, int w, uint64_t raw) {
+  const uint32_t cw = cnt_of(L, w);
+  return leaf ? 1.0 / (double)(cw ? (uint64_t)cw : raw) : __builtin_bit_cast(double, raw);
 }
 
-// Slot word of record r in lanes where `need` holds (other lanes get a value
-// that their callers do not use).  The load runs under the lane mask, so lanes
-// that need nothing move no data through the memory pipeline, and a step
-// where no lane needs it skips it.  The pipelined steps wait for such loads
-// with explicit waits (wait_vm_left), which hold whether or not they issued.
-__device__ __forceinline__ uint64_t slot_if(bool need, uint32_t r, const Lane &L) {
-  uint64_t x = 1;
+// Slot word of record r in lanes where `need` holds, dflt in the others.  The
+// load runs under the lane mask, so lanes that need nothing move no data
+// through the memory pipeline, and a step where no lane needs it skips it.
+// The pipelined steps wait for such loads with explicit waits (wait_vm_left),
+// which hold whether or not they issued.
+__device__ __forceinline__ uint64_t slot_or(bool need, uint32_t r, const Lane &L, uint64_t dflt) {
+  uint64_t x = dflt;
   if (need) x = SS(L, r);
   return x;
 }
@@ -347,7 +363,8 @@ struct Segs {
 // with one set of loads; the wavefront then handles kNodesF pieces per
 // step.  Load set 1 of a step loads the distance words of the pieces' entries;
 // load set 2 the up-label (first piece, where the node is an ancestor of the
-// lane's source) and the path-count rows of tight candidates not flagged 1.
+// lane's source) and the count slots of tight candidates whose distance words
+// hold no count.
 // The steps are pipelined: a step issues the next step's load set 1 after its
 // own load set 2 and before its stores, so every wait is for loads that are
 // already old (stores and loads share one completion counter).  A node's
@@ -394,17 +411,15 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
       // This step's rows have arrived once only the stores issued after them
       // (the last step's, or the sink stores) are outstanding.
       wait_vm_left<2 * kNodesF>();
-      int cand[kNodesF][kWidth], best[kNodesF], onem[kNodesF];
+      int cand[kNodesF][kWidth], best[kNodesF];
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
-        onem[u] = 0;
         best[u] = 0x7fffffff;
 #pragma unroll
         for (int r = 0; r < kWidth; ++r) {
           const int len = rl(slr[r], j);  // kNoLen in unused places
-          cand[u][r] = dist_of(w[u][r]) + len;
-          onem[u] |= (int)one_of(w[u][r]) << r;
+          cand[u][r] = dist_of(L, w[u][r]) + len;
           best[u] = min(best[u], cand[u][r]);
         }
       }
@@ -435,16 +450,16 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
           upc[u] = ix.cnt[o];
         }
       }
-      // Then the count rows of tight candidates not flagged 1.
+      // Then the count slots of tight candidates whose words hold no count
+      // (m is the candidate's count in its tight lanes).
       uint64_t m[kNodesF][kWidth];
 #pragma unroll
       for (int u = 0; u < kNodesF; ++u) {
         const int j = min(j0 + u, jn - 1);
 #pragma unroll
         for (int r = 0; r < kWidth; ++r) {
-          const bool need = (cand[u][r] == best[u]) & !((onem[u] >> r) & 1);
-          const uint64_t raw = slot_if(need, roff(rl(sur[r], j)), L);
-          m[u][r] = need ? raw : 1;
+          const uint32_t cw = cnt_of(L, w[u][r]);
+          m[u][r] = slot_or((cand[u][r] == best[u]) & (cw == 0), roff(rl(sur[r], j)), L, cw);
         }
       }
       rows1(j0 + kNodesF, w);
@@ -500,7 +515,7 @@ __device__ __forceinline__ void fwd_level_wave(const DevIndex &ix, const DevTree
 // derived from its count by the readers), with the same staging by pieces
 // (of the inverse bag) and kNodesB pieces per step.  Load set 1 loads the node's
 // distance word and the piece's entries' distance words; load set 2 the node's
-// count (lanes not flagged 1) and the tight entries' quotients; steps are
+// count (lanes whose word holds none) and the tight entries' quotients; steps are
 // pipelined as in the forward pass.  The node's dependency part is summed
 // piece by piece; its last piece writes Q and the node flow.  The flows of a
 // piece's entries and the node flows of the step are each one four-way
@@ -548,9 +563,10 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
 #pragma unroll
         for (int r = 0; r < kWidth; ++r) wx[u][r] = wxn[u][r];
       }
-      // Load set 2: raw slot words only (counts of nodes not flagged 1, and the
-      // tight entries' slots); the arithmetic follows once all are issued,
-      // and the phase order is kept as written (see the forward pass).
+      // Load set 2: raw slot words only (counts of nodes whose words hold
+      // none, and the tight entries' slots); the arithmetic follows once all
+      // are issued, and the phase order is kept as written (see the forward
+      // pass).
       __builtin_amdgcn_sched_barrier(0);
       uint64_t svr[kNodesB];
       uint64_t qr[kNodesB][kWidth];
@@ -558,8 +574,9 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
 #pragma unroll
       for (int u = 0; u < kNodesB; ++u) {
         const int j = min(j0 + u, jn - 1);
-        svr[u] = slot_if(!one_of(wv[u]), roff(rl(nd.x, j)), L);  // 1 where flagged
-        const int dv = dist_of(wv[u]);
+        const uint32_t cv = cnt_of(L, wv[u]);
+        svr[u] = slot_or(cv == 0, roff(rl(nd.x, j)), L, cv);  // sigma(v)
+        const int dv = dist_of(L, wv[u]);
         const int leafm = rl(nd.w, j);
         tm[u] = 0;
 #pragma unroll
@@ -567,9 +584,11 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
           const int len = rl(slr[r], j);  // kNoLen in unused places
           // Padding lanes (sources past the batch) count as never tight, so
           // their flows are 0 and the lane sums below need no mask.
-          const bool t = L.valid & (dv + len == dist_of(wx[u][r]));
+          const bool t = L.valid & (dv + len == dist_of(L, wx[u][r]));
           tm[u] |= (int)t << r;
-          qr[u][r] = slot_if(t & slot_needed((leafm >> r) & 1, wx[u][r]), roff(rl(sxr[r], j)), L);
+          // Q(x) where x has descendants, else sigma(x) (from the word or the slot).
+          qr[u][r] = slot_or(t & slot_needed(L, (leafm >> r) & 1, wx[u][r]), roff(rl(sxr[r], j)), L,
+                             cnt_of(L, wx[u][r]));
         }
       }
       rows1(j0 + kNodesB, wvn, wxn);
@@ -592,8 +611,8 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
           // or, where x has no descendants (a uniform bit), 1 / sigma(x).
           double q = __builtin_bit_cast(double, qr[u][r]);
           if ((leafm >> r) & 1)  // the division only where some tight lane's count is not 1
-            q = __builtin_amdgcn_ballot_w64(((tm[u] >> r) & 1) & !one_of(wx[u][r]))
-                    ? 1.0 / (double)(one_of(wx[u][r]) ? 1 : qr[u][r])
+            q = __builtin_amdgcn_ballot_w64(((tm[u] >> r) & 1) & (qr[u][r] != 1))
+                    ? 1.0 / (double)qr[u][r]
                     : 1.0;
           qx[u][r] = ((tm[u] >> r) & 1) ? q : 0.0;
         }
@@ -636,7 +655,7 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
         } else {
           double q = 0.0;
           if (fin)  // uniform branch: only the last piece divides, where some count is not 1
-            q = __builtin_amdgcn_ballot_w64(!one_of(wv[u])) ? (1.0 + tot) / svv : 1.0 + tot;
+            q = __builtin_amdgcn_ballot_w64(svv != 1.0) ? (1.0 + tot) / svv : 1.0 + tot;
           put_q_all(L, fin, roff(v), q);
           ns_in[u] = (fin & L.valid & (L.s != v)) ? tot : 0.0;
           fin_mask |= (int)fin << u;
@@ -779,7 +798,7 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
       const bool anc = x <= L.s && L.s < x + cld(tr.subtree, x);
       const uint32_t ax = roff(x);
       const int wx = SD(L, ax);
-      const int dx = dist_of(wx);
+      const int dx = dist_of(L, wx);
       uint64_t sa = 0;
       if (anc) {
         const int64_t o = L.rs + cld(ix.depth, x);
@@ -797,7 +816,7 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
             const int eq = e + q * kSweepWaves;
             const int ec = eq < e1 ? eq : e;
             ay[q] = roff(cld(tr.bag_v, ec));
-            dy[q] = dist_of(SD(L, ay[q]));
+            dy[q] = dist_of(L, SD(L, ay[q]));
           }
           uint64_t qa[4];
           bool t[4];
@@ -833,8 +852,8 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
         for (int w = 0; w < kSweepWaves; ++w) u += sh_acc[w][L.lane];
         const bool xleaf = cld(tr.ibag_off, x) == cld(tr.ibag_off, x + 1);
         uint64_t raw = 0;
-        if (slot_needed(xleaf, wx)) raw = SS(L, ax);
-        const double qx = q_from_slot(xleaf, wx, raw);
+        if (slot_needed(L, xleaf, wx)) raw = SS(L, ax);
+        const double qx = q_from_slot(L, xleaf, wx, raw);
         setQ(L, ax, sa != 0 ? qx + u : 0.0);
         const double ns = wave_sum_dpp((L.valid && x != L.s) ? sax * u : 0.0);
         const int xs = cld(tr.bnode_slot, x);  // -1 without descendants, where ns is 0
@@ -868,7 +887,7 @@ TileKernel tile_kernel(int form) {
 // Distance and count rows of a launch's tiles after a forward-only launch:
 // row r (tile r / 64, lane r % 64) of node p goes to d[r * n + p], s[r * n + p]
 // (tree ids), for r below nrows.
-__global__ __launch_bounds__(256) void sweep_extract_kernel(SweepWork sw, int n, int *d,
+__global__ __launch_bounds__(256) void sweep_extract_kernel(SweepWork sw, int n, int dbits, int *d,
                                                             uint64_t *s) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -876,8 +895,9 @@ __global__ __launch_bounds__(256) void sweep_extract_kernel(SweepWork sw, int n,
   const char *rec = sw.state + ((int64_t)(r / kWave) * (n + 1) + p) * kRec;
   const int lane = r % kWave;
   const int w = *(const int *)(rec + lane * 4);
-  const uint64_t c = one_of(w) ? 1 : *(const uint64_t *)(rec + kSlotOff + lane * 8);
-  d[(int64_t)r * n + p] = dist_of(w);
+  const uint32_t cw = (uint32_t)w >> dbits;  // the count, or 0 where it is in the slot
+  const uint64_t c = cw ? cw : *(const uint64_t *)(rec + kSlotOff + lane * 8);
+  d[(int64_t)r * n + p] = w & (int)((1u << dbits) - 1u);
   s[(int64_t)r * n + p] = c;
 }
 
@@ -1403,7 +1423,7 @@ int sweep_query(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t *d, u
       const int rows = std::min(nt * kWave, count - t0 * kWave);
       sweep_tiles_launch(c, t0, nt, count, form, true, nullptr, nullptr, false, c->profiling);
       sweep_extract_kernel<<<dim3((unsigned)((n + 255) / 256), (unsigned)rows), 256, 0, c->stream>>>(
-          c->sw, n, dd, ds);
+          c->sw, n, c->tree.dbits, dd, ds);
       HIP_OK(hipGetLastError());
       hd.resize((size_t)rows * n);
       hs.resize((size_t)rows * n);
