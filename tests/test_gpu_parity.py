@@ -450,6 +450,39 @@ def test_rows_unit_grid_saturation_every_form(k):
         _rows_match(g, csr, (d, s, h, ovf), src)
 
 
+def test_rows_wide_distances_small_count_field():
+    """The sweep keeps a lane's path count in its distance word when it fits
+    above the distance bits, else in the count slot.  A 34 x 34 unit grid with
+    every weight 7,500,000 needs 30 distance bits (twice the farthest root
+    distance, 37 hops, is 555,000,000 >= 2^29), leaving counts 1 to 3 in the
+    word and the binomial counts up to C(66, 33) in the slot: rows in both
+    forms, and the label scan's, equal the oracle's bit for bit, and so does
+    full BC."""
+    from math import comb
+
+    k, wgt = 34, 7_500_000
+    u = gg.unit_grid(k)
+    g = gg.ArcGraph(u.n, u.u, u.v, (u.w.astype(np.int64) * wgt).astype(np.int32))
+    G, ix, ctx = build(g)
+    csr = oracle.CSR.from_arcgraph(g)
+    src = np.array([0, k - 1, k * k - 1, k * (k // 2) + k // 2, 5 * k + 3], np.int32)
+    runs = [(bcx.MODE_BATCHED, None)] + [(bcx.MODE_SWEEP, f) for f in SWEEP_FORMS]
+    for mode, form in runs:
+        if form is not None:
+            ctx.set_tuning(sweep_form=form)
+        d, s, h, ovf = ctx.query_rows(src, mode)
+        assert not ovf, f"mode {mode} form {form}"
+        assert int(d[0, k * k - 1]) == 2 * (k - 1) * wgt
+        assert int(s[0, k * k - 1]) == comb(2 * k - 2, k - 1)
+        _rows_match(g, csr, (d, s, h, ovf), src)
+    ref, _ = oracle.brandes(csr)
+    for form in SWEEP_FORMS:
+        ctx.set_tuning(sweep_form=form)
+        bc, ovf = ctx.full_bc(bcx.MODE_SWEEP)
+        assert not ovf and close_bc(bc, ref), f"form {form}"
+    ctx.close()
+
+
 def test_rows_road_every_form(road):
     g, csr, ctx = road
     src = np.concatenate([np.arange(200, 264), gg.permutation(31, 1, g.n)[:16]]).astype(np.int32)
