@@ -100,6 +100,7 @@ _SIGS = {
     "bcx_ctx_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "bcx_ctx_autotune": (C.c_int, [_P, _I32P, C.c_int32, C.POINTER(Tuning)]),
     "bcx_ctx_sweep_capacity": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "bcx_ctx_sweep_slots": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "bcx_ctx_workspace_bytes": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_size_t)]),
     "bcx_ctx_mem_info": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "bcx_ctx_timing": (C.c_int, [_P, C.POINTER(Timing)]),
@@ -376,6 +377,12 @@ class Context:
         k = C.c_int32()
         _check(lib().bcx_ctx_sweep_capacity(self._h, C.byref(k)))
         return k.value
+
+    def sweep_slots(self) -> tuple[int, int]:
+        """(bag-sweep state slots allocated, compute units of the device)."""
+        slots, cus = C.c_int32(0), C.c_int32(0)
+        _check(lib().bcx_ctx_sweep_slots(self._h, C.byref(slots), C.byref(cus)))
+        return slots.value, cus.value
 
     def workspace_bytes(self, batch: int) -> int:
         b = C.c_size_t()

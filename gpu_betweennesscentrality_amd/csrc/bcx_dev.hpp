@@ -143,6 +143,12 @@ struct DevTree {
 };
 
 constexpr int kStageEntries = 4;
+// Default bag-sweep launch: tiles of 64 sources per compute unit.
+constexpr int kSweepTilesPerCu = 8;
+// Accelerator dies (XCDs) of the device: the state-slot pool is split among
+// them.  Overflow-flag bit set by a workgroup that found no free slot.
+constexpr int kSweepDies = 8;
+constexpr int kSweepNoSlot = 2;
 constexpr int kNarrowEntries = 2;
 // Length of unused staged places: a distance plus it exceeds every true
 // distance (below 999999999) without overflowing, so such an entry is never
@@ -154,8 +160,9 @@ constexpr int kSweepMaxSegments = 48;
 // Per-tile state of the sweep: one record per (tile, node) with the lanes'
 // values, per-tile sums [tile][node] and [tile][entry].
 struct SweepWork {
-  char *state;     // per (tile, node) record: distance[64], then 64 8-byte
-                   // slots (path count, then dependency quotient)
+  char *state;     // per (state slot, node) record: distance[64], then 64
+                   // 8-byte slots (path count, then dependency quotient); a
+                   // launch's workgroup w runs its tiles in slot w
   double *node_t;  // [tile][bslots]: node flow summed over the tile's lanes, at
                    // the slot of the node's last backward piece (tree.bnode_slot)
   double *phi_t;   // [tile][B]: shortcut flow summed over the tile's lanes,
@@ -169,8 +176,12 @@ struct SweepWork {
   int *cand_off;   // [job tiles + 1]
   int64_t cand_cap;
   int64_t src_cap; // tiles tile_src and cand_off hold
-  int tstride;     // tiles the state arrays are laid out for
-  int tile0;       // job tile of a launch's first workgroup (per launch)
+  int *slot_busy;  // [slot]: 1 while a workgroup holds the slot (pooled launches)
+  int *tile_slot;  // [tile]: state slot that ran the launch's tile
+  int slots;       // state slots allocated
+  int pool;        // per launch: slots per accelerator die when the launch has
+                   // more tiles than slots and draws on them as a pool, else 0
+  int tile0;       // job tile of a launch's tile 0 (per launch)
 };
 
 // Saturating count arithmetic: ~0 marks "overflowed" and is absorbing under
@@ -239,7 +250,11 @@ struct bcx_ctx {
   // bag-sweep form: tree arrays (device) and host copies used for planning
   bcx_dev::DevTree tree{};
   std::vector<int32_t> h_parent, h_depth, h_subtree, h_lvl_off, h_lvl_pair_off;
-  int sweep_tiles = 0;
+  int sweep_tiles = 0;  // tiles per launch the flow sums are allocated for
+  int sweep_alloc = 0;  // state slots allocated
+  int sweep_slots = 0;  // state slots the current launch shape uses (<= sweep_alloc)
+  int sweep_pool = 0;   // slots per die when launches may hold more tiles than slots
+  int occ_one = 1, occ_two = 2;  // resident workgroups per compute unit of each tile-kernel form
   bcx_dev::SweepWork sw{};
   // A sweep job's plan on the host (tile sources, ascending-pass candidate
   // lists); it stays alive until the next job starts, which first waits for
@@ -277,7 +292,7 @@ struct bcx_ctx {
     if (stream) (void)hipStreamDestroy(stream);
   }
   void free_sweep();
-  void ensure_sweep(int tiles);
+  void ensure_sweep(int slots, int tiles);
   void free_ws() {
     if (graph_exec) {
       (void)hipGraphExecDestroy(graph_exec);
@@ -317,7 +332,7 @@ struct bcx_ctx {
     try {
       alloc_ws(slots);
     } catch (const bcx::Error &e) {
-      if (e.code != BCX_ENOMEM || sweep_tiles == 0) throw;
+      if (e.code != BCX_ENOMEM || sweep_alloc == 0) throw;
       (void)hipGetLastError();  // clear the failed allocation's error
       if (stream) HIP_OK(hipStreamSynchronize(stream));
       free_sweep();
@@ -363,8 +378,13 @@ void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_ino
 int sweep_query(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t *d, uint64_t *sigma);
 // Pieces of the bag sweep, for callers that keep flows per batch.
 // Tiles per launch for `count` sources (tuning batch, capped by device
-// memory); allocates the sweep buffers for that many tiles.
+// memory); allocates the sweep buffers: state slots for the launch's
+// resident workgroups and flow sums for that many tiles.
 int sweep_tiles_for(bcx_ctx *c, int32_t count);
+// Tile-kernel form of a launch of nt tiles with the allocated slots.
+int sweep_launch_form(const bcx_ctx *c, int nt);
+// Resident workgroups per compute unit of each tile-kernel form (ctx occ_*).
+void sweep_occupancy(bcx_ctx *c);
 // One launch over the sorted tree ids pre[0 .. nsrc) (at most sweep_tiles
 // tiles of 64): node flows into node_out[n], shortcut flows into phi_out[B],
 // added when `accumulate`, else stored.  Sets ctx d_ovf on count overflow.
@@ -376,4 +396,7 @@ void sweep_sum_batches(bcx_ctx *c, const double *nodeb, const double *phi_b, int
 // (tree ids).
 void sweep_finish(bcx_ctx *c, double *bc_dev);
 void sweep_zero(bcx_ctx *c, double *p, int64_t cnt);
+// Throws if a pooled launch since the flags were last cleared found no free
+// state slot (its tiles then went unswept); call after the stream is idle.
+void sweep_check_slots(bcx_ctx *c);
 }  // namespace bcx_dev

@@ -578,12 +578,14 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
   HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto &e : c->ev) HIP_OK(hipEventCreate(&e));
   {
-    // Three sweep tiles per compute unit per launch (two run at a time on a
-    // unit; memory may cut the launch to fewer).
+    // Eight sweep tiles per compute unit per launch (two run at a time on a
+    // unit, each workgroup taking tiles until the launch's are all taken;
+    // memory may cut the launch to fewer).
     int cus = 0;
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     c->cus = std::max(1, cus);
-    c->tn.sweep_batch = 3 * c->cus * kWave;
+    c->tn.sweep_batch = kSweepTilesPerCu * c->cus * kWave;
+    bcx_dev::sweep_occupancy(c.get());
   }
   const int n = g.n;
   c->perm = ix.perm;
@@ -1035,6 +1037,13 @@ int bcx_ctx_sweep_capacity(bcx_ctx *c, int32_t *sources_per_launch) {
   *sources_per_launch = sweep_tiles_for(c, c->tn.sweep_batch) * kWave;
   return BCX_OK;
   BCX_GUARD_END
+}
+
+int bcx_ctx_sweep_slots(const bcx_ctx *c, int32_t *slots, int32_t *compute_units) {
+  if (!c) return bcx::fail(BCX_EINVAL, "null argument");
+  if (slots) *slots = c->sweep_slots;
+  if (compute_units) *compute_units = c->cus;
+  return BCX_OK;
 }
 
 int bcx_ctx_workspace_bytes(const bcx_ctx *c, int32_t batch, size_t *bytes) {
@@ -1576,6 +1585,7 @@ int bcx_rq_step(bcx_rq *rq, int32_t *active) {
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(c->stream));
+    bcx_dev::sweep_check_slots(c);
     c->timing.sources += cnt;
     rq->cur = (rq->cur + 1) % rq->nb;
     for (auto &r : rq->reqs)
@@ -1680,14 +1690,15 @@ extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t coun
         best.bp_threads = bp;
       }
     }
-    // Bag sweep: every candidate runs the same job, 12 x compute units x 64
-    // consecutive tree-order sources (a whole number of launches for 1 to 4
-    // tiles per unit), timed by wall clock after a warm-up run, so launch
-    // counts, per-launch work and the unpack all count.  The context's own
-    // setting is timed first and is kept unless a candidate is at least 2%
-    // faster (close candidates would otherwise be picked by timing noise).
+    // Bag sweep: every candidate runs the same job, 16 x compute units x 64
+    // consecutive tree-order sources (a whole number of launches for 1, 2,
+    // 4, 8 and 16 tiles per unit), timed by wall clock after a warm-up run,
+    // so launch counts, per-launch work and the unpack all count.  The
+    // context's own setting is timed first and is kept unless a candidate is
+    // at least 2% faster (close candidates would otherwise be picked by
+    // timing noise).
     const int32_t n = c->ix.n;
-    const int32_t total = (int32_t)std::min<int64_t>((int64_t)12 * c->cus * kWave, n);
+    const int32_t total = (int32_t)std::min<int64_t>((int64_t)16 * c->cus * kWave, n);
     std::vector<int32_t> job((size_t)total);
     for (int32_t i = 0; i < total; ++i) job[(size_t)i] = c->perm[(size_t)((n / 4 + (int64_t)i) % n)];
     double best_s = 1e300;
@@ -1705,7 +1716,7 @@ extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t coun
       const double sec = time_sweep(best);
       if (sec > 0) best_s = sec;
     }
-    for (int tpc = 1; tpc <= 4; ++tpc) {
+    for (int tpc : {1, 2, 4, 8, 16}) {
       for (int form : {BCX_SWEEP_FORM_ONE_PER_CU, BCX_SWEEP_FORM_TWO_PER_CU}) {
         bcx_tuning t = best;
         t.sweep_batch = tpc * c->cus * kWave;

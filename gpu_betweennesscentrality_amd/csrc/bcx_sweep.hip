@@ -69,17 +69,18 @@ struct Lane {
 };
 
 // Tile t of a launch is tile sw.tile0 + t of the job: its sources and
-// candidate lists are the job's, its state and flow sums the launch's.
+// candidate lists are the job's, its flow sums the launch's.  Its records
+// are those of the state slot of the workgroup that runs it.
 __device__ __forceinline__ Lane lane_setup(const DevIndex &ix, const DevTree &tr, const SweepWork &sw,
-                                           int t, int nsrc, int n) {
+                                           int t, int slot, int nsrc, int n) {
   Lane L;
   L.t = t;
   L.lane = threadIdx.x & (kWave - 1);
-  const int64_t slot = (int64_t)(sw.tile0 + t) * kWave + L.lane;
-  L.valid = slot < nsrc;
-  L.s = sw.tile_src[slot];  // padded with a valid source beyond nsrc
+  const int64_t src = (int64_t)(sw.tile0 + t) * kWave + L.lane;
+  L.valid = src < nsrc;
+  L.s = sw.tile_src[src];  // padded with a valid source beyond nsrc
   L.rs = ix.row_off[L.s];
-  L.tb = sw.state + (int64_t)t * (n + 1) * kRec;
+  L.tb = sw.state + (int64_t)slot * (n + 1) * kRec;
   L.rec = __builtin_amdgcn_make_buffer_rsrc(L.tb, 0, (int)((uint32_t)(n + 1) * (uint32_t)kRec), kBufferWord3);
   L.phi = __builtin_amdgcn_make_buffer_rsrc(sw.phi_t + (int64_t)t * tr.bags, 0, (int)(uint32_t)(tr.bags * 8),
                                             kBufferWord3);
@@ -676,31 +677,26 @@ __device__ __forceinline__ void bwd_level_wave(const DevTree &tr, const SweepWor
   }
 }
 
-// One workgroup per tile of 64 sources runs the whole sweep for its tile:
-// forward over depth levels, backward over depth levels, then the ascending
-// states.  Every level runs the pipelined steps over its staged pieces; a
-// level split in runs (few nodes, many pieces) then merges each node's
-// per-wavefront partial results from LDS in a fixed order, so results are
-// deterministic.
-// With fwd_only, the kernel stops after the forward pass (distances and path
-// counts left in the tile state, for probes of that pass).
-template <int kNodesF, int kNodesB, int kWavesPerSimd>
-__global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kernel(DevIndex ix, DevTree tr,
-                                                                   SweepWork sw, int nsrc,
-                                                                   int *ovf_flag,
-                                                                   uint64_t *stamps,
-                                                                   int fwd_only) {
-  __shared__ double sh_acc[kSweepWaves][kWave];
-  // Segments of levels split in runs: per-node partial results.
-  __shared__ uint64_t seg_x[kSweepMaxSegments][kWave], seg_y[kSweepMaxSegments][kWave];
-  __shared__ int seg_node[kSweepMaxSegments];
-  const Segs sg{seg_x, seg_y, seg_node};
-  const Lane L = lane_setup(ix, tr, sw, blockIdx.x, nsrc, tr.n);
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+// The whole sweep of tile t by one workgroup, in state slot `slot`: forward
+// over depth levels, backward over depth levels, then the ascending states.
+// Every level runs the pipelined steps over its staged pieces; a level split
+// in runs (few nodes, many pieces) then merges each node's per-wavefront
+// partial results from LDS in a fixed order, so results are deterministic.
+// Every record a tile reads it has written itself, so the result does not
+// depend on the slot or on what an earlier tile left there.
+// With fwd_only, the tile stops after the forward pass (distances and path
+// counts left in its slot, for probes of that pass).
+template <int kNodesF, int kNodesB>
+__device__ __forceinline__ void sweep_tile(const DevIndex &ix, const DevTree &tr, const SweepWork &sw,
+                                           int t, int slot, int nsrc, uint64_t *stamps, int fwd_only,
+                                           const Segs &sg, double (*sh_acc)[kWave], int wave, int &ovf) {
+  uint64_t(*seg_x)[kWave] = sg.x;
+  uint64_t(*seg_y)[kWave] = sg.y;
+  int *seg_node = sg.node;
+  const Lane L = lane_setup(ix, tr, sw, t, slot, nsrc, tr.n);
   const int H = tr.height;
-  int ovf = 0;
   // Optional phase timestamps (constant 100 MHz clock), one row per tile.
-  if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x] = __builtin_amdgcn_s_memrealtime();
+  if (stamps && threadIdx.x == 0) stamps[4 * t] = __builtin_amdgcn_s_memrealtime();
 
   // ---- forward: distance and path count ----
   for (int k = 0; k < H; ++k) {
@@ -725,9 +721,9 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
         if (s > 0 && __builtin_amdgcn_readfirstlane(seg_node[s - 1]) == v) continue;
         int b = (int)seg_y[s][L.lane];
         uint64_t c = seg_x[s][L.lane];
-        for (int t = s + 1; t < nseg && __builtin_amdgcn_readfirstlane(seg_node[t]) == v; ++t) {
-          const int bt = (int)seg_y[t][L.lane];
-          const uint64_t ct = seg_x[t][L.lane];
+        for (int s2 = s + 1; s2 < nseg && __builtin_amdgcn_readfirstlane(seg_node[s2]) == v; ++s2) {
+          const int bt = (int)seg_y[s2][L.lane];
+          const uint64_t ct = seg_x[s2][L.lane];
           c = bt < b ? ct : bt == b ? add_sat(c, ct) : c;
           b = min(b, bt);
         }
@@ -738,11 +734,8 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
     }
   }
 
-  if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
-  if (fwd_only) {  // uniform
-    if (ovf) atomicOr(ovf_flag, 1);
-    return;
-  }
+  if (stamps && threadIdx.x == 0) stamps[4 * t + 1] = __builtin_amdgcn_s_memrealtime();
+  if (fwd_only) return;  // uniform
 
   // ---- backward: dependencies, shortcut flows, node flows ----
   for (int k = H - 1; k >= 0; --k) {
@@ -766,8 +759,8 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
         if (s > 0 && __builtin_amdgcn_readfirstlane(seg_node[s - 1]) == v) continue;
         double tot = __builtin_bit_cast(double, seg_x[s][L.lane]);
         const double svv = __builtin_bit_cast(double, seg_y[s][L.lane]);
-        for (int t = s + 1; t < nseg && __builtin_amdgcn_readfirstlane(seg_node[t]) == v; ++t)
-          tot += __builtin_bit_cast(double, seg_x[t][L.lane]);
+        for (int s2 = s + 1; s2 < nseg && __builtin_amdgcn_readfirstlane(seg_node[s2]) == v; ++s2)
+          tot += __builtin_bit_cast(double, seg_x[s2][L.lane]);
         setQ(L, roff(v), (1.0 + tot) / svv);
         const double ns = wave_sum_dpp((L.valid && L.s != v) ? tot : 0.0);
         if (L.lane == 0) sw.node_t[(int64_t)L.t * tr.bslots + cld(tr.bnode_slot, v)] = ns;
@@ -776,7 +769,7 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
     }
   }
 
-  if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x + 2] = __builtin_amdgcn_s_memrealtime();
+  if (stamps && threadIdx.x == 0) stamps[4 * t + 2] = __builtin_amdgcn_s_memrealtime();
 
   // ---- ascending states: the union of the tile's root paths in top-down
   // order, each node's bag entries split over all wavefronts and merged in
@@ -862,22 +855,89 @@ __global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kerne
       __syncthreads();
     }
   }
-  if (stamps && threadIdx.x == 0) stamps[4 * blockIdx.x + 3] = __builtin_amdgcn_s_memrealtime();
+  if (stamps && threadIdx.x == 0) stamps[4 * t + 3] = __builtin_amdgcn_s_memrealtime();
+}
+
+// State slot of a workgroup's tile.  A launch of no more tiles than slots
+// gives workgroup w slot w.  A longer launch (sw.pool > 0) draws on the slots
+// as a pool: the hardware keeps starting workgroups as others finish, and
+// each takes a free slot with an atomic flag and returns it at its end, so
+// the slots, not the launch's tiles, bound the state memory.  The slots are
+// split by accelerator die (XCC_ID), sw.pool per die: the dies' L2 caches are
+// not kept coherent with each other inside a kernel, so a slot's records
+// never move between dies within a launch.  A die's share holds as many
+// slots as workgroups can be resident on it, so a free one appears as soon
+// as the workgroup that held it has ended; the wait is still bounded, and a
+// workgroup that finds none returns -1 (the launch then reports an error).
+__device__ __forceinline__ int take_slot(const SweepWork &sw) {
+  const int die = (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7u);  // XCC_ID[3:0]
+  const int base = (die % kSweepDies) * sw.pool;
+  const int first = (int)(blockIdx.x % (unsigned)sw.pool);
+  for (int round = 0; round < (1 << 14); ++round) {
+    for (int i = 0; i < sw.pool; ++i) {
+      const int s = base + (first + i) % sw.pool;
+      if (atomicCAS(&sw.slot_busy[s], 0, 1) == 0) return s;
+    }
+    __builtin_amdgcn_s_sleep(64);
+  }
+  return -1;
+}
+
+// One workgroup per tile of 64 sources (tile blockIdx.x of the launch).
+template <int kNodesF, int kNodesB, int kWavesPerSimd>
+__global__ __launch_bounds__(kSweepThreads, kWavesPerSimd) void sweep_tile_kernel(DevIndex ix, DevTree tr,
+                                                                   SweepWork sw, int nsrc,
+                                                                   int *ovf_flag,
+                                                                   uint64_t *stamps,
+                                                                   int fwd_only) {
+  __shared__ double sh_acc[kSweepWaves][kWave];
+  // Segments of levels split in runs: per-node partial results.
+  __shared__ uint64_t seg_x[kSweepMaxSegments][kWave], seg_y[kSweepMaxSegments][kWave];
+  __shared__ int seg_node[kSweepMaxSegments];
+  __shared__ int sh_slot;
+  const Segs sg{seg_x, seg_y, seg_node};
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int t = blockIdx.x;
+  if (threadIdx.x == 0) sh_slot = sw.pool > 0 ? take_slot(sw) : t;
+  __syncthreads();
+  const int slot = __builtin_amdgcn_readfirstlane(sh_slot);
+  if (slot < 0) {  // uniform: no slot came free (the host reports it)
+    if (threadIdx.x == 0) atomicOr(ovf_flag, kSweepNoSlot);
+    return;
+  }
+  if (threadIdx.x == 0) sw.tile_slot[t] = slot;
+  int ovf = 0;
+  sweep_tile<kNodesF, kNodesB>(ix, tr, sw, t, slot, nsrc, stamps, fwd_only, sg, sh_acc, wave, ovf);
   if (ovf) atomicOr(ovf_flag, 1);
+  if (sw.pool > 0) {
+    // Every store of every wave to the slot has reached the die's L2 (a
+    // store's count drops when the L2 takes it) before the slot is
+    // returned there: the next workgroup to take it, on the same die, writes
+    // the same records and reads only what it wrote itself.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicExch(&sw.slot_busy[slot], 0);
+  }
+}
+
+__global__ void sweep_pool_reset_kernel(int *busy, int slots) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < slots) busy[i] = 0;
 }
 
 using TileKernel = void (*)(DevIndex, DevTree, SweepWork, int, int *, uint64_t *, int);
 
-// Two forms of the tile kernel.  Up to one tile per compute unit: 2 pieces per
-// step in each pass, registers for 5 waves per SIMD (one 16-wave workgroup per
-// unit).  More tiles than units: 1 piece per step, registers capped at 64 for
-// 8 waves per SIMD, so two workgroups share each unit and one's waits are
-// covered by the other's work (on road graphs 11-16% more sources per second,
-// with three tiles per unit the best, while one such workgroup alone is 8%
-// slower than the first form).  The tuning may force either form.
-int tile_form(int tiles, int cus, int form) {
+// Two forms of the tile kernel.  Up to one workgroup per compute unit: 2
+// pieces per step in each pass, registers for 5 waves per SIMD (one 16-wave
+// workgroup per unit).  More workgroups than units: 1 piece per step,
+// registers capped at 64 for 8 waves per SIMD, so two workgroups share each
+// unit and one's waits are covered by the other's work (one such workgroup
+// alone is 8% slower than the first form; two sharing a unit run about 1.9
+// times the tiles per second of one alone).  `groups` is the launch's
+// resident workgroups (state slots in use).  The tuning may force either form.
+int tile_form(int groups, int cus, int form) {
   if (form == BCX_SWEEP_FORM_ONE_PER_CU || form == BCX_SWEEP_FORM_TWO_PER_CU) return form;
-  return tiles > cus ? BCX_SWEEP_FORM_TWO_PER_CU : BCX_SWEEP_FORM_ONE_PER_CU;
+  return groups > cus ? BCX_SWEEP_FORM_TWO_PER_CU : BCX_SWEEP_FORM_ONE_PER_CU;
 }
 TileKernel tile_kernel(int form) {
   if (form == BCX_SWEEP_FORM_TWO_PER_CU) return (TileKernel)sweep_tile_kernel<1, 1, 8>;
@@ -885,14 +945,14 @@ TileKernel tile_kernel(int form) {
 }
 
 // Distance and count rows of a launch's tiles after a forward-only launch:
-// row r (tile r / 64, lane r % 64) of node p goes to d[r * n + p], s[r * n + p]
-// (tree ids), for r below nrows.
+// row r (tile r / 64, lane r % 64, in the slot that ran the tile) of node p
+// goes to d[r * n + p], s[r * n + p] (tree ids), for r below nrows.
 __global__ __launch_bounds__(256) void sweep_extract_kernel(SweepWork sw, int n, int dbits, int *d,
                                                             uint64_t *s) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int r = blockIdx.y;
-  const char *rec = sw.state + ((int64_t)(r / kWave) * (n + 1) + p) * kRec;
+  const char *rec = sw.state + ((int64_t)sw.tile_slot[r / kWave] * (n + 1) + p) * kRec;
   const int lane = r % kWave;
   const int w = *(const int *)(rec + lane * 4);
   const uint32_t cw = (uint32_t)w >> dbits;  // the count, or 0 where it is in the slot
@@ -1072,28 +1132,34 @@ void bcx_ctx::free_sweep() {
   if (sweep_exec) (void)hipGraphExecDestroy(sweep_exec);
   sweep_exec = nullptr;
   sweep_exec_key.clear();
-  void *bufs[] = {sw.state, sw.node_t, sw.phi_t,
-                  sw.node, sw.phi,     sw.interior, sw.tile_src, sw.cand, sw.cand_off,
-                  sw.pf};
+  void *bufs[] = {sw.state, sw.node_t,   sw.phi_t,    sw.node,      sw.phi,
+                  sw.interior, sw.tile_src, sw.cand, sw.cand_off, sw.pf, sw.slot_busy, sw.tile_slot};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   sw = SweepWork{};
   sweep_tiles = 0;
+  sweep_alloc = 0;
+  sweep_slots = 0;
+  sweep_pool = 0;
 }
 
-void bcx_ctx::ensure_sweep(int tiles) {
-  if (tiles <= sweep_tiles) return;
+void bcx_ctx::ensure_sweep(int slots, int tiles) {
+  if (slots <= sweep_alloc && tiles <= sweep_tiles) return;
   free_sweep();
   const size_t n = (size_t)ix.n, B = (size_t)tree.bags, T = (size_t)tiles;
-  HIP_OK(hipMalloc(&sw.state, T * (n + 1) * kRec));
+  HIP_OK(hipMalloc(&sw.state, (size_t)slots * (n + 1) * kRec));
   HIP_OK(hipMalloc(&sw.node_t, T * (size_t)tree.bslots * 8));
   HIP_OK(hipMalloc(&sw.phi_t, T * std::max<size_t>(B, 1) * 8));
+  HIP_OK(hipMalloc(&sw.tile_slot, T * 4));
+  HIP_OK(hipMalloc(&sw.slot_busy, (size_t)slots * 4));
+  HIP_OK(hipMemset(sw.slot_busy, 0, (size_t)slots * 4));
   HIP_OK(hipMalloc(&sw.node, n * 8));
   HIP_OK(hipMalloc(&sw.phi, std::max<size_t>(B, 1) * 8));
   HIP_OK(hipMalloc(&sw.interior, n * 8));
   HIP_OK(hipMalloc(&sw.pf, std::max<size_t>((size_t)tree.pairs, 1) * 8));
   // The job arrays (tile sources, candidate lists) are sized by sweep_plan.
-  sw.tstride = tiles;
+  sw.slots = slots;
+  sweep_alloc = slots;
   sweep_tiles = tiles;
 }
 
@@ -1106,37 +1172,86 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
       (uint64_t)c->tree.bslots * 8 > (uint64_t)kDrop)
     throw bcx::Error{BCX_EGRAPH, "bag-sweep tile state exceeds its 32-bit offsets (more than 5242879 vertices)"};
   const int total_tiles = (int)(((int64_t)count + kWave - 1) / kWave);
-  int tiles = std::max(1, std::min(c->tn.sweep_batch / kWave, std::max(1, total_tiles)));
-  // Keep the per-tile state within 85% of the device memory not already held
-  // by this context's sweep buffers (and within the tuning's memory limit).
-  // When that cuts the launch short and the label-scan workspace is
-  // allocated, release it (it is re-allocated on its next use) and measure
-  // again.  A tile's sweep is a latency chain through the levels, so a launch
-  // of few tiles costs nearly as much as a full one: fitting the whole batch
-  // in one launch matters.
-  const size_t per_tile = ((size_t)n + 1) * kRec + (size_t)c->tree.bslots * 8 + (size_t)B * 8;
-  auto cap_now = [&]() {
+  const size_t want = (size_t)std::max(1, std::min(c->tn.sweep_batch / kWave, std::max(1, total_tiles)));
+  const size_t cus = (size_t)std::max(1, c->cus);
+  const int forced = c->tn.sweep_form;
+  // Device memory: a state slot's records (one per resident workgroup), and
+  // per tile of a launch its node and shortcut flow sums.  Both stay within
+  // 85% of the device memory not already held by this context's sweep
+  // buffers (and within the tuning's memory limit).
+  const size_t per_slot = ((size_t)n + 1) * kRec;
+  const size_t per_tile = (size_t)c->tree.bslots * 8 + (size_t)B * 8 + 4;
+  auto budget_now = [&]() {
     size_t fr = 0, tot = 0;
     HIP_OK(hipMemGetInfo(&fr, &tot));
-    const size_t held = (size_t)c->sweep_tiles * per_tile;
-    size_t cap = (size_t)((double)(fr + held) * 0.85) / std::max<size_t>(per_tile, 1);
-    if (c->tn.sweep_mem_limit > 0) cap = std::min(cap, (size_t)c->tn.sweep_mem_limit / per_tile);
+    const size_t held = (size_t)c->sweep_alloc * per_slot + (size_t)c->sweep_tiles * per_tile;
+    size_t cap = (size_t)((double)(fr + held) * 0.85);
+    if (c->tn.sweep_mem_limit > 0) cap = std::min(cap, (size_t)c->tn.sweep_mem_limit);
     return cap;
   };
-  size_t cap = cap_now();
-  if (cap < (size_t)tiles && c->ws_slots > 0) {
+  // The launch's shape for a memory budget:
+  //  * up to one tile per compute unit (unless the two-workgroup form is
+  //    forced), one slot per tile;
+  //  * more, a slot for every workgroup the form can keep resident (two per
+  //    unit in the two-workgroup form), and the launch's tiles drawing on
+  //    them as a pool, as many tiles as the flow sums' memory allows;
+  //  * where the two-workgroup form's slots do not fit, the one-workgroup
+  //    form's (chosen form permitting);
+  //  * where no pool fits, one slot per tile, a whole number of tiles per
+  //    unit beyond one per unit.
+  size_t slots = 0, tiles = 0;
+  auto shape = [&](size_t cap) {
+    tiles = want;
+    const size_t full1 = (size_t)c->occ_one * cus, full2 = (size_t)c->occ_two * cus;
+    const bool pool_ok = cus % kSweepDies == 0;
+    auto pooled = [&](size_t full) {  // slots for every resident workgroup, as many tiles as fit
+      if (!pool_ok || full * (per_slot + per_tile) > cap) return false;
+      slots = std::min(want, full);
+      tiles = std::max(slots, std::min(want, (cap - slots * per_slot) / per_tile));
+      return true;
+    };
+    if (forced != BCX_SWEEP_FORM_TWO_PER_CU && want <= cus) {
+      slots = want;
+      if (slots * (per_slot + per_tile) <= cap) return;
+    } else if (forced == BCX_SWEEP_FORM_ONE_PER_CU) {
+      if (pooled(full1)) return;
+    } else {
+      if (pooled(full2)) return;
+      if (forced == BCX_SWEEP_FORM_AUTO && pooled(full1)) return;
+    }
+    slots = std::min(want, cap / (per_slot + per_tile));
+    if (slots > cus) slots -= slots % cus;
+    tiles = slots;
+  };
+  size_t cap = budget_now();
+  shape(cap);
+  // Cut short by memory with the label-scan workspace allocated: release it
+  // (it is re-allocated on its next use) and measure again.
+  if ((tiles < want || slots < std::min(want, (size_t)c->occ_two * cus)) && c->ws_slots > 0) {
     HIP_OK(hipStreamSynchronize(c->stream));
     c->free_ws();
-    cap = cap_now();
+    cap = budget_now();
+    shape(cap);
   }
-  if (cap < 1) throw bcx::Error{BCX_ENOMEM, "not enough device memory for one sweep tile"};
-  // Cut short by memory, a launch of more tiles than compute units takes a
-  // whole number of tiles per unit, so that no unit runs a lone extra tile
-  // after the others are done.
-  if (cap < (size_t)tiles && c->cus > 0 && cap > (size_t)c->cus) cap -= cap % (size_t)c->cus;
-  tiles = (int)std::min<size_t>((size_t)tiles, cap);
-  c->ensure_sweep(tiles);
-  return tiles;
+  if (slots < 1) throw bcx::Error{BCX_ENOMEM, "not enough device memory for one sweep tile"};
+  c->ensure_sweep((int)slots, (int)tiles);
+  c->sweep_slots = (int)slots;
+  c->sweep_pool = tiles > slots ? (int)(slots / kSweepDies) : 0;
+  return (int)tiles;
+}
+
+int sweep_launch_form(const bcx_ctx *c, int nt) {
+  return tile_form(std::min(nt, c->sweep_slots), c->cus, c->tn.sweep_form);
+}
+
+void sweep_occupancy(bcx_ctx *c) {
+  int one = 0, two = 0;
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &one, (const void *)tile_kernel(BCX_SWEEP_FORM_ONE_PER_CU), kSweepThreads, 0));
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &two, (const void *)tile_kernel(BCX_SWEEP_FORM_TWO_PER_CU), kSweepThreads, 0));
+  c->occ_one = std::max(1, one);
+  c->occ_two = std::max(1, two);
 }
 
 // Plans a job over the sorted tree ids pre[0 .. cnt): tile t holds sources
@@ -1220,6 +1335,16 @@ void sweep_tiles_launch(bcx_ctx *c, int tile0, int nt, int nsrc, int form, bool 
     HIP_OK(hipMalloc(&stamps, (size_t)nt * 4 * 8));
     HIP_OK(hipEventRecord(c->ev[0], c->stream));
   }
+  // A launch of more tiles than slots draws on them as a pool; its flags are
+  // cleared first by a kernel (not a memset node, so that the launch sequence
+  // replays correctly from a captured graph).
+  sw.pool = nt > c->sweep_slots ? c->sweep_pool : 0;
+  if (nt > c->sweep_slots && sw.pool < 1) throw bcx::Error{BCX_EINVAL, "sweep launch exceeds its slots"};
+  if (sw.pool > 0) {
+    sweep_pool_reset_kernel<<<blocks_for(c->sweep_slots, 256), 256, 0, c->stream>>>(sw.slot_busy,
+                                                                                  c->sweep_slots);
+    HIP_OK(hipGetLastError());
+  }
   tile_kernel(form)<<<nt, kSweepThreads, 0, c->stream>>>(c->ix, c->tree, sw, nsrc, c->d_ovf, stamps,
                                                         fwd_only ? 1 : 0);
   HIP_OK(hipGetLastError());
@@ -1260,10 +1385,12 @@ void sweep_tiles_launch(bcx_ctx *c, int tile0, int nt, int nsrc, int form, bool 
 void sweep_launch(bcx_ctx *c, const int32_t *pre, int32_t nsrc, double *node_out,
                   double *phi_out, bool accumulate) {
   const int bt = (nsrc + kWave - 1) / kWave;
-  if (bt < 1 || bt > c->sweep_tiles) throw bcx::Error{BCX_EINVAL, "sweep launch exceeds its tiles"};
+  // The launch shape for this many sources (the buffers may have been
+  // released for the label-scan workspace since the caller sized its batch).
+  if (bt < 1 || bt > sweep_tiles_for(c, nsrc)) throw bcx::Error{BCX_EINVAL, "sweep launch exceeds its tiles"};
   sweep_plan(c, pre, nsrc);
-  sweep_tiles_launch(c, 0, bt, nsrc, tile_form(bt, c->cus, c->tn.sweep_form), false, node_out,
-                     phi_out, accumulate, c->profiling);
+  sweep_tiles_launch(c, 0, bt, nsrc, sweep_launch_form(c, bt), false, node_out, phi_out, accumulate,
+                     c->profiling);
   c->timing.scan_launches += 1;
   c->timing.dep_launches += 1;
 }
@@ -1321,7 +1448,7 @@ void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_ino
   const int tiles = sweep_tiles_for(c, count);
   sweep_plan(c, pre.data(), count);
   const int T = (int)(((int64_t)count + kWave - 1) / kWave);
-  const int form = tile_form(std::min(tiles, std::max(T, 1)), c->cus, c->tn.sweep_form);
+  const int form = sweep_launch_form(c, std::min(tiles, std::max(T, 1)));
   std::vector<double> bc_pre((size_t)n);
   for (int32_t v = 0; v < n; ++v) bc_pre[(size_t)c->iperm[(size_t)v]] = bc_inout[v];
   const int zero = 0;
@@ -1347,6 +1474,8 @@ void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_ino
                                       tiles,
                                       form,
                                       c->tn.unpack_wide,
+                                      c->sweep_slots,
+                                      c->sweep_pool,
                                       (int64_t)(intptr_t)c->sw.state,
                                       (int64_t)(intptr_t)c->sw.tile_src,
                                       (int64_t)(intptr_t)c->sw.cand,
@@ -1382,8 +1511,9 @@ void run_sweep(bcx_ctx *c, const int32_t *sources, int32_t count, double *bc_ino
   HIP_OK(hipMemcpyAsync(bc_pre.data(), c->d_bc, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipMemcpyAsync(&ovf, c->d_ovf, 4, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
+  if (ovf & kSweepNoSlot) throw bcx::Error{BCX_EDEVICE, "bag sweep: a workgroup found no free state slot"};
   for (int32_t v = 0; v < n; ++v) bc_inout[v] = bc_pre[(size_t)c->iperm[(size_t)v]];
-  if (overflow) *overflow = (uint32_t)ovf;
+  if (overflow) *overflow = (uint32_t)(ovf & 1);
 }
 
 int sweep_query(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t *d, uint64_t *sigma) {
@@ -1401,8 +1531,11 @@ int sweep_query(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t *d, u
   // Launches of up to `chunk` tiles, their rows extracted to about 1 GiB of
   // scratch at a time (and at most 65535 rows, the grid's second dimension).
   const int64_t rows_cap = std::max<int64_t>(kWave, (1ll << 30) / (12ll * n));
-  const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)tiles, rows_cap / kWave, 1023}));
-  const int form = tile_form(chunk, c->cus, c->tn.sweep_form);
+  // No pooled launches here: a tile's records must stay in its slot until
+  // they are extracted.
+  const int chunk = (int)std::max<int64_t>(
+      1, std::min<int64_t>({(int64_t)tiles, (int64_t)c->sweep_slots, rows_cap / kWave, 1023}));
+  const int form = sweep_launch_form(c, chunk);
   sweep_plan(c, pre.data(), count);
   const int zero = 0;
   HIP_OK(hipMemcpyAsync(c->d_ovf, &zero, 4, hipMemcpyHostToDevice, c->stream));
@@ -1446,7 +1579,14 @@ int sweep_query(bcx_ctx *c, const int32_t *sources, int32_t count, int32_t *d, u
   release();
   int ovf = 0;
   HIP_OK(hipMemcpy(&ovf, c->d_ovf, 4, hipMemcpyDeviceToHost));
-  return ovf ? 1 : 0;
+  if (ovf & kSweepNoSlot) throw bcx::Error{BCX_EDEVICE, "bag sweep: a workgroup found no free state slot"};
+  return (ovf & 1) ? 1 : 0;
+}
+
+void sweep_check_slots(bcx_ctx *c) {
+  int ovf = 0;
+  HIP_OK(hipMemcpy(&ovf, c->d_ovf, 4, hipMemcpyDeviceToHost));
+  if (ovf & kSweepNoSlot) throw bcx::Error{BCX_EDEVICE, "bag sweep: a workgroup found no free state slot"};
 }
 
 }  // namespace bcx_dev

@@ -76,9 +76,11 @@ typedef struct bcx_tuning {
   int32_t tchunk;        /* targets per wavefront work item                 */
   int32_t bp_threads;    /* threads per dependency group: 64..512, pow2   */
   int32_t batch;         /* sources per batched launch sequence             */
-  int32_t sweep_batch;   /* sources per bag-sweep launch (64 per tile, one
-                            tile per workgroup); default 3 x 64 x compute
-                            units (two workgroups per unit at a time)       */
+  int32_t sweep_batch;   /* sources per bag-sweep launch (64 per tile, each
+                            tile run by one workgroup, the launch's
+                            resident workgroups taking tiles in turn);
+                            default 8 x 64 x compute units (two workgroups
+                            per unit at a time)                             */
   int32_t sweep_form;    /* bag-sweep tile-kernel form: BCX_SWEEP_FORM_AUTO
                             (by tiles per launch), _ONE_PER_CU (2 pieces per
                             step, one workgroup per compute unit) or
@@ -230,9 +232,10 @@ int bcx_ctx_get_tuning(const bcx_ctx *ctx, bcx_tuning *tn);
 /* Autotuner (no reference counterpart: the reference fixes 256x256 launches
  * at compile time, MHLQBC.cu:3).  Times candidate label-scan target chunks
  * and dependency-group widths on `sources` (one batch at most), and bag-sweep
- * launch sizes (1 to 4 tiles of 64 sources per compute unit, those that fit
- * device memory) in both tile-kernel forms, each on the same job of 12 x 64
- * sources per compute unit taken in tree order; keeps the fastest of each in
+ * launch sizes (1, 2, 4, 8 and 16 tiles of 64 sources per compute unit,
+ * those that fit device memory) in both tile-kernel forms, each on the same
+ * job of 16 x 64 sources per compute unit taken in tree order (at most n);
+ * keeps the fastest of each in
  * the context (the current sweep setting unless a candidate is 2% faster) and
  * copies the tuning to *best (may be NULL). */
 int bcx_ctx_autotune(bcx_ctx *ctx, const int32_t *sources, int32_t count,
@@ -240,6 +243,11 @@ int bcx_ctx_autotune(bcx_ctx *ctx, const int32_t *sources, int32_t count,
 /* Sources one bag-sweep launch takes (the tuning's sweep_batch, cut to whole
  * tiles of 64 by device memory); allocates the launch state. */
 int bcx_ctx_sweep_capacity(bcx_ctx *ctx, int32_t *sources_per_launch);
+/* Bag-sweep state slots allocated (resident workgroups of a full launch:
+ * two per compute unit in the two-workgroup form, each with its own tile
+ * records) and the device's compute units; no reference counterpart (the
+ * reference gives every source its own block, MHLQBC.cu:508). */
+int bcx_ctx_sweep_slots(const bcx_ctx *ctx, int32_t *slots, int32_t *compute_units);
 /* Bytes of device memory one batched sequence of `batch` sources needs. */
 int bcx_ctx_workspace_bytes(const bcx_ctx *ctx, int32_t batch, size_t *bytes);
 int bcx_ctx_mem_info(const bcx_ctx *ctx, size_t *free_bytes, size_t *total_bytes);

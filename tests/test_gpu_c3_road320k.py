@@ -40,8 +40,8 @@ def test_sampled_sources_vs_oracle(cfg):
 def test_full_launch_two_forms_bitwise_and_label_scan(cfg):
     g, csr, ix, ctx = cfg
     batch = ctx.tuning()["sweep_batch"]
-    cus = batch // (3 * 64)
-    assert batch // 64 > cus, "the headline launch holds more tiles than compute units"
+    _, cus = ctx.sweep_slots()
+    assert batch // 64 > 2 * cus, "the headline launch holds more tiles than state slots"
     src = ix.tree_order()[:batch].astype(np.int32)
     ctx.reset_timing()
     two, o1 = ctx.bc_sources(src, bcx.MODE_SWEEP)

@@ -36,7 +36,7 @@ def test_memory_capped_job_equals_fitting_launches(cfg):
     g, csr, ix, ctx = cfg
     tree = ix.tree_order()
     batch = ctx.tuning()["sweep_batch"]
-    cus = batch // (3 * 64)
+    _, cus = ctx.sweep_slots()
     ctx.bc_sources(tree[:4096].astype(np.int32), bcx.MODE_BATCHED)  # holds the workspace
     src = tree[:batch].astype(np.int32)
     ctx.reset_timing()

@@ -359,16 +359,45 @@ def test_sweep_two_kernel_forms_agree_bitwise(road):
     # fewer tiles use the first.  Same tiles, same summation orders: the
     # results must be bitwise equal.
     g, csr, ctx = road
-    default = ctx.tuning()["sweep_batch"]  # three tiles of 64 per compute unit
-    cus = default // (3 * 64)
+    default = ctx.tuning()["sweep_batch"]
+    _, cus = ctx.sweep_slots()
     src = np.arange(g.n, dtype=np.int32)
     assert (g.n + 63) // 64 > cus, "the launch must hold more tiles than compute units"
     ctx.set_tuning(sweep_batch=default)
     two, o1 = ctx.bc_sources(src, bcx.MODE_SWEEP)
     ctx.set_tuning(sweep_batch=64 * max(1, cus // 2))
     one, o2 = ctx.bc_sources(src, bcx.MODE_SWEEP)
+    ctx.set_tuning(sweep_batch=default)
     assert not o1 and not o2
     assert np.array_equal(two.view(np.uint64), one.view(np.uint64))
+
+
+def test_sweep_pooled_slots_bitwise(road):
+    """A launch of more tiles than state slots: the one-workgroup form keeps
+    one slot per compute unit and the launch's workgroups take free slots as
+    others finish (a slot's records reused by several tiles in one launch).
+    Same tiles, same summation orders: bitwise equal to launches of at most
+    one tile per slot, and the oracle's BC."""
+    g, csr, ctx = road
+    default = ctx.tuning()["sweep_batch"]
+    _, cus = ctx.sweep_slots()
+    tiles = (g.n + 63) // 64
+    assert tiles > cus, "needs more tiles than compute units"
+    ctx.set_tuning(sweep_batch=64 * tiles, sweep_form=bcx.SWEEP_FORM_ONE_PER_CU)
+    ctx.reset_timing()
+    pooled, o1 = ctx.full_bc(bcx.MODE_SWEEP)
+    assert ctx.timing()["scan_launches"] == 1
+    slots, _ = ctx.sweep_slots()
+    assert slots == cus, f"{slots} slots for {tiles} tiles"
+    ctx.set_tuning(sweep_batch=64 * cus)
+    ctx.reset_timing()
+    split, o2 = ctx.full_bc(bcx.MODE_SWEEP)
+    assert ctx.timing()["scan_launches"] == -(-tiles // cus)
+    ctx.set_tuning(sweep_batch=default, sweep_form=bcx.SWEEP_FORM_AUTO)
+    assert not o1 and not o2
+    assert np.array_equal(pooled.view(np.uint64), split.view(np.uint64))
+    ref, _ = oracle.brandes(csr, threads=oracle.default_threads())
+    assert close_bc(pooled, ref)
 
 
 def test_sweep_scattered_sources(road):
@@ -544,31 +573,42 @@ def test_sweep_wide_unpack_path(maker):
 
 
 def test_sweep_memory_capped_launches(road):
-    """A launch cut short by the memory cap: the label-scan workspace is
-    released first, the launch rounds down to whole tiles per compute unit,
-    and the job runs as several launches; BC still matches the oracle."""
+    """Launches cut short by the memory cap, the label-scan workspace released
+    first: a cap that holds one state slot per compute unit but not two runs
+    the job as one pooled launch in the one-workgroup form; a cap below one
+    slot per unit runs it as several launches of one tile per slot.  Same
+    tiles, same summation orders: both bitwise equal to the uncapped job, and
+    the oracle's BC."""
     g, csr, _ = road
     G, ix, ctx = build(g)  # library default tuning (the shared context's is changed by other tests)
-    default = ctx.tuning()["sweep_batch"]
-    cus = default // (3 * 64)
+    _, cus = ctx.sweep_slots()
     tiles = (g.n + 63) // 64
     if tiles <= cus + 8:
         pytest.skip("needs more tiles than compute units")
     ctx.set_tuning(batch=256)
     ctx.bc_sources(np.arange(256, dtype=np.int32), bcx.MODE_BATCHED)  # holds the workspace
-    want_cap = (cus + tiles) // 2  # between the compute units and the job's tiles
-    ctx.set_tuning(sweep_batch=default, sweep_mem_limit=want_cap * (g.n + 1) * 768)
+    per_slot = (g.n + 1) * 768
     ctx.reset_timing()
-    capped, ovf = ctx.full_bc(bcx.MODE_SWEEP)
-    launches = ctx.timing()["scan_launches"]
-    ctx.set_tuning(sweep_mem_limit=0)
-    ctx.reset_timing()
-    whole, _ = ctx.full_bc(bcx.MODE_SWEEP)
+    whole, o0 = ctx.full_bc(bcx.MODE_SWEEP)
     assert ctx.timing()["scan_launches"] == 1
-    assert launches == -(-tiles // cus), f"{launches} launches for {tiles} tiles"
+    ctx.set_tuning(sweep_mem_limit=int(1.5 * cus * per_slot))  # one slot per unit, not two
+    ctx.bc_sources(np.arange(256, dtype=np.int32), bcx.MODE_BATCHED)  # holds the workspace again
+    ctx.reset_timing()
+    pooled, o1 = ctx.full_bc(bcx.MODE_SWEEP)
+    assert ctx.timing()["scan_launches"] == 1
+    assert ctx.sweep_slots()[0] == cus
+    want = cus // 2
+    ctx.set_tuning(sweep_mem_limit=want * per_slot)  # fewer slots than units
+    ctx.reset_timing()
+    split, o2 = ctx.full_bc(bcx.MODE_SWEEP)
+    launches = ctx.timing()["scan_launches"]
+    slots, _ = ctx.sweep_slots()
+    ctx.set_tuning(sweep_mem_limit=0)
+    assert 1 <= slots < want and launches == -(-tiles // slots), f"{launches} launches, {slots} slots"
+    assert not o0 and not o1 and not o2
+    assert np.array_equal(pooled.view(np.uint64), whole.view(np.uint64))
+    assert np.array_equal(split.view(np.uint64), whole.view(np.uint64))
     ref, _ = oracle.brandes(csr, threads=oracle.default_threads())
-    assert not ovf
-    assert close_bc(capped, ref)
     assert close_bc(whole, ref)
     ctx.close()
 
@@ -721,11 +761,12 @@ def test_autotune_picks_a_sweep_launch_and_stays_exact(road):
     tuning it keeps gives the oracle's BC."""
     g, csr, _ = road
     G, ix, ctx = build(g)
+    _, cus = ctx.sweep_slots()
     default = ctx.tuning()["sweep_batch"]
-    cus = default // (3 * 64)
     best = ctx.autotune(gg.permutation(12, 1, g.n)[:4096].astype(np.int32))
     assert best["sweep_form"] in (bcx.SWEEP_FORM_AUTO, bcx.SWEEP_FORM_ONE_PER_CU, bcx.SWEEP_FORM_TWO_PER_CU)
-    assert best["sweep_batch"] % (64 * cus) == 0 and 1 <= best["sweep_batch"] // (64 * cus) <= 4
+    assert best["sweep_batch"] == default or (
+        best["sweep_batch"] % (64 * cus) == 0 and best["sweep_batch"] // (64 * cus) in (1, 2, 4, 8, 16))
     assert ctx.tuning() == best
     src = gg.permutation(13, 1, g.n)[:3000].astype(np.int32)
     got, ovf = ctx.bc_sources(src, bcx.MODE_SWEEP)
