@@ -143,8 +143,11 @@ struct DevTree {
 };
 
 constexpr int kStageEntries = 4;
-// Default bag-sweep launch: tiles of 64 sources per compute unit.
-constexpr int kSweepTilesPerCu = 8;
+// Default bag-sweep launch: tiles of 64 sources per compute unit (a full BC
+// of a few hundred thousand sources is one launch), and the RQ engine's
+// steps (each admits requests at its start).
+constexpr int kSweepTilesPerCu = 32;
+constexpr int kRqTilesPerCu = 8;
 // Accelerator dies (XCDs) of the device: the state-slot pool is split among
 // them.  Overflow-flag bit set by a workgroup that found no free slot.
 constexpr int kSweepDies = 8;

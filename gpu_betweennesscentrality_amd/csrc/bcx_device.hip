@@ -578,9 +578,9 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
   HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto &e : c->ev) HIP_OK(hipEventCreate(&e));
   {
-    // Eight sweep tiles per compute unit per launch (two run at a time on a
-    // unit, each workgroup taking tiles until the launch's are all taken;
-    // memory may cut the launch to fewer).
+    // 32 sweep tiles per compute unit per launch (two run at a time on a
+    // unit, workgroups starting as others end; memory may cut the launch
+    // to fewer).
     int cus = 0;
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     c->cus = std::max(1, cus);
@@ -1444,7 +1444,11 @@ int bcx_rq_create_mode(bcx_ctx *c, int mode, size_t hbm_budget_bytes, bcx_rq **o
   r->budget = hbm_budget_bytes;
   const int32_t n = c->ix.n;
   if (mode == BCX_MODE_SWEEP) {
-    r->batch = bcx_dev::sweep_tiles_for(c, n) * bcx_dev::kWave;
+    // Steps of at most kRqTilesPerCu tiles per compute unit: a request is
+    // admitted at a step's start and served after one cycle of steps.
+    r->batch = bcx_dev::sweep_tiles_for(
+                   c, (int32_t)std::min<int64_t>(n, (int64_t)bcx_dev::kRqTilesPerCu * c->cus * bcx_dev::kWave)) *
+               bcx_dev::kWave;
     r->nb = (n + r->batch - 1) / r->batch;
     const size_t B = (size_t)std::max<int64_t>(c->tree.bags, 1);
     r->fixed = (size_t)r->nb * ((size_t)n + B) * 8 + (size_t)n * 8;
@@ -1690,22 +1694,21 @@ extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t coun
         best.bp_threads = bp;
       }
     }
-    // Bag sweep: every candidate runs the same job, 16 x compute units x 64
-    // consecutive tree-order sources (a whole number of launches for 1, 2,
-    // 4, 8 and 16 tiles per unit), timed by wall clock after a warm-up run,
-    // so launch counts, per-launch work and the unpack all count.  The
+    // Bag sweep: every candidate runs the same job, 32 x compute units x 64
+    // consecutive tree-order sources or all n (whole launches for 2 to 32
+    // tiles per unit where n allows), timed by wall clock after a warm-up
+    // run, so launch counts, per-launch work and the unpack all count.  The
     // context's own setting is timed first and is kept unless a candidate is
     // at least 2% faster (close candidates would otherwise be picked by
     // timing noise).
     const int32_t n = c->ix.n;
-    const int32_t total = (int32_t)std::min<int64_t>((int64_t)16 * c->cus * kWave, n);
+    const int32_t total = (int32_t)std::min<int64_t>((int64_t)32 * c->cus * kWave, n);
     std::vector<int32_t> job((size_t)total);
     for (int32_t i = 0; i < total; ++i) job[(size_t)i] = c->perm[(size_t)((n / 4 + (int64_t)i) % n)];
     double best_s = 1e300;
     c->profiling = false;
     auto time_sweep = [&](const bcx_tuning &t) {
       c->tn = t;
-      if (t.sweep_batch > total) return -1.0;  // more sources per launch than the job
       if (sweep_tiles_for(c, t.sweep_batch) * kWave < t.sweep_batch) return -1.0;  // cut by memory
       run_sweep(c, job.data(), total, bc_tmp.data(), nullptr);  // warm-up (and capture)
       const auto t0 = std::chrono::steady_clock::now();
@@ -1716,7 +1719,7 @@ extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t coun
       const double sec = time_sweep(best);
       if (sec > 0) best_s = sec;
     }
-    for (int tpc : {1, 2, 4, 8, 16}) {
+    for (int tpc : {2, 4, 8, 16, 32}) {
       for (int form : {BCX_SWEEP_FORM_ONE_PER_CU, BCX_SWEEP_FORM_TWO_PER_CU}) {
         bcx_tuning t = best;
         t.sweep_batch = tpc * c->cus * kWave;

@@ -1208,6 +1208,9 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
       if (!pool_ok || full * (per_slot + per_tile) > cap) return false;
       slots = std::min(want, full);
       tiles = std::max(slots, std::min(want, (cap - slots * per_slot) / per_tile));
+      // Cut short by memory, a whole number of tiles per slot, so that the
+      // launch's last round does not leave slots idle.
+      if (tiles < want) tiles -= tiles % slots;
       return true;
     };
     if (forced != BCX_SWEEP_FORM_TWO_PER_CU && want <= cus) {

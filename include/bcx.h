@@ -79,7 +79,7 @@ typedef struct bcx_tuning {
   int32_t sweep_batch;   /* sources per bag-sweep launch (64 per tile, each
                             tile run by one workgroup, the launch's
                             resident workgroups taking tiles in turn);
-                            default 8 x 64 x compute units (two workgroups
+                            default 32 x 64 x compute units (two workgroups
                             per unit at a time)                             */
   int32_t sweep_form;    /* bag-sweep tile-kernel form: BCX_SWEEP_FORM_AUTO
                             (by tiles per launch), _ONE_PER_CU (2 pieces per
@@ -232,9 +232,9 @@ int bcx_ctx_get_tuning(const bcx_ctx *ctx, bcx_tuning *tn);
 /* Autotuner (no reference counterpart: the reference fixes 256x256 launches
  * at compile time, MHLQBC.cu:3).  Times candidate label-scan target chunks
  * and dependency-group widths on `sources` (one batch at most), and bag-sweep
- * launch sizes (1, 2, 4, 8 and 16 tiles of 64 sources per compute unit,
+ * launch sizes (2, 4, 8, 16 and 32 tiles of 64 sources per compute unit,
  * those that fit device memory) in both tile-kernel forms, each on the same
- * job of 16 x 64 sources per compute unit taken in tree order (at most n);
+ * job of 32 x 64 sources per compute unit taken in tree order (at most n);
  * keeps the fastest of each in
  * the context (the current sweep setting unless a candidate is 2% faster) and
  * copies the tuning to *best (may be NULL). */

@@ -35,8 +35,10 @@ def test_sampled_sources_vs_oracle_with_workspace_held(cfg):
 def test_memory_capped_job_equals_fitting_launches(cfg):
     g, csr, ix, ctx = cfg
     tree = ix.tree_order()
-    batch = ctx.tuning()["sweep_batch"]
     _, cus = ctx.sweep_slots()
+    default = ctx.tuning()["sweep_batch"]
+    batch = 4 * cus * 64  # four tiles per compute unit: more than device memory holds at this size
+    ctx.set_tuning(sweep_batch=batch)
     ctx.bc_sources(tree[:4096].astype(np.int32), bcx.MODE_BATCHED)  # holds the workspace
     src = tree[:batch].astype(np.int32)
     ctx.reset_timing()
@@ -50,6 +52,7 @@ def test_memory_capped_job_equals_fitting_launches(cfg):
         part, o2 = ctx.bc_sources(src[k:k + step], bcx.MODE_SWEEP)
         assert not o2
         parts += part
+    ctx.set_tuning(sweep_batch=default)
     assert not o1
     assert close_bc(capped, parts), f"max rel err {max_rel(capped, parts):.3e}"
 

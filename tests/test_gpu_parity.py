@@ -766,7 +766,7 @@ def test_autotune_picks_a_sweep_launch_and_stays_exact(road):
     best = ctx.autotune(gg.permutation(12, 1, g.n)[:4096].astype(np.int32))
     assert best["sweep_form"] in (bcx.SWEEP_FORM_AUTO, bcx.SWEEP_FORM_ONE_PER_CU, bcx.SWEEP_FORM_TWO_PER_CU)
     assert best["sweep_batch"] == default or (
-        best["sweep_batch"] % (64 * cus) == 0 and best["sweep_batch"] // (64 * cus) in (1, 2, 4, 8, 16))
+        best["sweep_batch"] % (64 * cus) == 0 and best["sweep_batch"] // (64 * cus) in (2, 4, 8, 16, 32))
     assert ctx.tuning() == best
     src = gg.permutation(13, 1, g.n)[:3000].astype(np.int32)
     got, ovf = ctx.bc_sources(src, bcx.MODE_SWEEP)
