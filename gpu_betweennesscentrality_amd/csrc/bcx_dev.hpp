@@ -148,8 +148,9 @@ constexpr int kStageEntries = 4;
 // steps (each admits requests at its start).
 constexpr int kSweepTilesPerCu = 32;
 constexpr int kRqTilesPerCu = 8;
-// Accelerator dies (XCDs) of the device: the state-slot pool is split among
-// them.  Overflow-flag bit set by a workgroup that found no free slot.
+// Accelerator dies (XCDs) of an unpartitioned MI355X: the state-slot pool is
+// split among them (a device of one die keeps one pool; other die counts do
+// not pool).  Overflow-flag bit set by a workgroup that found no free slot.
 constexpr int kSweepDies = 8;
 constexpr int kSweepNoSlot = 2;
 constexpr int kNarrowEntries = 2;
@@ -184,6 +185,7 @@ struct SweepWork {
   int slots;       // state slots allocated
   int pool;        // per launch: slots per accelerator die when the launch has
                    // more tiles than slots and draws on them as a pool, else 0
+  int dies;        // accelerator dies the pool is split among (1 or kSweepDies)
   int tile0;       // job tile of a launch's tile 0 (per launch)
 };
 
@@ -258,6 +260,7 @@ struct bcx_ctx {
   int sweep_slots = 0;  // state slots the current launch shape uses (<= sweep_alloc)
   int sweep_pool = 0;   // slots per die when launches may hold more tiles than slots
   int occ_one = 1, occ_two = 2;  // resident workgroups per compute unit of each tile-kernel form
+  int dies = 1;                  // accelerator dies (XCCs) of the device
   bcx_dev::SweepWork sw{};
   // A sweep job's plan on the host (tile sources, ascending-pass candidate
   // lists); it stays alive until the next job starts, which first waits for

@@ -863,15 +863,15 @@ __device__ __forceinline__ void sweep_tile(const DevIndex &ix, const DevTree &tr
 // as a pool: the hardware keeps starting workgroups as others finish, and
 // each takes a free slot with an atomic flag and returns it at its end, so
 // the slots, not the launch's tiles, bound the state memory.  The slots are
-// split by accelerator die (XCC_ID), sw.pool per die: the dies' L2 caches are
-// not kept coherent with each other inside a kernel, so a slot's records
-// never move between dies within a launch.  A die's share holds as many
+// split by accelerator die (XCC_ID, on a device of kSweepDies dies), sw.pool
+// per die: the dies' L2 caches are not kept coherent with each other inside
+// a kernel, so a slot's records never move between dies within a launch.  A die's share holds as many
 // slots as workgroups can be resident on it, so a free one appears as soon
 // as the workgroup that held it has ended; the wait is still bounded, and a
 // workgroup that finds none returns -1 (the launch then reports an error).
 __device__ __forceinline__ int take_slot(const SweepWork &sw) {
-  const int die = (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7u);  // XCC_ID[3:0]
-  const int base = (die % kSweepDies) * sw.pool;
+  const int die = sw.dies == 1 ? 0 : (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7u);  // XCC_ID
+  const int base = die * sw.pool;
   const int first = (int)(blockIdx.x % (unsigned)sw.pool);
   for (int round = 0; round < (1 << 14); ++round) {
     for (int i = 0; i < sw.pool; ++i) {
@@ -1203,7 +1203,7 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   auto shape = [&](size_t cap) {
     tiles = want;
     const size_t full1 = (size_t)c->occ_one * cus, full2 = (size_t)c->occ_two * cus;
-    const bool pool_ok = cus % kSweepDies == 0;
+    const bool pool_ok = (c->dies == 1 || c->dies == kSweepDies) && cus % (size_t)c->dies == 0;
     auto pooled = [&](size_t full) {  // slots for every resident workgroup, as many tiles as fit
       if (!pool_ok || full * (per_slot + per_tile) > cap) return false;
       slots = std::min(want, full);
@@ -1239,7 +1239,7 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   if (slots < 1) throw bcx::Error{BCX_ENOMEM, "not enough device memory for one sweep tile"};
   c->ensure_sweep((int)slots, (int)tiles);
   c->sweep_slots = (int)slots;
-  c->sweep_pool = tiles > slots ? (int)(slots / kSweepDies) : 0;
+  c->sweep_pool = tiles > slots ? (int)(slots / (size_t)c->dies) : 0;
   return (int)tiles;
 }
 
@@ -1255,6 +1255,9 @@ void sweep_occupancy(bcx_ctx *c) {
       &two, (const void *)tile_kernel(BCX_SWEEP_FORM_TWO_PER_CU), kSweepThreads, 0));
   c->occ_one = std::max(1, one);
   c->occ_two = std::max(1, two);
+  int dies = 0;
+  HIP_OK(hipDeviceGetAttribute(&dies, hipDeviceAttributeNumberOfXccs, c->device));
+  c->dies = std::max(1, dies);
 }
 
 // Plans a job over the sorted tree ids pre[0 .. cnt): tile t holds sources
@@ -1342,6 +1345,7 @@ void sweep_tiles_launch(bcx_ctx *c, int tile0, int nt, int nsrc, int form, bool 
   // cleared first by a kernel (not a memset node, so that the launch sequence
   // replays correctly from a captured graph).
   sw.pool = nt > c->sweep_slots ? c->sweep_pool : 0;
+  sw.dies = c->dies;
   if (nt > c->sweep_slots && sw.pool < 1) throw bcx::Error{BCX_EINVAL, "sweep launch exceeds its slots"};
   if (sw.pool > 0) {
     sweep_pool_reset_kernel<<<blocks_for(c->sweep_slots, 256), 256, 0, c->stream>>>(sw.slot_busy,

@@ -613,21 +613,35 @@ def test_sweep_memory_capped_launches(road):
     ctx.close()
 
 
-def test_sweep_captured_job_replays(road):
+@pytest.mark.parametrize("pooled", [False, True])
+def test_sweep_captured_job_replays(road, pooled):
     """The captured launch sequence of a job, replayed several times with
     eager (profiled, uncaptured) jobs in between, gives the eager result bit
-    for bit every time."""
+    for bit every time.  pooled: the one-workgroup form's 256 slots under
+    launches of 300 tiles (the slot flags cleared by a kernel in the graph)."""
     g, csr, ctx = road
-    src = np.arange(0, g.n, 3, dtype=np.int32)
-    ctx.set_profiling(True)
-    eager, _ = ctx.bc_sources(src, bcx.MODE_SWEEP)
-    ctx.set_profiling(False)
-    for rep in range(4):
-        got, _ = ctx.bc_sources(src, bcx.MODE_SWEEP)
-        assert np.array_equal(got.view(np.uint64), eager.view(np.uint64)), f"replay {rep}"
+    saved = ctx.tuning()
+    if pooled:
+        src = np.arange(g.n, dtype=np.int32)
+        ctx.set_tuning(sweep_batch=300 * 64, sweep_form=bcx.SWEEP_FORM_ONE_PER_CU)
+        assert ctx.sweep_capacity() == 300 * 64
+        slots, cus = ctx.sweep_slots()
+        assert slots == cus < 300, f"{slots} slots"
+    else:
+        src = np.arange(0, g.n, 3, dtype=np.int32)
+    try:
         ctx.set_profiling(True)
-        other, _ = ctx.bc_sources(src[:777], bcx.MODE_SWEEP)
+        eager, _ = ctx.bc_sources(src, bcx.MODE_SWEEP)
         ctx.set_profiling(False)
+        for rep in range(4):
+            got, _ = ctx.bc_sources(src, bcx.MODE_SWEEP)
+            assert np.array_equal(got.view(np.uint64), eager.view(np.uint64)), f"replay {rep}"
+            ctx.set_profiling(True)
+            other, _ = ctx.bc_sources(src[:777], bcx.MODE_SWEEP)
+            ctx.set_profiling(False)
+    finally:
+        ctx.set_profiling(False)
+        ctx.set_tuning(sweep_batch=saved["sweep_batch"], sweep_form=saved["sweep_form"])
     ref, _ = oracle.brandes(csr, src, threads=oracle.default_threads())
     assert close_bc(eager, ref)
 
