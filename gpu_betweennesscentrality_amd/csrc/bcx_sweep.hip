@@ -1200,8 +1200,10 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   //  * where no pool fits, one slot per tile, a whole number of tiles per
   //    unit beyond one per unit.
   size_t slots = 0, tiles = 0;
+  bool cut = false;  // the shape is smaller than the memory-free one
   auto shape = [&](size_t cap) {
     tiles = want;
+    cut = false;
     const size_t full1 = (size_t)c->occ_one * cus, full2 = (size_t)c->occ_two * cus;
     const bool pool_ok = (c->dies == 1 || c->dies == kSweepDies) && cus % (size_t)c->dies == 0;
     auto pooled = [&](size_t full) {  // slots for every resident workgroup, as many tiles as fit
@@ -1210,7 +1212,10 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
       tiles = std::max(slots, std::min(want, (cap - slots * per_slot) / per_tile));
       // Cut short by memory, a whole number of tiles per slot, so that the
       // launch's last round does not leave slots idle.
-      if (tiles < want) tiles -= tiles % slots;
+      if (tiles < want) {
+        tiles -= tiles % slots;
+        cut = true;
+      }
       return true;
     };
     if (forced != BCX_SWEEP_FORM_TWO_PER_CU && want <= cus) {
@@ -1220,8 +1225,10 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
       if (pooled(full1)) return;
     } else {
       if (pooled(full2)) return;
+      cut = true;
       if (forced == BCX_SWEEP_FORM_AUTO && pooled(full1)) return;
     }
+    cut = true;
     slots = std::min(want, cap / (per_slot + per_tile));
     if (slots > cus) slots -= slots % cus;
     tiles = slots;
@@ -1230,7 +1237,7 @@ int sweep_tiles_for(bcx_ctx *c, int32_t count) {
   shape(cap);
   // Cut short by memory with the label-scan workspace allocated: release it
   // (it is re-allocated on its next use) and measure again.
-  if ((tiles < want || slots < std::min(want, (size_t)c->occ_two * cus)) && c->ws_slots > 0) {
+  if (cut && c->ws_slots > 0) {
     HIP_OK(hipStreamSynchronize(c->stream));
     c->free_ws();
     cap = budget_now();
