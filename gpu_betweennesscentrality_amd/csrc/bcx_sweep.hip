@@ -867,13 +867,20 @@ __device__ __forceinline__ void sweep_tile(const DevIndex &ix, const DevTree &tr
 // per die: the dies' L2 caches are not kept coherent with each other inside
 // a kernel, so a slot's records never move between dies within a launch.  A die's share holds as many
 // slots as workgroups can be resident on it, so a free one appears as soon
-// as the workgroup that held it has ended; the wait is still bounded, and a
-// workgroup that finds none returns -1 (the launch then reports an error).
+// as the workgroup that held it has ended (its return may land a moment after
+// the next workgroup starts on the unit, hence the retries).  The wait is
+// bounded, kSlotRounds rounds of at most sw.pool attempts and a short sleep
+// (about a second), and a workgroup that finds none returns -1 (the launch
+// then reports an error).
+// s_getreg operand for the die index: hardware register 20 (XCC_ID on gfx950),
+// bits [3:0] (offset 0 at bit 6, size 4 encoded as size - 1 at bit 11).
+constexpr int kHwRegXccId = 20 | (0 << 6) | ((4 - 1) << 11);
+constexpr int kSlotRounds = 1 << 14;
 __device__ __forceinline__ int take_slot(const SweepWork &sw) {
-  const int die = sw.dies == 1 ? 0 : (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7u);  // XCC_ID
+  const int die = sw.dies == 1 ? 0 : (int)(__builtin_amdgcn_s_getreg(kHwRegXccId) & 7u);
   const int base = die * sw.pool;
   const int first = (int)(blockIdx.x % (unsigned)sw.pool);
-  for (int round = 0; round < (1 << 14); ++round) {
+  for (int round = 0; round < kSlotRounds; ++round) {
     for (int i = 0; i < sw.pool; ++i) {
       const int s = base + (first + i) % sw.pool;
       if (atomicCAS(&sw.slot_busy[s], 0, 1) == 0) return s;
