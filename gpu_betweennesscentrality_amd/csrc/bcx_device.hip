@@ -1715,19 +1715,31 @@ extern "C" int bcx_ctx_autotune(bcx_ctx *c, const int32_t *sources, int32_t coun
       run_sweep(c, job.data(), total, bc_tmp.data(), nullptr);
       return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     };
+    // Settings that run the job as the same launches (launch sizes at or
+    // above the job's tiles, in the same form) are timed once: timing noise
+    // between identical runs must not move the setting.
+    const int job_tiles = (total + kWave - 1) / kWave;
+    std::vector<std::pair<int, int>> timed;  // (tiles per launch on this job, form)
+    auto shape_of = [&](const bcx_tuning &t) {
+      const int eff = std::min(t.sweep_batch / kWave, job_tiles);
+      const int form = t.sweep_form != BCX_SWEEP_FORM_AUTO ? t.sweep_form
+                       : eff > c->cus                      ? BCX_SWEEP_FORM_TWO_PER_CU
+                                                           : BCX_SWEEP_FORM_ONE_PER_CU;
+      return std::make_pair(eff, form);
+    };
     {
       const double sec = time_sweep(best);
       if (sec > 0) best_s = sec;
+      timed.push_back(shape_of(best));
     }
     for (int tpc : {2, 4, 8, 16, 32}) {
       for (int form : {BCX_SWEEP_FORM_ONE_PER_CU, BCX_SWEEP_FORM_TWO_PER_CU}) {
         bcx_tuning t = best;
         t.sweep_batch = tpc * c->cus * kWave;
         t.sweep_form = form;
-        const int start_form = start.sweep_form != BCX_SWEEP_FORM_AUTO ? start.sweep_form
-                               : start.sweep_batch / kWave > c->cus  ? BCX_SWEEP_FORM_TWO_PER_CU
-                                                                      : BCX_SWEEP_FORM_ONE_PER_CU;
-        if (t.sweep_batch == start.sweep_batch && form == start_form) continue;  // already timed
+        const auto sh = shape_of(t);
+        if (std::find(timed.begin(), timed.end(), sh) != timed.end()) continue;  // same launches
+        timed.push_back(sh);
         const double sec = time_sweep(t);
         if (sec > 0 && sec < 0.98 * best_s) {
           best_s = sec;

@@ -236,7 +236,8 @@ int bcx_ctx_get_tuning(const bcx_ctx *ctx, bcx_tuning *tn);
  * those that fit device memory) in both tile-kernel forms, each on the same
  * job of 32 x 64 sources per compute unit taken in tree order (at most n);
  * keeps the fastest of each in
- * the context (the current sweep setting unless a candidate is 2% faster) and
+ * the context (the current sweep setting unless a candidate is 2% faster;
+ * settings that run that job as the same launches are timed once) and
  * copies the tuning to *best (may be NULL). */
 int bcx_ctx_autotune(bcx_ctx *ctx, const int32_t *sources, int32_t count,
                      bcx_tuning *best);
