@@ -266,6 +266,7 @@ struct bcx_ctx {
   // lists); it stays alive until the next job starts, which first waits for
   // the stream, so the uploads from it may run asynchronously.
   std::vector<int32_t> plan_src, plan_cand, plan_cand_off;
+  std::vector<int32_t> plan_pre;  // the sorted sources of the plan in the job arrays
   // The launch sequence of the last sweep job, captured once and replayed
   // while the job's shape and buffers stay the same.
   hipGraphExec_t sweep_exec = nullptr;

@@ -1144,6 +1144,7 @@ void bcx_ctx::free_sweep() {
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   sw = SweepWork{};
+  plan_pre.clear();
   sweep_tiles = 0;
   sweep_alloc = 0;
   sweep_slots = 0;
@@ -1280,9 +1281,15 @@ void sweep_occupancy(bcx_ctx *c) {
 // ancestor first (a source's path stops where it meets the previous
 // source's).  Uploads both to the job arrays, growing them when needed.  The
 // host plan lives in the context until the next plan, which first waits for
-// the stream, so the uploads need no wait of their own.
+// the stream, so the uploads need no wait of their own.  A job over the same
+// sources as the last plan (a repeated full BC, say) keeps the job arrays as
+// they are: only plans write them.
 void sweep_plan(bcx_ctx *c, const int32_t *pre, int64_t cnt) {
+  if (c->sw.tile_src && c->sw.cand && c->sw.cand_off && (size_t)cnt == c->plan_pre.size() &&
+      (cnt == 0 || std::memcmp(pre, c->plan_pre.data(), (size_t)cnt * 4) == 0))
+    return;
   HIP_OK(hipStreamSynchronize(c->stream));
+  c->plan_pre.clear();  // the job arrays change below; recorded again at the end
   const int64_t T = (cnt + kWave - 1) / kWave;
   const auto &par = c->h_parent;
   const auto &sub = c->h_subtree;
@@ -1336,6 +1343,7 @@ void sweep_plan(bcx_ctx *c, const int32_t *pre, int64_t cnt) {
   HIP_OK(hipMemcpyAsync(sw.cand_off, coff.data(), coff.size() * 4, hipMemcpyHostToDevice, c->stream));
   if (!cand.empty())
     HIP_OK(hipMemcpyAsync(sw.cand, cand.data(), cand.size() * 4, hipMemcpyHostToDevice, c->stream));
+  c->plan_pre.assign(pre, pre + cnt);
 }
 
 // One launch over tiles [tile0, tile0 + nt) of the planned job of nsrc

@@ -400,6 +400,23 @@ def test_sweep_pooled_slots_bitwise(road):
     assert close_bc(pooled, ref)
 
 
+def test_sweep_repeated_job_keeps_its_plan(road):
+    """A job over the same sources as the last one keeps the uploaded plan (a
+    repeated full BC); another job in between plans anew.  Every full BC is
+    bitwise the first."""
+    g, csr, ctx = road
+    first, o1 = ctx.full_bc(bcx.MODE_SWEEP)
+    again, o2 = ctx.full_bc(bcx.MODE_SWEEP)
+    src = np.arange(0, g.n, 7, dtype=np.int32)
+    part, o3 = ctx.bc_sources(src, bcx.MODE_SWEEP)
+    last, o4 = ctx.full_bc(bcx.MODE_SWEEP)
+    assert not (o1 or o2 or o3 or o4)
+    assert np.array_equal(again.view(np.uint64), first.view(np.uint64))
+    assert np.array_equal(last.view(np.uint64), first.view(np.uint64))
+    want, _ = oracle.brandes(csr, src, threads=oracle.default_threads())
+    assert close_bc(part, want)
+
+
 def test_sweep_scattered_sources(road):
     g, csr, ctx = road
     src = gg.permutation(6, 2, g.n)[:700].astype(np.int32)  # not tree-consecutive
