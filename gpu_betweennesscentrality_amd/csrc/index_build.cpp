@@ -65,12 +65,6 @@ inline uint64_t add_sat(uint64_t a, uint64_t b, int32_t &ovf) {
   return r;
 }
 
-inline std::vector<Arc>::iterator find_arc(std::vector<Arc> &a, int32_t nb) {
-  auto it = std::lower_bound(a.begin(), a.end(), nb,
-                             [](const Arc &x, int32_t k) { return x.nb < k; });
-  return (it != a.end() && it->nb == nb) ? it : a.end();
-}
-
 // Elimination; returns elimination sequence and bags (sorted by vertex id).
 void eliminate(const Graph &g, const std::vector<int32_t> &order,
                std::vector<int32_t> &elim, std::vector<std::vector<Arc>> &bags,
@@ -88,6 +82,7 @@ void eliminate(const Graph &g, const std::vector<int32_t> &order,
   bags.assign((size_t)n, {});
   width = 0;
   std::vector<char> alive((size_t)n, 1);
+  std::vector<Arc> merged;  // scratch for one neighbour's new adjacency list
 
   const bool min_degree = order.empty();
   std::vector<int32_t> DD, DD2, tDD, tDD2;
@@ -130,54 +125,56 @@ void eliminate(const Graph &g, const std::vector<int32_t> &order,
     alive[(size_t)x] = 0;
     std::vector<Arc> nb;
     nb.swap(adj[(size_t)x]);  // only live neighbours remain in adjacency lists
-    for (const Arc &y : nb) {
-      auto &ay = adj[(size_t)y.nb];
-      auto it = find_arc(ay, x);
-      if (it != ay.end()) ay.erase(it);
-      if (min_degree) {
-        tDD[(size_t)y.nb] = (int32_t)ay.size();
-        changed[(size_t)y.nb] = 1;
-      }
-    }
+    // Each neighbour y loses its arc to x and gains, towards every other
+    // neighbour z, the shortcut through x: length l(y,x) + l(x,z) (kept at
+    // most kInf: a shortcut that long is on no shortest path, every distance
+    // staying below kInf as checked after the labels, and the clamp keeps
+    // every sum of two lengths in int32), count c(y,x) c(x,z), hop sum.  An
+    // existing arc y-z keeps the shorter length, adding counts and keeping
+    // the larger hop on a tie (Reduce, TreeDecomp.cpp:157-193).  Both lists
+    // are sorted by vertex id, so y's new list is one merge of the two: the
+    // same arcs as one pair at a time with searches and inserts, in time
+    // linear in the lists (the wide top of the tree has neighbour lists of
+    // hundreds).  Each unordered pair is updated from both of its ends with
+    // the same values, so the adjacency stays symmetric.
     for (size_t i = 0; i < nb.size(); ++i) {
-      for (size_t j = i + 1; j < nb.size(); ++j) {
-        const Arc &a = nb[i], &b = nb[j];
-        // Lengths are kept at most kInf: a shortcut that long is on no
-        // shortest path (every distance stays below kInf, checked after the
-        // labels), and the clamp keeps every sum of two lengths in int32.
+      const Arc &a = nb[i];
+      auto &ay = adj[(size_t)a.nb];
+      merged.clear();
+      merged.reserve(ay.size() + nb.size());
+      size_t p = 0;
+      int32_t added = 0;
+      for (size_t j = 0; j <= nb.size(); ++j) {
+        const int32_t z = j < nb.size() ? nb[j].nb : INT32_MAX;
+        for (; p < ay.size() && ay[p].nb < z; ++p)
+          if (ay[p].nb != x) merged.push_back(ay[p]);
+        if (j == nb.size()) break;
+        if (j == i) continue;
+        const Arc &b = nb[j];
         const int32_t L = (int32_t)std::min<int64_t>((int64_t)a.len + b.len, kInf);
         const uint64_t c = mul_sat(a.cnt, b.cnt, ovf);
         const uint32_t h = a.hop + b.hop;
-        auto &au = adj[(size_t)a.nb];
-        auto it = find_arc(au, b.nb);
-        if (it != au.end()) {
-          auto &av = adj[(size_t)b.nb];
-          auto jt = find_arc(av, a.nb);
-          if (it->len == L) {
-            it->cnt = jt->cnt = add_sat(it->cnt, c, ovf);
-            if (h > it->hop) it->hop = jt->hop = h;
-          } else if (it->len > L) {
-            it->len = jt->len = L;
-            it->cnt = jt->cnt = c;
-            it->hop = jt->hop = h;
+        if (p < ay.size() && ay[p].nb == z) {
+          Arc e = ay[p++];
+          if (e.len == L) {
+            e.cnt = add_sat(e.cnt, c, ovf);
+            if (h > e.hop) e.hop = h;
+          } else if (e.len > L) {
+            e.len = L;
+            e.cnt = c;
+            e.hop = h;
           }
+          merged.push_back(e);
         } else {
-          Arc ab{b.nb, L, c, h}, ba{a.nb, L, c, h};
-          au.insert(std::lower_bound(au.begin(), au.end(), b.nb,
-                                     [](const Arc &p, int32_t k) { return p.nb < k; }),
-                    ab);
-          auto &av = adj[(size_t)b.nb];
-          av.insert(std::lower_bound(av.begin(), av.end(), a.nb,
-                                     [](const Arc &p, int32_t k) { return p.nb < k; }),
-                    ba);
-          if (min_degree) {
-            tDD[(size_t)a.nb] = (int32_t)au.size();
-            tDD[(size_t)b.nb] = (int32_t)av.size();
-            ++tDD2[(size_t)a.nb];
-            ++tDD2[(size_t)b.nb];
-            changed[(size_t)a.nb] = changed[(size_t)b.nb] = 1;
-          }
+          merged.push_back(Arc{z, L, c, h});
+          ++added;
         }
+      }
+      ay.swap(merged);
+      if (min_degree) {
+        tDD[(size_t)a.nb] = (int32_t)ay.size();
+        tDD2[(size_t)a.nb] += added;
+        changed[(size_t)a.nb] = 1;
       }
     }
     if ((int32_t)nb.size() > width) width = (int32_t)nb.size();
