@@ -316,34 +316,42 @@ void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix) {
       rc[D] = 1;
       rh[D] = 0;
       const auto &b = bags[(size_t)ix.perm[(size_t)p]];
-      for (const Arc &a : b) {  // direct shortcut to each bag ancestor
-        const int32_t kpos = ix.depth[(size_t)ix.iperm[(size_t)a.nb]];
-        rd[kpos] = a.len;
-        rc[kpos] = a.cnt;
-        rh[kpos] = a.hop;
+      // Entry j of the row (the ancestor at depth j) is the minimum over the
+      // direct shortcut, where that ancestor is in the bag, and every bag
+      // entry x's shortcut plus x's own entry j; its count is the saturating
+      // sum, and its hop the maximum, over the candidates attaining the
+      // minimum.  Sums of saturating products and maxima do not depend on the
+      // order of their terms, so two passes over the bag (the minimum first,
+      // in a branch-free loop the compiler vectorizes, then counts and hops
+      // from the attaining candidates) give the entries of one running
+      // min-and-count pass (makeIndexDFS, TreeDecomp.cpp:355-477).  Lengths
+      // and label distances are at most kInf (999999999), so a candidate sum
+      // stays below 2^31; candidates through an entry at kInf (no restricted
+      // path) are left out, as they never attain a distance below kInf.
+      for (const Arc &a : b) rd[(size_t)ix.depth[(size_t)ix.iperm[(size_t)a.nb]]] = a.len;
+      for (const Arc &a : b) {
+        const int32_t x = ix.iperm[(size_t)a.nb];
+        const int32_t kpos = ix.depth[(size_t)x];
+        const int32_t *xd = ix.dis.data() + ix.row_off[(size_t)x];
+        const int32_t len = a.len;
+        for (int32_t j = 0; j < kpos; ++j) rd[j] = std::min(rd[j], len + xd[j]);
       }
       int32_t ovf = 0;
       for (const Arc &a : b) {
         const int32_t x = ix.iperm[(size_t)a.nb];
         const int32_t kpos = ix.depth[(size_t)x];
+        if (rd[kpos] == a.len) {  // the direct shortcut attains it
+          rc[kpos] = add_sat(rc[kpos], a.cnt, ovf);
+          rh[kpos] = std::max(rh[kpos], a.hop);
+        }
         const int32_t *xd = ix.dis.data() + ix.row_off[(size_t)x];
         const uint64_t *xc = ix.cnt.data() + ix.row_off[(size_t)x];
         const uint32_t *xh = ix.hop.data() + ix.row_off[(size_t)x];
+        const int32_t len = a.len;
         for (int32_t j = 0; j < kpos; ++j) {
-          if (xd[j] >= kInf) continue;
-          const int64_t cand64 = (int64_t)a.len + xd[j];
-          if (cand64 > rd[j]) continue;
-          const int32_t cand = (int32_t)cand64;
-          const uint64_t c = mul_sat(a.cnt, xc[j], ovf);
-          const uint32_t h = a.hop + xh[j];
-          if (cand < rd[j]) {
-            rd[j] = cand;
-            rc[j] = c;
-            rh[j] = h;
-          } else {
-            rc[j] = add_sat(rc[j], c, ovf);
-            if (h > rh[j]) rh[j] = h;
-          }
+          if (len + xd[j] != rd[j] || xd[j] >= kInf) continue;
+          rc[j] = add_sat(rc[j], mul_sat(a.cnt, xc[j], ovf), ovf);
+          rh[j] = std::max(rh[j], a.hop + xh[j]);
         }
       }
       ovf_all |= ovf;
