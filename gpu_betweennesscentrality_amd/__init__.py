@@ -29,6 +29,7 @@ MODE_SWEEP = 2
 SWEEP_FORM_AUTO = 0
 SWEEP_FORM_ONE_PER_CU = 1
 SWEEP_FORM_TWO_PER_CU = 2
+INDEX_DEVICE_LABELS = 1
 _PKG = Path(__file__).resolve().parent
 _LIB = None
 
@@ -76,6 +77,7 @@ _SIGS = {
     "bcx_graph_free": (None, [_P]),
     "bcx_index_build_mhl": (C.c_int, [_P, C.POINTER(_P)]),
     "bcx_index_build_order": (C.c_int, [_P, C.c_char_p, C.POINTER(_P)]),
+    "bcx_index_build_flags": (C.c_int, [_P, C.c_char_p, C.c_void_p, C.c_uint32, C.POINTER(_P)]),
     "bcx_index_build_with_order": (C.c_int, [_P, _I32P, C.POINTER(_P)]),
     "bcx_index_stats_get": (C.c_int, [_P, C.POINTER(IndexStats)]),
     "bcx_index_query": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
@@ -95,6 +97,7 @@ _SIGS = {
                                         _U64P, _U32P, _P, C.POINTER(_P)]),
     "bcx_index_export": (C.c_int, [_P, _I32P, _I64P, _I32P, _U64P, _U32P, _I64P, _I32P, _I32P, _U64P, _U32P]),
     "bcx_ctx_create": (C.c_int, [C.c_int, _P, _P, C.POINTER(_P)]),
+    "bcx_ctx_export_labels": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bcx_ctx_free": (None, [_P]),
     "bcx_ctx_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "bcx_ctx_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
@@ -214,16 +217,30 @@ class Index:
         self._h = _P(handle)
 
     @classmethod
-    def build_mhl(cls, g: Graph) -> "Index":
+    def build_mhl(cls, g: Graph, device_labels: bool = False) -> "Index":
+        """device_labels: tree and bags only, the label rows computed by the
+        context on the device (no host label rows)."""
         h = _P()
-        _check(lib().bcx_index_build_mhl(g._h, C.byref(h)))
+        if device_labels:
+            _check(lib().bcx_index_build_flags(g._h, None, None, INDEX_DEVICE_LABELS, C.byref(h)))
+        else:
+            _check(lib().bcx_index_build_mhl(g._h, C.byref(h)))
         return cls(h.value)
 
     @classmethod
-    def build_order(cls, g: Graph, order) -> "Index":
-        """order: path of an order file, or an array order[v] = contraction rank."""
+    def build_order(cls, g: Graph, order, device_labels: bool = False) -> "Index":
+        """order: path of an order file, or an array order[v] = contraction rank.
+        device_labels: as for build_mhl."""
         h = _P()
-        if isinstance(order, (str, os.PathLike)):
+        if device_labels:
+            if isinstance(order, (str, os.PathLike)):
+                _check(lib().bcx_index_build_flags(g._h, str(order).encode(), None, INDEX_DEVICE_LABELS,
+                                                   C.byref(h)))
+            else:
+                o = np.ascontiguousarray(order, dtype=np.int32)
+                _check(lib().bcx_index_build_flags(g._h, None, o.ctypes.data_as(C.c_void_p),
+                                                   INDEX_DEVICE_LABELS, C.byref(h)))
+        elif isinstance(order, (str, os.PathLike)):
             _check(lib().bcx_index_build_order(g._h, str(order).encode(), C.byref(h)))
         else:
             o = np.ascontiguousarray(order, dtype=np.int32)
@@ -377,6 +394,19 @@ class Context:
         k = C.c_int32()
         _check(lib().bcx_ctx_sweep_capacity(self._h, C.byref(k)))
         return k.value
+
+    def export_labels(self) -> dict:
+        """The context's label rows (as Index.export lays them out), from the device."""
+        n = self.n
+        off = np.zeros(n + 1, np.int64)
+        ptr = lambda x: x.ctypes.data_as(C.c_void_p)
+        _check(lib().bcx_ctx_export_labels(self._h, ptr(off), None, None, None))
+        E = int(off[-1])
+        a = {"label_off": off, "dis": np.zeros(E, np.int32), "cnt": np.zeros(E, np.uint64),
+             "hop": np.zeros(E, np.uint32)}
+        _check(lib().bcx_ctx_export_labels(self._h, ptr(a["label_off"]), ptr(a["dis"]), ptr(a["cnt"]),
+                                           ptr(a["hop"])))
+        return a
 
     def sweep_slots(self) -> tuple[int, int]:
         """(bag-sweep state slots allocated, compute units of the device)."""

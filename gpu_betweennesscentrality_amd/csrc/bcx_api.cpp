@@ -74,12 +74,13 @@ int bcx_graph_connected(const bcx_graph *g) {
 
 void bcx_graph_free(bcx_graph *g) { delete g; }
 
-static int build_common(const bcx_graph *g, const std::vector<int32_t> &order, bcx_index **out) {
+static int build_common(const bcx_graph *g, const std::vector<int32_t> &order, bcx_index **out,
+                        uint32_t flags = 0) {
   if (!bcx::graph_connected(g->g))
     return bcx::fail(BCX_EGRAPH, "graph is disconnected; the index needs a connected graph");
   auto *h = new bcx_index();
   try {
-    bcx::build_index(g->g, order, h->ix);
+    bcx::build_index(g->g, order, h->ix, (flags & BCX_INDEX_DEVICE_LABELS) == 0);
   } catch (...) {
     delete h;
     throw;
@@ -104,17 +105,37 @@ int bcx_index_build_order(const bcx_graph *g, const char *order_path, bcx_index 
   BCX_GUARD_END
 }
 
-int bcx_index_build_with_order(const bcx_graph *g, const int32_t *order, bcx_index **out) {
-  BCX_GUARD_BEGIN
-  if (!g || !order || !out) return bcx::fail(BCX_EINVAL, "null argument");
+// order[v] = contraction rank of vertex v, checked to be a permutation.
+static int copy_order(const bcx_graph *g, const int32_t *order, std::vector<int32_t> &ord) {
   const int32_t n = g->g.n;
-  std::vector<int32_t> ord(order, order + n);
+  ord.assign(order, order + n);
   std::vector<char> used((size_t)n, 0);
   for (int32_t v = 0; v < n; ++v) {
     if (ord[(size_t)v] < 0 || ord[(size_t)v] >= n || used[(size_t)ord[(size_t)v]])
       return bcx::fail(BCX_EINVAL, "order is not a permutation of 0..n-1");
     used[(size_t)ord[(size_t)v]] = 1;
   }
+  return BCX_OK;
+}
+
+int bcx_index_build_flags(const bcx_graph *g, const char *order_path, const int32_t *order, uint32_t flags,
+                          bcx_index **out) {
+  BCX_GUARD_BEGIN
+  if (!g || !out) return bcx::fail(BCX_EINVAL, "null argument");
+  if (order_path && order) return bcx::fail(BCX_EINVAL, "give an order file or an order array, not both");
+  if (flags & ~(uint32_t)BCX_INDEX_DEVICE_LABELS) return bcx::fail(BCX_EINVAL, "unknown index flags");
+  std::vector<int32_t> ord;
+  if (order_path) bcx::read_order_file(order_path, g->g.n, ord);
+  if (order && copy_order(g, order, ord) != BCX_OK) return BCX_EINVAL;
+  return build_common(g, ord, out, flags);
+  BCX_GUARD_END
+}
+
+int bcx_index_build_with_order(const bcx_graph *g, const int32_t *order, bcx_index **out) {
+  BCX_GUARD_BEGIN
+  if (!g || !order || !out) return bcx::fail(BCX_EINVAL, "null argument");
+  std::vector<int32_t> ord;
+  if (copy_order(g, order, ord) != BCX_OK) return BCX_EINVAL;
   return build_common(g, ord, out);
   BCX_GUARD_END
 }
@@ -180,6 +201,8 @@ int bcx_index_query(const bcx_index *ix, int32_t s, int32_t t, int32_t *d, uint6
                     uint32_t *hop) {
   if (!ix) return bcx::fail(BCX_EINVAL, "null index");
   const auto &x = ix->ix;
+  if (!x.host_labels)
+    return bcx::fail(BCX_EINVAL, "index query needs the label rows on the host; this index leaves them to the device");
   if (s < 0 || s >= x.n || t < 0 || t >= x.n) return bcx::fail(BCX_EINVAL, "vertex out of range");
   int32_t dd;
   uint64_t ss;

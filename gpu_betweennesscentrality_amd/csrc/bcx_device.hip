@@ -558,6 +558,131 @@ void run_sources(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, do
 
 }  // namespace
 
+// Label rows of the nodes at one depth (tree ids lvl[0 .. cnt)), one
+// wavefront per node, lanes over the row's entries.  Entry j of node p's row
+// (the ancestor at depth j) is the minimum over the direct shortcut, where
+// that ancestor is in p's bag, and every bag entry x's shortcut plus x's
+// entry j; its count is the saturating sum, and its hop the maximum, over the
+// candidates attaining the minimum; the last entry is p itself (0, 1, 0).
+// This is the host build's rule (build_index), so the rows are the same bit
+// for bit.  Bag entries are ancestors at distinct depths, and rows of
+// shallower nodes come from earlier levels.  Lengths and distances are at
+// most kInf, so candidate sums stay in int32; candidates through an entry at
+// kInf are left out.  A saturated count sets *sat.
+__global__ __launch_bounds__(256) void label_level_kernel(const int *__restrict__ lvl, int cnt,
+                                                          const int *__restrict__ depth,
+                                                          const int64_t *__restrict__ row_off,
+                                                          const int64_t *__restrict__ bag_off,
+                                                          const int *__restrict__ bag,
+                                                          const int *__restrict__ bag_len,
+                                                          const uint64_t *__restrict__ bag_cnt,
+                                                          const uint32_t *__restrict__ bag_hop, int *dis,
+                                                          uint64_t *lcnt, uint32_t *lhop, int *sat) {
+  const int w = blockIdx.x * (blockDim.x / kWave) + (int)(threadIdx.x / kWave);
+  const int lane = threadIdx.x & (kWave - 1);
+  if (w >= cnt) return;  // uniform over the wavefront
+  const int p = lvl[w];
+  const int D = depth[p];
+  const int64_t rp = row_off[p], e0 = bag_off[p], e1 = bag_off[p + 1];
+  int any_sat = 0;
+  for (int j = lane; j <= D; j += kWave) {
+    if (j == D) {
+      dis[rp + D] = 0;
+      lcnt[rp + D] = 1;
+      lhop[rp + D] = 0;
+      continue;
+    }
+    int m = kInfDev;
+    for (int64_t e = e0; e < e1; ++e) {
+      const int x = bag[e], kx = depth[x], len = bag_len[e];
+      if (kx == j) m = min(m, len);
+      else if (j < kx) m = min(m, len + dis[row_off[x] + j]);
+    }
+    uint64_t c = 0;
+    uint32_t h = 0;
+    for (int64_t e = e0; e < e1; ++e) {
+      const int x = bag[e], kx = depth[x], len = bag_len[e];
+      if (kx == j) {
+        if (len == m) {
+          c = add_sat(c, bag_cnt[e]);
+          h = max(h, bag_hop[e]);
+        }
+      } else if (j < kx) {
+        const int64_t o = row_off[x] + j;
+        const int v = dis[o];
+        if (v < kInfDev && len + v == m) {
+          c = add_sat(c, mul_sat(bag_cnt[e], lcnt[o]));
+          h = max(h, bag_hop[e] + lhop[o]);
+        }
+      }
+    }
+    dis[rp + j] = m;
+    lcnt[rp + j] = c;
+    lhop[rp + j] = h;
+    any_sat |= c == ~0ull;
+  }
+  if (any_sat) atomicOr(sat, 1);
+}
+
+// The largest label entry 0 over nodes 1 .. n-1 (each node's distance to the
+// tree root) into *far.
+__global__ __launch_bounds__(256) void root_far_kernel(int n, const int64_t *__restrict__ row_off,
+                                                       const int *__restrict__ dis, int *far) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= 1 && p < n) atomicMax(far, dis[row_off[p]]);
+}
+
+// Computes the index's label rows into d (dis/cnt/hop of ix.entries()
+// entries each) level by level on the context stream; returns the largest
+// root distance and whether a count saturated.
+static void device_label_rows(bcx_ctx *c, const bcx::Index &ix, int *dis, uint64_t *lcnt, uint32_t *lhop,
+                              int64_t &far, int &sat) {
+  const int32_t n = ix.n, H = ix.tree_height;
+  std::vector<int32_t> lvl_off((size_t)H + 1, 0), lvl((size_t)n);
+  for (int32_t p = 0; p < n; ++p) lvl_off[(size_t)ix.depth[(size_t)p] + 1]++;
+  for (int32_t k = 0; k < H; ++k) lvl_off[(size_t)k + 1] += lvl_off[(size_t)k];
+  {
+    std::vector<int32_t> f(lvl_off.begin(), lvl_off.end() - 1);
+    for (int32_t p = 0; p < n; ++p) lvl[(size_t)f[(size_t)ix.depth[(size_t)p]]++] = p;
+  }
+  std::vector<void *> tmp;
+  auto up = [&](const auto &h) {
+    void *p = dev_upload(h);
+    tmp.push_back(p);
+    return p;
+  };
+  try {
+    const int *d_lvl = (const int *)up(lvl);
+    const int *d_depth = (const int *)up(ix.depth);
+    const int64_t *d_row = (const int64_t *)up(ix.row_off);
+    const int64_t *d_boff = (const int64_t *)up(ix.bag_off);
+    const int *d_bag = (const int *)up(ix.bag);
+    const int *d_blen = (const int *)up(ix.bag_len);
+    const uint64_t *d_bcnt = (const uint64_t *)up(ix.bag_cnt);
+    const uint32_t *d_bhop = (const uint32_t *)up(ix.bag_hop);
+    int *d_flags = (int *)up(std::vector<int32_t>{0, 0});  // saturation, farthest root distance
+    for (int32_t k = 0; k < H; ++k) {
+      const int cnt = lvl_off[(size_t)k + 1] - lvl_off[(size_t)k];
+      if (cnt == 0) continue;
+      label_level_kernel<<<(unsigned)((cnt + 3) / 4), 256, 0, c->stream>>>(
+          d_lvl + lvl_off[(size_t)k], cnt, d_depth, d_row, d_boff, d_bag, d_blen, d_bcnt, d_bhop, dis, lcnt,
+          lhop, d_flags);
+      HIP_OK(hipGetLastError());
+    }
+    root_far_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(n, d_row, dis, d_flags + 1);
+    HIP_OK(hipGetLastError());
+    int32_t flags[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(flags, d_flags, sizeof flags, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    sat = flags[0];
+    far = flags[1];
+  } catch (...) {
+    for (void *p : tmp) (void)hipFree(p);
+    throw;
+  }
+  for (void *p : tmp) HIP_OK(hipFree(p));
+}
+
 extern "C" {
 
 int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx **out) {
@@ -638,10 +763,47 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     HIP_OK(hipMemcpy(p + h.size(), &tail, sizeof(T), hipMemcpyHostToDevice));
     return (const T *)p;
   };
-  d.dis = upload_tail(ix.dis, (int)0x7fffffff);
-  d.cnt = upload_tail(ix.cnt, (uint64_t)0);
-  d.hop = upload_tail(ix.hop, (uint32_t)0);
-  d.none = (int64_t)ix.dis.size();
+  // The farthest vertex's distance to the tree root (label entry 0 of its
+  // row): every shortest distance is at most twice it.
+  int64_t far = 0;
+  if (ix.host_labels) {
+    d.dis = upload_tail(ix.dis, (int)0x7fffffff);
+    d.cnt = upload_tail(ix.cnt, (uint64_t)0);
+    d.hop = upload_tail(ix.hop, (uint32_t)0);
+    for (int32_t p = 1; p < n; ++p) far = std::max<int64_t>(far, ix.dis[(size_t)ix.row_off[(size_t)p]]);
+  } else {
+    // Rows computed here, in arrays with the same extra entry.
+    const size_t E = (size_t)ix.entries();
+    int *dd = nullptr;
+    uint64_t *dc = nullptr;
+    uint32_t *dh = nullptr;
+    HIP_OK(hipMalloc(&dd, (E + 1) * sizeof(int)));
+    keep(dd);
+    HIP_OK(hipMalloc(&dc, (E + 1) * sizeof(uint64_t)));
+    keep(dc);
+    HIP_OK(hipMalloc(&dh, (E + 1) * sizeof(uint32_t)));
+    keep(dh);
+    const int tail_d = 0x7fffffff;
+    const uint64_t tail_c = 0;
+    const uint32_t tail_h = 0;
+    HIP_OK(hipMemcpy(dd + E, &tail_d, sizeof tail_d, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dc + E, &tail_c, sizeof tail_c, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dh + E, &tail_h, sizeof tail_h, hipMemcpyHostToDevice));
+    int sat = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    device_label_rows(c.get(), ix, dd, dc, dh, far, sat);
+    c->label_build_ms = 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    c->label_sat = sat;
+    d.dis = dd;
+    d.cnt = dc;
+    d.hop = dh;
+  }
+  // The host build's distance bound (shortest distances below the label INF
+  // and within the sweep's 30-bit distance field).
+  if (2 * far >= bcx::kInf)
+    return bcx::fail(BCX_EGRAPH, "shortest distances may reach the label INF 999999999 (farthest vertex from "
+                                 "the tree root at " + std::to_string(far) + ")");
+  d.none = ix.entries();
   d.rmq = (const int *)keep(dev_upload(rmq));
   d.grow = (const int *)keep(dev_upload(grow));
   d.gcol = (const int *)keep(dev_upload(gcol));
@@ -702,9 +864,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     {
       // Every shortest distance is at most the path through the tree root,
       // twice the farthest vertex's root distance (label entry 0 of its row),
-      // which the index's distance bound keeps below 2^30.
-      int64_t far = 0;
-      for (int32_t p = 1; p < n; ++p) far = std::max<int64_t>(far, ix.dis[(size_t)ix.row_off[(size_t)p]]);
+      // which the distance bound keeps below 2^30.
       int b = 1;
       while (b < 30 && (int64_t{1} << b) <= 2 * far) ++b;
       t.dbits = b;
@@ -1004,6 +1164,42 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
 }
 
 void bcx_ctx_free(bcx_ctx *ctx) { delete ctx; }
+
+int bcx_ctx_export_labels(const bcx_ctx *c, int64_t *label_off, int32_t *dis, uint64_t *cnt, uint32_t *hop) {
+  BCX_GUARD_BEGIN
+  if (!c) return bcx::fail(BCX_EINVAL, "null context");
+  HIP_OK(hipSetDevice(c->device));
+  const int32_t n = c->ix.n;
+  std::vector<int32_t> depth((size_t)n);
+  std::vector<int64_t> row((size_t)n + 1);
+  HIP_OK(hipMemcpy(depth.data(), c->ix.depth, (size_t)n * 4, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(row.data(), c->ix.row_off, ((size_t)n + 1) * 8, hipMemcpyDeviceToHost));
+  const size_t E = (size_t)row[(size_t)n];
+  std::vector<int32_t> hd(dis ? E : 0);
+  std::vector<uint64_t> hc(cnt ? E : 0);
+  std::vector<uint32_t> hh(hop ? E : 0);
+  if (dis) HIP_OK(hipMemcpy(hd.data(), c->ix.dis, E * 4, hipMemcpyDeviceToHost));
+  if (cnt) HIP_OK(hipMemcpy(hc.data(), c->ix.cnt, E * 8, hipMemcpyDeviceToHost));
+  if (hop) HIP_OK(hipMemcpy(hh.data(), c->ix.hop, E * 4, hipMemcpyDeviceToHost));
+  // Rows by vertex id, the ancestors' entries only (root first), as
+  // bcx_index_export lays them out.
+  int64_t lo = 0;
+  for (int32_t v = 0; v < n; ++v) {
+    const int32_t p = c->iperm[(size_t)v];
+    const int32_t D = depth[(size_t)p];
+    if (label_off) label_off[v] = lo;
+    const size_t r = (size_t)row[(size_t)p];
+    for (int32_t j = 0; j < D; ++j) {
+      if (dis) dis[lo + j] = hd[r + (size_t)j];
+      if (cnt) cnt[lo + j] = hc[r + (size_t)j];
+      if (hop) hop[lo + j] = hh[r + (size_t)j];
+    }
+    lo += D;
+  }
+  if (label_off) label_off[n] = lo;
+  return BCX_OK;
+  BCX_GUARD_END
+}
 
 int bcx_ctx_set_tuning(bcx_ctx *c, const bcx_tuning *tn) {
   if (!c || !tn) return bcx::fail(BCX_EINVAL, "null argument");

@@ -70,7 +70,11 @@ struct Index {
   int32_t tree_width = 0;   // max bag size
   int32_t count_overflow = 0;
   double build_seconds = 0.0;
-  int64_t entries() const { return (int64_t)dis.size(); }
+  // Label rows are computed on the host (the default), or left to the device
+  // (bcx_ctx_create computes them there; dis/cnt/hop then stay empty and the
+  // host functions that read them refuse the index).
+  bool host_labels = true;
+  int64_t entries() const { return row_off.empty() ? 0 : row_off.back(); }
 };
 
 // Error carrier for the host side; the C ABI maps it to codes + message.
@@ -91,7 +95,10 @@ void read_order_file(const char *path, int32_t n, std::vector<int32_t> &order);
 // index_build.cpp
 // Elimination order: empty -> min-degree (reference Reduce policy); else
 // order[v] = contraction rank of v (0 contracts first).
-void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix);
+void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix,
+                 bool host_labels = true);
+// Throws BCX_EINVAL when the index's label rows are not on the host.
+void need_host_labels(const Index &ix, const char *what);
 void index_query(const Index &ix, int32_t s, int32_t t, int32_t &d,
                  uint64_t &sigma, uint32_t &hop, bool &ovf);
 int32_t lca_depth(const Index &ix, int32_t p, int32_t q);  // preorder ids

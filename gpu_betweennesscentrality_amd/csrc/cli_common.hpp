@@ -53,6 +53,13 @@ inline int apply_env_tuning(bcx_ctx *ctx) {
   return BCX_MODE_SWEEP;
 }
 
+// Index build flags: the label rows are left to the device (computed by the
+// context) unless an image is to be saved, which needs them on the host.
+inline uint32_t index_flags() {
+  const char *save = std::getenv("BCX_INDEX_SAVE");
+  return (save && *save) ? 0u : (uint32_t)BCX_INDEX_DEVICE_LABELS;
+}
+
 // The MHL index: loaded from an image when BCX_INDEX_LOAD names one (a
 // restart without rebuilding; the context refuses an image of another
 // graph), else built; saved when BCX_INDEX_SAVE names a path.
@@ -62,7 +69,7 @@ inline bcx_index *mhl_index(const bcx_graph *g) {
   if (load && *load)
     check(bcx_index_load(load, &ix), "loading index");
   else
-    check(bcx_index_build_mhl(g, &ix), "building index");
+    check(bcx_index_build_flags(g, nullptr, nullptr, index_flags(), &ix), "building index");
   const char *save = std::getenv("BCX_INDEX_SAVE");
   if (save && *save) check(bcx_index_save(ix, save), "saving index");
   return ix;

@@ -26,7 +26,8 @@ int main(int argc, char *argv[]) {
 
   auto t0 = std::chrono::high_resolution_clock::now();
   bcx_index *ix = nullptr;
-  cli::check(bcx_index_build_order(g, order.c_str(), &ix), "building index from order file");
+  cli::check(bcx_index_build_flags(g, order.c_str(), nullptr, BCX_INDEX_DEVICE_LABELS, &ix),
+             "building index from order file");
   bcx_ctx *ctx = nullptr;
   cli::check(bcx_ctx_create(device, g, ix, &ctx), "creating device context");
   const int mode = cli::apply_env_tuning(ctx);

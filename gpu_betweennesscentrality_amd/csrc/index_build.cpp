@@ -198,7 +198,13 @@ void check_distance_bound(const Index &ix) {
                             "tree root at " + std::to_string(far) + ")"};
 }
 
-void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix) {
+void need_host_labels(const Index &ix, const char *what) {
+  if (!ix.host_labels)
+    throw Error{BCX_EINVAL, std::string(what) +
+                                " needs the label rows on the host; this index leaves them to the device"};
+}
+
+void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix, bool host_labels) {
   auto t0 = std::chrono::steady_clock::now();
   const int32_t n = g.n;
   if (!order.empty() && (int32_t)order.size() != n)
@@ -287,6 +293,13 @@ void build_index(const Graph &g, const std::vector<int32_t> &order, Index &ix) {
   for (int32_t p = 0; p < n; ++p)
     ix.row_off[(size_t)p + 1] = ix.row_off[(size_t)p] + ix.depth[(size_t)p] + 1;
   const int64_t E = ix.row_off[(size_t)n];
+  if (!host_labels) {  // the context computes the rows (and the distance bound) on the device
+    ix.host_labels = false;
+    ix.fp_arcs = g.arcs();
+    ix.fp_hash = graph_fingerprint(g);
+    ix.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return;
+  }
   ix.dis.resize((size_t)E);
   ix.cnt.resize((size_t)E);
   ix.hop.resize((size_t)E);
@@ -646,7 +659,7 @@ void build_p2h(Index &ix, int k, int d, int64_t R, uint64_t seed) {
 
 void build_h2h(Index &ix) {
   const int32_t n = ix.n, H = ix.tree_height;
-  ix.gdis.assign(ix.dis.size(), kInf);
+  ix.gdis.assign((size_t)ix.entries(), kInf);
   std::vector<int32_t> lvl_off((size_t)H + 1, 0), lvl((size_t)n);
   for (int32_t p = 0; p < n; ++p) lvl_off[(size_t)ix.depth[(size_t)p] + 1]++;
   for (int32_t k = 0; k < H; ++k) lvl_off[(size_t)k + 1] += lvl_off[(size_t)k];
@@ -997,6 +1010,7 @@ void validate_index(Index &ix, const std::string &what) {
 }
 
 void save_index(const Index &ix, const char *path) {
+  need_host_labels(ix, "saving an index image");
   FILE *f = std::fopen(path, "wb");
   if (!f) throw Error{BCX_EIO, std::string("cannot write ") + path};
   try {
@@ -1200,6 +1214,7 @@ void index_from_arrays(int32_t n, const int32_t *parent, const int64_t *label_of
 void index_export(const Index &ix, int32_t *parent, int64_t *label_off, int32_t *dis, uint64_t *cnt,
                   uint32_t *hop, int64_t *bag_off, int32_t *bag_v, int32_t *bag_len, uint64_t *bag_cnt,
                   uint32_t *bag_hop) {
+  if (dis || cnt || hop) need_host_labels(ix, "exporting label rows");
   const int32_t n = ix.n;
   int64_t lo = 0, bo = 0;
   for (int32_t v = 0; v < n; ++v) {

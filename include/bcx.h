@@ -142,6 +142,24 @@ int bcx_index_build_order(const bcx_graph *g, const char *order_path,
 /* Same, from an in-memory order: order[v] = contraction rank of vertex v. */
 int bcx_index_build_with_order(const bcx_graph *g, const int32_t *order,
                                bcx_index **out);
+/* Index build flags. */
+enum {
+  /* Build the tree and bags on the host and leave the label rows to the
+   * device: bcx_ctx_create computes them there, level by level (the label
+   * pass of TreeDecomp::MakeIndex, TreeDecomp.cpp:355-477, is the larger
+   * part of a host build at a million vertices, and its rows are the largest
+   * host array).  Such an index has no host label rows: bcx_index_query,
+   * bcx_index_save and the label rows of bcx_index_export return
+   * BCX_EINVAL, and its count_overflow covers the bags only (the contexts'
+   * results flag their own saturated counts as ever). */
+  BCX_INDEX_DEVICE_LABELS = 1,
+};
+/* bcx_index_build_mhl (order_path and order NULL), bcx_index_build_order
+ * (order_path) or bcx_index_build_with_order (order), with flags
+ * (BCX_INDEX_DEVICE_LABELS or 0). */
+int bcx_index_build_flags(const bcx_graph *g, const char *order_path,
+                          const int32_t *order, uint32_t flags,
+                          bcx_index **out);
 int bcx_index_stats_get(const bcx_index *ix, bcx_index_stats *st);
 /* Host-side (d, sigma, hop) query on the labels; used by tools and tests. */
 int bcx_index_query(const bcx_index *ix, int32_t s, int32_t t, int32_t *d,
@@ -224,8 +242,17 @@ int bcx_index_export(const bcx_index *ix, int32_t *parent, int64_t *label_off,
 void bcx_index_free(bcx_index *ix);
 
 /* ---- device context --------------------------------------------------- */
+/* Uploads the index (moveCountIndexToDevice, MHLQBC.cu:6-128), computing the
+ * label rows on the device when the index leaves them there
+ * (BCX_INDEX_DEVICE_LABELS); refuses an index of another graph. */
 int bcx_ctx_create(int device, const bcx_graph *g, const bcx_index *ix,
                    bcx_ctx **out);
+/* The context's label rows in bcx_index_export's layout (label_off[n + 1],
+ * rows of label_entries - n entries; each pointer may be NULL), copied from
+ * the device; for checking rows computed there against a host build.  No
+ * reference counterpart. */
+int bcx_ctx_export_labels(const bcx_ctx *ctx, int64_t *label_off, int32_t *dis,
+                          uint64_t *cnt, uint32_t *hop);
 void bcx_ctx_free(bcx_ctx *ctx);
 int bcx_ctx_set_tuning(bcx_ctx *ctx, const bcx_tuning *tn);
 int bcx_ctx_get_tuning(const bcx_ctx *ctx, bcx_tuning *tn);

@@ -804,3 +804,63 @@ def test_autotune_picks_a_sweep_launch_and_stays_exact(road):
     want, _ = oracle.brandes(csr, src, threads=oracle.default_threads())
     assert not ovf and close_bc(got, want)
     ctx.close()
+
+
+# ---- label rows computed on the device -----------------------------------------
+
+def _la_graph():
+    la = np.load(ROOT / "tests" / "golden" / "la_graph.npz")
+    return la, bcx.Graph.from_arcs(int(la["n"]), la["u"], la["v"], la["w"])
+
+
+@pytest.mark.parametrize("case", ["grid1k", "ties", "unit35", "road3k", "la", "la_order"])
+def test_device_label_rows_equal_host_rows(case):
+    """An index that leaves its label rows to the device (BCX_INDEX_DEVICE_LABELS):
+    the rows the context computes equal the host build's bit for bit
+    (distance, saturating uint64 count, max hop), on a tie-heavy grid, a
+    unit grid whose counts saturate, a road graph and LosAngeles with the
+    min-degree and the shipped order; BC is bitwise the same in the bag-sweep
+    and label-scan forms."""
+    order = None
+    if case in ("la", "la_order"):
+        la, G = _la_graph()
+        order = la["order"] if case == "la_order" else None
+    else:
+        g = {"grid1k": lambda: gg.make("grid1k"), "ties": lambda: gg.grid_graph(k=20, seed=3, wmax=2),
+             "unit35": lambda: gg.unit_grid(35), "road3k": lambda: gg.road_graph(3000, 3750, seed=21)}[case]()
+        G = bcx.Graph.from_arcs(g.n, g.u, g.v, g.w)
+    if order is None:
+        host, dev = bcx.Index.build_mhl(G), bcx.Index.build_mhl(G, device_labels=True)
+    else:
+        host, dev = bcx.Index.build_order(G, order), bcx.Index.build_order(G, order, device_labels=True)
+    want = host.export()
+    ch, cd = bcx.Context(G, host, 0), bcx.Context(G, dev, 0)
+    got = cd.export_labels()
+    for k in ("label_off", "dis", "cnt", "hop"):
+        assert np.array_equal(got[k], want[k]), f"{case}: device rows differ in {k}"
+    assert np.array_equal(ch.export_labels()["cnt"], want["cnt"])  # the export path itself
+    for mode in (bcx.MODE_SWEEP, bcx.MODE_BATCHED):
+        a, oa = ch.full_bc(mode)
+        b, ob = cd.full_bc(mode)
+        assert oa == ob and np.array_equal(a.view(np.uint64), b.view(np.uint64)), f"{case}, mode {mode}"
+    ch.close()
+    cd.close()
+
+
+def test_device_label_rows_distance_bound():
+    """The device build refuses graphs whose distances may reach the label INF,
+    as the host build does, and accepts those the host build accepts."""
+    n = 3001
+    e = np.array([(0, i) for i in range(1, n)], np.int32)
+    star = gg._both_ways(n, e[:, 0], e[:, 1], np.full(n - 1, 400_000, np.int64))
+    G = bcx.Graph.from_arcs(star.n, star.u, star.v, star.w)
+    ctx = bcx.Context(G, bcx.Index.build_mhl(G, device_labels=True), 0)
+    bc, ovf = ctx.full_bc(bcx.MODE_SWEEP)
+    assert not ovf and bc[0] == (n - 1) * (n - 2) / 2
+    ctx.close()
+    far = gg._both_ways(3, np.array([0, 1]), np.array([1, 2]), np.array([600_000_000, 600_000_000]))
+    G = bcx.Graph.from_arcs(far.n, far.u, far.v, far.w)
+    with pytest.raises(bcx.BcxError):
+        bcx.Index.build_mhl(G)
+    with pytest.raises(bcx.BcxError):
+        bcx.Context(G, bcx.Index.build_mhl(G, device_labels=True), 0)

@@ -273,3 +273,33 @@ def test_p2h_tables_on_the_shipped_order_file():
         assert ix.h2h_query(s - 1, t - 1) == d, (s, t)
     sep, p2h = ix.h2h_scan_stats(200_000, 7)
     assert 55 <= sep <= 67 and 22 <= p2h <= 30, (sep, p2h)
+
+
+# ---- index whose label rows are left to the device ---------------------------
+
+def test_device_label_index_same_tree_and_bags_no_host_rows(la):
+    """BCX_INDEX_DEVICE_LABELS: the same tree, bags and statistics as a host
+    build (min-degree and the shipped order), no host label rows (queries,
+    images and row exports refused), and H2H labels still build on it."""
+    G = bcx.Graph.from_arcs(int(la["n"]), la["u"], la["v"], la["w"])
+    for build in (lambda d: bcx.Index.build_mhl(G, device_labels=d),
+                  lambda d: bcx.Index.build_order(G, la["order"], device_labels=d)):
+        host = build(False)
+        dev = build(True)
+        hs, ds = host.stats(), dev.stats()
+        for k in ("n", "tree_height", "tree_width", "label_entries", "bag_entries", "count_overflow"):
+            assert hs[k] == ds[k], k
+        for a, b in zip(host.tree(), dev.tree()):
+            assert np.array_equal(a, b)
+        assert np.array_equal(host.tree_order(), dev.tree_order())
+        with pytest.raises(bcx.BcxError):
+            dev.query(0, 1)
+        with pytest.raises(bcx.BcxError):
+            dev.export()
+        with pytest.raises(bcx.BcxError):
+            dev.save("/tmp/bcx_device_label_index.img")
+        host.build_h2h()
+        dev.build_h2h()
+        rng = np.random.default_rng(7)
+        for s, t in rng.integers(0, G.n, size=(200, 2)):
+            assert host.h2h_query(int(s), int(t)) == dev.h2h_query(int(s), int(t))
