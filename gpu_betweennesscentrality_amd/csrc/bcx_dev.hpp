@@ -261,8 +261,6 @@ struct bcx_ctx {
   int sweep_pool = 0;   // slots per die when launches may hold more tiles than slots
   int occ_one = 1, occ_two = 2;  // resident workgroups per compute unit of each tile-kernel form
   int dies = 1;                  // accelerator dies (XCCs) of the device
-  double label_build_ms = 0;     // device label rows: time to build them (0: uploaded)
-  int label_sat = 0;             // device label rows: some count saturated
   bcx_dev::SweepWork sw{};
   // A sweep job's plan on the host (tile sources, ascending-pass candidate
   // lists); it stays alive until the next job starts, which first waits for

@@ -568,7 +568,8 @@ void run_sources(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, do
 // for bit.  Bag entries are ancestors at distinct depths, and rows of
 // shallower nodes come from earlier levels.  Lengths and distances are at
 // most kInf, so candidate sums stay in int32; candidates through an entry at
-// kInf are left out.  A saturated count sets *sat.
+// kInf are left out.  Saturated counts stay saturated, and results through
+// them carry their own overflow flag.
 __global__ __launch_bounds__(256) void label_level_kernel(const int *__restrict__ lvl, int cnt,
                                                           const int *__restrict__ depth,
                                                           const int64_t *__restrict__ row_off,
@@ -577,14 +578,13 @@ __global__ __launch_bounds__(256) void label_level_kernel(const int *__restrict_
                                                           const int *__restrict__ bag_len,
                                                           const uint64_t *__restrict__ bag_cnt,
                                                           const uint32_t *__restrict__ bag_hop, int *dis,
-                                                          uint64_t *lcnt, uint32_t *lhop, int *sat) {
+                                                          uint64_t *lcnt, uint32_t *lhop) {
   const int w = blockIdx.x * (blockDim.x / kWave) + (int)(threadIdx.x / kWave);
   const int lane = threadIdx.x & (kWave - 1);
   if (w >= cnt) return;  // uniform over the wavefront
   const int p = lvl[w];
   const int D = depth[p];
   const int64_t rp = row_off[p], e0 = bag_off[p], e1 = bag_off[p + 1];
-  int any_sat = 0;
   for (int j = lane; j <= D; j += kWave) {
     if (j == D) {
       dis[rp + D] = 0;
@@ -619,9 +619,7 @@ __global__ __launch_bounds__(256) void label_level_kernel(const int *__restrict_
     dis[rp + j] = m;
     lcnt[rp + j] = c;
     lhop[rp + j] = h;
-    any_sat |= c == ~0ull;
   }
-  if (any_sat) atomicOr(sat, 1);
 }
 
 // The largest label entry 0 over nodes 1 .. n-1 (each node's distance to the
@@ -634,9 +632,9 @@ __global__ __launch_bounds__(256) void root_far_kernel(int n, const int64_t *__r
 
 // Computes the index's label rows into d (dis/cnt/hop of ix.entries()
 // entries each) level by level on the context stream; returns the largest
-// root distance and whether a count saturated.
+// root distance.
 static void device_label_rows(bcx_ctx *c, const bcx::Index &ix, int *dis, uint64_t *lcnt, uint32_t *lhop,
-                              int64_t &far, int &sat) {
+                              int64_t &far) {
   const int32_t n = ix.n, H = ix.tree_height;
   std::vector<int32_t> lvl_off((size_t)H + 1, 0), lvl((size_t)n);
   for (int32_t p = 0; p < n; ++p) lvl_off[(size_t)ix.depth[(size_t)p] + 1]++;
@@ -660,22 +658,21 @@ static void device_label_rows(bcx_ctx *c, const bcx::Index &ix, int *dis, uint64
     const int *d_blen = (const int *)up(ix.bag_len);
     const uint64_t *d_bcnt = (const uint64_t *)up(ix.bag_cnt);
     const uint32_t *d_bhop = (const uint32_t *)up(ix.bag_hop);
-    int *d_flags = (int *)up(std::vector<int32_t>{0, 0});  // saturation, farthest root distance
+    int *d_far = (int *)up(std::vector<int32_t>{0});
     for (int32_t k = 0; k < H; ++k) {
       const int cnt = lvl_off[(size_t)k + 1] - lvl_off[(size_t)k];
       if (cnt == 0) continue;
       label_level_kernel<<<(unsigned)((cnt + 3) / 4), 256, 0, c->stream>>>(
           d_lvl + lvl_off[(size_t)k], cnt, d_depth, d_row, d_boff, d_bag, d_blen, d_bcnt, d_bhop, dis, lcnt,
-          lhop, d_flags);
+          lhop);
       HIP_OK(hipGetLastError());
     }
-    root_far_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(n, d_row, dis, d_flags + 1);
+    root_far_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(n, d_row, dis, d_far);
     HIP_OK(hipGetLastError());
-    int32_t flags[2] = {0, 0};
-    HIP_OK(hipMemcpyAsync(flags, d_flags, sizeof flags, hipMemcpyDeviceToHost, c->stream));
+    int32_t f = 0;
+    HIP_OK(hipMemcpyAsync(&f, d_far, sizeof f, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
-    sat = flags[0];
-    far = flags[1];
+    far = f;
   } catch (...) {
     for (void *p : tmp) (void)hipFree(p);
     throw;
@@ -789,11 +786,7 @@ int bcx_ctx_create(int device, const bcx_graph *gh, const bcx_index *ih, bcx_ctx
     HIP_OK(hipMemcpy(dd + E, &tail_d, sizeof tail_d, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(dc + E, &tail_c, sizeof tail_c, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(dh + E, &tail_h, sizeof tail_h, hipMemcpyHostToDevice));
-    int sat = 0;
-    const auto t0 = std::chrono::steady_clock::now();
-    device_label_rows(c.get(), ix, dd, dc, dh, far, sat);
-    c->label_build_ms = 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    c->label_sat = sat;
+    device_label_rows(c.get(), ix, dd, dc, dh, far);
     d.dis = dd;
     d.cnt = dc;
     d.hop = dh;
