@@ -210,6 +210,9 @@ WORKLOADS = {
     # vertex (1.6M arc lines at 320k) -- a near-triangulation, far denser
     # than road networks (tree height 2,368 against 526).
     "road320k_m800k": dict(kind="road", n=320_000, m=800_000, seed=3),
+    # The same reading at 100k vertices: 250,000 undirected edges (tree
+    # height 1,235 against 294).
+    "road100k_m250k": dict(kind="road", n=100_000, m=250_000, seed=2),
 }
 
 
