@@ -96,6 +96,11 @@ _SIGS = {
     "bcx_index_from_arrays": (C.c_int, [C.c_int32, _I32P, _I64P, _I32P, _U64P, _U32P, _I64P, _I32P, _I32P,
                                         _U64P, _U32P, _P, C.POINTER(_P)]),
     "bcx_index_export": (C.c_int, [_P, _I32P, _I64P, _I32P, _U64P, _U32P, _I64P, _I32P, _I32P, _U64P, _U32P]),
+    "bcx_index_h2h_from_arrays": (C.c_int, [_P, _I32P, _I64P, _I32P, _I64P, _I32P, C.c_int32, _I32P, _I64P,
+                                            _I32P]),
+    "bcx_index_h2h_sizes": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int64)]),
+    "bcx_index_h2h_export": (C.c_int, [_P, _I64P, _I32P, _I64P, _I32P, _I32P, _I64P, _I32P]),
     "bcx_ctx_create": (C.c_int, [C.c_int, _P, _P, C.POINTER(_P)]),
     "bcx_ctx_export_labels": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bcx_ctx_free": (None, [_P]),
@@ -310,6 +315,32 @@ class Index:
         """QBC's H2H distance labels and separators (buildIndex.cpp:222-278,
         313-386); needed before creating a Context for h2h_distances."""
         _check(lib().bcx_index_build_h2h(self._h))
+
+    def h2h_from_arrays(self, a: dict) -> None:
+        """QBC's H2H index from the reference's host structure (keys as
+        h2h_export(), plus "parent"), installed on this index's tree."""
+        n = self.stats()["n"]
+        br = np.ascontiguousarray(a.get("branch", np.zeros(0, np.int32)), np.int32)
+        args = [np.ascontiguousarray(a[k], dt) for k, dt in (
+            ("parent", np.int32), ("gdis_off", np.int64), ("gdis", np.int32), ("sep_off", np.int64),
+            ("sep", np.int32))]
+        p2h_off = np.ascontiguousarray(a.get("p2h_off", np.zeros(len(br) * n + 1, np.int64)), np.int64)
+        p2h = np.ascontiguousarray(a.get("p2h", np.zeros(0, np.int32)), np.int32)
+        _check(lib().bcx_index_h2h_from_arrays(self._h, *args, len(br), br, p2h_off, p2h))
+
+    def h2h_export(self) -> dict:
+        """The H2H distance rows (no own entry), separators and P2H tables,
+        indexed by vertex id (the layout h2h_from_arrays takes)."""
+        n = self.stats()["n"]
+        g, s, b, p = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int64()
+        _check(lib().bcx_index_h2h_sizes(self._h, C.byref(g), C.byref(s), C.byref(b), C.byref(p)))
+        a = {"gdis_off": np.zeros(n + 1, np.int64), "gdis": np.zeros(g.value, np.int32),
+             "sep_off": np.zeros(n + 1, np.int64), "sep": np.zeros(s.value, np.int32),
+             "branch": np.zeros(b.value, np.int32), "p2h_off": np.zeros(b.value * n + 1, np.int64),
+             "p2h": np.zeros(p.value, np.int32)}
+        _check(lib().bcx_index_h2h_export(self._h, a["gdis_off"], a["gdis"], a["sep_off"], a["sep"], a["branch"],
+                                          a["p2h_off"], a["p2h"]))
+        return a
 
     def h2h_query(self, s: int, t: int) -> int:
         d = C.c_int32()

@@ -303,6 +303,43 @@ int bcx_index_export(const bcx_index *ix, int32_t *parent, int64_t *label_off, i
   BCX_GUARD_END
 }
 
+int bcx_index_h2h_from_arrays(bcx_index *ix, const int32_t *parent, const int64_t *gdis_off,
+                              const int32_t *gdis, const int64_t *sep_off, const int32_t *sep,
+                              int32_t nbranch, const int32_t *branch, const int64_t *p2h_off,
+                              const int32_t *p2h) {
+  BCX_GUARD_BEGIN
+  if (!ix || !parent || !gdis_off || !gdis || !sep_off || !sep ||
+      (nbranch > 0 && (!branch || !p2h_off || !p2h)))
+    return bcx::fail(BCX_EINVAL, "null argument");
+  bcx::index_h2h_from_arrays(ix->ix, parent, gdis_off, gdis, sep_off, sep, nbranch, branch, p2h_off, p2h);
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_index_h2h_sizes(const bcx_index *ix, int64_t *gdis_entries, int64_t *sep_entries, int32_t *nbranch,
+                        int64_t *p2h_entries) {
+  BCX_GUARD_BEGIN
+  if (!ix) return bcx::fail(BCX_EINVAL, "null index");
+  int64_t a = 0, b = 0, d = 0;
+  int32_t c = 0;
+  bcx::index_h2h_sizes(ix->ix, a, b, c, d);
+  if (gdis_entries) *gdis_entries = a;
+  if (sep_entries) *sep_entries = b;
+  if (nbranch) *nbranch = c;
+  if (p2h_entries) *p2h_entries = d;
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
+int bcx_index_h2h_export(const bcx_index *ix, int64_t *gdis_off, int32_t *gdis, int64_t *sep_off, int32_t *sep,
+                         int32_t *branch, int64_t *p2h_off, int32_t *p2h) {
+  BCX_GUARD_BEGIN
+  if (!ix) return bcx::fail(BCX_EINVAL, "null index");
+  bcx::index_h2h_export(ix->ix, gdis_off, gdis, sep_off, sep, branch, p2h_off, p2h);
+  return BCX_OK;
+  BCX_GUARD_END
+}
+
 void bcx_index_free(bcx_index *ix) { delete ix; }
 
 }  // extern "C"

@@ -125,6 +125,8 @@ void load_index(const char *path, Index &ix);
 void check_distance_bound(const Index &ix);
 // Validates an index from outside; derives iperm and subtree.
 void validate_index(Index &ix, const std::string &what);
+// Checks the H2H members of an index with a valid tree.
+void check_h2h(const Index &ix, const std::string &what);
 // The reference's host index structure, indexed by vertex id (see
 // index_build.cpp), in and out.
 void index_from_arrays(int32_t n, const int32_t *parent, const int64_t *label_off, const int32_t *dis,
@@ -133,5 +135,14 @@ void index_from_arrays(int32_t n, const int32_t *parent, const int64_t *label_of
 void index_export(const Index &ix, int32_t *parent, int64_t *label_off, int32_t *dis, uint64_t *cnt,
                   uint32_t *hop, int64_t *bag_off, int32_t *bag_v, int32_t *bag_len, uint64_t *bag_cnt,
                   uint32_t *bag_hop);
+// QBC's H2H index (labels, separators, P2H tables) from the reference's host
+// structure, indexed by vertex id, onto ix's tree; and out.
+void index_h2h_from_arrays(Index &ix, const int32_t *parent, const int64_t *gdis_off, const int32_t *gdis,
+                           const int64_t *sep_off, const int32_t *sep, int32_t nbranch, const int32_t *branch,
+                           const int64_t *p2h_off, const int32_t *p2h);
+void index_h2h_sizes(const Index &ix, int64_t &gdis_entries, int64_t &sep_entries, int32_t &nbranch,
+                     int64_t &p2h_entries);
+void index_h2h_export(const Index &ix, int64_t *gdis_off, int32_t *gdis, int64_t *sep_off, int32_t *sep,
+                      int32_t *branch, int64_t *p2h_off, int32_t *p2h);
 
 }  // namespace bcx

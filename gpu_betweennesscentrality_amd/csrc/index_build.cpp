@@ -903,6 +903,42 @@ void rvec(FILE *f, std::vector<T> &v, int64_t max_len) {
 }
 }  // namespace
 
+// Checks the H2H labels, separators and P2H tables of an index whose tree
+// is valid (iperm and subtree derived); throws BCX_EIO naming `what`.
+void check_h2h(const Index &ix, const std::string &what) {
+  auto bad = [&](const std::string &m) { throw Error{BCX_EIO, what + ": " + m}; };
+  const int32_t n = ix.n;
+  const int64_t E = ix.row_off[(size_t)n];
+  if ((int64_t)ix.gdis.size() != E) bad("H2H labels of the wrong length");
+  if ((int64_t)ix.sep_off.size() != (int64_t)n + 1 || ix.sep_off[0] != 0) bad("separator offsets");
+  for (int32_t p = 0; p < n; ++p) {
+    if (ix.sep_off[(size_t)p + 1] <= ix.sep_off[(size_t)p]) bad("a node without its own separator entry");
+    for (int64_t i = ix.sep_off[(size_t)p]; i < ix.sep_off[(size_t)p + 1]; ++i)
+      if ((int64_t)ix.sep.size() <= i || ix.sep[(size_t)i] < 0 || ix.sep[(size_t)i] > ix.depth[(size_t)p])
+        bad("a separator depth out of range");
+  }
+  if ((int64_t)ix.sep.size() != ix.sep_off[(size_t)n]) bad("separator array of the wrong length");
+  for (int64_t i = 0; i < E; ++i)
+    if (ix.gdis[(size_t)i] < 0 || ix.gdis[(size_t)i] > kInf) bad("an H2H distance outside [0, INF]");
+  if (ix.p2h_base.size() != ix.p2h_node.size()) bad("P2H tables of the wrong length");
+  int64_t next = 0;
+  for (size_t b = 0; b < ix.p2h_node.size(); ++b) {
+    const int32_t v = ix.p2h_node[b];
+    if (v < 0 || v >= n || (b > 0 && v <= ix.p2h_node[b - 1])) bad("P2H branch nodes out of order");
+    if (ix.p2h_base[b] != next) bad("P2H table offsets");
+    const int64_t size = ix.subtree[(size_t)v];
+    next += size + 1;
+    if ((int64_t)ix.p2h_off.size() < next) bad("P2H offsets of the wrong length");
+    for (int64_t r = 0; r < size; ++r) {
+      const int64_t o0 = ix.p2h_off[(size_t)(ix.p2h_base[b] + r)], o1 = ix.p2h_off[(size_t)(ix.p2h_base[b] + r + 1)];
+      if (o0 < 0 || o1 < o0 || o1 > (int64_t)ix.p2h.size()) bad("P2H offsets are not monotone");
+      for (int64_t i = o0; i < o1; ++i)
+        if (ix.p2h[(size_t)i] < 0 || ix.p2h[(size_t)i] > ix.depth[(size_t)v]) bad("a P2H depth out of range");
+    }
+  }
+  if ((int64_t)ix.p2h_off.size() != next) bad("P2H offsets of the wrong length");
+}
+
 // Checks an index assembled from outside (a loaded image or caller arrays)
 // against every property the kernels and the host planning rely on, and
 // derives iperm and subtree.  Throws BCX_EIO naming `what` on the first
@@ -976,36 +1012,7 @@ void validate_index(Index &ix, const std::string &what) {
     if (p > 0 && ix.bag[(size_t)(e1 - 1)] != ix.parent[(size_t)p]) bad("a node's deepest bag entry is not its parent");
   }
   if (ix.tree_width != width) bad("tree width is not the largest bag");
-  if (!ix.sep_off.empty() || !ix.gdis.empty() || !ix.sep.empty()) {
-    if ((int64_t)ix.gdis.size() != E) bad("H2H labels of the wrong length");
-    if ((int64_t)ix.sep_off.size() != (int64_t)n + 1 || ix.sep_off[0] != 0) bad("separator offsets");
-    for (int32_t p = 0; p < n; ++p) {
-      if (ix.sep_off[(size_t)p + 1] <= ix.sep_off[(size_t)p]) bad("a node without its own separator entry");
-      for (int64_t i = ix.sep_off[(size_t)p]; i < ix.sep_off[(size_t)p + 1]; ++i)
-        if ((int64_t)ix.sep.size() <= i || ix.sep[(size_t)i] < 0 || ix.sep[(size_t)i] > ix.depth[(size_t)p])
-          bad("a separator depth out of range");
-    }
-    if ((int64_t)ix.sep.size() != ix.sep_off[(size_t)n]) bad("separator array of the wrong length");
-    for (int64_t i = 0; i < E; ++i)
-      if (ix.gdis[(size_t)i] < 0 || ix.gdis[(size_t)i] > kInf) bad("an H2H distance outside [0, INF]");
-    if (ix.p2h_base.size() != ix.p2h_node.size()) bad("P2H tables of the wrong length");
-    int64_t next = 0;
-    for (size_t b = 0; b < ix.p2h_node.size(); ++b) {
-      const int32_t v = ix.p2h_node[b];
-      if (v < 0 || v >= n || (b > 0 && v <= ix.p2h_node[b - 1])) bad("P2H branch nodes out of order");
-      if (ix.p2h_base[b] != next) bad("P2H table offsets");
-      const int64_t size = ix.subtree[(size_t)v];
-      next += size + 1;
-      if ((int64_t)ix.p2h_off.size() < next) bad("P2H offsets of the wrong length");
-      for (int64_t r = 0; r < size; ++r) {
-        const int64_t o0 = ix.p2h_off[(size_t)(ix.p2h_base[b] + r)], o1 = ix.p2h_off[(size_t)(ix.p2h_base[b] + r + 1)];
-        if (o0 < 0 || o1 < o0 || o1 > (int64_t)ix.p2h.size()) bad("P2H offsets are not monotone");
-        for (int64_t i = o0; i < o1; ++i)
-          if (ix.p2h[(size_t)i] < 0 || ix.p2h[(size_t)i] > ix.depth[(size_t)v]) bad("a P2H depth out of range");
-      }
-    }
-    if ((int64_t)ix.p2h_off.size() != next) bad("P2H offsets of the wrong length");
-  }
+  if (!ix.sep_off.empty() || !ix.gdis.empty() || !ix.sep.empty()) check_h2h(ix, what);
   check_distance_bound(ix);
 }
 
@@ -1239,6 +1246,136 @@ void index_export(const Index &ix, int32_t *parent, int64_t *label_off, int32_t 
   }
   if (label_off) label_off[n] = lo;
   if (bag_off) bag_off[n] = bo;
+}
+
+
+// QBC's H2H index from the reference's host structure (Graph::Tree after
+// makeIndex, optimizeVS and SVSP_P2H, buildIndex.cpp:208-278, 313-386,
+// 565-719; the arrays moveDisIndexToDevice flattens, QBC.cu:11-127), indexed
+// by vertex id, installed on ix's tree:
+//   parent[v]          v's tree parent vertex (-1 at the root); must be ix's
+//   gdis row of v      [gdis_off[v], gdis_off[v + 1]): depth(v) global
+//                      distances to v's ancestors, root first (Node::dis);
+//                      the own entry 0 is appended here
+//   sep row of v       [sep_off[v], sep_off[v + 1]): row positions (= ancestor
+//                      depths, v's own depth included) of the separator
+//                      scanned when v is the LCA (Node::posOptimize), any order
+//   branch[j]          the kept P2H branch vertices (Node::isBranchNode)
+//   list of s below    [p2h_off[j * n + s], p2h_off[j * n + s + 1]) of p2h:
+//   branch[j]          positions of branch[j]'s separator (Node::posP2H_3[s]);
+//                      empty for s outside branch[j]'s subtree
+// Positions must lie in the rows they index.  Every array is checked before
+// anything of ix changes (BCX_EINVAL for shape, BCX_EIO for ranges).
+void index_h2h_from_arrays(Index &ix, const int32_t *parent, const int64_t *gdis_off, const int32_t *gdis,
+                           const int64_t *sep_off, const int32_t *sep, int32_t nbranch, const int32_t *branch,
+                           const int64_t *p2h_off, const int32_t *p2h) {
+  const int32_t n = ix.n;
+  auto inval = [](const std::string &m) { throw Error{BCX_EINVAL, "H2H arrays: " + m}; };
+  for (int32_t v = 0; v < n; ++v) {
+    const int32_t pp = ix.parent[(size_t)ix.iperm[(size_t)v]];
+    if (parent[v] != (pp < 0 ? -1 : ix.perm[(size_t)pp]))
+      inval("the parent of vertex " + std::to_string(v) + " differs from the index's tree");
+  }
+  if (nbranch < 0 || nbranch > n) inval("branch count out of range");
+  if (gdis_off[0] != 0 || sep_off[0] != 0 || (nbranch > 0 && p2h_off[0] != 0)) inval("offsets do not start at 0");
+  Index h;  // the new H2H members, moved into ix once checked
+  h.gdis.resize((size_t)ix.entries());
+  h.sep_off.assign((size_t)n + 1, 0);
+  for (int32_t p = 0; p < n; ++p) {
+    const int32_t v = ix.perm[(size_t)p], D = ix.depth[(size_t)p];
+    if (gdis_off[v + 1] - gdis_off[v] != D)
+      inval("the distance row of vertex " + std::to_string(v) + " does not have depth(v) entries");
+    const int64_t r = ix.row_off[(size_t)p];
+    for (int32_t j = 0; j < D; ++j) h.gdis[(size_t)(r + j)] = gdis[gdis_off[v] + j];
+    h.gdis[(size_t)(r + D)] = 0;
+    if (sep_off[v + 1] < sep_off[v]) inval("separator offsets are not monotone");
+    h.sep.insert(h.sep.end(), sep + sep_off[v], sep + sep_off[v + 1]);
+    h.sep_off[(size_t)p + 1] = (int64_t)h.sep.size();
+  }
+  std::vector<std::pair<int32_t, int32_t>> br;  // (preorder id, j)
+  for (int32_t j = 0; j < nbranch; ++j) {
+    if (branch[j] < 0 || branch[j] >= n) inval("branch vertex out of range");
+    br.emplace_back(ix.iperm[(size_t)branch[j]], j);
+  }
+  std::sort(br.begin(), br.end());
+  for (size_t b = 1; b < br.size(); ++b)
+    if (br[b].first == br[b - 1].first) inval("a branch vertex listed twice");
+  for (auto [v, j] : br) {
+    const int64_t *o = p2h_off + (int64_t)j * n;
+    for (int32_t s = 0; s < n; ++s) {
+      if (o[s + 1] < o[s]) inval("P2H offsets are not monotone");
+      const int32_t q = ix.iperm[(size_t)s];
+      if (o[s + 1] > o[s] && !(v <= q && q < v + ix.subtree[(size_t)v]))
+        inval("a P2H list for a vertex outside its branch vertex's subtree");
+    }
+    h.p2h_node.push_back(v);
+    h.p2h_base.push_back((int64_t)h.p2h_off.size());
+    for (int32_t r = 0; r < ix.subtree[(size_t)v]; ++r) {
+      const int32_t s = ix.perm[(size_t)(v + r)];
+      h.p2h_off.push_back((int64_t)h.p2h.size());
+      h.p2h.insert(h.p2h.end(), p2h + o[s], p2h + o[s + 1]);
+    }
+    h.p2h_off.push_back((int64_t)h.p2h.size());
+  }
+  auto swap_in = [&]() {
+    std::swap(ix.gdis, h.gdis);
+    std::swap(ix.sep_off, h.sep_off);
+    std::swap(ix.sep, h.sep);
+    std::swap(ix.p2h_node, h.p2h_node);
+    std::swap(ix.p2h_base, h.p2h_base);
+    std::swap(ix.p2h_off, h.p2h_off);
+    std::swap(ix.p2h, h.p2h);
+  };
+  swap_in();
+  try {
+    check_h2h(ix, "H2H arrays");
+  } catch (...) {
+    swap_in();  // ix keeps what it had
+    throw;
+  }
+}
+
+void index_h2h_sizes(const Index &ix, int64_t &gdis_entries, int64_t &sep_entries, int32_t &nbranch,
+                     int64_t &p2h_entries) {
+  if (ix.sep_off.empty()) throw Error{BCX_EINVAL, "H2H labels not built"};
+  gdis_entries = ix.entries() - ix.n;
+  sep_entries = (int64_t)ix.sep.size();
+  nbranch = (int32_t)ix.p2h_node.size();
+  p2h_entries = (int64_t)ix.p2h.size();
+}
+
+void index_h2h_export(const Index &ix, int64_t *gdis_off, int32_t *gdis, int64_t *sep_off, int32_t *sep,
+                      int32_t *branch, int64_t *p2h_off, int32_t *p2h) {
+  if (ix.sep_off.empty()) throw Error{BCX_EINVAL, "H2H labels not built"};
+  const int32_t n = ix.n;
+  int64_t go = 0, so = 0;
+  for (int32_t v = 0; v < n; ++v) {
+    const int32_t p = ix.iperm[(size_t)v], D = ix.depth[(size_t)p];
+    if (gdis_off) gdis_off[v] = go;
+    if (gdis)
+      for (int32_t j = 0; j < D; ++j) gdis[go + j] = ix.gdis[(size_t)(ix.row_off[(size_t)p] + j)];
+    go += D;
+    if (sep_off) sep_off[v] = so;
+    for (int64_t i = ix.sep_off[(size_t)p]; i < ix.sep_off[(size_t)p + 1]; ++i, ++so)
+      if (sep) sep[so] = ix.sep[(size_t)i];
+  }
+  if (gdis_off) gdis_off[n] = go;
+  if (sep_off) sep_off[n] = so;
+  int64_t po = 0;
+  for (size_t b = 0; b < ix.p2h_node.size(); ++b) {
+    const int32_t v = ix.p2h_node[b];
+    if (branch) branch[b] = ix.perm[(size_t)v];
+    for (int32_t s = 0; s < n; ++s) {
+      const int32_t q = ix.iperm[(size_t)s];
+      if (p2h_off) p2h_off[(int64_t)b * n + s] = po;
+      if (v <= q && q < v + ix.subtree[(size_t)v]) {
+        const int64_t i = ix.p2h_base[b] + (q - v);
+        for (int64_t k = ix.p2h_off[(size_t)i]; k < ix.p2h_off[(size_t)i + 1]; ++k, ++po)
+          if (p2h) p2h[po] = ix.p2h[(size_t)k];
+      }
+    }
+  }
+  if (p2h_off) p2h_off[(int64_t)ix.p2h_node.size() * n] = po;
 }
 
 }  // namespace bcx

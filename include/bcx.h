@@ -239,6 +239,45 @@ int bcx_index_export(const bcx_index *ix, int32_t *parent, int64_t *label_off,
                      int32_t *dis, uint64_t *cnt, uint32_t *hop,
                      int64_t *bag_off, int32_t *bag_v, int32_t *bag_len,
                      uint64_t *bag_cnt, uint32_t *bag_hop);
+/* The host-array seam of moveDisIndexToDevice (QBC.cu:11-127): QBC's H2H
+ * distance index as the reference's Graph holds it after makeIndex,
+ * optimizeVS and SVSP_P2H (src/buildIndex.cpp:208-278, 313-386, 565-719;
+ * fields of Node, include/Graph.h:16-62), indexed by vertex id (0-based),
+ * installed on this index's tree (in place of bcx_index_build_h2h):
+ *   parent[v]             tree parent vertex, -1 for the root (Node::pa);
+ *                         must equal the index's tree (an index built from
+ *                         the same order file has it)
+ *   gdis_off[n + 1], gdis v's row [gdis_off[v], gdis_off[v + 1]): depth(v)
+ *                         shortest distances to its ancestors, root first
+ *                         (Node::dis)
+ *   sep_off[n + 1], sep   the row positions scanned when v is the LCA of a
+ *                         pair, v's own position included (Node::posOptimize;
+ *                         a position is an ancestor depth)
+ *   branch[nbranch]       the kept P2H branch vertices (Node::isBranchNode)
+ *   p2h_off[nbranch * n + 1], p2h
+ *                         for branch[j] and vertex s, the positions
+ *                         [p2h_off[j * n + s], p2h_off[j * n + s + 1]) of p2h
+ *                         (Node::posP2H_3[s]); empty unless s is below
+ *                         branch[j]
+ * Every array is checked first (BCX_EINVAL for shape, BCX_EIO for ranges);
+ * the index is unchanged by a call that fails.  nbranch may be 0 (branch,
+ * p2h_off and p2h may then be NULL). */
+int bcx_index_h2h_from_arrays(bcx_index *ix, const int32_t *parent,
+                              const int64_t *gdis_off, const int32_t *gdis,
+                              const int64_t *sep_off, const int32_t *sep,
+                              int32_t nbranch, const int32_t *branch,
+                              const int64_t *p2h_off, const int32_t *p2h);
+/* Sizes of the H2H arrays of an index with H2H labels: gdis entries
+ * (label_entries - n), separator entries, kept branch vertices and P2H list
+ * entries (each pointer may be NULL). */
+int bcx_index_h2h_sizes(const bcx_index *ix, int64_t *gdis_entries,
+                        int64_t *sep_entries, int32_t *nbranch,
+                        int64_t *p2h_entries);
+/* The same arrays out, in bcx_index_h2h_from_arrays's layout (each pointer
+ * may be NULL; p2h_off has nbranch * n + 1 entries). */
+int bcx_index_h2h_export(const bcx_index *ix, int64_t *gdis_off, int32_t *gdis,
+                         int64_t *sep_off, int32_t *sep, int32_t *branch,
+                         int64_t *p2h_off, int32_t *p2h);
 void bcx_index_free(bcx_index *ix);
 
 /* ---- device context --------------------------------------------------- */

@@ -11,6 +11,10 @@
 //   tc_ref tcfull <graph>               TreeCount index, every node's label row
 //                                       and bag (the data moveCountIndexToDevice
 //                                       reads, MHLQBC.cu:6-128, plus the bags)
+//   tc_ref h2hfull <graph> <order>      QBC's H2H index with its optimizeVS
+//                                       separators and P2H branch tables (the
+//                                       data moveDisIndexToDevice reads,
+//                                       QBC.cu:11-127)
 // Output (text): a stats line, the elimination order, one "V" line per vertex
 // (id, parent vertex id, tree height, label length; 1-based ids), label rows
 // of the vertices listed in the pairs file, and one "Q s t d count hop" line
@@ -202,12 +206,55 @@ static int run_h2h(const char *graph, const char *order, const char *pairs_path)
   return 0;
 }
 
+// QBC's whole host H2H index, built as the QBC driver builds it before its
+// upload (QBC.cu:421-434: order-file contraction, tree, global-distance labels,
+// optimizeVS, SVSP_P2H with k = 10, d = 5, R = 1e6), i.e. the data
+// moveDisIndexToDevice (QBC.cu:11-127) reads.  Per vertex (1-based ids):
+//   "V v parent_vertex height row_length"
+//   "D v k d_0 ... d_{k-1}"   Node::dis, root first (no own entry)
+//   "S v k p_0 ... p_{k-1}"   Node::posOptimize, in the reference's order
+// and per kept branch node b, per vertex s with a non-empty list:
+//   "L b s k p_0 ... p_{k-1}" Node::posP2H_3[s]
+static int run_h2hfull(const char *graph, const char *order) {
+  Graph g;
+  g.ReadGraph(graph);
+  g.CHconsorderMT(order);
+  g.makeTree();
+  g.makeIndex();
+  g.optimizeVS();
+  std::vector<int> kept;
+  g.SVSP_P2H(10, 5, 1000000, kept);
+  const int n = g.nodenum;
+  std::printf("n %d\n", n);
+  for (int v = 0; v < n; ++v) {
+    const auto &nd = g.Tree[g.rank[v]];
+    int pv = nd.pa < 0 ? 0 : g.Tree[nd.pa].uniqueVertex + 1;
+    std::printf("V %d %d %d %zu\nD %d %zu", v + 1, pv, nd.height, nd.dis.size(), v + 1, nd.dis.size());
+    for (int d : nd.dis) std::printf(" %d", d);
+    std::printf("\nS %d %zu", v + 1, nd.posOptimize.size());
+    for (int p : nd.posOptimize) std::printf(" %d", p);
+    std::printf("\n");
+  }
+  for (const auto &nd : g.Tree) {
+    if (!nd.isBranchNode) continue;
+    for (int s = 0; s < (int)nd.posP2H_3.size(); ++s) {
+      const auto &l = nd.posP2H_3[s];
+      if (l.empty()) continue;
+      std::printf("L %d %d %zu", nd.uniqueVertex + 1, s + 1, l.size());
+      for (int p : l) std::printf(" %d", p);
+      std::printf("\n");
+    }
+  }
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc >= 3 && std::string(argv[1]) == "tc") return run_tc(argv[2], argc > 3 ? argv[3] : nullptr);
   if (argc >= 3 && std::string(argv[1]) == "tcfull") return run_tcfull(argv[2]);
   if (argc >= 4 && std::string(argv[1]) == "h2h")
     return run_h2h(argv[2], argv[3], argc > 4 ? argv[4] : nullptr);
+  if (argc >= 4 && std::string(argv[1]) == "h2hfull") return run_h2hfull(argv[2], argv[3]);
   std::fprintf(stderr, "usage: tc_ref tc <graph> [pairs] | tc_ref tcfull <graph> | "
-                       "tc_ref h2h <graph> <order> [pairs]\n");
+                       "tc_ref h2h <graph> <order> [pairs] | tc_ref h2hfull <graph> <order>\n");
   return 2;
 }

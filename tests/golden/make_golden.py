@@ -7,10 +7,11 @@ buildIndex.cpp) compiled by oracle/build_ref.sh with a dumping harness:
 tree shape (parent, height, label length per vertex), the elimination order
 and (distance, count, max-hop) for seeded vertex pairs evaluated with the
 reference's scan on the reference's labels; for the grid also the reference's
-whole host index structure (label rows and bags), the input of the host-array
-seam.  Fixtures are data only.
+whole host index structures, the inputs of the host-array seams: TreeDecomp's
+label rows and bags, and QBC's H2H distance rows, optimizeVS separators and P2H
+branch tables.  Fixtures are data only.
 
-  python tests/golden/make_golden.py [--reference /root/reference]
+  python tests/golden/make_golden.py [--reference /root/reference] [--only-h2hfull]
 """
 from __future__ import annotations
 
@@ -98,6 +99,68 @@ def run_ref_full(graph_path, n: int) -> dict:
                 bag_hop=Bg[:, 3].astype(np.uint32))
 
 
+def run_ref_h2hfull(graph_path, order_path, n: int) -> dict:
+    """QBC's whole host H2H index (tc_ref h2hfull): parent, distance rows over
+    ancestors, optimizeVS separators and the kept P2H branch tables, indexed
+    by 0-based vertex id (the layout of bcx_index_h2h_from_arrays)."""
+    exe = ROOT / "oracle" / "_ref" / "tc_ref"
+    out = subprocess.run([str(exe), "h2hfull", str(graph_path), str(order_path)], capture_output=True,
+                         text=True, check=True).stdout
+    parent = np.full(n, -1, np.int32)
+    rows = [None] * n
+    seps = [None] * n
+    lists = {}
+    for line in out.splitlines():
+        f = line.split()
+        if not f:
+            continue
+        if f[0] == "V":
+            parent[int(f[1]) - 1] = int(f[2]) - 1
+        elif f[0] == "D":
+            rows[int(f[1]) - 1] = [int(x) for x in f[3:]]
+        elif f[0] == "S":
+            seps[int(f[1]) - 1] = [int(x) for x in f[3:]]
+        elif f[0] == "L":
+            lists.setdefault(int(f[1]) - 1, {})[int(f[2]) - 1] = [int(x) for x in f[4:]]
+    go = np.zeros(n + 1, np.int64)
+    so = np.zeros(n + 1, np.int64)
+    for v in range(n):
+        go[v + 1] = go[v] + len(rows[v])
+        so[v + 1] = so[v] + len(seps[v])
+    branch = np.array(sorted(lists), np.int32)
+    po = np.zeros(len(branch) * n + 1, np.int64)
+    flat = []
+    for j, b in enumerate(branch.tolist()):
+        for v in range(n):
+            po[j * n + v] = len(flat)
+            flat += lists[b].get(v, [])
+    po[-1] = len(flat)
+    return dict(parent=parent, gdis_off=go, gdis=np.array([x for r in rows for x in r], np.int32),
+                sep_off=so, sep=np.array([x for r in seps for x in r], np.int32), branch=branch,
+                p2h_off=po, p2h=np.array(flat, np.int32))
+
+
+def grid_h2hfull() -> None:
+    """grid1k_h2hfull.npz: the reference's QBC H2H index of the grid on its
+    min-degree order (the order GraphConstract writes, restated by
+    bcx_min_degree_order and checked against the shipped LosAngeles order)."""
+    import gpu_betweennesscentrality_amd as bcx
+
+    g = gg.make("grid1k")
+    order = bcx.Graph.from_arcs(g.n, g.u, g.v, g.w).min_degree_order()
+    with tempfile.TemporaryDirectory() as d:
+        gp, op = Path(d) / "grid.txt", Path(d) / "order.txt"
+        gg.write_graph(g, gp)
+        with open(op, "w") as f:
+            f.write(f"{g.n}\n")
+            for v in range(g.n):
+                f.write(f"{v} {int(order[v])}\n")
+        full = run_ref_h2hfull(gp, op, g.n)
+    np.savez_compressed(HERE / "grid1k_h2hfull.npz", order=order.astype(np.int32), **full)
+    print("grid1k full H2H:", len(full["gdis"]), "distance entries,", len(full["sep"]), "separator entries,",
+          len(full["branch"]), "branch vertices,", len(full["p2h"]), "P2H entries")
+
+
 def seeded_pairs(n: int, k: int, seed: int) -> np.ndarray:
     a = gg.randint(seed, 1, k, 1, n)
     b = gg.randint(seed, 2, k, 1, n)
@@ -109,8 +172,12 @@ def seeded_pairs(n: int, k: int, seed: int) -> np.ndarray:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
+    ap.add_argument("--only-h2hfull", action="store_true", help="write grid1k_h2hfull.npz only")
     args = ap.parse_args()
     ref = Path(args.reference)
+    grid_h2hfull()
+    if args.only_h2hfull:
+        return
 
     # Seeded grid workload.
     g = gg.make("grid1k")

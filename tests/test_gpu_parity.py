@@ -327,6 +327,31 @@ def test_h2h_distances_in_several_chunks():
     assert np.array_equal(got.reshape(reps, len(uniq), g.n), np.broadcast_to(ref, (reps, len(uniq), g.n)))
 
 
+def test_h2h_distances_from_the_reference_h2h_index(golden_dir):
+    """QBC's H2H index as the reference's host code builds it for the grid
+    (distance rows, optimizeVS separators, its own P2H branch tables), handed
+    over through bcx_index_h2h_from_arrays (the seam of moveDisIndexToDevice,
+    QBC.cu:11-127): the device scan from every source equals Dijkstra, and the
+    distances of bcx's own H2H build."""
+    g = gg.make("grid1k")
+    G = bcx.Graph.from_arcs(g.n, g.u, g.v, g.w)
+    ref = dict(np.load(golden_dir / "grid1k_h2hfull.npz"))
+    ix = bcx.Index.build_order(G, ref["order"])
+    ix.h2h_from_arrays(ref)
+    ctx = bcx.Context(G, ix, 0)
+    src = np.arange(g.n, dtype=np.int32)
+    got = ctx.h2h_distances(src)
+    ctx.close()
+    csr = oracle.CSR.from_arcgraph(g)
+    for s in range(g.n):
+        assert np.array_equal(got[s].astype(np.int64), oracle.sssp(csr, s)[0]), f"source {s}"
+    own = bcx.Index.build_order(G, ref["order"])
+    own.build_h2h()
+    ctx = bcx.Context(G, own, 0)
+    assert np.array_equal(ctx.h2h_distances(src), got)
+    ctx.close()
+
+
 def test_h2h_distances_need_labels(grid):
     g, csr, ctx = grid
     with pytest.raises(bcx.BcxError):
