@@ -104,6 +104,84 @@ inline void maybe_dump_bc(const std::vector<double> &bc) {
   std::fclose(f);
 }
 
+// Source range of a partial run (BCX_SOURCES=begin:end, 0-based vertex ids,
+// half-open): the BC file and dump then hold that range's share of BC, and
+// the shares of ranges that cover every vertex once sum to the full BC, so a
+// long run can be split into ranges and resumed from the last one finished.
+// Returns false (all sources) when unset.
+inline bool source_range(int32_t n, std::vector<int32_t> &src) {
+  const char *v = std::getenv("BCX_SOURCES");
+  if (!v || !*v) return false;
+  long long a = -1, b = -1;
+  char tail = 0;
+  if (std::sscanf(v, "%lld:%lld%c", &a, &b, &tail) != 2 || a < 0 || b < a || b > n) {
+    std::cerr << "error: BCX_SOURCES must be begin:end with 0 <= begin <= end <= " << n << ", got " << v
+              << std::endl;
+    std::exit(1);
+  }
+  src.resize((size_t)(b - a));
+  for (long long i = a; i < b; ++i) src[(size_t)(i - a)] = (int32_t)i;
+  return true;
+}
+
+// BC of every source, or of a BCX_SOURCES range (`range`, parsed before the
+// index is built so that a bad value fails early); returns the sources used.
+inline int64_t compute_bc(bcx_ctx *ctx, int mode, int32_t n, const std::vector<int32_t> *range,
+                          std::vector<double> &bc, uint32_t &ovf) {
+  bc.assign((size_t)n, 0.0);
+  ovf = 0;
+  if (range) {
+    check(bcx_bc_sources(ctx, mode, range->data(), (int32_t)range->size(), bc.data(), &ovf), "computing BC");
+    return (int64_t)range->size();
+  }
+  check(bcx_full_bc(ctx, mode, bc.data(), &ovf), "computing BC");
+  return n;
+}
+
+inline const char *mode_name(int mode) {
+  return mode == BCX_MODE_PLAIN ? "plain" : mode == BCX_MODE_BATCHED ? "batched" : "sweep";
+}
+
+// Machine-readable metrics beside the reference's stdout lines: with
+// BCX_METRICS=path, one JSON object per run is appended to that file (phase
+// seconds, sources or queries per second, overflow, kernel times).
+struct Metrics {
+  std::string tool, graph;
+  int32_t n = 0;
+  int64_t m = 0;
+  int device = 0, mode = BCX_MODE_SWEEP;
+  int64_t work = 0;          // sources (BC) or query vertices (RQ)
+  const char *unit = "sources";
+  double index_s = 0, context_s = 0, compute_s = 0, total_s = 0;
+  uint32_t overflow = 0;
+};
+inline void write_metrics(const Metrics &mt, const bcx_ctx *ctx) {
+  const char *p = std::getenv("BCX_METRICS");
+  if (!p || !*p) return;
+  bcx_timing tm{};
+  if (ctx) bcx_ctx_timing(ctx, &tm);
+  std::string graph;
+  for (char c : mt.graph) {
+    if (c == '"' || c == '\\') graph += '\\';
+    graph += c;
+  }
+  FILE *f = std::fopen(p, "a");
+  if (!f) {
+    std::cerr << "error: cannot append to " << p << std::endl;
+    std::exit(1);
+  }
+  std::fprintf(f,
+               "{\"tool\": \"%s\", \"graph\": \"%s\", \"n\": %d, \"m\": %lld, \"device\": %d, "
+               "\"mode\": \"%s\", \"%s\": %lld, \"index_s\": %.6f, \"context_s\": %.6f, "
+               "\"compute_s\": %.6f, \"total_s\": %.6f, \"%s_per_s\": %.3f, \"overflow\": %s, "
+               "\"kernel_ms\": {\"scan\": %.3f, \"dep\": %.3f, \"reduce\": %.3f}}\n",
+               mt.tool.c_str(), graph.c_str(), mt.n, (long long)mt.m, mt.device, mode_name(mt.mode), mt.unit,
+               (long long)mt.work, mt.index_s, mt.context_s, mt.compute_s, mt.total_s, mt.unit,
+               mt.compute_s > 0 ? (double)mt.work / mt.compute_s : 0.0, mt.overflow ? "true" : "false",
+               tm.scan_ms, tm.dep_ms, tm.reduce_ms);
+  std::fclose(f);
+}
+
 inline double seconds_since(std::chrono::high_resolution_clock::time_point t0) {
   return std::chrono::duration_cast<std::chrono::duration<double>>(
              std::chrono::high_resolution_clock::now() - t0)

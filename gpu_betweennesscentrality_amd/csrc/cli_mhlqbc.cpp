@@ -20,18 +20,27 @@ int main(int argc, char *argv[]) {
 
   auto t0 = std::chrono::high_resolution_clock::now();
   std::cout << "Read Weighted Graph_ OK!\nn = " << n << ", m = " << m << std::endl;
+  cli::Metrics mt{"MHLQBC", road, n, m, device};
+  std::vector<int32_t> range;
+  const bool partial = cli::source_range(n, range);
   bcx_index *ix = cli::mhl_index(g);
+  mt.index_s = cli::seconds_since(t0);
   bcx_ctx *ctx = nullptr;
   cli::check(bcx_ctx_create(device, g, ix, &ctx), "creating device context");
-  const int mode = cli::apply_env_tuning(ctx);
-  std::vector<double> bc((size_t)n);
+  mt.context_s = cli::seconds_since(t0) - mt.index_s;
+  mt.mode = cli::apply_env_tuning(ctx);
+  std::vector<double> bc;
   uint32_t ovf = 0;
-  cli::check(bcx_full_bc(ctx, mode, bc.data(), &ovf), "computing BC");
+  mt.work = cli::compute_bc(ctx, mt.mode, n, partial ? &range : nullptr, bc, ovf);
   const double secs = cli::seconds_since(t0);
+  mt.compute_s = secs - mt.index_s - mt.context_s;
+  mt.total_s = secs;
+  mt.overflow = ovf;
   std::cout << secs << "  seconds (---totalTime---)" << std::endl;
   if (ovf) std::cerr << "warning: a shortest-path count overflowed uint64" << std::endl;
   if (argc >= 4) cli::write_bc_file(argv[3], bc);
   cli::maybe_dump_bc(bc);
+  cli::write_metrics(mt, ctx);
   bcx_ctx_free(ctx);
   bcx_index_free(ix);
   bcx_graph_free(g);

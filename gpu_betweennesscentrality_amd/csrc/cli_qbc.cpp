@@ -25,20 +25,29 @@ int main(int argc, char *argv[]) {
   std::cout << "roadDataPath:" << road << "\t" << n << "\t" << m << "\n";
 
   auto t0 = std::chrono::high_resolution_clock::now();
+  cli::Metrics mt{"QBC", road, n, m, device};
+  std::vector<int32_t> range;
+  const bool partial = cli::source_range(n, range);
   bcx_index *ix = nullptr;
   cli::check(bcx_index_build_flags(g, order.c_str(), nullptr, BCX_INDEX_DEVICE_LABELS, &ix),
              "building index from order file");
+  mt.index_s = cli::seconds_since(t0);
   bcx_ctx *ctx = nullptr;
   cli::check(bcx_ctx_create(device, g, ix, &ctx), "creating device context");
-  const int mode = cli::apply_env_tuning(ctx);
-  std::vector<double> bc((size_t)n);
+  mt.context_s = cli::seconds_since(t0) - mt.index_s;
+  mt.mode = cli::apply_env_tuning(ctx);
+  std::vector<double> bc;
   uint32_t ovf = 0;
-  cli::check(bcx_full_bc(ctx, mode, bc.data(), &ovf), "computing BC");
+  mt.work = cli::compute_bc(ctx, mt.mode, n, partial ? &range : nullptr, bc, ovf);
   const double secs = cli::seconds_since(t0);
+  mt.compute_s = secs - mt.index_s - mt.context_s;
+  mt.total_s = secs;
+  mt.overflow = ovf;
   std::cout << secs << "  seconds (---totalTime---)" << std::endl;
   if (ovf) std::cerr << "warning: a shortest-path count overflowed uint64" << std::endl;
   if (argc >= 5) cli::write_bc_file(argv[4], bc);
   cli::maybe_dump_bc(bc);
+  cli::write_metrics(mt, ctx);
   if (argc >= 6) {
     std::ifstream ifs(argv[5]);
     if (!ifs) {

@@ -49,14 +49,25 @@ int main(int argc, char *argv[]) {
 
   auto t0 = std::chrono::high_resolution_clock::now();
   std::cout << "Read Weighted Graph_ OK!\nn = " << n << ", m = " << m << std::endl;
+  cli::Metrics mt{"RQ", road, n, m, device};
+  mt.unit = "queries";
   bcx_index *ix = cli::mhl_index(g);
+  mt.index_s = cli::seconds_since(t0);
   bcx_ctx *ctx = nullptr;
   cli::check(bcx_ctx_create(device, g, ix, &ctx), "creating device context");
+  mt.context_s = cli::seconds_since(t0) - mt.index_s;
+  mt.mode = BCX_MODE_SWEEP;  // bcx_subset_bc answers from a bag sweep
   cli::apply_env_tuning(ctx);
   std::vector<double> out((size_t)q);
-  cli::check(bcx_subset_bc(ctx, pts.data(), q, out.data()), "computing subset BC");
+  const int rc = bcx_subset_bc(ctx, pts.data(), q, out.data());  // 1: a count overflowed
+  cli::check(rc, "computing subset BC");
   const double secs = cli::seconds_since(t0);
+  mt.work = q;
+  mt.overflow = rc == 1;
+  mt.compute_s = secs - mt.index_s - mt.context_s;
+  mt.total_s = secs;
   std::cout << secs << "  seconds (---totalTime---)" << std::endl;
+  cli::write_metrics(mt, ctx);
 
   std::cout << std::left;
   std::cout << std::endl;

@@ -1,4 +1,5 @@
 """C ABI surface, input contracts and CLI behaviour that need no GPU."""
+import os
 import re
 import subprocess
 from pathlib import Path
@@ -109,6 +110,19 @@ def test_graphgen_is_deterministic():
     g = gg.make("grid1k")
     assert g.n == 1024 and g.m == 1834  # 1984 - 198 deleted + 48 diagonals
     assert g.w.min() >= 1 and g.w.max() <= 1000
+
+
+@pytest.mark.parametrize("value", ["5:2", "x", "0:99999", "3:4:5", "-1:4"])
+def test_cli_refuses_a_bad_source_range(tmp_path, value):
+    """BCX_SOURCES is checked right after the graph is read, before any index
+    or device work."""
+    g = gg.make("grid1k")
+    gp = tmp_path / "g.txt"
+    gg.write_graph(g, gp)
+    env = dict(os.environ, BCX_SOURCES=value)
+    for cmd in ([BIN / "MHLQBC", gp, "0"], [BIN / "QBC", gp, tmp_path / "order.txt", "0"]):
+        r = subprocess.run(cmd, capture_output=True, text=True, env=env)
+        assert r.returncode == 1 and "BCX_SOURCES must be begin:end" in r.stderr, r.stderr
 
 
 def test_cli_usage_and_query_file_contract(tmp_path):

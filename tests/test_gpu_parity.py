@@ -4,6 +4,7 @@ Bars: distances, uint64 path counts and max-hop layers bit-exact; BC within
 1e-9 relative (|a - b| <= 1e-9 * max(|b|, 1)); plain and batched forms
 bitwise identical (same accumulation order by construction).
 """
+import json
 import os
 import subprocess
 from pathlib import Path
@@ -182,6 +183,42 @@ def test_cli_mhlqbc_and_qbc(tmp_path, grid):
     assert out.returncode == 0, out.stdout + out.stderr
     assert "BC value is correct" in out.stdout
     assert close_bc(np.fromfile(tmp_path / "bc.bin", np.float64), ref)
+
+
+def test_cli_source_ranges_and_metrics(tmp_path, grid):
+    """A run split into source ranges (BCX_SOURCES, for resuming a long run
+    range by range): each range's dump equals the oracle over those sources,
+    and the ranges' dumps sum to the full BC.  BCX_METRICS appends one JSON
+    line per run (the reference's stdout is unchanged)."""
+    g, csr, ctx = grid
+    ref, _ = oracle.brandes(csr)
+    gp = tmp_path / "g.txt"
+    gg.write_graph(g, gp)
+    exe = ROOT / "gpu_betweennesscentrality_amd" / "bin"
+    metrics = tmp_path / "metrics.jsonl"
+    total = np.zeros(g.n)
+    for a, b in ((0, 300), (300, 301), (301, g.n)):
+        dump = tmp_path / f"bc_{a}.bin"
+        env = dict(os.environ, BCX_SOURCES=f"{a}:{b}", BCX_BC_DUMP=str(dump), BCX_METRICS=str(metrics))
+        out = subprocess.run([exe / "MHLQBC", gp, "0"], env=env, capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        part = np.fromfile(dump, np.float64)
+        want, _ = oracle.brandes(csr, np.arange(a, b, dtype=np.int32))
+        assert close_bc(part, want), f"range {a}:{b}"
+        total += part
+    assert close_bc(total, ref)
+    qp = tmp_path / "q.txt"
+    qp.write_text("3\n1\n2\n3\n")
+    env = dict(os.environ, BCX_METRICS=str(metrics))
+    out = subprocess.run([exe / "RQ", gp, qp, "0"], env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = [json.loads(line) for line in metrics.read_text().splitlines()]
+    assert [r["tool"] for r in rows] == ["MHLQBC"] * 3 + ["RQ"]
+    assert [r["sources"] for r in rows[:3]] == [300, 1, g.n - 301]
+    assert rows[3]["queries"] == 3 and rows[3]["n"] == g.n and rows[3]["overflow"] is False
+    for r in rows[:3]:
+        assert r["mode"] == "sweep" and r["compute_s"] > 0 and r["sources_per_s"] > 0
+        assert abs(r["total_s"] - (r["index_s"] + r["context_s"] + r["compute_s"])) < 1e-3
 
 
 def test_cli_rq_table(tmp_path, grid):
