@@ -160,8 +160,14 @@ inline void write_metrics(const Metrics &mt, const bcx_ctx *ctx) {
   if (!p || !*p) return;
   bcx_timing tm{};
   if (ctx) bcx_ctx_timing(ctx, &tm);
-  std::string graph;
+  std::string graph;  // the path as a JSON string
   for (char c : mt.graph) {
+    if ((unsigned char)c < 0x20) {
+      char u[8];
+      std::snprintf(u, sizeof u, "\\u%04x", (unsigned)(unsigned char)c);
+      graph += u;
+      continue;
+    }
     if (c == '"' || c == '\\') graph += '\\';
     graph += c;
   }
