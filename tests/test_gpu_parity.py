@@ -306,6 +306,47 @@ def test_closed_forms_every_form(name, n, edges, want):
         assert np.array_equal(dist[s].astype(np.int64), oracle.sssp(csr, s)[0]), f"{name}, source {s}"
 
 
+def _random_small_graph(seed):
+    """Seeded connected graph of 2..200 vertices: a random spanning tree plus
+    up to 2n extra edges, weights from a small range (many equal-length
+    paths) or a wide one, and a random contraction order."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 201))
+    edges = {(min(v, int(p)), max(v, int(p))) for v in range(1, n) for p in [rng.integers(0, v)]}
+    for a, b in rng.integers(0, n, size=(int(rng.integers(0, 2 * n + 1)), 2)).tolist():
+        if a != b:
+            edges.add((min(a, b), max(a, b)))
+    e = np.array(sorted(edges), np.int32).reshape(-1, 2)
+    w = rng.integers(1, int(rng.choice([1, 2, 3, 1000])) + 1, size=len(e)).astype(np.int32)
+    g = gg.ArcGraph(n, np.concatenate([e[:, 0], e[:, 1]]), np.concatenate([e[:, 1], e[:, 0]]),
+                    np.concatenate([w, w]))
+    return g, rng.permutation(n).astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_small_graphs_every_form(seed):
+    """Tree shapes the road graphs do not have (paths of bags, one-entry
+    bags, wide top levels) and tie-heavy weights: full BC in the three forms
+    within 1e-9 of the oracle, on the min-degree index and on a random
+    contraction order, and the bag sweep's forward rows bit for bit."""
+    g, order = _random_small_graph(seed)
+    csr = oracle.CSR.from_arcgraph(g)
+    ref, ovf = oracle.brandes(csr)
+    assert not ovf
+    G = bcx.Graph.from_arcs(g.n, g.u, g.v, g.w)
+    for ix in (bcx.Index.build_mhl(G), bcx.Index.build_order(G, order)):
+        ctx = bcx.Context(G, ix, 0)
+        for mode in FORMS:
+            bc, o = ctx.full_bc(mode)
+            assert not o and close_bc(bc, ref), f"seed {seed} n {g.n}, mode {mode}"
+        src = np.arange(g.n, dtype=np.int32)
+        d, s, _, _ = ctx.query_rows(src, bcx.MODE_SWEEP)
+        for i in range(g.n):
+            od, osg, _, _ = oracle.sssp(csr, i)
+            assert np.array_equal(d[i].astype(np.int64), od) and np.array_equal(s[i], osg), f"seed {seed} row {i}"
+        ctx.close()
+
+
 def test_ragged_and_empty_source_lists(grid):
     g, csr, ctx = grid
     ctx.set_tuning(sweep_batch=4096, batch=256)
