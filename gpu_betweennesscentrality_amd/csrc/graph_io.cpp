@@ -76,14 +76,21 @@ void graph_from_arcs(int32_t n, int64_t m, const int32_t *u, const int32_t *v,
     if (w[i] <= 0)
       throw Error{BCX_EGRAPH, "arc " + std::to_string(i) + " has weight <= 0"};
   }
-  // Sort arc ids by (u, v, file position) to find duplicates and reverses.
+  // Arc ids bucketed by tail in file order (a counting sort), then, per tail,
+  // ordered by (head, file position) to find duplicates and reverses: the same
+  // (u, v, file position) order as one global sort, at linear cost.
+  std::vector<int64_t> off((size_t)n + 1, 0);
+  for (int64_t i = 0; i < m; ++i) off[(size_t)u[i] + 1]++;
+  for (int32_t x = 0; x < n; ++x) off[(size_t)x + 1] += off[(size_t)x];
   std::vector<int64_t> ord((size_t)m);
-  std::iota(ord.begin(), ord.end(), 0);
-  std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
-    if (u[a] != u[b]) return u[a] < u[b];
-    if (v[a] != v[b]) return v[a] < v[b];
-    return a < b;
-  });
+  {
+    std::vector<int64_t> fill(off.begin(), off.end() - 1);
+    for (int64_t i = 0; i < m; ++i) ord[(size_t)fill[(size_t)u[i]]++] = i;
+  }
+  for (int32_t x = 0; x < n; ++x)
+    std::sort(ord.begin() + off[(size_t)x], ord.begin() + off[(size_t)x + 1], [&](int64_t a, int64_t b) {
+      return v[a] != v[b] ? v[a] < v[b] : a < b;
+    });
   std::vector<char> keep((size_t)m, 1);
   for (size_t k = 1; k < ord.size(); ++k) {
     int64_t a = ord[k - 1], b = ord[k];
@@ -94,17 +101,14 @@ void graph_from_arcs(int32_t n, int64_t m, const int32_t *u, const int32_t *v,
       keep[(size_t)b] = 0;  // first listing wins
     }
   }
-  // Every kept arc needs its reverse with the same weight.
-  std::vector<int64_t> uniq;
-  uniq.reserve((size_t)m);
-  for (int64_t a : ord)
-    if (keep[(size_t)a]) uniq.push_back(a);
-  for (int64_t a : uniq) {
-    int32_t ru = v[a], rv = u[a];
-    auto it = std::lower_bound(uniq.begin(), uniq.end(), (int64_t)-1, [&](int64_t x, int64_t) {
-      return u[x] < ru || (u[x] == ru && v[x] < rv);
-    });
-    if (it == uniq.end() || u[*it] != ru || v[*it] != rv)
+  // Every kept arc needs its reverse with the same weight: the first listing
+  // of (v, u) in v's bucket.
+  for (int64_t a : ord) {
+    if (!keep[(size_t)a]) continue;
+    const int32_t ru = v[a], rv = u[a];
+    const auto lo = ord.begin() + off[(size_t)ru], hi = ord.begin() + off[(size_t)ru + 1];
+    auto it = std::lower_bound(lo, hi, rv, [&](int64_t x, int32_t key) { return v[x] < key; });
+    if (it == hi || v[*it] != rv)
       throw Error{BCX_EGRAPH, "arc " + std::to_string(u[a] + 1) + " " + std::to_string(v[a] + 1) +
                                   " has no reverse arc (edges must be listed both ways)"};
     if (w[*it] != w[a])
