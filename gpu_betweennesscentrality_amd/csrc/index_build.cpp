@@ -25,6 +25,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <queue>
 #include <set>
 #include <tuple>
 
@@ -440,24 +441,21 @@ void min_degree_order(const Graph &g, std::vector<int32_t> &order, int64_t *shor
     std::sort(a.begin(), a.end());
     a.erase(std::unique(a.begin(), a.end()), a.end());
   }
-  auto has = [&](int32_t u, int32_t v) {
-    const auto &a = adj[(size_t)u];
-    return std::binary_search(a.begin(), a.end(), v);
-  };
-  auto add = [&](int32_t u, int32_t v) {
-    auto &a = adj[(size_t)u];
-    a.insert(std::lower_bound(a.begin(), a.end(), v), v);
-  };
   auto drop = [&](int32_t u, int32_t v) {
     auto &a = adj[(size_t)u];
     auto it = std::lower_bound(a.begin(), a.end(), v);
     if (it != a.end() && *it == v) a.erase(it);
   };
-  std::vector<int32_t> key((size_t)n);
-  std::set<std::pair<int32_t, int32_t>> q;
+  std::vector<int32_t> key((size_t)n), merged;
+  // Least (degree, id) first.  A re-keyed vertex gets a new entry; an old
+  // one is skipped when it surfaces (its degree no longer matches, or the
+  // vertex is gone), so the valid top is always the least live (key, id).
+  std::priority_queue<int64_t, std::vector<int64_t>, std::greater<int64_t>> q;
+  auto entry = [](int32_t d, int32_t v) { return ((int64_t)d << 32) | (int64_t)(uint32_t)v; };
+  std::vector<char> gone((size_t)n, 0);
   for (int32_t v = 0; v < n; ++v) {
     key[(size_t)v] = (int32_t)adj[(size_t)v].size();
-    q.insert({key[(size_t)v], v});
+    q.push(entry(key[(size_t)v], v));
   }
   order.assign((size_t)n, 0);
   int64_t arcs_out = 0;
@@ -465,27 +463,48 @@ void min_degree_order(const Graph &g, std::vector<int32_t> &order, int64_t *shor
   if (width_log) width_log->clear();
   for (int32_t k = 0; k < n; ++k) {
     if (width_log && k % 100000 == 0) width_log->push_back(width);
-    const int32_t x = q.begin()->second;
-    q.erase(q.begin());
+    int32_t x;
+    for (;;) {
+      const int64_t top = q.top();
+      q.pop();
+      x = (int32_t)(uint32_t)top;
+      if (!gone[(size_t)x] && key[(size_t)x] == (int32_t)(top >> 32)) break;
+    }
+    gone[(size_t)x] = 1;
     order[(size_t)x] = k;
     const std::vector<int32_t> nb = std::move(adj[(size_t)x]);
     adj[(size_t)x].clear();
     arcs_out += (int64_t)nb.size();
     width = std::max(width, (int32_t)nb.size());
-    for (int32_t y : nb) drop(y, x);
-    if (nb.size() <= 1000)
-      for (size_t i = 0; i < nb.size(); ++i)
-        for (size_t j = i + 1; j < nb.size(); ++j)
-          if (!has(nb[i], nb[j])) {
-            add(nb[i], nb[j]);
-            add(nb[j], nb[i]);
-          }
+    // Each neighbour y loses x and, unless the neighbourhood is over 1000,
+    // gains the other neighbours: one merge of two sorted lists per y.
+    for (int32_t y : nb) {
+      if (nb.size() > 1000) {
+        drop(y, x);
+        continue;
+      }
+      auto &a = adj[(size_t)y];
+      merged.clear();
+      size_t i = 0, j = 0;
+      while (i < a.size() || j < nb.size()) {
+        int32_t z;
+        if (j == nb.size() || (i < a.size() && a[i] < nb[j])) {
+          z = a[i++];
+        } else if (i == a.size() || nb[j] < a[i]) {
+          z = nb[j++];
+        } else {
+          z = a[i++];
+          ++j;
+        }
+        if (z != x && z != y) merged.push_back(z);
+      }
+      a.swap(merged);
+    }
     for (int32_t y : nb) {
       const int32_t d = (int32_t)adj[(size_t)y].size();
       if (d != key[(size_t)y]) {
-        q.erase({key[(size_t)y], y});
         key[(size_t)y] = d;
-        q.insert({d, y});
+        q.push(entry(d, y));
       }
     }
   }
