@@ -88,7 +88,12 @@ void eliminate(const Graph &g, const std::vector<int32_t> &order,
   const bool min_degree = order.empty();
   std::vector<int32_t> DD, DD2, tDD, tDD2;
   std::vector<char> changed;
-  std::set<std::tuple<int32_t, int32_t, int32_t>> pq;
+  // Least (DD, DD2, id) first, as the reference's ordered set: a lazy binary
+  // heap whose entries that no longer match their vertex's keys (re-keyed or
+  // eliminated) are dropped when they surface, so the valid top is the least
+  // live (DD, DD2, id).
+  using Key = std::tuple<int32_t, int32_t, int32_t>;
+  std::priority_queue<Key, std::vector<Key>, std::greater<Key>> pq;
   std::vector<int32_t> seq;
   if (min_degree) {
     DD.resize((size_t)n);
@@ -99,7 +104,7 @@ void eliminate(const Graph &g, const std::vector<int32_t> &order,
     for (int32_t x = 0; x < n; ++x) {
       DD[(size_t)x] = DD2[(size_t)x] = tDD[(size_t)x] = tDD2[(size_t)x] =
           (int32_t)adj[(size_t)x].size();
-      pq.insert(std::make_tuple(DD[(size_t)x], DD2[(size_t)x], x));
+      pq.push(Key{DD[(size_t)x], DD2[(size_t)x], x});
     }
   } else {
     seq.assign((size_t)n, -1);
@@ -109,16 +114,24 @@ void eliminate(const Graph &g, const std::vector<int32_t> &order,
   for (int32_t step = 0; step < n; ++step) {
     int32_t x;
     if (min_degree) {
-      x = std::get<2>(*pq.begin());
-      while (changed[(size_t)x]) {  // lazy re-key, TreeDecomp.cpp:103-112
-        pq.erase(std::make_tuple(DD[(size_t)x], DD2[(size_t)x], x));
-        DD[(size_t)x] = tDD[(size_t)x];
-        DD2[(size_t)x] = tDD2[(size_t)x];
-        pq.insert(std::make_tuple(DD[(size_t)x], DD2[(size_t)x], x));
-        changed[(size_t)x] = 0;
-        x = std::get<2>(*pq.begin());
+      for (;;) {
+        const Key top = pq.top();
+        x = std::get<2>(top);
+        if (!alive[(size_t)x] || std::get<0>(top) != DD[(size_t)x] || std::get<1>(top) != DD2[(size_t)x]) {
+          pq.pop();  // stale
+          continue;
+        }
+        if (changed[(size_t)x]) {  // lazy re-key, TreeDecomp.cpp:103-112
+          pq.pop();
+          DD[(size_t)x] = tDD[(size_t)x];
+          DD2[(size_t)x] = tDD2[(size_t)x];
+          changed[(size_t)x] = 0;
+          pq.push(Key{DD[(size_t)x], DD2[(size_t)x], x});
+          continue;
+        }
+        pq.pop();
+        break;
       }
-      pq.erase(pq.begin());
     } else {
       x = seq[(size_t)step];
     }
