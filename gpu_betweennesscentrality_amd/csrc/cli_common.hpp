@@ -38,8 +38,11 @@ inline int parse_device(const char *s) {
 
 // Knobs that the reference fixes at compile time come from the environment so
 // the positional command line stays identical: BCX_MODE=plain|batched|sweep
-// (default sweep), BCX_TCHUNK, BCX_BP_THREADS, BCX_BATCH, BCX_SWEEP_BATCH.
+// (default sweep), BCX_TCHUNK, BCX_BP_THREADS, BCX_BATCH, BCX_SWEEP_BATCH;
+// BCX_PROFILE=1 launches kernel by kernel with events between phases, so
+// that the metrics line carries kernel times.
 inline int apply_env_tuning(bcx_ctx *ctx) {
+  if (env_int("BCX_PROFILE", 0)) check(bcx_ctx_set_profiling(ctx, 1), "bcx_ctx_set_profiling");
   bcx_tuning tn;
   check(bcx_ctx_get_tuning(ctx, &tn), "bcx_ctx_get_tuning");
   tn.tchunk = env_int("BCX_TCHUNK", tn.tchunk);
