@@ -212,7 +212,13 @@ def test_cli_source_ranges_and_metrics(tmp_path, grid):
     env = dict(os.environ, BCX_METRICS=str(metrics))
     out = subprocess.run([exe / "RQ", gp, qp, "0"], env=env, capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
+    env = dict(os.environ, BCX_METRICS=str(metrics), BCX_PROFILE="1", BCX_BC_DUMP=str(tmp_path / "prof.bin"))
+    out = subprocess.run([exe / "MHLQBC", gp, "0"], env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert close_bc(np.fromfile(tmp_path / "prof.bin", np.float64), ref)
     rows = [json.loads(line) for line in metrics.read_text().splitlines()]
+    assert rows[4]["tool"] == "MHLQBC" and rows[4]["kernel_ms"]["scan"] > 0  # kernel times under BCX_PROFILE
+    rows = rows[:4]
     assert [r["tool"] for r in rows] == ["MHLQBC"] * 3 + ["RQ"]
     assert [r["sources"] for r in rows[:3]] == [300, 1, g.n - 301]
     assert rows[3]["queries"] == 3 and rows[3]["n"] == g.n and rows[3]["overflow"] is False
