@@ -62,6 +62,34 @@ def test_full_bc_grid_plain_and_batched(grid):
     assert np.array_equal(plain.view(np.uint64), batched.view(np.uint64))
 
 
+@pytest.mark.parametrize("scale", [1, 4000])
+def test_label_scan_key_forms(scale):
+    """The label-scan tile kernel packs (distance, entry) into one key where
+    distances and tree height leave room (the seeded grid) and keeps the
+    compare-and-select pass where they do not (the same grid with weights
+    times 4,000, distances near 2^28): rows bit-exact against the oracle from
+    every source, and full BC bitwise equal to the one-source-per-launch
+    form, in both."""
+    g = gg.make("grid1k")
+    g = gg.ArcGraph(g.n, g.u, g.v, (g.w.astype(np.int64) * scale).astype(np.int32))
+    csr = oracle.CSR.from_arcgraph(g)
+    G, ix, ctx = build(g)
+    src = np.arange(g.n, dtype=np.int32)
+    ctx.set_tuning(batch=256)
+    d, s, h, _ = ctx.query_rows(src, bcx.MODE_BATCHED)
+    for i in range(g.n):
+        od, osg, ohp, _ = oracle.sssp(csr, i)
+        assert np.array_equal(d[i].astype(np.int64), od), f"scale {scale}, distances of row {i}"
+        assert np.array_equal(s[i], osg) and np.array_equal(h[i], ohp), f"scale {scale}, row {i}"
+    batched, o1 = ctx.full_bc(bcx.MODE_BATCHED)
+    plain, o2 = ctx.full_bc(bcx.MODE_PLAIN)
+    assert not o1 and not o2
+    assert np.array_equal(batched.view(np.uint64), plain.view(np.uint64))
+    ref, _ = oracle.brandes(csr)
+    assert close_bc(batched, ref)
+    ctx.close()
+
+
 @pytest.mark.parametrize("batch", [1, 64, 100, 4096])
 def test_batch_size_invariance(grid, batch):
     g, csr, ctx = grid
