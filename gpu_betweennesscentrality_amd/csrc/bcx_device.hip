@@ -200,26 +200,13 @@ __global__ __launch_bounds__(256) void scan_plain_kernel(DevIndex ix, DevBatch b
 // count/hop of that entry are fetched afterwards, and only lanes with a tie
 // (first != last, i.e. shortest paths through different top vertices) rescan
 // their range to sum counts and max hops.
-//
-// Packed form (kPacked): where every shortest distance is below 2^dbits and
-// dbits + kb <= 30 (2^kb > the largest entry index), pass 1 keeps two keys
-// per lane instead of (minimum, first, last): min over entries of
-// (sum << kb) | i and of (sum << kb) | (2^kb - 1 - i), whose high bits are the
-// minimum and whose low bits give the first and the last entry attaining it.
-// Each term is clamped to 2^dbits first (the LDS copy, already shifted, once
-// per tile; the target entry on the scalar unit), so a sum with a clamped term
-// is at least 2^dbits and never ties the minimum, which is a true distance;
-// the keys stay below 2^31.  Per entry that is two adds and two minimums on
-// the vector unit against an add, two compares, two selects and a minimum;
-// the results are the same bits.
 constexpr int kTileDepth = 512;
 
-template <bool kPacked>
 __global__ __launch_bounds__(1024) void scan_tile_kernel(
     DevIndex ix, const int *__restrict__ gdis, const uint64_t *__restrict__ gcnt,
     const uint32_t *__restrict__ ghop, const int64_t *__restrict__ grow_off, DevBatch bt,
     int *__restrict__ out_d, uint64_t *__restrict__ out_s, uint32_t *__restrict__ out_h,
-    int batch, int tchunk, int *ovf_flag, uint32_t cap, uint32_t kb) {
+    int batch, int tchunk, int *ovf_flag) {
   __shared__ int sdis[kTileDepth * kWave];
   const int base = bt.cursor[0];
   const int total = bt.total[0];
@@ -233,10 +220,8 @@ __global__ __launch_bounds__(1024) void scan_tile_kernel(
   const int s = active ? bt.srcs[base + slot] : bt.srcs[base + tile0];
   const int64_t rs = grow_off[s];
   const int sdepth = ix.depth[s];
-  for (int i = wave; i < kTileDepth; i += nwaves) {
-    const int x = i <= sdepth ? gdis[rs + i] : 0;
-    sdis[i * kWave + lane] = kPacked ? (int)(min((uint32_t)x, cap) << kb) : x;  // packed: key-aligned
-  }
+  for (int i = wave; i < kTileDepth; i += nwaves)
+    sdis[i * kWave + lane] = i <= sdepth ? gdis[rs + i] : 0;
   __syncthreads();
   const int n = ix.n;
   const int t0 = (blockIdx.y * nwaves + wave) * tchunk;
@@ -252,49 +237,23 @@ __global__ __launch_bounds__(1024) void scan_tile_kernel(
     int m = 0x7fffffff, first = 0, last = 0;
     const int e1 = min(Lmin + 1, kTileDepth);
     int i = 0;
-    if constexpr (kPacked) {
-      const uint32_t low = (1u << kb) - 1u;
-      uint32_t k1 = 0xffffffffu, k2 = 0xffffffffu;
-      // Unmasked: every lane's range covers i (i <= Lmin), source entry in LDS.
+    // Unmasked: every lane's range covers i (i <= Lmin), source entry in LDS.
 #pragma unroll 4
-      for (; i < e1; ++i) {
-        const uint32_t tv = min((uint32_t)dt[i], cap) << kb;  // uniform: scalar unit
-        const uint32_t sv = (uint32_t)sdis[i * kWave + lane];
-        k1 = min(k1, sv + (tv | (uint32_t)i));
-        k2 = min(k2, sv + (tv | (low - (uint32_t)i)));
-      }
-      // Masked tail, as below.
-      for (; i <= Lmax; ++i) {
-        if (i <= L) {
-          const uint32_t tv = min((uint32_t)dt[i], cap) << kb;
-          const uint32_t sv = i < kTileDepth ? (uint32_t)sdis[i * kWave + lane]
-                                             : min((uint32_t)gdis[rs + i], cap) << kb;
-          k1 = min(k1, sv + (tv | (uint32_t)i));
-          k2 = min(k2, sv + (tv | (low - (uint32_t)i)));
-        }
-      }
-      m = (int)(k1 >> kb);
-      first = (int)(k1 & low);
-      last = (int)(low - (k2 & low));
-    } else {
-      // Unmasked: every lane's range covers i (i <= Lmin), source entry in LDS.
-#pragma unroll 4
-      for (; i < e1; ++i) {
-        const int v = sdis[i * kWave + lane] + dt[i];
+    for (; i < e1; ++i) {
+      const int v = sdis[i * kWave + lane] + dt[i];
+      first = v < m ? i : first;
+      m = min(m, v);
+      last = v == m ? i : last;
+    }
+    // Masked tail (lanes whose LCA is deeper than the wave minimum), and
+    // entries past the LDS window read from the label row in HBM.
+    for (; i <= Lmax; ++i) {
+      if (i <= L) {
+        const int sv = i < kTileDepth ? sdis[i * kWave + lane] : gdis[rs + i];
+        const int v = sv + dt[i];
         first = v < m ? i : first;
         m = min(m, v);
         last = v == m ? i : last;
-      }
-      // Masked tail (lanes whose LCA is deeper than the wave minimum), and
-      // entries past the LDS window read from the label row in HBM.
-      for (; i <= Lmax; ++i) {
-        if (i <= L) {
-          const int sv = i < kTileDepth ? sdis[i * kWave + lane] : gdis[rs + i];
-          const int v = sv + dt[i];
-          first = v < m ? i : first;
-          m = min(m, v);
-          last = v == m ? i : last;
-        }
       }
     }
     uint64_t c = mul_sat(gcnt[rs + first], gcnt[rt + first]);
@@ -482,26 +441,6 @@ namespace {
 
 DevBatch batch_of(bcx_ctx *c) { return DevBatch{c->d_srcs, c->d_cursor, c->d_total}; }
 
-// The label-scan tile kernel over a batch, in its packed form where the
-// distances and the tree height leave room for the keys (dbits + kb <= 30).
-void launch_scan_tile(bcx_ctx *c, const DevBatch &bt, int batch) {
-  const int n = c->ix.n;
-  const int waves = 16;
-  dim3 grid((batch + kWave - 1) / kWave, (n + waves * c->tn.tchunk - 1) / (waves * c->tn.tchunk));
-  uint32_t kb = 1;
-  while ((1u << kb) < (uint32_t)c->tree.height) ++kb;
-  const uint32_t db = (uint32_t)c->tree.dbits;
-  if (db + kb <= 30)
-    scan_tile_kernel<true><<<grid, waves * kWave, 0, c->stream>>>(
-        c->ix, c->ix.dis, c->ix.cnt, c->ix.hop, c->ix.row_off, bt, c->ws.d, c->ws.sigma, c->ws.hop, batch,
-        c->tn.tchunk, c->d_ovf, 1u << db, kb);
-  else
-    scan_tile_kernel<false><<<grid, waves * kWave, 0, c->stream>>>(
-        c->ix, c->ix.dis, c->ix.cnt, c->ix.hop, c->ix.row_off, bt, c->ws.d, c->ws.sigma, c->ws.hop, batch,
-        c->tn.tchunk, c->d_ovf, 0u, 0u);
-  HIP_OK(hipGetLastError());
-}
-
 // Launch the kernels of one batch on the context stream.  With `timed`,
 // events bracket each phase and the durations are accumulated.
 void launch_batch(bcx_ctx *c, int mode, int batch, bool timed, double *bsum = nullptr,
@@ -512,7 +451,12 @@ void launch_batch(bcx_ctx *c, int mode, int batch, bool timed, double *bsum = nu
   if (mode == BCX_MODE_PLAIN) {
     scan_plain_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(c->ix, bt, c->ws, c->d_ovf);
   } else {
-    launch_scan_tile(c, bt, batch);
+    const int waves = 16;
+    dim3 grid((batch + kWave - 1) / kWave,
+              (n + waves * c->tn.tchunk - 1) / (waves * c->tn.tchunk));
+    scan_tile_kernel<<<grid, waves * kWave, 0, c->stream>>>(
+        c->ix, c->ix.dis, c->ix.cnt, c->ix.hop, c->ix.row_off, bt, c->ws.d, c->ws.sigma,
+        c->ws.hop, batch, c->tn.tchunk, c->d_ovf);
   }
   if (timed) HIP_OK(hipEventRecord(c->ev[1], c->stream));
   switch (c->tn.bp_threads) {
@@ -1428,7 +1372,12 @@ int bcx_query_dist_count_mode(bcx_ctx *c, int mode, const int32_t *sources, int3
   for (int32_t b0 = 0; b0 < count; b0 += batch) {
     const int32_t nb = std::min(batch, count - b0);
     HIP_OK(hipMemcpyAsync(c->d_cursor, &b0, 4, hipMemcpyHostToDevice, c->stream));
-    launch_scan_tile(c, batch_of(c), batch);
+    const int waves = 16;
+    dim3 grid((batch + kWave - 1) / kWave, (n + waves * c->tn.tchunk - 1) / (waves * c->tn.tchunk));
+    scan_tile_kernel<<<grid, waves * kWave, 0, c->stream>>>(c->ix, c->ix.dis, c->ix.cnt, c->ix.hop, c->ix.row_off,
+                                                            batch_of(c), c->ws.d, c->ws.sigma, c->ws.hop, batch,
+                                                            c->tn.tchunk, c->d_ovf);
+    HIP_OK(hipGetLastError());
     hd.resize((size_t)nb * n);
     hs.resize((size_t)nb * n);
     hh.resize((size_t)nb * n);

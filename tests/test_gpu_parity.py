@@ -63,13 +63,11 @@ def test_full_bc_grid_plain_and_batched(grid):
 
 
 @pytest.mark.parametrize("scale", [1, 4000])
-def test_label_scan_key_forms(scale):
-    """The label-scan tile kernel packs (distance, entry) into one key where
-    distances and tree height leave room (the seeded grid) and keeps the
-    compare-and-select pass where they do not (the same grid with weights
-    times 4,000, distances near 2^28): rows bit-exact against the oracle from
-    every source, and full BC bitwise equal to the one-source-per-launch
-    form, in both."""
+def test_label_scan_small_and_large_distances(scale):
+    """The label-scan tile kernel on the seeded grid and on the same grid
+    with weights times 4,000 (distances near 2^28, sums of two label entries
+    near 2^29): rows bit-exact against the oracle from every source, and
+    full BC bitwise equal to the one-source-per-launch form."""
     g = gg.make("grid1k")
     g = gg.ArcGraph(g.n, g.u, g.v, (g.w.astype(np.int64) * scale).astype(np.int32))
     csr = oracle.CSR.from_arcgraph(g)
