@@ -1,9 +1,9 @@
 """Benchmark of the betweenness hot path on one MI355X.
 
-A step is one bag-sweep launch over one batch of source vertices (49,152 on
-MI355X: three tiles of 64 sources per compute unit, fewer where device
-memory caps the launch) on a seeded road-like graph with its MHL index
-resident in HBM.  W untimed steps, then exactly K timed steps as one full-BC
+A step is one bag-sweep launch over one batch of source vertices (524,288 on
+MI355X: 32 tiles of 64 sources per compute unit, drawing on two state slots
+per unit; fewer where device memory caps the launch, or as the autotuner
+picks) on a seeded road-like graph with its MHL index resident in HBM.  W untimed steps, then exactly K timed steps as one full-BC
 call: K launches (tile kernel + reduction over tiles), then the shortcut
 unpacking once; the sources walk the tree order cyclically.  `value` is
 sources/s over that call.  Beside it: a full BC of every source, the
@@ -52,9 +52,9 @@ def sweep_algorithmic_bytes(n: int, bags: int, inner: int, height: int, sources:
     quotient (12 * inner).  Every inner node that is not an ancestor of the
     source (at most `height` are) reaches its distance through at least one
     tight bag entry, whose ancestor's backward step reads the node's quotient
-    (8 * (inner - height)).  Count words (written and read only where a path
-    count is not 1) and the quotients of further tight entries are
-    data-dependent and not counted.
+    (8 * (inner - height)).  Count slots (written and read only where a path
+    count is too large for its distance word) and the quotients of further
+    tight entries are data-dependent and not counted.
     """
     return float(sources) * (8.0 * bags + 4.0 * n + 12.0 * inner + 8.0 * max(0, inner - height))
 
@@ -192,7 +192,7 @@ def main() -> int:
     ap.add_argument("--workload", default="road320k")
     ap.add_argument("--batch", type=int, default=4096, help="label-scan sources per batch")
     ap.add_argument("--sweep-batch", type=int, default=0,
-                    help="bag-sweep sources per launch (0: the library default, 3 tiles of 64 per CU)")
+                    help="bag-sweep sources per launch (0: the library default, 32 tiles of 64 per CU)")
     ap.add_argument("--labelscan-full-max-n", type=int, default=400_000,
                     help="run the label scan over every source (the cross-check of the full BC) up to this n")
     ap.add_argument("--plain-sources", type=int, default=64)
