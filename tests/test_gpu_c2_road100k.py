@@ -42,8 +42,8 @@ def test_rows_bit_exact_every_form(cfg):
 
 
 def test_full_bc_sweep_equals_label_scan(cfg):
-    """All 100,000 sources: the bag sweep (two launches of 49,152 and one of
-    1,696) against the reference's label-scan method."""
+    """All 100,000 sources: the bag sweep (one launch of 1,563 tiles on 512
+    state slots) against the reference's label-scan method."""
     g, csr, ix, ctx = cfg
     sweep, o1 = ctx.full_bc(bcx.MODE_SWEEP)
     scan, o2 = ctx.full_bc(bcx.MODE_BATCHED)
