@@ -3,8 +3,8 @@ workload (batched MHL-QBC, MHLQBC.cu:495-537), and RQ serving on it
 (RQ.cu:465-531): sampled sources against the oracle (every wide unpack level
 runs), one headline launch over every source (5,000 tiles drawing on 512
 state slots) in the two-workgroups-per-unit tile-kernel form against the
-one-per-unit form (bitwise) and against the label scan, the forward pass's rows bit for bit, and an RQ request alone and
-co-batched."""
+one-per-unit form (bitwise) and against the label scan, the forward pass's
+rows bit for bit, and an RQ request alone and co-batched."""
 import numpy as np
 import pytest
 
