@@ -559,7 +559,7 @@ void run_sources(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, do
 }  // namespace
 
 // Label rows of the nodes at one depth (tree ids lvl[0 .. cnt)), one
-// wavefront per node, lanes over the row's entries.  Entry j of node p's row
+// workgroup per node, threads over the row's entries.  Entry j of node p's row
 // (the ancestor at depth j) is the minimum over the direct shortcut, where
 // that ancestor is in p's bag, and every bag entry x's shortcut plus x's
 // entry j; its count is the saturating sum, and its hop the maximum, over the
@@ -570,22 +570,58 @@ void run_sources(bcx_ctx *c, int mode, const int32_t *sources, int32_t count, do
 // most kInf, so candidate sums stay in int32; candidates through an entry at
 // kInf are left out.  Saturated counts stay saturated, and results through
 // them carry their own overflow flag.
-__global__ __launch_bounds__(256) void label_level_kernel(const int *__restrict__ lvl, int cnt,
-                                                          const int *__restrict__ depth,
-                                                          const int64_t *__restrict__ row_off,
-                                                          const int64_t *__restrict__ bag_off,
-                                                          const int *__restrict__ bag,
-                                                          const int *__restrict__ bag_len,
-                                                          const uint64_t *__restrict__ bag_cnt,
-                                                          const uint32_t *__restrict__ bag_hop, int *dis,
-                                                          uint64_t *lcnt, uint32_t *lhop) {
-  const int w = blockIdx.x * (blockDim.x / kWave) + (int)(threadIdx.x / kWave);
-  const int lane = threadIdx.x & (kWave - 1);
-  if (w >= cnt) return;  // uniform over the wavefront
-  const int p = lvl[w];
+// The bag's fields (the ancestor's depth and row offset, the shortcut's
+// length, count and hop) are staged in LDS once per node, up to
+// kLabelBagLds entries (wider bags read the rest from HBM), so each entry of
+// the row costs the bag's loads of the ancestors' row entries only.  Levels
+// are launched one after another and the deep levels of tall trees hold few
+// nodes, so a workgroup per node (instead of a wavefront) is what keeps the
+// level's latency down: the nodes of a level share one depth, and the
+// workgroup has a thread per row entry up to kLabelThreads.
+constexpr int kLabelThreads = 1024;
+constexpr int kLabelBagLds = 1024;
+
+__global__ __launch_bounds__(kLabelThreads) void label_level_kernel(const int *__restrict__ lvl,
+                                                                    const int *__restrict__ depth,
+                                                                    const int64_t *__restrict__ row_off,
+                                                                    const int64_t *__restrict__ bag_off,
+                                                                    const int *__restrict__ bag,
+                                                                    const int *__restrict__ bag_len,
+                                                                    const uint64_t *__restrict__ bag_cnt,
+                                                                    const uint32_t *__restrict__ bag_hop,
+                                                                    int *dis, uint64_t *lcnt, uint32_t *lhop) {
+  __shared__ int64_t s_row[kLabelBagLds];
+  __shared__ uint64_t s_cnt[kLabelBagLds];
+  __shared__ int s_kx[kLabelBagLds], s_len[kLabelBagLds];
+  __shared__ uint32_t s_hop[kLabelBagLds];
+  const int p = lvl[blockIdx.x];
   const int D = depth[p];
-  const int64_t rp = row_off[p], e0 = bag_off[p], e1 = bag_off[p + 1];
-  for (int j = lane; j <= D; j += kWave) {
+  const int64_t rp = row_off[p], e0 = bag_off[p];
+  const int nb = (int)(bag_off[p + 1] - e0);
+  const int staged = min(nb, kLabelBagLds);
+  for (int i = threadIdx.x; i < staged; i += blockDim.x) {
+    const int x = bag[e0 + i];
+    s_kx[i] = depth[x];
+    s_row[i] = row_off[x];
+    s_len[i] = bag_len[e0 + i];
+    s_cnt[i] = bag_cnt[e0 + i];
+    s_hop[i] = bag_hop[e0 + i];
+  }
+  __syncthreads();
+  // Bag entry i: (ancestor depth, ancestor row offset, shortcut length).
+  auto entry = [&](int i, int &kx, int64_t &rx, int &len) {
+    if (i < kLabelBagLds) {
+      kx = s_kx[i];
+      rx = s_row[i];
+      len = s_len[i];
+    } else {
+      const int x = bag[e0 + i];
+      kx = depth[x];
+      rx = row_off[x];
+      len = bag_len[e0 + i];
+    }
+  };
+  for (int j = threadIdx.x; j <= D; j += blockDim.x) {
     if (j == D) {
       dis[rp + D] = 0;
       lcnt[rp + D] = 1;
@@ -593,26 +629,32 @@ __global__ __launch_bounds__(256) void label_level_kernel(const int *__restrict_
       continue;
     }
     int m = kInfDev;
-    for (int64_t e = e0; e < e1; ++e) {
-      const int x = bag[e], kx = depth[x], len = bag_len[e];
+    for (int i = 0; i < nb; ++i) {
+      int kx, len;
+      int64_t rx;
+      entry(i, kx, rx, len);
       if (kx == j) m = min(m, len);
-      else if (j < kx) m = min(m, len + dis[row_off[x] + j]);
+      else if (j < kx) m = min(m, len + dis[rx + j]);
     }
     uint64_t c = 0;
     uint32_t h = 0;
-    for (int64_t e = e0; e < e1; ++e) {
-      const int x = bag[e], kx = depth[x], len = bag_len[e];
+    for (int i = 0; i < nb; ++i) {
+      int kx, len;
+      int64_t rx;
+      entry(i, kx, rx, len);
+      const uint64_t bc = i < kLabelBagLds ? s_cnt[i] : bag_cnt[e0 + i];
+      const uint32_t bh = i < kLabelBagLds ? s_hop[i] : bag_hop[e0 + i];
       if (kx == j) {
         if (len == m) {
-          c = add_sat(c, bag_cnt[e]);
-          h = max(h, bag_hop[e]);
+          c = add_sat(c, bc);
+          h = max(h, bh);
         }
       } else if (j < kx) {
-        const int64_t o = row_off[x] + j;
+        const int64_t o = rx + j;
         const int v = dis[o];
         if (v < kInfDev && len + v == m) {
-          c = add_sat(c, mul_sat(bag_cnt[e], lcnt[o]));
-          h = max(h, bag_hop[e] + lhop[o]);
+          c = add_sat(c, mul_sat(bc, lcnt[o]));
+          h = max(h, bh + lhop[o]);
         }
       }
     }
@@ -662,9 +704,9 @@ static void device_label_rows(bcx_ctx *c, const bcx::Index &ix, int *dis, uint64
     for (int32_t k = 0; k < H; ++k) {
       const int cnt = lvl_off[(size_t)k + 1] - lvl_off[(size_t)k];
       if (cnt == 0) continue;
-      label_level_kernel<<<(unsigned)((cnt + 3) / 4), 256, 0, c->stream>>>(
-          d_lvl + lvl_off[(size_t)k], cnt, d_depth, d_row, d_boff, d_bag, d_blen, d_bcnt, d_bhop, dis, lcnt,
-          lhop);
+      const int threads = std::min(kLabelThreads, (k + 1 + kWave - 1) / kWave * kWave);  // row length k + 1
+      label_level_kernel<<<(unsigned)cnt, threads, 0, c->stream>>>(
+          d_lvl + lvl_off[(size_t)k], d_depth, d_row, d_boff, d_bag, d_blen, d_bcnt, d_bhop, dis, lcnt, lhop);
       HIP_OK(hipGetLastError());
     }
     root_far_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(n, d_row, dis, d_far);
