@@ -151,45 +151,62 @@ void eliminate(const Graph &g, const std::vector<int32_t> &order,
     // linear in the lists (the wide top of the tree has neighbour lists of
     // hundreds).  Each unordered pair is updated from both of its ends with
     // the same values, so the adjacency stays symmetric.
-    for (size_t i = 0; i < nb.size(); ++i) {
+    // Neighbour i's new list.  The neighbours' lists are independent of each
+    // other, so a large neighbourhood (the dense top of the tree) merges
+    // them in parallel; each list is the same either way.
+    auto merge_one = [&](size_t i, std::vector<Arc> &out, int32_t &of) {
       const Arc &a = nb[i];
       auto &ay = adj[(size_t)a.nb];
-      merged.clear();
-      merged.reserve(ay.size() + nb.size());
+      out.clear();
+      out.reserve(ay.size() + nb.size());
       size_t p = 0;
       int32_t added = 0;
       for (size_t j = 0; j <= nb.size(); ++j) {
         const int32_t z = j < nb.size() ? nb[j].nb : INT32_MAX;
         for (; p < ay.size() && ay[p].nb < z; ++p)
-          if (ay[p].nb != x) merged.push_back(ay[p]);
+          if (ay[p].nb != x) out.push_back(ay[p]);
         if (j == nb.size()) break;
         if (j == i) continue;
         const Arc &b = nb[j];
         const int32_t L = (int32_t)std::min<int64_t>((int64_t)a.len + b.len, kInf);
-        const uint64_t c = mul_sat(a.cnt, b.cnt, ovf);
+        const uint64_t c = mul_sat(a.cnt, b.cnt, of);
         const uint32_t h = a.hop + b.hop;
         if (p < ay.size() && ay[p].nb == z) {
           Arc e = ay[p++];
           if (e.len == L) {
-            e.cnt = add_sat(e.cnt, c, ovf);
+            e.cnt = add_sat(e.cnt, c, of);
             if (h > e.hop) e.hop = h;
           } else if (e.len > L) {
             e.len = L;
             e.cnt = c;
             e.hop = h;
           }
-          merged.push_back(e);
+          out.push_back(e);
         } else {
-          merged.push_back(Arc{z, L, c, h});
+          out.push_back(Arc{z, L, c, h});
           ++added;
         }
       }
-      ay.swap(merged);
+      ay.swap(out);
       if (min_degree) {
         tDD[(size_t)a.nb] = (int32_t)ay.size();
         tDD2[(size_t)a.nb] += added;
         changed[(size_t)a.nb] = 1;
       }
+    };
+    int64_t work = 0;
+    for (const Arc &a : nb) work += (int64_t)adj[(size_t)a.nb].size() + (int64_t)nb.size();
+    if (nb.size() >= 16 && work >= (int64_t)1 << 16) {
+      int32_t of_any = 0;
+#pragma omp parallel reduction(| : of_any)
+      {
+        std::vector<Arc> out;
+#pragma omp for schedule(dynamic, 2)
+        for (int64_t i = 0; i < (int64_t)nb.size(); ++i) merge_one((size_t)i, out, of_any);
+      }
+      ovf |= of_any;
+    } else {
+      for (size_t i = 0; i < nb.size(); ++i) merge_one(i, merged, ovf);
     }
     if ((int32_t)nb.size() > width) width = (int32_t)nb.size();
     bags[(size_t)x] = std::move(nb);
@@ -490,14 +507,12 @@ void min_degree_order(const Graph &g, std::vector<int32_t> &order, int64_t *shor
     arcs_out += (int64_t)nb.size();
     width = std::max(width, (int32_t)nb.size());
     // Each neighbour y loses x and, unless the neighbourhood is over 1000,
-    // gains the other neighbours: one merge of two sorted lists per y.
-    for (int32_t y : nb) {
-      if (nb.size() > 1000) {
-        drop(y, x);
-        continue;
-      }
+    // gains the other neighbours: one merge of two sorted lists per y.  The
+    // neighbours' lists are independent, so a large neighbourhood merges
+    // them in parallel.
+    auto merge_one = [&](int32_t y, std::vector<int32_t> &out) {
       auto &a = adj[(size_t)y];
-      merged.clear();
+      out.clear();
       size_t i = 0, j = 0;
       while (i < a.size() || j < nb.size()) {
         int32_t z;
@@ -509,9 +524,25 @@ void min_degree_order(const Graph &g, std::vector<int32_t> &order, int64_t *shor
           z = a[i++];
           ++j;
         }
-        if (z != x && z != y) merged.push_back(z);
+        if (z != x && z != y) out.push_back(z);
       }
-      a.swap(merged);
+      a.swap(out);
+    };
+    if (nb.size() > 1000) {
+      for (int32_t y : nb) drop(y, x);
+    } else {
+      int64_t work = 0;
+      for (int32_t y : nb) work += (int64_t)adj[(size_t)y].size() + (int64_t)nb.size();
+      if (nb.size() >= 16 && work >= (int64_t)1 << 16) {
+#pragma omp parallel
+        {
+          std::vector<int32_t> out;
+#pragma omp for schedule(dynamic, 2)
+          for (int64_t k = 0; k < (int64_t)nb.size(); ++k) merge_one(nb[(size_t)k], out);
+        }
+      } else {
+        for (int32_t y : nb) merge_one(y, merged);
+      }
     }
     for (int32_t y : nb) {
       const int32_t d = (int32_t)adj[(size_t)y].size();
