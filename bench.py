@@ -244,7 +244,9 @@ def main() -> int:
     lbar = ix.mean_scan(1 << 18, 1)
     log(f"[bench] index: height={st['tree_height']} width={st['tree_width']} "
         f"entries={st['label_entries']} bags={st['bag_entries']} Lbar={lbar:.1f} build={st['build_seconds']:.1f}s")
+    t0 = time.time()
     ctx = bcx.Context(G, ix, 0)
+    ctx_s = time.time() - t0
     n = g.n
     B = min(args.batch, n)
     ctx.set_tuning(batch=B)
@@ -398,6 +400,7 @@ def main() -> int:
             "timed_sources": K * Bs, "timed_seconds": round(el, 4),
             "full_bc_sources_per_s": round(n / full_el, 2), "full_bc_seconds": round(full_el, 4),
             "full_bc_first_call_seconds": round(full_first, 4),
+            "index_build_seconds": round(st["build_seconds"], 3), "context_create_seconds": round(ctx_s, 3),
             "labelscan_batch": B, "tchunk": tuning["tchunk"], "dep_threads": tuning["bp_threads"],
             "tree_height": st["tree_height"], "tree_width": st["tree_width"],
             "label_entries": st["label_entries"], "bag_entries": bags, "inner_nodes": inner,
